@@ -1,0 +1,154 @@
+/*
+ * trlhip.h -- device layer of the MI355X thick-restart Lanczos solver.
+ *
+ * Plain C ABI (pointers, sizes, status codes; no HIP or torch types), used
+ * by the host driver behind trlan()/ztrlan() and callable directly.  These
+ * entry points are additive: the reference has no device layer.  Each one
+ * replaces a group of reference BLAS/communication calls on the hot path,
+ * cited per function.
+ *
+ * Data layout in HBM: one Lanczos basis slab per solve, column-major,
+ * `ld` elements per column (n rounded up to 64 elements = 512 B for FP64),
+ * complex bases store interleaved (re, im) doubles.  Column k of the slab is
+ * logical basis vector k: the reference's evec(:,k) for k < mev, else
+ * base(:,k-mev) (SRC/trlcore.c:883-910).
+ *
+ * Status: 0 on success; TRLHIP_E* (negative) on failure.  The solver maps a
+ * failure to info->stat = TRLHIP_STAT_DEVICE.
+ */
+#ifndef NUTRLAN_AMD_TRLHIP_H
+#define NUTRLAN_AMD_TRLHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TRLHIP_R64 = 0,   /* real FP64 values / basis       */
+    TRLHIP_C128 = 1,  /* complex FP64 (interleaved)     */
+    TRLHIP_R32 = 2    /* real FP32 values / basis       */
+};
+
+enum {
+    TRLHIP_OK = 0,
+    TRLHIP_EINVAL = -1,   /* bad argument                          */
+    TRLHIP_ENOMEM = -2,   /* device or pinned allocation failed    */
+    TRLHIP_EHIP = -3,     /* a HIP runtime call failed             */
+    TRLHIP_ENODEV = -4    /* no usable GPU                          */
+};
+
+/* info->stat value for a device failure inside trlan()/ztrlan(). */
+#define TRLHIP_STAT_DEVICE (-1201)
+
+typedef struct trlhip_csr trlhip_csr;   /* device CSR operator       */
+typedef struct trlhip_ctx trlhip_ctx;   /* device basis + workspace  */
+
+/* ---- device CSR operator ------------------------------------------ */
+
+/* Copy a CSR matrix (row_ptr[n+1], col_idx[nnz] int32, values[nnz] of
+ * `vtype`: double, interleaved complex double, or float) to the device.
+ * Rows must be sorted by column for best locality (not required). */
+int trlhip_csr_create(trlhip_csr **A, int n, long nnz, const int *row_ptr,
+                      const int *col_idx, const void *values, int vtype);
+int trlhip_csr_destroy(trlhip_csr *A);
+
+/* trl_matvec-compatible operator for a trlhip_csr passed as mvparam.
+ * Called with host vectors it stages x to the device, multiplies there and
+ * copies y back.  trlan() recognises this function and applies the matrix
+ * to its device-resident basis directly, with no host staging.
+ * Replaces: the user `op` callback (SRC/trlcore.c:938-942, 1098-1102). */
+void trlhip_csr_matvec(const int nrow, const int ncol, const double *x,
+                       const int ldx, double *y, const int ldy,
+                       void *mvparam);
+/* Same for complex (ztrl_matvec-compatible; x, y are trl_dcomplex*). */
+void trlhip_csr_zmatvec(const int nrow, const int ncol, const void *x,
+                        const int ldx, void *y, const int ldy, void *mvparam);
+
+/* ---- basis context -------------------------------------------------- */
+
+/* Allocate an n x ncols basis of `vtype` plus reduction scratch. */
+int trlhip_ctx_create(trlhip_ctx **c, int n, int ncols, int vtype);
+int trlhip_ctx_destroy(trlhip_ctx *c);
+int trlhip_ctx_ld(const trlhip_ctx *c);
+
+/* Host <-> device column transfers (elements of the context's vtype;
+ * complex columns are 2n doubles).  Replace the reference's memcpy of
+ * vectors between evec/base (SRC/trlcore.c:1395-1403). */
+int trlhip_col_set(trlhip_ctx *c, int col, const void *x);
+int trlhip_col_get(trlhip_ctx *c, int col, void *x);
+int trlhip_col_copy(trlhip_ctx *c, int dst, int src);
+int trlhip_col_scale(trlhip_ctx *c, int col, double s);
+
+/* y = A x for basis columns x, y; *xdoty = Re(x^H y) (reduced on device).
+ * Replaces: op + trl_ddot + trl_g_sum (SRC/trlcore.c:1098-1108). */
+int trlhip_spmv(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol,
+                double *xdoty);
+
+/* One Lanczos three-term step after r = A q_a is in column rcol:
+ *   alpha = Re(q_a^H r);  r -= alpha q_a + beta_prev q_b;  *rr2 = ||r||^2.
+ * If A != NULL the product is formed on the device first (y = A q_a into
+ * rcol); q_b < 0 means no q_b term.
+ * Replaces: SRC/trlcore.c:1098-1117 (op, ddot, g_sum, 2 daxpy) and the
+ * ||r||^2 of trl_orth (1865-1871). */
+int trlhip_lanczos_step(trlhip_ctx *c, const trlhip_csr *A, int qa, int qb,
+                        double beta_prev, int rcol, double *alpha,
+                        double *rr2);
+
+/* First step after a restart: r = A q_a (if A != NULL),
+ * alpha = Re(q_a^H r), then r -= sum_{i<jnd} coef_i V(:,i) with
+ * coef[0:jnd-1) = beta[0:jnd-1) and coef[jnd-1] = alpha; *rr2 = ||r||^2.
+ * Replaces: SRC/trlcore.c:938-1017. */
+int trlhip_restart_step(trlhip_ctx *c, const trlhip_csr *A, int qa,
+                        const double *beta, int jnd, int rcol,
+                        double *alpha, double *rr2);
+
+/* One classical Gram-Schmidt pass against columns [c0, c0+m):
+ *   h = V^H r;  r -= V h;  *hh = ||h||^2;  *rr2 = ||r||^2.
+ * h is kept on the device; if h_out != NULL it is also returned (m
+ * doubles, 2m for complex).
+ * Replaces: one trip of the trl_cgs loop (SRC/trlcore.c:2993-3032:
+ * trl_g_dot_, two dgemv, two ddot, g_sum). */
+int trlhip_cgs_pass(trlhip_ctx *c, int c0, int m, int rcol, double *hh,
+                    double *rr2, double *h_out);
+
+/* h = V(:, c0:c0+m)^H r (no update).  Replaces trl_g_dot_
+ * (SRC/trl_comm_none.c:238-329). */
+int trlhip_proj(trlhip_ctx *c, int c0, int m, int rcol, double *h);
+
+/* r -= V(:, c0:c0+m) h for a host h; *rr2 = ||r||^2 (may be NULL). */
+int trlhip_update(trlhip_ctx *c, int c0, int m, const double *h, int rcol,
+                  double *rr2);
+
+/* In-place restart update V(:, lck:lck+k) <- V(:, lck:lck+jl) Y with
+ * Y jl x k column-major (ldy >= jl), real FP64; complex bases are
+ * multiplied as 2n x jl real matrices.  FP64 MFMA.
+ * Replaces: trl_ritz_vectors (SRC/trlcore.c:2761-2910). */
+int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y,
+                int ldy);
+
+/* ---- measurement ---------------------------------------------------- */
+
+/* Kernel classes timed with HIP events on the solver stream when enabled. */
+enum {
+    TRLHIP_K_SPMV = 0,
+    TRLHIP_K_RECUR = 1,
+    TRLHIP_K_PROJ = 2,
+    TRLHIP_K_UPDATE = 3,
+    TRLHIP_K_RITZ = 4,
+    TRLHIP_K_MISC = 5,
+    TRLHIP_K_COUNT = 6
+};
+
+/* Enable (1) / disable (0) per-kernel event timing; resets counters. */
+void trlhip_stats_enable(int on);
+/* out[k*3 + 0] = launches, out[k*3+1] = total ms, out[k*3+2] = algorithmic
+ * bytes, for k < TRLHIP_K_COUNT.  Returns TRLHIP_K_COUNT. */
+int trlhip_stats_get(double *out, int nout);
+
+/* Number of visible GPUs (0 when none); never aborts. */
+int trlhip_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NUTRLAN_AMD_TRLHIP_H */

@@ -1,0 +1,927 @@
+/*
+ * trlan.c -- host driver of the MI355X thick-restart Lanczos solver.
+ *
+ * trlan() keeps the reference's contract (SRC/trlan.c:21-301) and makes the
+ * same decisions as the reference iteration (SRC/trlcore.c:723-1532) so the
+ * two produce the same matvec/restart counts, but its state is split
+ * differently: the Lanczos basis, the residual vector and every O(n)
+ * operation live on the GPU (trlhip.h); the host keeps only the projected
+ * problem (alpha, beta, the arrowhead/tridiagonal and its eigenvectors,
+ * at most maxlan^2 doubles) and the scalars read back from device
+ * reductions.  No MPI: the reference's trl_g_sum/trl_g_dot_ reductions are
+ * the device's block/last-block reductions.
+ *
+ * Basis columns: logical column k is device column k (the reference splits
+ * them between evec and base, SRC/trlcore.c:883-910).
+ */
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "trl_kit.h"
+#include "trlan.h"
+#include "trlan_internal.h"
+#include "ztrlan.h"
+#include "trlhip.h"
+
+#define DMIN(a, b) ((a) < (b) ? (a) : (b))
+#define DMAX(a, b) ((a) > (b) ? (a) : (b))
+
+/* ------------------------------------------------------------- helpers */
+
+double trl_rand_uniform(uint64_t *state)
+{
+    /* xorshift64*: 53 random bits -> [0, 1) */
+    uint64_t x = *state;
+    x ^= x >> 12;
+    x ^= x << 25;
+    x ^= x >> 27;
+    *state = x;
+    return (double)((x * 0x2545F4914F6CDD1Dull) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static void add_ticks(trl_info *info, clock_t *acc, double *spill, clock_t t0)
+{
+    const clock_t t1 = clock();
+    const clock_t d = t1 >= t0 ? t1 - t0 : 0;
+    if (*acc + d >= *acc) {
+        *acc += d;
+    } else {
+        *spill += (*acc + (double)d) / (double)info->clk_rate;
+        *acc = 0;
+    }
+}
+
+typedef struct {
+    trl_info *info;
+    trlhip_ctx *dev;
+    const trlhip_csr *A;   /* device operator, or NULL for a host op */
+    trl_matvec op;
+    ztrl_matvec zop;       /* complex Hermitian operator (ztrlan) */
+    trl_uo uo;
+    int cplx, E;           /* complex basis?  doubles per element */
+    int n, mev, nbas, ncols, maxlan;
+    double *alpha, *beta, *alfrot, *betrot; /* maxlan each */
+    double *rot;                            /* maxlan^2 */
+    double *lam, *res;                      /* maxlan + 1 */
+    int *tag;                               /* maxlan + 1 */
+    double *yy;                             /* maxlan^2 */
+    double *zq;                             /* maxlan^2 scratch */
+    double *dsc, *esc;                      /* maxlan scratch */
+    int *iwrk;                              /* 4 maxlan, index map leftovers */
+    double *hx, *hy;                        /* host staging for host ops */
+    uint64_t rng;
+    int rot_n;                              /* order of rot */
+} lz_t;
+
+#define DEV(call)                                 \
+    do {                                          \
+        if ((call) != 0) {                        \
+            z->info->stat = TRLHIP_STAT_DEVICE;   \
+            return TRLHIP_STAT_DEVICE;            \
+        }                                         \
+    } while (0)
+
+/* y column <- A * x column, counting time in clk_op. */
+static int lz_apply_host_op(lz_t *z, int xcol, int ycol)
+{
+    DEV(trlhip_col_get(z->dev, xcol, z->hx));
+    if (z->cplx)
+        z->zop(z->n, 1, (const trl_dcomplex *)z->hx, z->n, (trl_dcomplex *)z->hy, z->n,
+               z->info->mvparam);
+    else
+        z->op(z->n, 1, z->hx, z->n, z->hy, z->n, z->info->mvparam);
+    DEV(trlhip_col_set(z->dev, ycol, z->hy));
+    return 0;
+}
+
+/* Hand the residual to the caller's extra orthogonalization (host). */
+static int lz_user_ortho(lz_t *z, int rcol, double *change)
+{
+    DEV(trlhip_col_get(z->dev, rcol, z->hx));
+    z->uo(z->n, z->hx, change);
+    DEV(trlhip_col_set(z->dev, rcol, z->hx));
+    return 0;
+}
+
+/* Complex smoothing: the reference's trl_smooth_zz (SRC/ztrlcore.c:
+ * 1289-1332), including its last element taking the real part of itself
+ * into the imaginary sum. */
+static void smooth_zz(int n, double *z)
+{
+    if (n <= 0) return;
+    double r2 = z[0], i2 = z[1];
+    z[0] = 2 * z[0] + z[2 * (2 % n)] + z[2 * (n - 1)];
+    z[1] = 2 * z[1] + z[2 * (2 % n) + 1] + z[2 * (n - 1) + 1];
+    for (int i = 1; i < n - 1; ++i) {
+        const double r1 = z[2 * i], i1 = z[2 * i + 1];
+        z[2 * i] = 2 * z[2 * i] + z[2 * (i + 1)] + r2;
+        z[2 * i + 1] = 2 * z[2 * i + 1] + z[2 * (i + 1) + 1] + i2;
+        r2 = r1;
+        i2 = i1;
+    }
+    const int l = n - 1;
+    const double re = z[2 * l];
+    z[2 * l] = 2 * re + z[2 * (1 % n)] + r2;
+    z[2 * l + 1] = 2 * re + z[2 * (1 % n) + 1] + i2;
+}
+
+static int lz_random_column(lz_t *z, int col)
+{
+    for (int i = 0; i < z->E * z->n; ++i) z->hx[i] = trl_rand_uniform(&z->rng);
+    if (z->cplx) smooth_zz(z->n, z->hx);
+    else trlkit_smooth_rr(z->n, z->hx);
+    DEV(trlhip_col_set(z->dev, col, z->hx));
+    return 0;
+}
+
+/* Complex retry of the reference's ztrl_cgs (SRC/ztrlcore.c:2117-2178):
+ * first a segment perturbation scaled by the residual, later a random
+ * segment (or a full random vector). */
+static int lz_perturb_complex(lz_t *z, int rcol, int irnd, double rnrm, double old_rnrm)
+{
+    const int n = z->n;
+    double *x = z->hx;
+    DEV(trlhip_col_get(z->dev, rcol, x));
+    if (irnd == 1 && rnrm > 0.0 && rnrm > DBL_EPSILON * old_rnrm) {
+        const int i = (int)(n * trl_rand_uniform(&z->rng));
+        const double w = n * sqrt(DBL_EPSILON * old_rnrm / rnrm);
+        const int k = i + (int)(w > 1.0 ? w : 1.0);
+        for (int j = i; j < k && j < n; ++j) {
+            double t = trl_rand_uniform(&z->rng);
+            while (fabs(t - 0.5) <= DBL_EPSILON) t = trl_rand_uniform(&z->rng);
+            x[2 * j] += rnrm * (t - 0.5);
+            x[2 * j + 1] += rnrm * (t - 0.5);
+        }
+    } else {
+        const int i = (int)(n * trl_rand_uniform(&z->rng));
+        const int j = (int)(n * trl_rand_uniform(&z->rng));
+        const int lo = i < j ? i : j, hi = i < j ? j : i;
+        const int a = (i == j) ? 0 : lo, b = (i == j) ? n - 1 : hi;
+        for (int k = a; k <= b; ++k) {
+            x[2 * k] = trl_rand_uniform(&z->rng);
+            x[2 * k + 1] = trl_rand_uniform(&z->rng);
+        }
+    }
+    smooth_zz(n, x);
+    DEV(trlhip_col_set(z->dev, rcol, x));
+    return 0;
+}
+
+/* Iterated classical Gram-Schmidt of column rcol against columns
+ * [0, nold): the decision logic of the reference's trl_cgs
+ * (SRC/trlcore.c:2913-3117) around device passes.  Returns 0, -203, -204
+ * or the device status. */
+static int lz_cgs_complex(lz_t *z, int nold, int rcol, double *rnrm, int *north);
+
+static int lz_cgs(lz_t *z, int nold, int rcol, double *rnrm, int *north)
+{
+    const double eps = DBL_EPSILON;
+    int irnd = 0;
+    double rr2 = 0.0, hh = 0.0;
+    if (z->cplx) return lz_cgs_complex(z, nold, rcol, rnrm, north);
+    if (nold > 0) {
+        int cnt = 0;
+        while (cnt <= 3) {
+            DEV(trlhip_cgs_pass(z->dev, 0, nold, rcol, &hh, &rr2, NULL));
+            if (z->uo) {
+                double change = 0.0;
+                if (lz_user_ortho(z, rcol, &change)) return TRLHIP_STAT_DEVICE;
+                DEV(trlhip_update(z->dev, 0, 0, &change, rcol, &rr2));
+                hh += change * change;
+            }
+            ++*north;
+            ++cnt;
+            *rnrm = sqrt(rr2);
+            if (eps * rr2 > hh) break;
+            if ((cnt > 1 && nold * eps * eps * (rr2 + hh) < hh) || !(rr2 > DBL_MIN)) {
+                /* no independent direction left: one fresh random vector */
+                if (irnd > 0) break;
+                cnt = 0;
+                ++irnd;
+                z->info->nrand++;
+                if (lz_random_column(z, rcol)) return TRLHIP_STAT_DEVICE;
+            }
+        }
+        if (rr2 < hh && z->uo == NULL) return -203;
+    }
+    if (*rnrm > DBL_MIN) {
+        DEV(trlhip_col_scale(z->dev, rcol, 1.0 / *rnrm));
+    } else {
+        return -204;
+    }
+    if (irnd > 0) *rnrm = 0.0;
+    return 0;
+}
+
+/* Complex Hermitian variant: the reference's ztrl_cgs
+ * (SRC/ztrlcore.c:1977-2203).  Its randomization test scales with ntot
+ * (not nold) and allows up to three retries.  The reference's
+ * ztrl_g_dot_ mis-computes V^H r when a basis segment has exactly two
+ * columns (SRC/ztrl_comm_none.c:100, 117-119); the device projection here
+ * is the exact V^H r. */
+static int lz_cgs_complex(lz_t *z, int nold, int rcol, double *rnrm, int *north)
+{
+    const double eps = DBL_EPSILON;
+    const int maxorth = 3;
+    int irnd = 0, ierr = 0;
+    double rr2 = 0.0, hh = 0.0;
+    if (nold > 0) {
+        int cnt = 0;
+        while (cnt <= maxorth) {
+            DEV(trlhip_cgs_pass(z->dev, 0, nold, rcol, &hh, &rr2, NULL));
+            ++*north;
+            ++cnt;
+            *rnrm = sqrt(rr2);
+            const double old_rnrm = sqrt(rr2 + hh);
+            if (eps * rr2 > hh) break;
+            if (((cnt > 1 && !(hh <= z->info->ntot * eps * eps * (rr2 + hh))) || !(rr2 > DBL_MIN)) &&
+                irnd < maxorth) {
+                cnt = 0;
+                ++irnd;
+                z->info->nrand++;
+                if (lz_perturb_complex(z, rcol, irnd, *rnrm, old_rnrm)) return TRLHIP_STAT_DEVICE;
+            }
+        }
+        if (!(rr2 >= hh)) ierr = -203;
+    }
+    if (ierr == 0) {
+        if (*rnrm > DBL_MIN) {
+            DEV(trlhip_col_scale(z->dev, rcol, 1.0 / *rnrm));
+        } else {
+            return -204;
+        }
+    }
+    if (irnd > 0) *rnrm = 0.0;
+    return ierr;
+}
+
+/* Re-orthogonalize and normalize the new residual in column rcol (basis
+ * size jnd): the reference's trl_orth (SRC/trlcore.c:1766-1975). */
+static void lz_orth(lz_t *z, int jnd, int kept, int rcol, double rr2)
+{
+    trl_info *info = z->info;
+    double *alpha = z->alpha, *beta = z->beta;
+    const double eps = DBL_EPSILON;
+    const int jm1 = jnd - 1;
+    if (!(rr2 >= 0.0) || !(rr2 <= DBL_MAX)) {
+        info->stat = -102;
+        return;
+    }
+    info->stat = 0;
+    beta[jnd - 1] = sqrt(rr2);
+    double tmp = alpha[jnd - 1] * alpha[jnd - 1];
+    if (jm1 > kept) {
+        tmp += beta[jm1 - 1] * beta[jm1 - 1];
+    } else if (kept > 0) {
+        for (int i = 0; i < jm1; ++i) tmp += beta[i] * beta[i];
+    }
+    int usecgs;
+    if (jm1 == kept) {
+        usecgs = 1;
+    } else if (jnd >= info->ntot) {
+        usecgs = 0;
+    } else if (eps * rr2 >= tmp) {
+        double anorm = 0.0;
+        for (int i = 0; i < jnd; ++i) anorm = DMAX(anorm, fabs(alpha[i]) + fabs(beta[i]));
+        usecgs = beta[jm1] < eps * anorm * info->ntot;
+    } else {
+        usecgs = 1;
+    }
+    if (usecgs) {
+        const int rc = lz_cgs(z, jnd, rcol, &beta[jnd - 1], &info->north);
+        if (rc == TRLHIP_STAT_DEVICE) return;
+        info->stat = rc;
+        info->flop += 4 * z->n * jnd + z->n;
+    } else {
+        /* local re-orthogonalization against the last one or two vectors */
+        double h[4];
+        const int c0 = jnd > 1 ? jnd - 2 : 0, m = jnd > 1 ? 2 : 1;
+        if (trlhip_proj(z->dev, c0, m, rcol, h) != 0) {
+            info->stat = TRLHIP_STAT_DEVICE;
+            return;
+        }
+        alpha[jnd - 1] += h[(m - 1) * z->E]; /* real part for complex */
+        if (trlhip_update(z->dev, c0, m, h, rcol, NULL) != 0) {
+            info->stat = TRLHIP_STAT_DEVICE;
+            return;
+        }
+        if (z->uo && lz_user_ortho(z, rcol, NULL)) return;
+        if (trlhip_col_scale(z->dev, rcol, 1.0 / beta[jnd - 1]) != 0) {
+            info->stat = TRLHIP_STAT_DEVICE;
+            return;
+        }
+        info->flop += 9 * z->n;
+    }
+    if (info->stat == 0) {
+        if (beta[jnd - 1] <= eps * fabs(alpha[jnd - 1])) beta[jnd - 1] = 0.0;
+        else if (jnd >= info->ntot) beta[jnd - 1] = 0.0;
+    }
+}
+
+/* Arrowhead (alpha, beta spike in the last row/column) of order nd to
+ * tridiagonal form (alfrot, betrot) with the orthogonal rot:
+ * reference trl_tridiag (SRC/trlcore.c:1978-2074). */
+static void lz_tridiag(lz_t *z, int nd, const double *alpha, const double *beta,
+                       double *alfrot, double *betrot)
+{
+    z->rot_n = nd;
+    if (nd == 0) return;
+    if (nd == 1) {
+        z->rot[0] = 1.0;
+        alfrot[0] = alpha[0];
+        betrot[0] = beta[0];
+        return;
+    }
+    double *a = z->rot;
+    memset(a, 0, sizeof(double) * (size_t)nd * nd);
+    for (int i = 0; i < nd; ++i) a[(size_t)i * nd + i] = alpha[i];
+    for (int i = 0; i < nd - 1; ++i) a[(size_t)(nd - 1) * nd + i] = beta[i];
+    trlkit_householder(nd, a, alfrot, betrot);
+    betrot[nd - 1] = beta[nd - 1];
+}
+
+/* Ritz values and residual estimates of the current tridiagonal
+ * (reference trl_get_eval, SRC/trlcore.c:2077-2146): lam ascending per
+ * block, res = beta_jnd * |last eigenvector component|, tag = index of the
+ * Ritz vector in the QL ordering (-1-i for locked i). */
+static int lz_get_eval(lz_t *z, int jnd, int locked)
+{
+    const int m = jnd - locked;
+    for (int i = 0; i < locked; ++i) {
+        z->lam[i] = z->alfrot[i];
+        z->res[i] = 0.0;
+        z->tag[i] = -1 - i;
+    }
+    memcpy(z->lam + locked, z->alfrot + locked, sizeof(double) * (size_t)m);
+    memcpy(z->esc, z->betrot + locked, sizeof(double) * (size_t)(m > 1 ? m - 1 : 0));
+    if (trlkit_tridiag_ql(m, z->lam + locked, z->esc, z->dsc, 1) != 0) return -122;
+    for (int i = 0; i < m; ++i) {
+        z->res[locked + i] = z->betrot[jnd - 1] * fabs(z->dsc[i]);
+        z->tag[locked + i] = i;
+    }
+    return 0;
+}
+
+/* Eigenvectors (columns of yy, order nd) of the tridiagonal alf/bet for
+ * the nlam selected Ritz values (tags), then the rotation rot applied to
+ * rows irot..irot+nrot-1: the role of the reference's trl_get_tvec
+ * (dstein + rot, SRC/trlcore.c:2381-2490).  Here the vectors come from
+ * the same QL iteration that produced the Ritz values. */
+static int lz_get_tvec(lz_t *z, int nd, int irot, int nrot, const double *alf,
+                       const double *bet, int nlam, const int *tag, double *yy)
+{
+    const int m = nd - irot; /* active block order (rows irot..nd-1) */
+    double *Zq = z->zq;
+    memset(Zq, 0, sizeof(double) * (size_t)m * m);
+    for (int i = 0; i < m; ++i) Zq[(size_t)i * m + i] = 1.0;
+    memcpy(z->dsc, alf + irot, sizeof(double) * (size_t)m);
+    memcpy(z->esc, bet + irot, sizeof(double) * (size_t)(m > 1 ? m - 1 : 0));
+    if (trlkit_tridiag_ql(m, z->dsc, z->esc, Zq, 2) != 0) return -132;
+    for (int j = 0; j < nlam; ++j) {
+        double *y = yy + (size_t)j * nd;
+        memset(y, 0, sizeof(double) * (size_t)nd);
+        if (tag[j] < 0) {
+            y[-1 - tag[j]] = 1.0;
+        } else {
+            memcpy(y + irot, Zq + (size_t)tag[j] * m, sizeof(double) * (size_t)m);
+        }
+    }
+    /* dstein bookkeeping the reference leaves in iwrk */
+    for (int i = 0; i < nd && i < 4 * z->maxlan; ++i) z->iwrk[i] = 1;
+    for (int i = nd; i < 2 * nd && i < 4 * z->maxlan; ++i) z->iwrk[i] = nd;
+    if (nrot > 1) {
+        double *tmp = z->dsc;
+        for (int j = 0; j < nlam; ++j) {
+            double *y = yy + (size_t)j * nd + irot;
+            for (int r = 0; r < nrot; ++r) {
+                double s = 0.0;
+                for (int q = 0; q < nrot; ++q) s += z->rot[(size_t)q * nrot + r] * y[q];
+                tmp[r] = s;
+            }
+            memcpy(y, tmp, sizeof(double) * (size_t)nrot);
+        }
+    }
+    return 0;
+}
+
+/* Eigen-decomposition of the projected arrowhead matrix (diag alpha, the
+ * first `kept` betas in column `kept`, the rest on the superdiagonal) and
+ * the mapping of the nlam wanted values lambda onto its eigenvalues: the
+ * reference's trl_get_tvec_a (SRC/trlcore.c:2493-2673), whose index loops
+ * stop one short inside multiplicity groups (`< i2`, 2639-2653); that
+ * behaviour is kept.  Overwrites alpha[0:nd) with the eigenvalues. */
+static int lz_get_tvec_a(lz_t *z, int nd, int kept, double *alpha, const double *beta,
+                         int nlam, const double *lambda, double *yy)
+{
+    if (nlam <= 0) return 0;
+    memset(yy, 0, sizeof(double) * (size_t)nd * nd);
+    for (int i = 0; i < nd; ++i) yy[(size_t)i * nd + i] = alpha[i];
+    for (int i = 0; i < kept; ++i) yy[(size_t)kept * nd + i] = beta[i];
+    for (int i = kept; i < nd - 1; ++i) yy[(size_t)(i + 1) * nd + i] = beta[i];
+    if (trlkit_syev(nd, yy, alpha) != 0) return -142;
+    if (nlam >= nd) return 0;
+    double tol = DMAX(alpha[nd - 1] - alpha[0], DMAX(fabs(alpha[nd - 1]), fabs(alpha[0])));
+    tol = DBL_EPSILON * tol * nd;
+    int *map = z->iwrk;
+    int i = 0, j = 0;
+    while (i < nlam) {
+        int ii = j;
+        j = nd - 1;
+        for (; ii < nd; ++ii)
+            if (!(alpha[ii] < lambda[i] - tol)) {
+                j = ii;
+                break;
+            }
+        if (alpha[j] > lambda[i] + tol) return -143;
+        int i2 = nlam - 1;
+        for (ii = i + 1; ii < nlam; ++ii)
+            if (!(lambda[ii] <= lambda[i] + tol)) {
+                i2 = ii - 1;
+                break;
+            }
+        int j2 = nd - 1;
+        for (ii = j + 1; ii < nd; ++ii)
+            if (!(alpha[ii] <= lambda[i] + tol)) {
+                j2 = ii - 1;
+                break;
+            }
+        if (j2 == j && i2 == i) {
+            map[i] = j;
+        } else if (j2 - j >= i2 - i) {
+            for (ii = i; ii < i2; ++ii) map[ii] = j + ii - i;
+        } else if (j2 < nd) {
+            i2 = i + j2 - j;
+            for (ii = i; ii < i2; ++ii) map[ii] = j + ii - i;
+        } else {
+            return -144;
+        }
+        i = i2 + 1;
+        j = j2 + 1;
+    }
+    for (i = 0; i < nlam; ++i) {
+        j = map[i];
+        if (j > i) {
+            alpha[i] = alpha[j];
+            memcpy(yy + (size_t)i * nd, yy + (size_t)j * nd, sizeof(double) * (size_t)nd);
+        }
+    }
+    return 0;
+}
+
+/* Initial vector in column nec (reference trl_initial_guess,
+ * SRC/trlcore.c:1577-1763, guess <= 1).  Random numbers come from this
+ * library's own generator (the reference seeds drand48 from clock()). */
+static int lz_initial_guess(lz_t *z, const double *evec, int lde, int *j1)
+{
+    trl_info *info = z->info;
+    const int j = info->nec;
+    double *x = z->hx;
+    const int E = z->E;
+    if (info->guess > 1) {
+        info->stat = TRLAN_STAT_NO_CHECKPOINT;
+        return info->stat;
+    }
+    if (info->guess == 1) {
+        memcpy(x, evec + (size_t)j * lde * E, sizeof(double) * E * (size_t)z->n);
+    } else if (z->cplx) {
+        /* ztrl_initial_guess (SRC/ztrlcore.c:1421-1490): (1,0), or (1,1)
+         * plus -guess random perturbations */
+        for (int k = 0; k < z->n; ++k) {
+            x[2 * k] = 1.0;
+            x[2 * k + 1] = info->guess == 0 ? 0.0 : 1.0;
+        }
+        if (info->guess < 0) {
+            int nran = DMIN(-info->guess, 4 * info->maxlan);
+            nran = 2 * (nran / 2);
+            if (nran < z->n) {
+                for (int k = 0; k < nran - 1; k += 2) {
+                    const int ii = (int)(z->n * trl_rand_uniform(&z->rng));
+                    x[2 * ii] += trl_rand_uniform(&z->rng) - 0.5;
+                    x[2 * ii + 1] += trl_rand_uniform(&z->rng) - 0.5;
+                }
+            } else {
+                for (int k = 0; k < 2 * z->n; ++k) x[k] = trl_rand_uniform(&z->rng);
+                smooth_zz(z->n, x);
+                info->nrand++;
+            }
+        }
+    } else {
+        for (int k = 0; k < z->n; ++k) x[k] = 1.0;
+        int nran = DMIN(1 - info->guess, 4 * info->maxlan);
+        nran = 2 * (nran / 2);
+        if (nran > 0 && nran < z->n) {
+            for (int k = 0; k < nran - 1; k += 2) {
+                const int ii = (int)(z->n * trl_rand_uniform(&z->rng));
+                x[ii] += trl_rand_uniform(&z->rng) - 0.5;
+            }
+        } else if (nran >= z->n) {
+            for (int k = 0; k < z->n; ++k) x[k] = trl_rand_uniform(&z->rng);
+            trlkit_smooth_rr(z->n, x);
+            info->nrand++;
+        }
+    }
+    DEV(trlhip_col_set(z->dev, j, x));
+    /* ||x||^2 on the device; a zero/inf/nan start is replaced by random */
+    double nr2 = 0.0;
+    DEV(trlhip_update(z->dev, 0, 0, &nr2, j, &nr2));
+    double rnrm = 0.0;
+    if (nr2 >= DBL_MIN && nr2 <= DBL_MAX) {
+        rnrm = sqrt(nr2);
+    } else {
+        if (lz_random_column(z, j)) return TRLHIP_STAT_DEVICE;
+        info->nrand++;
+        DEV(trlhip_update(z->dev, 0, 0, &nr2, j, &nr2));
+        rnrm = sqrt(nr2);
+    }
+    *j1 = info->nec;
+    int local_north = 0;
+    const int rc = lz_cgs(z, *j1, j, &rnrm, &local_north);
+    if (rc == TRLHIP_STAT_DEVICE) return rc;
+    info->stat = rc;
+    return rc;
+}
+
+/* ------------------------------------------------------------- driver */
+
+static int lz_alloc(lz_t *z, int maxlan)
+{
+    const size_t m = (size_t)maxlan + 2;
+    z->alpha = (double *)calloc(m, sizeof(double));
+    z->beta = (double *)calloc(m, sizeof(double));
+    z->alfrot = (double *)calloc(m, sizeof(double));
+    z->betrot = (double *)calloc(m, sizeof(double));
+    z->rot = (double *)calloc(m * m, sizeof(double));
+    z->lam = (double *)calloc(m, sizeof(double));
+    z->res = (double *)calloc(m, sizeof(double));
+    z->tag = (int *)calloc(m, sizeof(int));
+    z->yy = (double *)calloc(m * m, sizeof(double));
+    z->zq = (double *)calloc(m * m, sizeof(double));
+    z->dsc = (double *)calloc(m, sizeof(double));
+    z->esc = (double *)calloc(m, sizeof(double));
+    z->iwrk = (int *)calloc(4 * m, sizeof(int));
+    z->hx = (double *)calloc((size_t)z->n * z->E, sizeof(double));
+    z->hy = (double *)calloc((size_t)z->n * z->E, sizeof(double));
+    return (z->alpha && z->beta && z->alfrot && z->betrot && z->rot && z->lam && z->res &&
+            z->tag && z->yy && z->zq && z->dsc && z->esc && z->iwrk && z->hx && z->hy)
+               ? 0
+               : -1;
+}
+
+static void lz_free(lz_t *z)
+{
+    free(z->alpha);
+    free(z->beta);
+    free(z->alfrot);
+    free(z->betrot);
+    free(z->rot);
+    free(z->lam);
+    free(z->res);
+    free(z->tag);
+    free(z->yy);
+    free(z->zq);
+    free(z->dsc);
+    free(z->esc);
+    free(z->iwrk);
+    free(z->hx);
+    free(z->hy);
+}
+
+/* y = A x into column ycol from column xcol for either operator kind,
+ * then the three-term / restart-step recurrence on the device. */
+static int lz_matvec_step(lz_t *z, int first, int qa, int qb, double beta_prev, int jnd,
+                          int rcol, double *alpha, double *rr2)
+{
+    clock_t t0 = clock();
+    if (!z->A) {
+        if (lz_apply_host_op(z, qa, rcol)) return TRLHIP_STAT_DEVICE;
+    }
+    int rc;
+    if (first)
+        rc = trlhip_restart_step(z->dev, z->A, qa, z->beta, jnd, rcol, alpha, rr2);
+    else
+        rc = trlhip_lanczos_step(z->dev, z->A, qa, qb, beta_prev, rcol, alpha, rr2);
+    add_ticks(z->info, &z->info->clk_op, &z->info->tick_o, t0);
+    if (rc) {
+        z->info->stat = TRLHIP_STAT_DEVICE;
+        return TRLHIP_STAT_DEVICE;
+    }
+    z->info->matvec++;
+    return 0;
+}
+
+/* The thick-restart iteration; mirrors the decision sequence of the
+ * reference's trlanczos (SRC/trlcore.c:723-1532). */
+static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk, int lwrk)
+{
+    trl_info *info = z->info;
+    const int mev = z->mev;
+    double *alpha = z->alpha, *beta = z->beta, *alfrot = z->alfrot, *betrot = z->betrot;
+    int locked = info->nec;
+    if (locked > 0) {
+        memcpy(alpha, eval, sizeof(double) * (size_t)locked);
+        memset(beta, 0, sizeof(double) * (size_t)locked);
+        for (int i = 0; i < locked; ++i)
+            if (trlhip_col_set(z->dev, i, evec + (size_t)i * lde * z->E)) {
+                info->stat = TRLHIP_STAT_DEVICE;
+                return;
+            }
+    }
+    int j1 = 0;
+    if (lz_initial_guess(z, evec, lde, &j1) != 0) return;
+    int jnd = j1, kept = jnd;
+    int i1 = info->ned - jnd;
+    int next_test = i1 + DMIN(i1, DMIN(6, info->ned / 2));
+    int rr = jnd, qa = -1, qb = -1, i2 = 0;
+    while (info->matvec < info->maxmv && info->nec < info->ned) {
+        /* ---- first step of a cycle: r = A q - sum beta_i q_i */
+        ++jnd;
+        qa = jnd - 1;
+        rr = jnd;
+        if (rr >= z->ncols) {
+            info->stat = -1111;
+            return;
+        }
+        double a, rr2;
+        if (lz_matvec_step(z, 1, qa, -1, 0.0, jnd, rr, &a, &rr2)) return;
+        alpha[jnd - 1] = a;
+        beta[jnd - 1] = a;
+        clock_t t0 = clock();
+        lz_orth(z, jnd, kept, rr, rr2);
+        add_ticks(info, &info->clk_orth, &info->tick_h, t0);
+        if (info->stat != 0) return;
+        memcpy(alfrot, alpha, sizeof(double) * (size_t)locked);
+        memset(betrot, 0, sizeof(double) * (size_t)locked);
+        lz_tridiag(z, jnd - locked, alpha + locked, beta + locked, alfrot + locked, betrot + locked);
+        info->flop += 8 * (jnd - locked) * (jnd - locked) * (jnd - locked) / 3;
+        betrot[jnd - 1] = beta[jnd - 1];
+        /* ---- three-term recurrence until the basis is full */
+        while (jnd < info->klan && info->nec < info->ned) {
+            qb = qa;
+            qa = rr;
+            ++jnd;
+            rr = jnd;
+            if (rr >= z->ncols) {
+                info->stat = -1111;
+                return;
+            }
+            if (lz_matvec_step(z, 0, qa, qb, beta[jnd - 2], jnd, rr, &a, &rr2)) return;
+            alpha[jnd - 1] = a;
+            info->flop += 6 * z->n;
+            t0 = clock();
+            lz_orth(z, jnd, kept, rr, rr2);
+            add_ticks(info, &info->clk_orth, &info->tick_h, t0);
+            alfrot[jnd - 1] = alpha[jnd - 1];
+            betrot[jnd - 1] = beta[jnd - 1];
+            if (info->stat != 0) return;
+            if (info->matvec >= next_test) {
+                if ((info->stat = lz_get_eval(z, jnd, locked)) != 0) return;
+                memcpy(eval, z->lam, sizeof(double) * (size_t)DMIN(mev, jnd));
+                trlkit_convergence_test(jnd, z->lam, z->res, z->tag, info);
+                if (info->nec < info->ned && info->nec > 0) {
+                    next_test = (int)((double)(info->ned * info->matvec) / (double)info->nec);
+                } else if (info->nec == 0) {
+                    next_test += next_test;
+                    if (info->maxlan == info->ntot)
+                        next_test = (int)ceil(0.5 * (info->maxlan + info->matvec));
+                }
+            }
+        }
+        /* ---- restart or finish */
+        clock_t tr = clock();
+        const int prek = kept;
+        const int jml = jnd - locked;
+        i2 = kept - locked + 1;
+        if (info->nec < info->ned) {
+            if ((info->stat = lz_get_eval(z, jnd, locked)) != 0) return;
+            trlkit_convergence_test(jnd, z->lam, z->res, z->tag, info);
+        }
+        if (info->nec < info->ned && info->matvec < info->maxmv) {
+            /* lambda[jnd] aliases res[0] in the reference's workspace */
+            z->lam[jnd] = z->res[0];
+            kept = trlkit_shuffle_eig(jml, info->klan - locked, z->lam + locked, z->res + locked,
+                                      z->tag + locked, info, kept);
+            int st = 0;
+            if (kept * 3 < jml) {
+                st = lz_get_tvec(z, jml, 0, i2, alfrot + locked, betrot + locked, kept,
+                                 z->tag + locked, z->yy);
+                if (st == 0 && locked + kept > 0)
+                    memcpy(alpha, z->lam, sizeof(double) * (size_t)(locked + kept));
+            }
+            if (kept * 3 >= jml || st != 0) {
+                if (locked + kept > 0) memcpy(alfrot, z->lam, sizeof(double) * (size_t)(locked + kept));
+                st = lz_get_tvec_a(z, jml, prek - locked, alpha + locked, beta + locked, kept,
+                                   alfrot + locked, z->yy);
+            }
+            if (st != 0) {
+                info->stat = st;
+                return;
+            }
+            for (int i = 0; i < kept; ++i)
+                beta[locked + i] = z->yy[(size_t)(i + 1) * jml - 1] * betrot[jnd - 1];
+            if (jml > info->ned + (info->ned / 5 + 6))
+                i2 = trlkit_set_locking(jml, kept, alpha + locked, beta + locked, z->yy, info->anrm);
+            else
+                i2 = 0;
+            if (trlhip_ritz(z->dev, locked, jml, kept, z->yy, jml) != 0) {
+                info->stat = TRLHIP_STAT_DEVICE;
+                return;
+            }
+            info->flop += 2 * z->n * jml * kept;
+            kept += locked;
+            locked += i2;
+            info->locked = locked;
+            jnd = kept;
+            if (jnd > mev && jnd - mev >= z->nbas - 1) {
+                info->stat = -1111;
+                return;
+            }
+            if (info->nec > 0)
+                next_test = (int)(((double)(info->matvec * info->ned)) / ((double)info->nec));
+            else
+                next_test += info->maxlan;
+            if (DMIN(mev, jnd) > 0) memcpy(eval, z->lam, sizeof(double) * (size_t)DMIN(mev, jnd));
+            /* the residual becomes the vector after the kept Ritz vectors */
+            if (trlhip_col_copy(z->dev, jnd, rr) != 0) {
+                info->stat = TRLHIP_STAT_DEVICE;
+                return;
+            }
+            rr = jnd;
+        } else {
+            kept = DMIN(info->nec, DMAX(info->ned, mev - 1));
+            info->nec = kept;
+            if (kept == 0) kept = DMIN(mev - 1, info->ned);
+            trlkit_sort_eig(jnd, info->lohi, kept, info->ref, z->lam, z->res, z->tag);
+            memcpy(eval, z->lam, sizeof(double) * (size_t)kept);
+            int st = 0;
+            if (kept * 3 < jnd)
+                st = lz_get_tvec(z, jnd, locked, i2, alfrot, betrot, kept, z->tag, z->yy);
+            if (kept * 3 >= jnd || st != 0) {
+                st = lz_get_tvec_a(z, jnd, prek, alpha, beta, kept, eval, z->yy);
+                if (st != 0) {
+                    info->stat = st;
+                    return;
+                }
+            }
+            if (kept > 0) memcpy(alpha, eval, sizeof(double) * (size_t)kept);
+            for (int i = 0; i < kept; ++i) beta[i] = betrot[jnd - 1] * z->yy[(size_t)(i + 1) * jnd - 1];
+            if (kept > 0 && trlhip_ritz(z->dev, 0, jnd, kept, z->yy, jnd) != 0) {
+                info->stat = TRLHIP_STAT_DEVICE;
+                return;
+            }
+            info->flop += 2 * z->n * jml * kept;
+            jnd = kept;
+            j1 = kept;
+            /* Ritz vectors (and the last residual after them) to the caller */
+            for (int i = 0; i < kept && i < mev; ++i)
+                if (trlhip_col_get(z->dev, i, evec + (size_t)i * lde * z->E)) {
+                    info->stat = TRLHIP_STAT_DEVICE;
+                    return;
+                }
+            if (kept < mev && trlhip_col_get(z->dev, rr, evec + (size_t)kept * lde * z->E)) {
+                info->stat = TRLHIP_STAT_DEVICE;
+                return;
+            }
+        }
+        add_ticks(info, &info->clk_res, &info->tick_r, tr);
+        info->nloop++;
+    }
+    /* residual-norm estimates, then what the reference's workspace holds;
+     * complex: misc[i].r (stride 2) like ztrlanczos (ztrlcore.c:1166-1168) */
+    if (!wrk) return;
+    const int nw = DMIN(mev, lwrk);
+    for (int i = 0; i < nw; ++i) wrk[(size_t)i * z->E] = i < j1 ? fabs(beta[i]) : alpha[i];
+}
+
+/* Validation and derived sizes: the reference's trl_clear_counter
+ * (SRC/trlan.c:1217-1349). */
+static void check_and_clear(trl_info *info, int nrow, int mev, int lde)
+{
+    info->stat = 0;
+    if (nrow != info->nloc || nrow > info->ntot) {
+        printf("TRLAN: info not setup for this problem.\n");
+        info->stat = -1;
+    }
+    if (info->nec < 0) info->nec = 0;
+    if (lde < nrow) {
+        printf("TRLAN: leading dimension of EVEC to small.\n");
+        info->stat = -2;
+    }
+    if (info->tol >= 1.0) info->tol = sqrt(DBL_EPSILON);
+    else if (info->tol <= DBL_MIN) info->tol = DBL_EPSILON;
+    if (info->ned + info->ned >= info->ntot && info->ned > info->ntot) {
+        info->ned = DMIN(info->ntot - 1, info->maxlan - 3);
+        printf("TRLAN: ** reduced ned to %d **\n", info->ned);
+    }
+    if (mev < info->ned) {
+        printf("TRLAN: array EVAL and EVEC can not hold wanted number of eigenpairs.\n");
+        info->stat = -3;
+    }
+    if (info->ntot < 10) {
+        printf("TRLAN is not designed to work with such a small matrix(%dx%d).\n", info->ntot,
+               info->ntot);
+        info->stat = -4;
+    }
+    if (info->maxlan < info->ned + 3) {
+        info->maxlan = info->ned + DMIN(info->ned, 20) + (int)log((double)info->ntot);
+        info->maxlan = DMIN(info->maxlan, info->ntot);
+        printf("TRLAN: ** reset maxlan to %d! **\n", info->maxlan);
+    }
+    if (info->maxlan < mev) {
+        const int ntmp = DMIN(info->ntot, DMAX(100 + info->ned, 10 * info->ned));
+        info->maxlan = mev < ntmp ? mev : ntmp;
+    }
+    if (info->maxlan < 5) {
+        printf("TRLAN must have at least 5 basis vectors, it is currently %d.\n", info->maxlan);
+        info->stat = -5;
+    }
+    info->tmv = -1;
+    info->klan = DMIN(info->maxlan, info->ntot);
+    if (info->restart >= 7)
+        info->klan = DMIN(info->maxlan, DMAX(100, DMIN(info->klan, 2 * info->ned)));
+    info->locked = info->nec;
+    info->matvec = 0;
+    info->nloop = 0;
+    info->north = 0;
+    info->nrand = 0;
+    info->flop = 0;
+    info->rflp = 0;
+    info->flop_h = 0;
+    info->rflp_h = 0;
+    info->flop_r = 0;
+    info->rflp_r = 0;
+    info->tick_t = 0.0;
+    info->clk_op = 0;
+    info->tick_o = 0.0;
+    info->clk_orth = 0;
+    info->tick_h = 0.0;
+    info->clk_res = 0;
+    info->tick_r = 0.0;
+    info->clk_in = 0;
+    info->clk_out = 0;
+    info->wrds_in = 0;
+    info->wrds_out = 0;
+    info->avgm = 0.0;
+}
+
+/* Shared entry of trlan()/ztrlan(): the reference's trlan()
+ * (SRC/trlan.c:21-301) / ztrlan() (SRC/ztrlan.c:26-221) bookkeeping around
+ * one solve on a fresh device basis. */
+void trl_solve_device(trl_matvec op, ztrl_matvec zop, trl_uo user_ortho, trl_info *info,
+                      int nrow, int mev, double *eval, double *evec, int lde, int lwrk,
+                      double *wrk, int cplx)
+{
+    const clock_t t_start = clock();
+    info->clk_tot = t_start;
+    if (info->ned > mev) {
+        printf("info->ned (%d) is larger than mev (%d) reducing info->ned to %d\n", info->ned,
+               mev, mev);
+        info->ned = mev;
+    }
+    if (info->ned <= info->nec || info->ned <= 0) return;
+    info->stat = 0;
+    check_and_clear(info, nrow, mev, lde);
+    if (info->stat != 0) return;
+
+    lz_t z;
+    memset(&z, 0, sizeof z);
+    z.info = info;
+    z.op = op;
+    z.zop = zop;
+    z.uo = user_ortho;
+    z.cplx = cplx;
+    z.E = cplx ? 2 : 1;
+    z.n = nrow;
+    z.mev = mev;
+    z.maxlan = info->maxlan;
+    z.nbas = DMAX(1, info->maxlan - mev + 1);
+    z.ncols = DMAX(info->maxlan, mev) + 1;
+    z.rng = 0x9E3779B97F4A7C15ull;
+    const int vt = cplx ? TRLHIP_C128 : TRLHIP_R64;
+    z.A = trlhip_csr_from_op(cplx ? (void (*)(void))zop : (void (*)(void))op, info->mvparam, vt);
+    if (lz_alloc(&z, DMAX(info->maxlan, mev)) != 0) {
+        info->stat = -4;
+        lz_free(&z);
+        return;
+    }
+    if (trlhip_ctx_create(&z.dev, nrow, z.ncols, vt) != 0) {
+        info->stat = TRLHIP_STAT_DEVICE;
+        lz_free(&z);
+        return;
+    }
+    lz_iterate(&z, eval, evec, lde, wrk, lwrk);
+    trlhip_ctx_destroy(z.dev);
+    lz_free(&z);
+    const clock_t t_end = clock();
+    info->tick_t += (double)(t_end - t_start) / (double)info->clk_rate;
+    info->clk_tot = 0;
+}
+
+void trlan(trl_matvec op, trl_uo user_ortho, trl_info *info, int nrow, int mev, double *eval,
+           double *evec, int lde, int lwrk, double *wrk)
+{
+    trl_solve_device(op, NULL, user_ortho, info, nrow, mev, eval, evec, lde, lwrk, wrk, 0);
+}

@@ -1,0 +1,40 @@
+/*
+ * trlan_internal.h -- declarations shared between the host driver files
+ * and the device layer; not part of the public API.
+ */
+#ifndef NUTRLAN_AMD_TRLAN_INTERNAL_H
+#define NUTRLAN_AMD_TRLAN_INTERNAL_H
+
+#include <stdint.h>
+
+#include "trlan.h"
+#include "trlhip.h"
+#include "ztrlan.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* info->stat when guess > 1 asks for a checkpoint file that cannot be
+ * read (the reference's trl_read_checkpoint codes are -212..-215). */
+#define TRLAN_STAT_NO_CHECKPOINT (-212)
+
+/* The device CSR handle behind an operator callback, or NULL when `op` is
+ * not this library's device operator for `vtype` or mvparam is not a live
+ * handle of that type. */
+const trlhip_csr *trlhip_csr_from_op(void (*op)(void), void *mvparam, int vtype);
+
+/* Uniform [0,1) from a 64-bit xorshift state (the library's own generator;
+ * the reference uses drand48 seeded from clock()). */
+double trl_rand_uniform(uint64_t *state);
+
+/* One thick-restart solve on a device basis; cplx selects ztrlan (zop,
+ * evec/wrk interleaved complex) or trlan (op).  Implemented in trlan.c. */
+void trl_solve_device(trl_matvec op, ztrl_matvec zop, trl_uo user_ortho, trl_info *info,
+                      int nrow, int mev, double *eval, double *evec, int lde, int lwrk,
+                      double *wrk, int cplx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

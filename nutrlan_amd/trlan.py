@@ -1,0 +1,325 @@
+"""Python mirror of the library's C API (include/trlan.h, ztrlan.h, trlhip.h).
+
+Loads the in-tree libnutrlan_amd.so (built by `make -C nutrlan_amd/csrc`)
+and exposes:
+  TrlInfo                 ctypes mirror of trl_info (same 800-byte layout)
+  Csr                     a device CSR operator (trlhip_csr_create)
+  solve_csr(...)          trlan()/ztrlan() with the device operator
+  solve_hostop(...)       trlan() with a Python host operator callback
+  stats_enable/stats_get  per-kernel HIP-event timing of the device layer
+
+There is no fallback: if the library is missing or a GPU call fails, these
+raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libnutrlan_amd.so")
+
+TRLHIP_R64, TRLHIP_C128, TRLHIP_R32 = 0, 1, 2
+TRLHIP_STAT_DEVICE = -1201
+KERNELS = ["spmv", "recur", "proj", "update", "ritz", "misc"]
+
+_lib = None
+
+
+class TrlInfo(ctypes.Structure):
+    """Field-for-field mirror of trl_info (include/trlan.h)."""
+
+    _fields_ = [
+        ("stat", ctypes.c_int), ("lohi", ctypes.c_int), ("ned", ctypes.c_int),
+        ("nec", ctypes.c_int), ("tol", ctypes.c_double), ("mpicomp", ctypes.c_void_p),
+        ("maxlan", ctypes.c_int), ("klan", ctypes.c_int), ("maxmv", ctypes.c_int),
+        ("restart", ctypes.c_int), ("locked", ctypes.c_int), ("guess", ctypes.c_int),
+        ("matvec", ctypes.c_int), ("nloop", ctypes.c_int), ("north", ctypes.c_int),
+        ("nrand", ctypes.c_int), ("flop", ctypes.c_int), ("flop_h", ctypes.c_int),
+        ("flop_r", ctypes.c_int), ("rflp", ctypes.c_double), ("rflp_h", ctypes.c_double),
+        ("rflp_r", ctypes.c_double), ("clk_rate", ctypes.c_long), ("clk_max", ctypes.c_long),
+        ("clk_tot", ctypes.c_long), ("clk_op", ctypes.c_long), ("clk_orth", ctypes.c_long),
+        ("clk_res", ctypes.c_long), ("tick_t", ctypes.c_double), ("tick_o", ctypes.c_double),
+        ("tick_h", ctypes.c_double), ("tick_r", ctypes.c_double), ("clk_in", ctypes.c_int),
+        ("wrds_in", ctypes.c_int), ("clk_out", ctypes.c_int), ("wrds_out", ctypes.c_int),
+        ("anrm", ctypes.c_double), ("my_pe", ctypes.c_int), ("npes", ctypes.c_int),
+        ("nloc", ctypes.c_int), ("ntot", ctypes.c_int), ("verbose", ctypes.c_int),
+        ("crat", ctypes.c_double), ("trgt", ctypes.c_double), ("tres", ctypes.c_double),
+        ("tmv", ctypes.c_int), ("avgm", ctypes.c_double), ("cfac", ctypes.c_double),
+        ("ptres", ctypes.c_double), ("predicted_crate", ctypes.c_double),
+        ("mgap", ctypes.c_double), ("mgamma", ctypes.c_double), ("gamma0", ctypes.c_double),
+        ("old_target", ctypes.c_double), ("target_id", ctypes.c_int),
+        ("old_locked", ctypes.c_int), ("k1", ctypes.c_int), ("k2", ctypes.c_int),
+        ("k", ctypes.c_int), ("rfact", ctypes.c_double), ("ref", ctypes.c_double),
+        ("log_io", ctypes.c_int), ("log_fp", ctypes.c_void_p), ("log_file", ctypes.c_char * 128),
+        ("cpflag", ctypes.c_int), ("cpio", ctypes.c_int), ("cpt_fp", ctypes.c_void_p),
+        ("cpfile", ctypes.c_char * 128), ("oldcpf", ctypes.c_char * 128),
+        ("mvparam", ctypes.c_void_p),
+    ]
+
+
+MATVEC = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                          ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                          ctypes.c_void_p)
+
+
+def available():
+    return os.path.exists(LIBPATH)
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not available():
+            raise RuntimeError(
+                "libnutrlan_amd.so not built: run `make -C nutrlan_amd/csrc` "
+                "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIBPATH)
+        P, I, D, Lg = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+        L.trl_init_info.argtypes = [ctypes.POINTER(TrlInfo), I, I, I, I, D, I, I, P]
+        L.trl_init_info.restype = None
+        L.trl_set_iguess.argtypes = [ctypes.POINTER(TrlInfo), I, I, I, ctypes.c_char_p]
+        L.trl_set_iguess.restype = None
+        L.trl_set_restart.argtypes = [ctypes.POINTER(TrlInfo), D]
+        L.trl_set_restart.restype = None
+        L.trlan.argtypes = [P, P, ctypes.POINTER(TrlInfo), I, I, P, P, I, I, P]
+        L.trlan.restype = None
+        L.ztrlan.argtypes = [P, ctypes.POINTER(TrlInfo), I, I, P, P, I, P, I, P, I]
+        L.ztrlan.restype = None
+        L.trlhip_csr_create.argtypes = [ctypes.POINTER(P), I, Lg, P, P, P, I]
+        L.trlhip_csr_create.restype = I
+        L.trlhip_csr_destroy.argtypes = [P]
+        L.trlhip_csr_destroy.restype = I
+        L.trlhip_stats_enable.argtypes = [I]
+        L.trlhip_stats_enable.restype = None
+        L.trlhip_stats_get.argtypes = [P, I]
+        L.trlhip_stats_get.restype = I
+        L.trlhip_device_count.argtypes = []
+        L.trlhip_device_count.restype = I
+        L.trl_check_ritz.argtypes = [P, ctypes.POINTER(TrlInfo), I, I, P, I, P,
+                                     ctypes.POINTER(I), P, P, P, I]
+        L.trl_check_ritz.restype = None
+        L.trl_rayleigh_quotients.argtypes = [P, ctypes.POINTER(TrlInfo), I, P, I, P, P]
+        L.trl_rayleigh_quotients.restype = None
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def fn_addr(name):
+    return ctypes.cast(getattr(lib(), name), ctypes.c_void_p).value
+
+
+class Csr:
+    """Device CSR operator (trlhip_csr); usable as mvparam with the op
+    returned by .op (trlhip_csr_matvec / trlhip_csr_zmatvec)."""
+
+    def __init__(self, rp, ci, val, complex_=False):
+        n = len(rp) - 1
+        self.n = n
+        self.complex = complex_
+        rp = np.ascontiguousarray(rp, dtype=np.int32)
+        ci = np.ascontiguousarray(ci, dtype=np.int32)
+        if complex_:
+            v = np.ascontiguousarray(np.asarray(val, dtype=np.complex128)).view(np.float64)
+            vt = TRLHIP_C128
+        else:
+            v = np.ascontiguousarray(val, dtype=np.float64)
+            vt = TRLHIP_R64
+        h = ctypes.c_void_p()
+        rc = lib().trlhip_csr_create(ctypes.byref(h), n, len(ci), _ptr(rp), _ptr(ci), _ptr(v), vt)
+        if rc != 0:
+            raise RuntimeError(f"trlhip_csr_create failed ({rc})")
+        self.handle = h
+        self.op = fn_addr("trlhip_csr_zmatvec" if complex_ else "trlhip_csr_matvec")
+
+    def close(self):
+        if self.handle:
+            lib().trlhip_csr_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def init_info(n, maxlan, lohi, ned, tol, restart, maxmv):
+    info = TrlInfo()
+    lib().trl_init_info(ctypes.byref(info), n, maxlan, lohi, ned, tol, restart, maxmv, None)
+    return info
+
+
+def _result(info, ev, evec, wrk, wall):
+    keys = ["stat", "nec", "matvec", "nloop", "north", "nrand", "locked", "klan", "maxlan", "ned"]
+    out = {k: int(getattr(info, k)) for k in keys}
+    out.update(eval=ev, evec=evec, resid=wrk, anrm=float(info.anrm), wall_s=wall,
+               crat=float(info.crat), tol=float(info.tol))
+    return out
+
+
+def solve_csr(A, lohi, ned, maxlan, tol, restart, maxmv, mev, start, rfact=0.0):
+    """trlan()/ztrlan() on the device CSR operator A with start vector
+    `start` (guess = 1).  Returns the same dict keys as oracle/ref.py."""
+    import time
+
+    n = A.n
+    info = init_info(n, maxlan, lohi, ned, tol, restart, maxmv)
+    lib().trl_set_iguess(ctypes.byref(info), 0, 1, 0, None)
+    if rfact:
+        lib().trl_set_restart(ctypes.byref(info), rfact)
+    info.mvparam = A.handle.value
+    ev = np.zeros(mev)
+    if A.complex:
+        evec = np.zeros((mev, n), dtype=np.complex128)
+        evec[0] = start
+        misc = np.zeros(max(mev, 16), dtype=np.complex128)
+        t0 = time.perf_counter()
+        lib().ztrlan(A.op, ctypes.byref(info), n, mev, _ptr(ev), _ptr(evec), n, _ptr(misc),
+                     len(misc), None, 0)
+        wall = time.perf_counter() - t0
+        wrk = misc.real[:mev].copy()
+    else:
+        evec = np.zeros((mev, n))
+        evec[0] = start
+        wrk = np.zeros(mev)
+        t0 = time.perf_counter()
+        lib().trlan(A.op, None, ctypes.byref(info), n, mev, _ptr(ev), _ptr(evec), n, mev,
+                    _ptr(wrk))
+        wall = time.perf_counter() - t0
+    if info.stat == TRLHIP_STAT_DEVICE:
+        raise RuntimeError("device failure inside trlan()")
+    return _result(info, ev, evec.T, wrk, wall)
+
+
+def solve_hostop(matvec, n, lohi, ned, maxlan, tol, restart, maxmv, mev, start):
+    """trlan() with a Python host operator y = matvec(x) (staged through
+    host memory every step; the basis still lives on the device)."""
+    import time
+
+    def cb(nrow, ncol, x, ldx, y, ldy, _p):
+        for j in range(ncol):
+            xv = np.ctypeslib.as_array(x, shape=(ldx * (j + 1),))[j * ldx:j * ldx + nrow]
+            yv = np.ctypeslib.as_array(y, shape=(ldy * (j + 1),))
+            yv[j * ldy:j * ldy + nrow] = matvec(xv.copy())
+
+    cfn = MATVEC(cb)
+    info = init_info(n, maxlan, lohi, ned, tol, restart, maxmv)
+    lib().trl_set_iguess(ctypes.byref(info), 0, 1, 0, None)
+    ev = np.zeros(mev)
+    evec = np.zeros((mev, n))
+    evec[0] = start
+    wrk = np.zeros(mev)
+    t0 = time.perf_counter()
+    lib().trlan(ctypes.cast(cfn, ctypes.c_void_p), None, ctypes.byref(info), n, mev, _ptr(ev),
+                _ptr(evec), n, mev, _ptr(wrk))
+    wall = time.perf_counter() - t0
+    if info.stat == TRLHIP_STAT_DEVICE:
+        raise RuntimeError("device failure inside trlan()")
+    return _result(info, ev, evec.T, wrk, wall)
+
+
+def stats_enable(on=True):
+    lib().trlhip_stats_enable(1 if on else 0)
+
+
+def stats_get():
+    """{kernel: (launches, total_ms, algorithmic_bytes)} since stats_enable."""
+    out = np.zeros(3 * len(KERNELS))
+    lib().trlhip_stats_get(_ptr(out), len(out))
+    return {k: (int(out[3 * i]), float(out[3 * i + 1]), float(out[3 * i + 2]))
+            for i, k in enumerate(KERNELS)}
+
+
+def device_count():
+    return int(lib().trlhip_device_count())
+
+
+class Ctx:
+    """A device basis context (trlhip_ctx) for kernel-level checks."""
+
+    def __init__(self, n, ncols, complex_=False):
+        L = lib()
+        P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        for name, args in [
+            ("trlhip_ctx_create", [ctypes.POINTER(P), I, I, I]),
+            ("trlhip_ctx_destroy", [P]),
+            ("trlhip_col_set", [P, I, P]),
+            ("trlhip_col_get", [P, I, P]),
+            ("trlhip_col_copy", [P, I, I]),
+            ("trlhip_col_scale", [P, I, D]),
+            ("trlhip_spmv", [P, P, I, I, P]),
+            ("trlhip_lanczos_step", [P, P, I, I, D, I, P, P]),
+            ("trlhip_restart_step", [P, P, I, P, I, I, P, P]),
+            ("trlhip_cgs_pass", [P, I, I, I, P, P, P]),
+            ("trlhip_proj", [P, I, I, I, P]),
+            ("trlhip_update", [P, I, I, P, I, P]),
+            ("trlhip_ritz", [P, I, I, I, P, I]),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = I
+        self.n, self.ncols, self.complex = n, ncols, complex_
+        self.E = 2 if complex_ else 1
+        h = ctypes.c_void_p()
+        rc = L.trlhip_ctx_create(ctypes.byref(h), n, ncols, TRLHIP_C128 if complex_ else TRLHIP_R64)
+        if rc != 0:
+            raise RuntimeError(f"trlhip_ctx_create failed ({rc})")
+        self.h = h
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc})")
+
+    def set(self, col, x):
+        x = np.ascontiguousarray(x, dtype=np.complex128 if self.complex else np.float64)
+        self._chk(lib().trlhip_col_set(self.h, col, _ptr(x)), "col_set")
+
+    def get(self, col):
+        x = np.zeros(self.n, dtype=np.complex128 if self.complex else np.float64)
+        self._chk(lib().trlhip_col_get(self.h, col, _ptr(x)), "col_get")
+        return x
+
+    def proj(self, c0, m, rcol):
+        h = np.zeros(m * self.E)
+        self._chk(lib().trlhip_proj(self.h, c0, m, rcol, _ptr(h)), "proj")
+        return h.view(np.complex128) if self.complex else h
+
+    def update(self, c0, m, h, rcol):
+        hv = np.ascontiguousarray(h, dtype=np.complex128 if self.complex else np.float64)
+        hv = hv if m > 0 else np.zeros(2)
+        rr2 = ctypes.c_double()
+        self._chk(lib().trlhip_update(self.h, c0, m, _ptr(hv), rcol, ctypes.byref(rr2)), "update")
+        return rr2.value
+
+    def cgs_pass(self, c0, m, rcol):
+        hh, rr2 = ctypes.c_double(), ctypes.c_double()
+        h = np.zeros(m * self.E)
+        self._chk(lib().trlhip_cgs_pass(self.h, c0, m, rcol, ctypes.byref(hh), ctypes.byref(rr2),
+                                        _ptr(h)), "cgs_pass")
+        return hh.value, rr2.value, (h.view(np.complex128) if self.complex else h)
+
+    def spmv(self, A, xcol, ycol):
+        a = ctypes.c_double()
+        self._chk(lib().trlhip_spmv(self.h, A.handle, xcol, ycol, ctypes.byref(a)), "spmv")
+        return a.value
+
+    def ritz(self, lck, jl, k, Y):
+        Y = np.asfortranarray(Y, dtype=np.float64)
+        self._chk(lib().trlhip_ritz(self.h, lck, jl, k, _ptr(Y), Y.shape[0]), "ritz")
+
+    def close(self):
+        if self.h:
+            lib().trlhip_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

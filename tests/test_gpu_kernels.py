@@ -1,0 +1,91 @@
+"""Kernel-level checks of the device layer against numpy FP64 on the same
+data (each kernel replaces a reference BLAS/communication group)."""
+import numpy as np
+import pytest
+
+from nutrlan_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("complex_", [False, True])
+@pytest.mark.parametrize("n,m", [(1000, 1), (1000, 7), (4097, 16), (20000, 33), (70001, 64)])
+def test_proj_update_cgs(gpu_lib, n, m, complex_):
+    T = gpu_lib
+    rng = np.random.default_rng(n + m)
+    dt = np.complex128 if complex_ else np.float64
+    V = rng.standard_normal((n, m + 1))
+    if complex_:
+        V = V + 1j * rng.standard_normal((n, m + 1))
+    print(f"proj/update n={n} m={m} complex={complex_}", flush=True)
+    c = T.Ctx(n, m + 2, complex_)
+    for j in range(m + 1):
+        c.set(j, V[:, j])
+    r = V[:, m].astype(dt)
+    h = c.proj(0, m, m)
+    href = V[:, :m].conj().T @ r
+    assert np.abs(h - href).max() <= 1e-12 * np.abs(href).max() * np.sqrt(n)
+    hh, rr2, h2 = c.cgs_pass(0, m, m)
+    rnew = r - V[:, :m] @ href
+    got = c.get(m)
+    assert np.abs(got - rnew).max() <= 1e-11 * np.abs(r).max() * m
+    assert abs(rr2 - np.vdot(rnew, rnew).real) <= 1e-11 * abs(np.vdot(rnew, rnew))
+    assert abs(hh - np.vdot(href, href).real) <= 1e-11 * abs(np.vdot(href, href))
+    # explicit update with a host h
+    c.set(m, r)
+    hx = (rng.standard_normal(m) + (1j * rng.standard_normal(m) if complex_ else 0)).astype(dt)
+    rr2b = c.update(0, m, hx, m)
+    ref = r - V[:, :m] @ hx
+    assert np.abs(c.get(m) - ref).max() <= 1e-11 * np.abs(ref).max() * m
+    assert abs(rr2b - np.vdot(ref, ref).real) <= 1e-11 * np.vdot(ref, ref).real
+    c.close()
+
+
+@pytest.mark.parametrize("complex_", [False, True])
+@pytest.mark.parametrize("n,lck,jl,k", [(100, 0, 5, 3), (1000, 2, 17, 9), (5000, 0, 64, 33),
+                                         (33333, 3, 40, 40), (4099, 0, 128, 128)])
+def test_ritz_mfma_inplace(gpu_lib, n, lck, jl, k, complex_):
+    T = gpu_lib
+    rng = np.random.default_rng(jl * 7 + k)
+    ncols = lck + jl + 1
+    V = rng.standard_normal((n, ncols))
+    if complex_:
+        V = V + 1j * rng.standard_normal((n, ncols))
+    Y = rng.standard_normal((jl, k))
+    print(f"ritz n={n} lck={lck} jl={jl} k={k} complex={complex_}", flush=True)
+    c = T.Ctx(n, ncols, complex_)
+    for j in range(ncols):
+        c.set(j, V[:, j])
+    c.ritz(lck, jl, k, Y)
+    ref = V[:, lck:lck + jl] @ Y
+    for j in range(k):
+        got = c.get(lck + j)
+        assert np.abs(got - ref[:, j]).max() <= 1e-12 * np.abs(ref).max() * np.sqrt(jl), j
+    for j in list(range(lck)) + list(range(lck + k, ncols)):
+        assert np.array_equal(c.get(j), V[:, j]), j  # untouched columns
+    c.close()
+
+
+@pytest.mark.parametrize("builder", ["lap", "rand", "diag", "anderson3d"])
+def test_spmv_alpha(gpu_lib, builder):
+    T = gpu_lib
+    cplx = builder == "anderson3d"
+    if builder == "lap":
+        rp, ci, v = W.laplace3d_csr(20, seed=1)
+    elif builder == "rand":
+        rp, ci, v = W.random_sym_csr(50000, seed=2)
+    elif builder == "diag":
+        rp, ci, v = W.diag_csr(12345)
+    else:
+        rp, ci, v = W.anderson3d_csr(10, seed=3)
+    n = len(rp) - 1
+    x = W.start_vector(n, seed=5, complex_=cplx)
+    A = T.Csr(rp, ci, v, complex_=cplx)
+    c = T.Ctx(n, 2, cplx)
+    c.set(0, x)
+    a = c.spmv(A, 0, 1)
+    y = W.csr_matvec(rp, ci, v)(x)
+    assert np.abs(c.get(1) - y).max() <= 1e-13 * np.abs(y).max()
+    assert abs(a - np.vdot(x, y).real) <= 1e-12 * abs(np.vdot(x, y))
+    c.close()
+    A.close()

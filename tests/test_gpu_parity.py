@@ -1,0 +1,93 @@
+"""GPU parity: the device-resident trlan()/ztrlan() against the oracle
+(numpy restatement of the reference) on the same operator and start vector.
+
+Bars (per the FP64 path): eigenvalues within 1e-10*||A|| of the oracle's,
+residual ||A v - lambda v|| <= tol*||A|| (plus roundoff), ||V^T V - I||
+<= 1e-12 on the returned Ritz vectors, and the integer trajectory
+(matvec, nloop, north, locked) equal to the oracle's.
+"""
+import numpy as np
+import pytest
+
+from nutrlan_amd import workloads as W
+from oracle import trlan_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_solve(rp, ci, v, lohi, ned, maxlan, tol, restart, maxmv, mev, x0):
+    n = len(rp) - 1
+    info = O.init_info(n, maxlan, lohi, ned, tol, restart, maxmv)
+    O.set_iguess(info, 0, 1)
+    o = O.trlan(W.csr_matvec(rp, ci, v), info, n, mev, x0)
+    return o
+
+
+def check_against_oracle(g, o, A_mv, tol, exact_counts=True):
+    info = o["info"]
+    assert g["stat"] == 0, g["stat"]
+    assert g["nec"] == info.nec
+    nec = g["nec"]
+    anrm = max(abs(info.anrm), 1e-300)
+    dl = np.abs(g["eval"][:nec] - o["eval"][:nec]).max()
+    assert dl <= 1e-10 * anrm, (dl, anrm)
+    if exact_counts:
+        assert (g["matvec"], g["nloop"], g["north"], g["locked"]) == (
+            info.matvec, info.nloop, info.north, info.locked)
+    X = g["evec"][:, :nec]
+    G = X.conj().T @ X
+    assert np.abs(G - np.eye(nec)).max() <= 1e-12
+    for i in range(nec):
+        r = A_mv(X[:, i]) - g["eval"][i] * X[:, i]
+        assert np.linalg.norm(r) <= 1.01 * tol * anrm + 1e-12 * anrm
+
+
+CASES = [
+    # name, builder, lohi, ned, maxlan, tol, restart, maxmv, mev, seed
+    ("diag1000_lg5", lambda: W.diag_csr(1000), 1, 5, 40, 1.4e-8, 1, 5000, 5, 1),
+    ("diag2000_sm10", lambda: W.diag_csr(2000), -1, 10, 100, 1.4901e-8, 1, 2000, 10, 3),
+    ("lap10_lg8", lambda: W.laplace3d_csr(10, seed=3), 1, 8, 40, 1e-8, 1, 20000, 8, 2),
+    ("lap12_sm6", lambda: W.laplace3d_csr(12, seed=4), -1, 6, 32, 1e-8, 1, 20000, 6, 5),
+    ("rand3000_lg10", lambda: W.random_sym_csr(3000, seed=4), 1, 10, 48, 1e-8, 1, 20000, 10, 5),
+    ("and2d48_sm8", lambda: W.anderson2d_csr(48, 6.0, seed=6)[0], -1, 8, 32, 1e-8, 1, 20000, 8, 7),
+    ("lap10_both8", lambda: W.laplace3d_csr(10, seed=3), 0, 8, 40, 1e-8, 1, 20000, 8, 2),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_trlan_device_csr_vs_oracle(gpu_lib, case):
+    T = gpu_lib
+    name, build, lohi, ned, maxlan, tol, restart, maxmv, mev, seed = case
+    rp, ci, v = build()
+    n = len(rp) - 1
+    x0 = W.start_vector(n, seed=seed)
+    print(f"case {name} n={n}", flush=True)
+    A = T.Csr(rp, ci, v)
+    g = T.solve_csr(A, lohi, ned, maxlan, tol, restart, maxmv, mev, x0)
+    A.close()
+    o = oracle_solve(rp, ci, v, lohi, ned, maxlan, tol, restart, maxmv, mev, x0)
+    check_against_oracle(g, o, W.csr_matvec(rp, ci, v), tol)
+
+
+def test_trlan_host_op_matches_device_op(gpu_lib):
+    """A Python host operator (staged every step) gives the same solve."""
+    T = gpu_lib
+    rp, ci, v = W.laplace3d_csr(8, seed=9)
+    n = len(rp) - 1
+    x0 = W.start_vector(n, seed=10)
+    mv = W.csr_matvec(rp, ci, v)
+    h = T.solve_hostop(mv, n, 1, 6, 30, 1e-8, 1, 20000, 6, x0)
+    o = oracle_solve(rp, ci, v, 1, 6, 30, 1e-8, 1, 20000, 6, x0)
+    check_against_oracle(h, o, mv, 1e-8)
+
+
+@pytest.mark.parametrize("restart", [0, 2, 4, 5, 6, 8])
+def test_restart_schemes_vs_oracle(gpu_lib, restart):
+    T = gpu_lib
+    rp, ci, v = W.diag_csr(2000)
+    x0 = np.ones(2000)
+    A = T.Csr(rp, ci, v)
+    g = T.solve_csr(A, -1, 10, 100, 1.4901e-8, restart, 2000, 10, x0)
+    A.close()
+    o = oracle_solve(rp, ci, v, -1, 10, 100, 1.4901e-8, restart, 2000, 10, x0)
+    check_against_oracle(g, o, W.csr_matvec(rp, ci, v), 1.4901e-8)
