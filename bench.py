@@ -1,0 +1,176 @@
+"""Throughput of the device-resident thick-restart Lanczos solver on one GPU.
+
+Workload (one step = one complete trlan() solve, inputs resident in HBM):
+  periodic 3D 7-point Laplacian on 128^3 (n = 2,097,152, nnz = 14,680,064),
+  off-diagonal -1, diagonal 6 + U[0,1) (seed 0), FP64 CSR on the device;
+  nev = 16 largest, maxlan = 64, tol = 1e-8, restart scheme 1,
+  start vector U[0,1) (seed 1), plain FP64 path.
+value = eigenpairs with ||A v - lambda v|| <= 1e-8 ||A|| per second.
+
+Prints one JSON line (see README/DESIGN for the fields), including
+  roofline: the dominant kernel's algorithmic bytes / HIP-event time vs the
+            8 TB/s HBM peak, plus the SpMV + recurrence + Gram-Schmidt
+            aggregate;
+  cpu_baseline: the compiled reference trlan (oracle/_ref, OpenBLAS +
+            OpenMP CSR operator) on a bounded sample of the same solve.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+L = 128
+NEV, MAXLAN, TOL, RESTART, MAXMV = 16, 64, 1e-8, 1, 20000
+
+
+def build_problem():
+    from nutrlan_amd import workloads as W
+
+    rp, ci, v = W.laplace3d_csr(L, seed=0)
+    x0 = W.start_vector(L ** 3, seed=1)
+    return rp, ci, v, x0
+
+
+def residuals(rp, ci, v, evec, ev):
+    import scipy.sparse as sp
+
+    n = len(rp) - 1
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    R = A @ evec - evec * ev[None, :]
+    return (R * R).sum(axis=0) ** 0.5
+
+
+def cpu_baseline(sample_matvec, full_matvec, nthreads):
+    """Reference trlan on the same problem capped at `sample_matvec`
+    operator applications, in a child process (OpenBLAS thread count is
+    fixed when the library loads).  Returns (eigenpairs/s extrapolated to a
+    full solve, seconds, sample description) or None."""
+    from oracle import ref
+
+    if not ref.available():
+        return None
+    code = (
+        "import sys, json; sys.path.insert(0, %r)\n"
+        "import bench\n"
+        "from oracle import ref\n"
+        "rp, ci, v, x0 = bench.build_problem()\n"
+        "r = ref.solve_csr(rp, ci, v, 1, bench.NEV, bench.MAXLAN, bench.TOL, bench.RESTART, %d,"
+        " bench.NEV, x0, nthreads=%d)\n"
+        "print(json.dumps(dict(wall=r['wall_s'], matvec=r['matvec'], north=r['north'],"
+        " stat=r['stat'])))\n" % (ROOT, sample_matvec, nthreads))
+    env = dict(os.environ, OPENBLAS_NUM_THREADS=str(nthreads), OMP_NUM_THREADS=str(nthreads))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                         timeout=600)
+    if out.returncode != 0:
+        print(out.stderr[-2000:], file=sys.stderr)
+        return None
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    per_mv = r["wall"] / max(r["matvec"], 1)
+    full_s = per_mv * full_matvec
+    return NEV / full_s, r["wall"], (
+        f"reference trlan, first {r['matvec']} operator applications of the same solve "
+        f"({r['wall']:.1f} s, {r['north']} Gram-Schmidt passes); eigenpairs/s extrapolated to "
+        f"the full solve's {full_matvec} applications")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=96)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): the multi-GPU "
+              "row-block sharding is a design, not built here.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from nutrlan_amd import trlan as T
+
+    assert T.device_count() > 0, "no GPU"
+    rp, ci, v, x0 = build_problem()
+    n = len(rp) - 1
+    A = T.Csr(rp, ci, v)
+
+    def solve():
+        return T.solve_csr(A, 1, NEV, MAXLAN, TOL, RESTART, MAXMV, NEV, x0)
+
+    for _ in range(args.warmup):
+        solve()
+    T.stats_enable(True)
+    t0 = time.perf_counter()
+    outs = [solve() for _ in range(args.steps)]
+    t1 = time.perf_counter()
+    stats = T.stats_get()
+    T.stats_enable(False)
+    g = outs[-1]
+    assert g["stat"] == 0, g["stat"]
+    nec = g["nec"]
+    res = residuals(rp, ci, v, g["evec"][:, :nec], g["eval"][:nec])
+    anrm = g["anrm"]
+    good = int((res <= 1e-8 * anrm).sum())
+    elapsed = t1 - t0
+    value = good * args.steps / elapsed
+
+    # roofline: dominant kernel by time, plus the path aggregate
+    dom = max(stats, key=lambda k: stats[k][1])
+    launches, ms, byts = stats[dom]
+    achieved = byts / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    path = ["spmv", "recur", "proj", "update"]
+    pb = sum(stats[k][2] for k in path)
+    pt = sum(stats[k][1] for k in path)
+    path_gbs = pb / (pt * 1e-3) / 1e9 if pt > 0 else 0.0
+    roofline = {
+        "bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "avg_launch_us": round(ms * 1e3 / max(launches, 1), 2),
+        "bytes_per_launch": round(byts / max(launches, 1)),
+        "path_spmv_recur_cgs": {"achieved": round(path_gbs, 1),
+                                "frac": round(path_gbs / HBM_PEAK_GBS, 4)},
+        "kernels": {k: {"launches": s[0], "ms": round(s[1], 3),
+                        "GBps": round(s[2] / (s[1] * 1e-3) / 1e9, 1) if s[1] > 0 else 0.0}
+                    for k, s in stats.items()},
+    }
+    cpu = None
+    if not args.no_cpu_baseline:
+        nthreads = min(os.cpu_count() or 1, 16)
+        cb = cpu_baseline(args.cpu_sample, g["matvec"], nthreads)
+        if cb is not None:
+            cpu = {"value": round(cb[0], 4), "unit": "eigenpairs/s", "cores": nthreads,
+                   "kind": "reference", "sample": cb[2]}
+    line = {
+        "metric": "eigenpairs/sec at resid<=1e-8||A||",
+        "value": round(value, 3),
+        "unit": "eigenpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "laplace3d_128_periodic_fp64", "n": n, "nnz": int(len(ci)),
+                   "nev": NEV, "lohi": 1, "maxlan": MAXLAN, "tol": TOL, "restart": RESTART,
+                   "path": "plain FP64 device basis", "matvec": g["matvec"],
+                   "nloop": g["nloop"], "north": g["north"], "converged": nec,
+                   "converged_at_1e-8": good},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    A.close()
+
+
+if __name__ == "__main__":
+    main()

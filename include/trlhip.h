@@ -45,9 +45,12 @@ typedef struct trlhip_ctx trlhip_ctx;   /* device basis + workspace  */
 
 /* ---- device CSR operator ------------------------------------------ */
 
-/* Copy a CSR matrix (row_ptr[n+1], col_idx[nnz] int32, values[nnz] of
- * `vtype`: double, interleaved complex double, or float) to the device.
- * Rows must be sorted by column for best locality (not required). */
+/* Copy a CSR matrix (row_ptr[n+1], col_idx[nnz] int32 in [0, n)) to the
+ * device.  values: nnz doubles for TRLHIP_R64, nnz interleaved complex
+ * doubles for TRLHIP_C128; for TRLHIP_R32 nnz doubles that are kept in
+ * FP64 and also rounded to an FP32 copy (the FP32 copy drives a
+ * mixed-precision solve, the FP64 one its refinement).  Invalid structure
+ * (decreasing row_ptr, column out of range) is rejected with EINVAL. */
 int trlhip_csr_create(trlhip_csr **A, int n, long nnz, const int *row_ptr,
                       const int *col_idx, const void *values, int vtype);
 int trlhip_csr_destroy(trlhip_csr *A);
@@ -101,6 +104,24 @@ int trlhip_lanczos_step(trlhip_ctx *c, const trlhip_csr *A, int qa, int qb,
 int trlhip_restart_step(trlhip_ctx *c, const trlhip_csr *A, int qa,
                         const double *beta, int jnd, int rcol,
                         double *alpha, double *rr2);
+
+/* One Lanczos step with the first Gram-Schmidt pass done speculatively,
+ * all in one device round trip:
+ *   r = A q_a (if A != NULL) in column rcol; alpha = Re(q_a^H r);
+ *   r -= alpha q_a + beta_prev q_b        (first == 0; q_b < 0: no term)
+ *   r -= sum_{i<jnd} coef_i V(:,i)        (first != 0; coef = beta, alpha last)
+ *   h = V(:,0:jnd)^H r; s = r - V(:,0:jnd) h into the context's scratch
+ *   column (column ncols), leaving r itself in rcol.
+ * out = {alpha, ||r||^2, ||h||^2, ||s||^2}.  The caller either adopts s
+ * (trlhip_col_scale_copy from column ncols) or keeps working on r.
+ * Replaces: SRC/trlcore.c:938-1017 / 1098-1117 plus the first trip of
+ * trl_cgs (2993-3032). */
+int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int qb,
+                    double beta_prev, const double *beta, int jnd, int rcol,
+                    double out[4]);
+
+/* dst = s * src; src may be the scratch column (index ncols). */
+int trlhip_col_scale_copy(trlhip_ctx *c, int dst, int src, double s);
 
 /* One classical Gram-Schmidt pass against columns [c0, c0+m):
  *   h = V^H r;  r -= V h;  *hh = ||h||^2;  *rr2 = ||r||^2.

@@ -89,9 +89,9 @@ void trl_set_iguess(trl_info *info, int nec, int iguess, int ncps, char *cpf)
     info->guess = iguess;
     info->stat = 0;
     if (info->guess > 1 && strlen(info->oldcpf) > 0) {
-        char name[160];
-        snprintf(name, sizeof name, "%s%0*d", info->oldcpf, 1, info->my_pe);
-        if (cpf) snprintf(cpf, 128, "%s", name);
+        char name[128];
+        snprintf(name, sizeof name, "%.118s%d", info->oldcpf, info->my_pe);
+        if (cpf) memcpy(cpf, name, sizeof name);
         FILE *f = fopen(name, "r");
         if (f == NULL) info->stat = -8;
         else if (fclose(f) != 0) info->stat = -9;

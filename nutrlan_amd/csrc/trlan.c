@@ -173,22 +173,44 @@ static int lz_perturb_complex(lz_t *z, int rcol, int irnd, double rnrm, double o
 
 /* Iterated classical Gram-Schmidt of column rcol against columns
  * [0, nold): the decision logic of the reference's trl_cgs
- * (SRC/trlcore.c:2913-3117) around device passes.  Returns 0, -203, -204
- * or the device status. */
-static int lz_cgs_complex(lz_t *z, int nold, int rcol, double *rnrm, int *north);
+ * (SRC/trlcore.c:2913-3117) around device passes.  When pass1 is given
+ * ({||h||^2, ||s||^2}), the first pass has already been applied into the
+ * device scratch column (trlhip_step_cgs) and the loop starts from its
+ * result.  Returns 0, -203, -204 or the device status. */
+static int lz_cgs_complex(lz_t *z, int nold, int rcol, double *rnrm, int *north,
+                          const double *pass1);
 
-static int lz_cgs(lz_t *z, int nold, int rcol, double *rnrm, int *north)
+/* Make column rcol hold the current vector (it may still be in scratch). */
+static int lz_adopt(lz_t *z, int rcol, int *src)
+{
+    if (*src != rcol) {
+        DEV(trlhip_col_scale_copy(z->dev, rcol, *src, 1.0));
+        *src = rcol;
+    }
+    return 0;
+}
+
+static int lz_cgs(lz_t *z, int nold, int rcol, double *rnrm, int *north, const double *pass1)
 {
     const double eps = DBL_EPSILON;
-    int irnd = 0;
+    int irnd = 0, src = rcol;
     double rr2 = 0.0, hh = 0.0;
-    if (z->cplx) return lz_cgs_complex(z, nold, rcol, rnrm, north);
+    if (z->cplx) return lz_cgs_complex(z, nold, rcol, rnrm, north, pass1);
     if (nold > 0) {
         int cnt = 0;
         while (cnt <= 3) {
-            DEV(trlhip_cgs_pass(z->dev, 0, nold, rcol, &hh, &rr2, NULL));
+            if (pass1) {
+                hh = pass1[0];
+                rr2 = pass1[1];
+                src = z->ncols;
+                pass1 = NULL;
+            } else {
+                if (lz_adopt(z, rcol, &src)) return TRLHIP_STAT_DEVICE;
+                DEV(trlhip_cgs_pass(z->dev, 0, nold, rcol, &hh, &rr2, NULL));
+            }
             if (z->uo) {
                 double change = 0.0;
+                if (lz_adopt(z, rcol, &src)) return TRLHIP_STAT_DEVICE;
                 if (lz_user_ortho(z, rcol, &change)) return TRLHIP_STAT_DEVICE;
                 DEV(trlhip_update(z->dev, 0, 0, &change, rcol, &rr2));
                 hh += change * change;
@@ -204,12 +226,13 @@ static int lz_cgs(lz_t *z, int nold, int rcol, double *rnrm, int *north)
                 ++irnd;
                 z->info->nrand++;
                 if (lz_random_column(z, rcol)) return TRLHIP_STAT_DEVICE;
+                src = rcol;
             }
         }
         if (rr2 < hh && z->uo == NULL) return -203;
     }
     if (*rnrm > DBL_MIN) {
-        DEV(trlhip_col_scale(z->dev, rcol, 1.0 / *rnrm));
+        DEV(trlhip_col_scale_copy(z->dev, rcol, src, 1.0 / *rnrm));
     } else {
         return -204;
     }
@@ -223,16 +246,25 @@ static int lz_cgs(lz_t *z, int nold, int rcol, double *rnrm, int *north)
  * ztrl_g_dot_ mis-computes V^H r when a basis segment has exactly two
  * columns (SRC/ztrl_comm_none.c:100, 117-119); the device projection here
  * is the exact V^H r. */
-static int lz_cgs_complex(lz_t *z, int nold, int rcol, double *rnrm, int *north)
+static int lz_cgs_complex(lz_t *z, int nold, int rcol, double *rnrm, int *north,
+                          const double *pass1)
 {
     const double eps = DBL_EPSILON;
     const int maxorth = 3;
-    int irnd = 0, ierr = 0;
+    int irnd = 0, ierr = 0, src = rcol;
     double rr2 = 0.0, hh = 0.0;
     if (nold > 0) {
         int cnt = 0;
         while (cnt <= maxorth) {
-            DEV(trlhip_cgs_pass(z->dev, 0, nold, rcol, &hh, &rr2, NULL));
+            if (pass1) {
+                hh = pass1[0];
+                rr2 = pass1[1];
+                src = z->ncols;
+                pass1 = NULL;
+            } else {
+                if (lz_adopt(z, rcol, &src)) return TRLHIP_STAT_DEVICE;
+                DEV(trlhip_cgs_pass(z->dev, 0, nold, rcol, &hh, &rr2, NULL));
+            }
             ++*north;
             ++cnt;
             *rnrm = sqrt(rr2);
@@ -243,6 +275,7 @@ static int lz_cgs_complex(lz_t *z, int nold, int rcol, double *rnrm, int *north)
                 cnt = 0;
                 ++irnd;
                 z->info->nrand++;
+                if (lz_adopt(z, rcol, &src)) return TRLHIP_STAT_DEVICE;
                 if (lz_perturb_complex(z, rcol, irnd, *rnrm, old_rnrm)) return TRLHIP_STAT_DEVICE;
             }
         }
@@ -250,7 +283,7 @@ static int lz_cgs_complex(lz_t *z, int nold, int rcol, double *rnrm, int *north)
     }
     if (ierr == 0) {
         if (*rnrm > DBL_MIN) {
-            DEV(trlhip_col_scale(z->dev, rcol, 1.0 / *rnrm));
+            DEV(trlhip_col_scale_copy(z->dev, rcol, src, 1.0 / *rnrm));
         } else {
             return -204;
         }
@@ -261,7 +294,7 @@ static int lz_cgs_complex(lz_t *z, int nold, int rcol, double *rnrm, int *north)
 
 /* Re-orthogonalize and normalize the new residual in column rcol (basis
  * size jnd): the reference's trl_orth (SRC/trlcore.c:1766-1975). */
-static void lz_orth(lz_t *z, int jnd, int kept, int rcol, double rr2)
+static void lz_orth(lz_t *z, int jnd, int kept, int rcol, double rr2, const double *pass1)
 {
     trl_info *info = z->info;
     double *alpha = z->alpha, *beta = z->beta;
@@ -292,7 +325,7 @@ static void lz_orth(lz_t *z, int jnd, int kept, int rcol, double rr2)
         usecgs = 1;
     }
     if (usecgs) {
-        const int rc = lz_cgs(z, jnd, rcol, &beta[jnd - 1], &info->north);
+        const int rc = lz_cgs(z, jnd, rcol, &beta[jnd - 1], &info->north, pass1);
         if (rc == TRLHIP_STAT_DEVICE) return;
         info->stat = rc;
         info->flop += 4 * z->n * jnd + z->n;
@@ -539,7 +572,7 @@ static int lz_initial_guess(lz_t *z, const double *evec, int lde, int *j1)
     }
     *j1 = info->nec;
     int local_north = 0;
-    const int rc = lz_cgs(z, *j1, j, &rnrm, &local_north);
+    const int rc = lz_cgs(z, *j1, j, &rnrm, &local_north, NULL);
     if (rc == TRLHIP_STAT_DEVICE) return rc;
     info->stat = rc;
     return rc;
@@ -590,25 +623,27 @@ static void lz_free(lz_t *z)
     free(z->hy);
 }
 
-/* y = A x into column ycol from column xcol for either operator kind,
- * then the three-term / restart-step recurrence on the device. */
+/* One operator application and recurrence, with the first Gram-Schmidt
+ * pass done speculatively on the device in the same round trip
+ * (trlhip_step_cgs); pass1 receives {||h||^2, ||s||^2} of that pass. */
 static int lz_matvec_step(lz_t *z, int first, int qa, int qb, double beta_prev, int jnd,
-                          int rcol, double *alpha, double *rr2)
+                          int rcol, double *alpha, double *rr2, double *pass1)
 {
     clock_t t0 = clock();
     if (!z->A) {
         if (lz_apply_host_op(z, qa, rcol)) return TRLHIP_STAT_DEVICE;
     }
-    int rc;
-    if (first)
-        rc = trlhip_restart_step(z->dev, z->A, qa, z->beta, jnd, rcol, alpha, rr2);
-    else
-        rc = trlhip_lanczos_step(z->dev, z->A, qa, qb, beta_prev, rcol, alpha, rr2);
+    double out[4];
+    const int rc = trlhip_step_cgs(z->dev, z->A, first, qa, qb, beta_prev, z->beta, jnd, rcol, out);
     add_ticks(z->info, &z->info->clk_op, &z->info->tick_o, t0);
     if (rc) {
         z->info->stat = TRLHIP_STAT_DEVICE;
         return TRLHIP_STAT_DEVICE;
     }
+    *alpha = out[0];
+    *rr2 = out[1];
+    pass1[0] = out[2];
+    pass1[1] = out[3];
     z->info->matvec++;
     return 0;
 }
@@ -645,12 +680,12 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
             info->stat = -1111;
             return;
         }
-        double a, rr2;
-        if (lz_matvec_step(z, 1, qa, -1, 0.0, jnd, rr, &a, &rr2)) return;
+        double a, rr2, pass1[2];
+        if (lz_matvec_step(z, 1, qa, -1, 0.0, jnd, rr, &a, &rr2, pass1)) return;
         alpha[jnd - 1] = a;
         beta[jnd - 1] = a;
         clock_t t0 = clock();
-        lz_orth(z, jnd, kept, rr, rr2);
+        lz_orth(z, jnd, kept, rr, rr2, pass1);
         add_ticks(info, &info->clk_orth, &info->tick_h, t0);
         if (info->stat != 0) return;
         memcpy(alfrot, alpha, sizeof(double) * (size_t)locked);
@@ -668,11 +703,11 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
                 info->stat = -1111;
                 return;
             }
-            if (lz_matvec_step(z, 0, qa, qb, beta[jnd - 2], jnd, rr, &a, &rr2)) return;
+            if (lz_matvec_step(z, 0, qa, qb, beta[jnd - 2], jnd, rr, &a, &rr2, pass1)) return;
             alpha[jnd - 1] = a;
             info->flop += 6 * z->n;
             t0 = clock();
-            lz_orth(z, jnd, kept, rr, rr2);
+            lz_orth(z, jnd, kept, rr, rr2, pass1);
             add_ticks(info, &info->clk_orth, &info->tick_h, t0);
             alfrot[jnd - 1] = alpha[jnd - 1];
             betrot[jnd - 1] = beta[jnd - 1];

@@ -24,6 +24,9 @@ extern "C" {
  * handle of that type. */
 const trlhip_csr *trlhip_csr_from_op(void (*op)(void), void *mvparam, int vtype);
 
+/* 1 when A carries an FP32 value copy for a mixed-precision solve. */
+int trlhip_csr_is_mixed(const trlhip_csr *A);
+
 /* Uniform [0,1) from a 64-bit xorshift state (the library's own generator;
  * the reference uses drand48 seeded from clock()). */
 double trl_rand_uniform(uint64_t *state);

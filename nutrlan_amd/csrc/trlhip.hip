@@ -3,6 +3,10 @@
 // kernels (SpMV + alpha, recurrence, classical Gram-Schmidt projection and
 // update, FP64-MFMA restart GEMM).  C ABI in include/trlhip.h.
 //
+// Storage: the basis is FP64 (real), FP64 complex (interleaved re/im) or
+// FP32 (mixed precision); every reduction and every Gram-Schmidt
+// coefficient is accumulated in FP64.
+//
 // Reductions (alpha, ||r||^2, V^H r) never leave the device until the final
 // scalar: each block reduces its rows (wave shuffles, then LDS) into a
 // partial slot, and the last block to finish (atomic ticket) sums the
@@ -14,22 +18,28 @@
 #include <cstring>
 #include <vector>
 
-#include "trlhip.h"
 #include "trlan_internal.h"
+#include "trlhip.h"
 
 namespace {
 
-constexpr int kBlock = 256;             // 4 waves of 64
-constexpr int kMaxGrid = 1024;          // partial slots per reduction
+constexpr int kBlock = 256;            // 4 waves of 64
+constexpr int kMaxGrid = 1024;         // partial slots per streaming reduction
+constexpr int kProjGrid = 512;         // blocks of the projection kernel
+constexpr int kNnzBlk = 2048;          // nonzeros staged per SpMV row block
+constexpr int kRowsBlk = kBlock;       // rows per SpMV row block (at most)
 constexpr uint64_t kCsrMagic = 0x7472686970637372ull;  // "trhipcsr"
 
-// ------------------------------------------------------------------ utils
+// ticket slots of a context (one per concurrent reduction site)
+enum { T_PROJ = 0, T_SPMV = 61, T_DOT = 62, T_UPDATE = 63, T_COUNT = 64 };
 
-#define TRY(expr)                                                   \
-    do {                                                            \
-        hipError_t e_ = (expr);                                     \
-        if (e_ != hipSuccess) return TRLHIP_EHIP;                   \
+#define TRY(expr)                                  \
+    do {                                           \
+        hipError_t e_ = (expr);                    \
+        if (e_ != hipSuccess) return TRLHIP_EHIP;  \
     } while (0)
+
+// ------------------------------------------------------------ reductions
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -95,60 +105,112 @@ __device__ __forceinline__ bool last_block(unsigned *ticket, unsigned nblocks)
     return amlast != 0;
 }
 
-// Ordered sum of `nb` scalar partials by the last block; result in thread 0.
+// Sum of part[b * stride + p] over b < nb, four independent accumulators
+// so the loads overlap; the order depends only on nb.
+__device__ __forceinline__ double strided_sum(const double *part, int nb, size_t stride, size_t p,
+                                              int b0, int step)
+{
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int b = b0;
+    for (; b + 3 * step < nb; b += 4 * step) {
+        s0 += part[(size_t)b * stride + p];
+        s1 += part[(size_t)(b + step) * stride + p];
+        s2 += part[(size_t)(b + 2 * step) * stride + p];
+        s3 += part[(size_t)(b + 3 * step) * stride + p];
+    }
+    for (; b < nb; b += step) s0 += part[(size_t)b * stride + p];
+    return (s0 + s1) + (s2 + s3);
+}
+
+// Ordered sum of nb scalar partials by the last block; result in thread 0.
 __device__ __forceinline__ double ordered_sum(const double *part, int nb)
 {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kBlock) s += part[b];
-    return block_sum(s);
+    return block_sum(strided_sum(part, nb, 1, 0, threadIdx.x, kBlock));
 }
 
-// ---------------------------------------------------------------- kernels
+__device__ __forceinline__ void write_scalar(double t, double *out0, double *out1, int E)
+{
+    out0[0] = t;
+    if (out1) {
+        out1[0] = t;
+        if (E == 2) out1[1] = 0.0;
+    }
+}
 
-// y = A x (CSR), one SW-lane subgroup per row; epilogue: Re(x^H y) reduced
-// to out0[0] (and out1[0], out1[1] = 0 for a complex coefficient slot).
-// E = doubles per element (1 real, 2 complex).
-template <int E, int SW>
+// ---------------------------------------------------------------- SpMV
+
+// y = A x, CSR-stream: a block owns a row block [rb[b], rb[b+1]) whose
+// nonzeros (at most kNnzBlk) it stages as products val*x[col] in LDS with
+// fully coalesced loads, then each thread sums one row from LDS.  A row
+// block holding one very long row is reduced by the whole block.
+// Epilogue: Re(x^H y) into out0 (and out1 as a coefficient slot).
+// S = vector storage, SV = matrix value storage, E = doubles per element.
+template <typename S, typename SV, int E>
 __global__ __launch_bounds__(kBlock) void k_spmv(
-    int n, const int *__restrict__ rp, const int *__restrict__ ci,
-    const double *__restrict__ val, const double *__restrict__ x,
-    double *__restrict__ y, double *__restrict__ part,
-    unsigned *__restrict__ ticket, double *__restrict__ out0,
+    const int *__restrict__ rb, const int *__restrict__ rp, const int *__restrict__ ci,
+    const SV *__restrict__ val, const S *__restrict__ x, S *__restrict__ y,
+    double *__restrict__ part, unsigned *__restrict__ ticket, double *__restrict__ out0,
     double *__restrict__ out1)
 {
-    constexpr int subs = kBlock / SW;
-    const int sl = threadIdx.x % SW;
-    const int sub = threadIdx.x / SW;
+    __shared__ double prod[kNnzBlk * E];
+    const int r0 = rb[blockIdx.x], r1 = rb[blockIdx.x + 1];
+    const int k0 = rp[r0], nz = rp[r1] - k0;
     double acc = 0.0;
-    for (int row = blockIdx.x * subs + sub; row < n; row += gridDim.x * subs) {
-        const int kb = rp[row], ke = rp[row + 1];
-        if constexpr (E == 1) {
-            double s = 0.0;
-            for (int k = kb + sl; k < ke; k += SW) s += val[k] * x[ci[k]];
-#pragma unroll
-            for (int o = SW / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
-            if (sl == 0) {
-                y[row] = s;
-                acc += x[row] * s;
-            }
-        } else {
-            double sr = 0.0, si = 0.0;
-            for (int k = kb + sl; k < ke; k += SW) {
-                const double ar = val[2 * k], ai = val[2 * k + 1];
-                const int c = ci[k];
+    if (r1 - r0 > 1 || nz <= kNnzBlk) {
+        for (int k = threadIdx.x; k < nz; k += kBlock) {
+            const int c = ci[k0 + k];
+            if constexpr (E == 1) {
+                prod[k] = (double)val[k0 + k] * (double)x[c];
+            } else {
+                const double ar = val[2 * (k0 + k)], ai = val[2 * (k0 + k) + 1];
                 const double xr = x[2 * c], xi = x[2 * c + 1];
-                sr += ar * xr - ai * xi;
-                si += ar * xi + ai * xr;
+                prod[2 * k] = ar * xr - ai * xi;
+                prod[2 * k + 1] = ar * xi + ai * xr;
             }
-#pragma unroll
-            for (int o = SW / 2; o > 0; o >>= 1) {
-                sr += __shfl_xor(sr, o);
-                si += __shfl_xor(si, o);
-            }
-            if (sl == 0) {
+        }
+        __syncthreads();
+        const int t = threadIdx.x;
+        if (t < r1 - r0) {
+            const int row = r0 + t;
+            const int a = rp[row] - k0, b = rp[row + 1] - k0;
+            if constexpr (E == 1) {
+                double s = 0.0;
+                for (int k = a; k < b; ++k) s += prod[k];
+                y[row] = (S)s;
+                acc = (double)x[row] * (double)(S)s;
+            } else {
+                double sr = 0.0, si = 0.0;
+                for (int k = a; k < b; ++k) {
+                    sr += prod[2 * k];
+                    si += prod[2 * k + 1];
+                }
                 y[2 * row] = sr;
                 y[2 * row + 1] = si;
-                acc += x[2 * row] * sr + x[2 * row + 1] * si;
+                acc = x[2 * row] * sr + x[2 * row + 1] * si;
+            }
+        }
+    } else {
+        double sr = 0.0, si = 0.0;
+        for (int k = threadIdx.x; k < nz; k += kBlock) {
+            const int c = ci[k0 + k];
+            if constexpr (E == 1) {
+                sr += (double)val[k0 + k] * (double)x[c];
+            } else {
+                const double ar = val[2 * (k0 + k)], ai = val[2 * (k0 + k) + 1];
+                sr += ar * x[2 * c] - ai * x[2 * c + 1];
+                si += ar * x[2 * c + 1] + ai * x[2 * c];
+            }
+        }
+        sr = block_sum(sr);
+        if (E == 2) si = block_sum(si);
+        if (threadIdx.x == 0) {
+            if constexpr (E == 1) {
+                y[r0] = (S)sr;
+                acc = (double)x[r0] * (double)(S)sr;
+            } else {
+                y[2 * r0] = sr;
+                y[2 * r0 + 1] = si;
+                acc = x[2 * r0] * sr + x[2 * r0 + 1] * si;
             }
         }
     }
@@ -156,155 +218,155 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
     if (last_block(ticket, gridDim.x)) {
         const double t = ordered_sum(part, gridDim.x);
-        if (threadIdx.x == 0) {
-            out0[0] = t;
-            if (out1) {
-                out1[0] = t;
-                if (E == 2) out1[1] = 0.0;
-            }
-        }
+        if (threadIdx.x == 0) write_scalar(t, out0, out1, E);
     }
 }
 
-// Re(x^H y) over n elements, same outputs as k_spmv's epilogue.
-template <int E>
+// Re(x^H y) over n elements, outputs as k_spmv's epilogue.
+template <typename S, int E>
 __global__ __launch_bounds__(kBlock) void k_dot(
-    int n, const double *__restrict__ x, const double *__restrict__ y,
-    double *__restrict__ part, unsigned *__restrict__ ticket,
-    double *__restrict__ out0, double *__restrict__ out1)
+    long ne, const S *__restrict__ x, const S *__restrict__ y, double *__restrict__ part,
+    unsigned *__restrict__ ticket, double *__restrict__ out0, double *__restrict__ out1)
 {
     double acc = 0.0;
-    const long ne = (long)n * E;
     for (long i = blockIdx.x * kBlock + threadIdx.x; i < ne; i += (long)gridDim.x * kBlock)
-        acc += x[i] * y[i];
+        acc += (double)x[i] * (double)y[i];
     const double bs = block_sum(acc);
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
     if (last_block(ticket, gridDim.x)) {
         const double t = ordered_sum(part, gridDim.x);
-        if (threadIdx.x == 0) {
-            out0[0] = t;
-            if (out1) {
-                out1[0] = t;
-                if (E == 2) out1[1] = 0.0;
-            }
-        }
+        if (threadIdx.x == 0) write_scalar(t, out0, out1, E);
     }
 }
 
-// h(c0+cb : c0+cb+CB) = V(:, ...)^H r for the column chunk blockIdx.y.
-// Each thread streams rows (grid stride), keeps CB*E running sums in
-// registers; the wave transposes-and-reduces them, waves combine in LDS,
-// the block writes one partial row, the last block of the chunk reduces the
-// partial rows in block order into hout.
-template <int E, int CB>
+// --------------------------------------------------------- Gram-Schmidt
+
+// h = V(:, c0:c0+m)^H r.  A block owns a grid-stride set of rows and walks
+// the m columns in chunks of CB: per row it issues all CB column loads
+// before the FMAs (column indices past the chunk's end are clamped to its
+// last column, so the extra loads hit cache and are discarded), keeps CB*E
+// running sums in registers, then transposes-and-reduces them over the
+// wave, combines waves in LDS and writes its partial row.  The last block
+// sums the partial rows in block order.
+template <typename S, int E, int CB>
 __global__ __launch_bounds__(kBlock) void k_proj(
-    int n, const double *__restrict__ V, long ld, int c0, int m,
-    const double *__restrict__ r, double *__restrict__ part,
-    unsigned *__restrict__ ticket, double *__restrict__ hout)
+    int n, const S *__restrict__ V, long ld, int c0, int m, const S *__restrict__ r,
+    double *__restrict__ part, unsigned *__restrict__ ticket, double *__restrict__ hout)
 {
     constexpr int N = CB * E;
     static_assert(N <= 64, "chunk too wide");
     __shared__ double sm[kBlock / 64][N];
-    const int chunk = blockIdx.y;
-    const int cb = chunk * CB;
-    const int ncol = min(CB, m - cb);
-    const double *Vc = V + (size_t)(c0 + cb) * ld * E;
-    double acc[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) acc[i] = 0.0;
-    for (int row = blockIdx.x * kBlock + threadIdx.x; row < n; row += gridDim.x * kBlock) {
-        if constexpr (E == 1) {
-            const double rv = r[row];
-#pragma unroll
-            for (int c = 0; c < CB; ++c)
-                if (c < ncol) acc[c] += Vc[(size_t)c * ld + row] * rv;
-        } else {
-            const double rr = r[2 * row], ri = r[2 * row + 1];
-#pragma unroll
-            for (int c = 0; c < CB; ++c)
-                if (c < ncol) {
-                    const double vr = Vc[((size_t)c * ld + row) * 2];
-                    const double vi = Vc[((size_t)c * ld + row) * 2 + 1];
-                    acc[2 * c] += vr * rr + vi * ri;      // conj(v) * r
-                    acc[2 * c + 1] += vr * ri - vi * rr;
-                }
-        }
-    }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const double ws = wave_transpose_sum<N>(acc, lane);
-    constexpr int grp = 64 / N;
-    if (lane % grp == 0) sm[w][lane / grp] = ws;
-    __syncthreads();
     const int G = gridDim.x;
-    double *prow = part + ((size_t)chunk * G + blockIdx.x) * N;
-    if (threadIdx.x < N) {
-        double t = 0.0;
+    const int mE = m * E;
+    double *prow = part + (size_t)blockIdx.x * mE;
+    for (int cb = 0; cb < m; cb += CB) {
+        const int last = min(CB, m - cb) - 1;
+        const S *Vc = V + (size_t)(c0 + cb) * ld * E;
+        double acc[N];
 #pragma unroll
-        for (int i = 0; i < kBlock / 64; ++i) t += sm[i][threadIdx.x];
-        prow[threadIdx.x] = t;
-    }
-    if (last_block(ticket + chunk, G)) {
-        // 256 threads: column index p = t % N, block phase q = t / N
-        constexpr int Q = kBlock / N;
-        const int p = threadIdx.x % N, q = threadIdx.x / N;
-        double s = 0.0;
-        for (int b = q; b < G; b += Q) s += part[((size_t)chunk * G + b) * N + p];
-        __shared__ double red[kBlock];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        if (threadIdx.x < N) {
-            double t = 0.0;
-            for (int i = 0; i < Q; ++i) t += red[i * N + threadIdx.x];
-            if (threadIdx.x < ncol * E) hout[cb * E + threadIdx.x] = t;
+        for (int i = 0; i < N; ++i) acc[i] = 0.0;
+        for (int row = blockIdx.x * kBlock + threadIdx.x; row < n; row += G * kBlock) {
+            if constexpr (E == 1) {
+                double vv[CB];
+#pragma unroll
+                for (int c = 0; c < CB; ++c) vv[c] = (double)Vc[(size_t)min(c, last) * ld + row];
+                const double rv = (double)r[row];
+#pragma unroll
+                for (int c = 0; c < CB; ++c) acc[c] += vv[c] * rv;
+            } else {
+                double2 vv[CB];
+                const double2 *V2 = reinterpret_cast<const double2 *>(Vc);
+#pragma unroll
+                for (int c = 0; c < CB; ++c) vv[c] = V2[(size_t)min(c, last) * ld + row];
+                const double2 rv = reinterpret_cast<const double2 *>(r)[row];
+#pragma unroll
+                for (int c = 0; c < CB; ++c) {
+                    acc[2 * c] += vv[c].x * rv.x + vv[c].y * rv.y;  // conj(v) * r
+                    acc[2 * c + 1] += vv[c].x * rv.y - vv[c].y * rv.x;
+                }
+            }
         }
+        const double ws = wave_transpose_sum<N>(acc, lane);
+        constexpr int grp = 64 / N;
+        if (lane % grp == 0) sm[w][lane / grp] = ws;
+        __syncthreads();
+        if (threadIdx.x < (last + 1) * E) {
+            double t = 0.0;
+#pragma unroll
+            for (int i = 0; i < kBlock / 64; ++i) t += sm[i][threadIdx.x];
+            prow[cb * E + threadIdx.x] = t;
+        }
+        __syncthreads();
+    }
+    if (last_block(ticket, G)) {
+        for (int p = threadIdx.x; p < mE; p += kBlock)
+            hout[p] = strided_sum(part, G, (size_t)mE, (size_t)p, 0, 1);
     }
 }
 
-// r -= V(:, c0:c0+m) h ; ||r||^2 -> out_rr2 ; ||h||^2 -> out_hh (block 0).
-// h (E*m doubles) is staged in LDS.
-template <int E>
+// rout = rin - V(:, c0:c0+m) h ; ||rout||^2 -> out_rr2 ; ||h||^2 -> out_hh.
+// h (E*m doubles, FP64) is staged in LDS; four independent partial sums
+// per row keep the FMA chains short.  rin may equal rout.
+template <typename S, int E>
 __global__ __launch_bounds__(kBlock) void k_update(
-    int n, const double *__restrict__ V, long ld, int c0, int m,
-    const double *__restrict__ h, double *__restrict__ r,
-    double *__restrict__ part, unsigned *__restrict__ ticket,
+    int n, const S *__restrict__ V, long ld, int c0, int m, const double *__restrict__ h,
+    const S *rin, S *rout, double *__restrict__ part, unsigned *__restrict__ ticket,
     double *__restrict__ out_rr2, double *__restrict__ out_hh)
 {
     extern __shared__ double hs[];
     for (int i = threadIdx.x; i < E * m; i += kBlock) hs[i] = h[i];
     __syncthreads();
-    const double *Vc = V + (size_t)c0 * ld * E;
+    const S *Vc = V + (size_t)c0 * ld * E;
     double acc = 0.0;
     for (int row = blockIdx.x * kBlock + threadIdx.x; row < n; row += gridDim.x * kBlock) {
         if constexpr (E == 1) {
-            double s = 0.0;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int c = 0;
+            for (; c + 8 <= m; c += 8) {
+                double v[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) v[q] = (double)Vc[(size_t)(c + q) * ld + row];
+                s0 += v[0] * hs[c];
+                s1 += v[1] * hs[c + 1];
+                s2 += v[2] * hs[c + 2];
+                s3 += v[3] * hs[c + 3];
+                s0 += v[4] * hs[c + 4];
+                s1 += v[5] * hs[c + 5];
+                s2 += v[6] * hs[c + 6];
+                s3 += v[7] * hs[c + 7];
+            }
+            for (; c < m; ++c) s0 += (double)Vc[(size_t)c * ld + row] * hs[c];
+            const S rn = (S)((double)rin[row] - ((s0 + s1) + (s2 + s3)));
+            rout[row] = rn;
+            acc += (double)rn * (double)rn;
+        } else {
+            const double2 *V2 = reinterpret_cast<const double2 *>(Vc);
+            double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
             int c = 0;
             for (; c + 4 <= m; c += 4) {
-                const double v0 = Vc[(size_t)c * ld + row];
-                const double v1 = Vc[(size_t)(c + 1) * ld + row];
-                const double v2 = Vc[(size_t)(c + 2) * ld + row];
-                const double v3 = Vc[(size_t)(c + 3) * ld + row];
-                s += v0 * hs[c];
-                s += v1 * hs[c + 1];
-                s += v2 * hs[c + 2];
-                s += v3 * hs[c + 3];
+                double2 v[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[q] = V2[(size_t)(c + q) * ld + row];
+#pragma unroll
+                for (int q = 0; q < 4; q += 2) {
+                    ar += v[q].x * hs[2 * (c + q)] - v[q].y * hs[2 * (c + q) + 1];
+                    ai += v[q].x * hs[2 * (c + q) + 1] + v[q].y * hs[2 * (c + q)];
+                    br += v[q + 1].x * hs[2 * (c + q + 1)] - v[q + 1].y * hs[2 * (c + q + 1) + 1];
+                    bi += v[q + 1].x * hs[2 * (c + q + 1) + 1] + v[q + 1].y * hs[2 * (c + q + 1)];
+                }
             }
-            for (; c < m; ++c) s += Vc[(size_t)c * ld + row] * hs[c];
-            const double rn = r[row] - s;
-            r[row] = rn;
-            acc += rn * rn;
-        } else {
-            double sr = 0.0, si = 0.0;
-            for (int c = 0; c < m; ++c) {
-                const double vr = Vc[((size_t)c * ld + row) * 2];
-                const double vi = Vc[((size_t)c * ld + row) * 2 + 1];
-                const double hr = hs[2 * c], hi = hs[2 * c + 1];
-                sr += vr * hr - vi * hi;
-                si += vr * hi + vi * hr;
+            for (; c < m; ++c) {
+                const double2 v = V2[(size_t)c * ld + row];
+                ar += v.x * hs[2 * c] - v.y * hs[2 * c + 1];
+                ai += v.x * hs[2 * c + 1] + v.y * hs[2 * c];
             }
-            const double rr = r[2 * row] - sr, ri = r[2 * row + 1] - si;
-            r[2 * row] = rr;
-            r[2 * row + 1] = ri;
-            acc += rr * rr + ri * ri;
+            const double2 ri = reinterpret_cast<const double2 *>(rin)[row];
+            double2 ro;
+            ro.x = ri.x - (ar + br);
+            ro.y = ri.y - (ai + bi);
+            reinterpret_cast<double2 *>(rout)[row] = ro;
+            acc += ro.x * ro.x + ro.y * ro.y;
         }
     }
     const double bs = block_sum(acc);
@@ -320,26 +382,31 @@ __global__ __launch_bounds__(kBlock) void k_update(
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scale(long ne, double *__restrict__ x, double s)
+// dst = s * src over ne storage elements (dst may equal src).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_scale_copy(long ne, const S *src, S *dst, double s)
 {
     for (long i = blockIdx.x * kBlock + threadIdx.x; i < ne; i += (long)gridDim.x * kBlock)
-        x[i] *= s;
+        dst[i] = (S)((double)src[i] * s);
 }
+
+// ------------------------------------------------------------- restart
 
 // In-place restart GEMM V(:, lck:lck+k) <- V(:, lck:lck+jl) * Y on FP64
 // MFMA (v_mfma_f64_16x16x4_f64).  Each wave owns 16-row tiles (rows
 // disjoint across waves), reads its rows over all jl input columns into
 // the accumulators before writing any output column, so the update is safe
 // in place.  Y (jl x k) is staged in LDS as Ys[kk * kp + col], padded with
-// zeros to jl4 = ceil4(jl) rows and kp = 16*KT columns.
+// zeros to jl4 = ceil4(jl) rows and kp = 16*KT columns.  FP32 bases are
+// widened to FP64 on load and rounded on store.
 // MFMA f64 16x16x4 operand layout (lane l): A[l%16][l/16], B[l/16][l%16],
 // D[l/16 + 4*r][l%16] for accumulator element r (measured on gfx950).
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-template <int KT>
+template <typename S, int KT>
 __global__ __launch_bounds__(kBlock) void k_ritz_mfma(
-    long nrows, double *__restrict__ V, long ld, int lck, int jl, int k,
-    const double *__restrict__ Y, int ldy, int tiles_per_wave)
+    long nrows, S *__restrict__ V, long ld, int lck, int jl, int k, const double *__restrict__ Y,
+    int ldy, int tiles_per_wave)
 {
     extern __shared__ double Ys[];
     constexpr int kp = 16 * KT;
@@ -351,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void k_ritz_mfma(
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int li = lane & 15, lq = lane >> 4;
-    double *Vin = V + (size_t)lck * ld;
+    S *Vin = V + (size_t)lck * ld;
     for (int t = 0; t < tiles_per_wave; ++t) {
         const long row0 = ((long)(blockIdx.x * (kBlock / 64) + w) * tiles_per_wave + t) * 16;
         if (row0 >= nrows) break;
@@ -361,7 +428,7 @@ __global__ __launch_bounds__(kBlock) void k_ritz_mfma(
         const long arow = row0 + li;
         for (int kk = 0; kk < jl4; kk += 4) {
             const int col = kk + lq;
-            const double a = (arow < nrows && col < jl) ? Vin[(size_t)col * ld + arow] : 0.0;
+            const double a = (arow < nrows && col < jl) ? (double)Vin[(size_t)col * ld + arow] : 0.0;
 #pragma unroll
             for (int q = 0; q < KT; ++q) {
                 const double b = Ys[(kk + lq) * kp + q * 16 + li];
@@ -374,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_ritz_mfma(
 #pragma unroll
             for (int rr = 0; rr < 4; ++rr) {
                 const long row = row0 + lq + 4 * rr;
-                if (row < nrows && col < k) Vin[(size_t)col * ld + row] = acc[q][rr];
+                if (row < nrows && col < k) Vin[(size_t)col * ld + row] = (S)acc[q][rr];
             }
         }
     }
@@ -420,6 +487,7 @@ void stats_resolve()
     g_stats.pending.clear();
 }
 
+// Brackets one kernel launch with HIP events on its stream when enabled.
 struct Timed {
     hipEvent_t a = nullptr, b = nullptr;
     int k;
@@ -450,166 +518,197 @@ struct trlhip_csr {
     int n;
     long nnz;
     int vtype;
-    int esz;       // doubles per value (complex 2); float storage flagged by vtype
-    int sw;        // subgroup width per row
+    int E;          // doubles per value (complex 2)
+    int nb;         // SpMV row blocks
+    int *rb;        // nb + 1 row-block starts
     int *rp;
     int *ci;
-    double *val;
+    double *val;    // FP64 values (E per nonzero)
+    float *val32;   // FP32 copy for a mixed-precision basis (R32 only)
     // staging for host-called matvec
-    double *dx;
-    double *dy;
-    double *part;
+    double *dx, *dy, *part, *dscal;
     unsigned *tick;
-    double *dscal;
     hipStream_t s;
 };
 
 struct trlhip_ctx {
     int n;
-    int ld;
-    int ncols;
+    int ld;         // elements per column
+    int ncols;      // user columns; column ncols is an internal scratch
     int vtype;
-    int E;          // doubles per element
-    double *V;
-    double *part;   // kMaxGrid * (E*ncols + 64) partial slots
-    unsigned *tick; // 64 tickets
+    int E;          // doubles per element (complex 2)
+    size_t esz;     // bytes per element
+    char *V;
+    double *part;
+    size_t partcap; // doubles
+    unsigned *tick; // T_COUNT tickets
     double *dscal;  // 16 device scalars
-    double *dcoef;  // 2*(ncols+2) coefficients (recurrence / restart step)
+    double *dcoef;  // 2*(ncols+2) recurrence / restart-step coefficients
     double *dh;     // 2*(ncols+2) CGS projection
     double *dY;     // restart matrix
     size_t dYcap;
-    double *hbuf;   // pinned host mirror: 16 scalars + 2*(ncols+2) + ...
+    double *hbuf;   // pinned host mirror: 16 scalars + coefficients
     size_t hcap;
     double *hY;     // pinned staging for Y
-    size_t hYcap;
     hipStream_t s;
 };
 
 namespace {
 
-enum { S_ALPHA = 0, S_RR2 = 1, S_HH = 2 };
+enum { S_ALPHA = 0, S_RR2 = 1, S_HH = 2, S_RR2B = 3 };
 
-int grid_for(long work, int per_block_min)
+int grid_for(long work, int per_block_min, int cap)
 {
     long g = (work + per_block_min - 1) / per_block_min;
     if (g < 1) g = 1;
-    if (g > kMaxGrid) g = kMaxGrid;
+    if (g > cap) g = cap;
     return (int)g;
 }
 
-int pick_sw(long nnz, int n)
+bool is_f32(const trlhip_ctx *c) { return c->vtype == TRLHIP_R32; }
+
+template <typename S>
+S *colp(trlhip_ctx *c, int col)
 {
-    const double avg = n > 0 ? (double)nnz / n : 1.0;
-    if (avg <= 2.0) return 2;
-    if (avg <= 4.5) return 4;
-    if (avg <= 12.0) return 8;
-    if (avg <= 24.0) return 16;
-    if (avg <= 48.0) return 32;
-    return 64;
+    return reinterpret_cast<S *>(c->V + (size_t)col * c->ld * c->esz);
 }
 
-template <int E>
-hipError_t launch_spmv(int sw, int n, const trlhip_csr *A, const double *x,
-                       double *y, double *part, unsigned *tick, double *o0,
-                       double *o1, hipStream_t s)
+int ensure_part(trlhip_ctx *c, size_t need)
 {
-    const int subs = kBlock / sw;
-    const int g = grid_for(n, subs * 8);
-#define SPMV_CASE(SW)                                                      \
-    case SW:                                                               \
-        hipLaunchKernelGGL((k_spmv<E, SW>), dim3(g), dim3(kBlock), 0, s, n, \
-                           A->rp, A->ci, A->val, x, y, part, tick, o0, o1); \
-        break;
-    switch (sw) {
-        SPMV_CASE(2)
-        SPMV_CASE(4)
-        SPMV_CASE(8)
-        SPMV_CASE(16)
-        SPMV_CASE(32)
-        SPMV_CASE(64)
-    default:
-        return hipErrorInvalidValue;
+    if (c->partcap >= need) return 0;
+    if (c->part) {
+        (void)hipStreamSynchronize(c->s);
+        (void)hipFree(c->part);
     }
-#undef SPMV_CASE
-    return hipGetLastError();
+    c->part = nullptr;
+    c->partcap = 0;
+    if (hipMalloc((void **)&c->part, sizeof(double) * need) != hipSuccess) return TRLHIP_ENOMEM;
+    c->partcap = need;
+    return 0;
 }
 
-double spmv_bytes(const trlhip_csr *A, int E)
+double spmv_bytes(const trlhip_csr *A, int E, size_t vsz, size_t xsz)
 {
     // values + col_idx + row_ptr + x read once + y written once
-    return (double)A->nnz * (8.0 * E + 4.0) + 4.0 * (A->n + 1) + 2.0 * 8.0 * E * A->n;
+    return (double)A->nnz * ((double)vsz * E + 4.0) + 4.0 * (A->n + 1) + 2.0 * (double)xsz * E * A->n;
+}
+
+// y = A x on stream s with the row-block kernel for the context's storage.
+int launch_spmv(const trlhip_csr *A, int vtype, const void *x, void *y, double *part,
+                unsigned *tick, double *o0, double *o1, hipStream_t s)
+{
+    const dim3 g(A->nb), b(kBlock);
+    if (vtype == TRLHIP_R64)
+        hipLaunchKernelGGL((k_spmv<double, double, 1>), g, b, 0, s, A->rb, A->rp, A->ci, A->val,
+                           (const double *)x, (double *)y, part, tick, o0, o1);
+    else if (vtype == TRLHIP_C128)
+        hipLaunchKernelGGL((k_spmv<double, double, 2>), g, b, 0, s, A->rb, A->rp, A->ci, A->val,
+                           (const double *)x, (double *)y, part, tick, o0, o1);
+    else
+        hipLaunchKernelGGL((k_spmv<float, float, 1>), g, b, 0, s, A->rb, A->rp, A->ci, A->val32,
+                           (const float *)x, (float *)y, part, tick, o0, o1);
+    return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
+}
+
+bool csr_fits(const trlhip_ctx *c, const trlhip_csr *A)
+{
+    if (!A || A->magic != kCsrMagic || A->n != c->n) return false;
+    if (c->vtype == TRLHIP_C128) return A->vtype == TRLHIP_C128;
+    if (c->vtype == TRLHIP_R32) return A->val32 != nullptr;
+    return A->vtype == TRLHIP_R64 || A->vtype == TRLHIP_R32;
 }
 
 int run_spmv(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol, double *o1)
 {
-    const size_t cs = (size_t)c->ld * c->E;
-    const double *x = c->V + (size_t)xcol * cs;
-    double *y = c->V + (size_t)ycol * cs;
-    Timed t(TRLHIP_K_SPMV, c->s, spmv_bytes(A, c->E));
-    hipError_t e = (c->E == 1)
-        ? launch_spmv<1>(A->sw, c->n, A, x, y, c->part, c->tick, c->dscal + S_ALPHA, o1, c->s)
-        : launch_spmv<2>(A->sw, c->n, A, x, y, c->part, c->tick, c->dscal + S_ALPHA, o1, c->s);
-    return e == hipSuccess ? 0 : TRLHIP_EHIP;
+    int rc = ensure_part(c, (size_t)A->nb);
+    if (rc) return rc;
+    const size_t vsz = is_f32(c) ? 4 : 8;
+    Timed t(TRLHIP_K_SPMV, c->s, spmv_bytes(A, c->E, vsz, vsz));
+    return launch_spmv(A, c->vtype, c->V + (size_t)xcol * c->ld * c->esz,
+                       c->V + (size_t)ycol * c->ld * c->esz, c->part, c->tick + T_SPMV,
+                       c->dscal + S_ALPHA, o1, c->s);
 }
 
 int run_dot(trlhip_ctx *c, int xcol, int ycol, double *o1)
 {
-    const size_t cs = (size_t)c->ld * c->E;
-    const int g = grid_for((long)c->n * c->E, kBlock * 8);
-    Timed t(TRLHIP_K_MISC, c->s, 2.0 * 8.0 * c->E * c->n);
-    if (c->E == 1)
-        hipLaunchKernelGGL((k_dot<1>), dim3(g), dim3(kBlock), 0, c->s, c->n,
-                           c->V + (size_t)xcol * cs, c->V + (size_t)ycol * cs, c->part,
-                           c->tick, c->dscal + S_ALPHA, o1);
+    const long ne = (long)c->n * c->E;
+    const int g = grid_for(ne, kBlock * 8, kMaxGrid);
+    Timed t(TRLHIP_K_MISC, c->s, 2.0 * (double)c->esz * c->n);
+    if (is_f32(c))
+        hipLaunchKernelGGL((k_dot<float, 1>), dim3(g), dim3(kBlock), 0, c->s, ne,
+                           colp<float>(c, xcol), colp<float>(c, ycol), c->part, c->tick + T_DOT,
+                           c->dscal + S_ALPHA, o1);
+    else if (c->E == 1)
+        hipLaunchKernelGGL((k_dot<double, 1>), dim3(g), dim3(kBlock), 0, c->s, ne,
+                           colp<double>(c, xcol), colp<double>(c, ycol), c->part,
+                           c->tick + T_DOT, c->dscal + S_ALPHA, o1);
     else
-        hipLaunchKernelGGL((k_dot<2>), dim3(g), dim3(kBlock), 0, c->s, c->n,
-                           c->V + (size_t)xcol * cs, c->V + (size_t)ycol * cs, c->part,
-                           c->tick, c->dscal + S_ALPHA, o1);
+        hipLaunchKernelGGL((k_dot<double, 2>), dim3(g), dim3(kBlock), 0, c->s, ne,
+                           colp<double>(c, xcol), colp<double>(c, ycol), c->part,
+                           c->tick + T_DOT, c->dscal + S_ALPHA, o1);
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
-int run_update(trlhip_ctx *c, int c0, int m, const double *dh, int rcol, double *o_rr2,
+// rout = rin - V(:, c0:c0+m) dh
+int run_update(trlhip_ctx *c, int c0, int m, const double *dh, int rin, int rout, double *o_rr2,
                double *o_hh, int kind)
 {
-    const size_t cs = (size_t)c->ld * c->E;
-    const int g = grid_for(c->n, kBlock * 4);
+    const int g = grid_for(c->n, kBlock * 4, kMaxGrid);
     const size_t shm = sizeof(double) * (size_t)c->E * (m > 0 ? m : 1);
-    Timed t(kind, c->s, 8.0 * c->E * (double)c->n * (m + 2));
-    if (c->E == 1)
-        hipLaunchKernelGGL((k_update<1>), dim3(g), dim3(kBlock), shm, c->s, c->n, c->V,
-                           (long)c->ld, c0, m, dh, c->V + (size_t)rcol * cs, c->part,
-                           c->tick + 63, o_rr2, o_hh);
+    Timed t(kind, c->s, (double)c->esz * c->n * (m + 2));
+    if (is_f32(c))
+        hipLaunchKernelGGL((k_update<float, 1>), dim3(g), dim3(kBlock), shm, c->s, c->n,
+                           colp<float>(c, 0), (long)c->ld, c0, m, dh, colp<float>(c, rin),
+                           colp<float>(c, rout), c->part, c->tick + T_UPDATE, o_rr2, o_hh);
+    else if (c->E == 1)
+        hipLaunchKernelGGL((k_update<double, 1>), dim3(g), dim3(kBlock), shm, c->s, c->n,
+                           colp<double>(c, 0), (long)c->ld, c0, m, dh, colp<double>(c, rin),
+                           colp<double>(c, rout), c->part, c->tick + T_UPDATE, o_rr2, o_hh);
     else
-        hipLaunchKernelGGL((k_update<2>), dim3(g), dim3(kBlock), shm, c->s, c->n, c->V,
-                           (long)c->ld, c0, m, dh, c->V + (size_t)rcol * cs, c->part,
-                           c->tick + 63, o_rr2, o_hh);
+        hipLaunchKernelGGL((k_update<double, 2>), dim3(g), dim3(kBlock), shm, c->s, c->n,
+                           colp<double>(c, 0), (long)c->ld, c0, m, dh, colp<double>(c, rin),
+                           colp<double>(c, rout), c->part, c->tick + T_UPDATE, o_rr2, o_hh);
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
-template <int E, int CB>
-void launch_proj(trlhip_ctx *c, int g, int c0, int m, const double *r, double *hout)
+template <typename S, int E, int CB>
+void launch_proj(trlhip_ctx *c, int g, int c0, int m, int rcol, double *hout)
 {
-    const int nch = (m + CB - 1) / CB;
-    hipLaunchKernelGGL((k_proj<E, CB>), dim3(g, nch), dim3(kBlock), 0, c->s, c->n, c->V,
-                       (long)c->ld, c0, m, r, c->part, c->tick, hout);
+    hipLaunchKernelGGL((k_proj<S, E, CB>), dim3(g), dim3(kBlock), 0, c->s, c->n, colp<S>(c, 0),
+                       (long)c->ld, c0, m, colp<S>(c, rcol), c->part, c->tick + T_PROJ, hout);
 }
 
 int run_proj(trlhip_ctx *c, int c0, int m, int rcol, double *hout)
 {
     if (m <= 0) return 0;
-    if ((m + 7) / 8 > 60) return TRLHIP_EINVAL;  // one ticket per column chunk
-    const size_t cs = (size_t)c->ld * c->E;
-    const double *r = c->V + (size_t)rcol * cs;
-    const int g = grid_for(c->n, kBlock * 4);
-    Timed t(TRLHIP_K_PROJ, c->s, 8.0 * c->E * (double)c->n * (m + 1));
-    if (c->E == 1) {
-        if (m <= 8) launch_proj<1, 8>(c, g, c0, m, r, hout);
-        else if (m <= 16) launch_proj<1, 16>(c, g, c0, m, r, hout);
-        else launch_proj<1, 32>(c, g, c0, m, r, hout);
+    const int g = grid_for(c->n, kBlock * 4, kProjGrid);
+    Timed t(TRLHIP_K_PROJ, c->s, (double)c->esz * c->n * (m + 1));
+    if (is_f32(c)) {
+        if (m <= 8) launch_proj<float, 1, 8>(c, g, c0, m, rcol, hout);
+        else if (m <= 16) launch_proj<float, 1, 16>(c, g, c0, m, rcol, hout);
+        else launch_proj<float, 1, 32>(c, g, c0, m, rcol, hout);
+    } else if (c->E == 1) {
+        if (m <= 8) launch_proj<double, 1, 8>(c, g, c0, m, rcol, hout);
+        else if (m <= 16) launch_proj<double, 1, 16>(c, g, c0, m, rcol, hout);
+        else launch_proj<double, 1, 32>(c, g, c0, m, rcol, hout);
     } else {
-        if (m <= 8) launch_proj<2, 8>(c, g, c0, m, r, hout);
-        else launch_proj<2, 16>(c, g, c0, m, r, hout);
+        if (m <= 8) launch_proj<double, 2, 8>(c, g, c0, m, rcol, hout);
+        else launch_proj<double, 2, 16>(c, g, c0, m, rcol, hout);
     }
+    return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
+}
+
+int run_scale_copy(trlhip_ctx *c, int dst, int src, double s)
+{
+    const long ne = (long)c->n * c->E;
+    const int g = grid_for(ne, kBlock * 8, kMaxGrid);
+    Timed t(TRLHIP_K_MISC, c->s, 2.0 * (double)c->esz * c->n);
+    if (is_f32(c))
+        hipLaunchKernelGGL((k_scale_copy<float>), dim3(g), dim3(kBlock), 0, c->s, ne,
+                           colp<float>(c, src), colp<float>(c, dst), s);
+    else
+        hipLaunchKernelGGL((k_scale_copy<double>), dim3(g), dim3(kBlock), 0, c->s, ne,
+                           colp<double>(c, src), colp<double>(c, dst), s);
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
@@ -622,9 +721,28 @@ int fetch(trlhip_ctx *c, const double *dsrc, double *hdst, size_t cnt)
     return 0;
 }
 
+bool valid_col(const trlhip_ctx *c, int col) { return col >= 0 && col < c->ncols; }
+
 bool valid_cols(const trlhip_ctx *c, int c0, int m)
 {
     return c0 >= 0 && m >= 0 && c0 + m <= c->ncols;
+}
+
+// Row blocks for the CSR-stream SpMV: at most kRowsBlk rows and kNnzBlk
+// nonzeros each; a longer row gets a block of its own.
+std::vector<int> row_blocks(int n, const int *rp)
+{
+    std::vector<int> rb;
+    rb.reserve((size_t)n / 16 + 2);
+    int r = 0;
+    rb.push_back(0);
+    while (r < n) {
+        int e = r + 1;
+        while (e < n && e - r < kRowsBlk && rp[e + 1] - rp[r] <= kNnzBlk) ++e;
+        rb.push_back(e);
+        r = e;
+    }
+    return rb;
 }
 
 }  // namespace
@@ -637,11 +755,16 @@ const trlhip_csr *trlhip_csr_from_op(void (*op)(void), void *mvparam, int vtype)
 {
     const trlhip_csr *A = (const trlhip_csr *)mvparam;
     if (!A) return nullptr;
-    if (vtype == TRLHIP_R64 && op != (void (*)(void))trlhip_csr_matvec) return nullptr;
-    if (vtype == TRLHIP_C128 && op != (void (*)(void))trlhip_csr_zmatvec) return nullptr;
-    if (A->magic != kCsrMagic || A->vtype != vtype) return nullptr;
+    if (vtype == TRLHIP_C128) {
+        if (op != (void (*)(void))trlhip_csr_zmatvec) return nullptr;
+        return (A->magic == kCsrMagic && A->vtype == TRLHIP_C128) ? A : nullptr;
+    }
+    if (op != (void (*)(void))trlhip_csr_matvec) return nullptr;
+    if (A->magic != kCsrMagic || A->vtype == TRLHIP_C128) return nullptr;
     return A;
 }
+
+int trlhip_csr_is_mixed(const trlhip_csr *A) { return A && A->vtype == TRLHIP_R32; }
 
 int trlhip_device_count(void)
 {
@@ -650,12 +773,17 @@ int trlhip_device_count(void)
     return n;
 }
 
-int trlhip_csr_create(trlhip_csr **out, int n, long nnz, const int *row_ptr,
-                      const int *col_idx, const void *values, int vtype)
+int trlhip_csr_create(trlhip_csr **out, int n, long nnz, const int *row_ptr, const int *col_idx,
+                      const void *values, int vtype)
 {
     if (!out || n <= 0 || nnz < 0 || !row_ptr || (nnz > 0 && (!col_idx || !values)))
         return TRLHIP_EINVAL;
-    if (vtype != TRLHIP_R64 && vtype != TRLHIP_C128) return TRLHIP_EINVAL;
+    if (vtype != TRLHIP_R64 && vtype != TRLHIP_C128 && vtype != TRLHIP_R32) return TRLHIP_EINVAL;
+    if (row_ptr[0] != 0 || row_ptr[n] != nnz) return TRLHIP_EINVAL;
+    for (int i = 0; i < n; ++i)
+        if (row_ptr[i + 1] < row_ptr[i]) return TRLHIP_EINVAL;
+    for (long k = 0; k < nnz; ++k)
+        if (col_idx[k] < 0 || col_idx[k] >= n) return TRLHIP_EINVAL;
     if (trlhip_device_count() <= 0) return TRLHIP_ENODEV;
     trlhip_csr *A = (trlhip_csr *)calloc(1, sizeof(trlhip_csr));
     if (!A) return TRLHIP_ENOMEM;
@@ -663,16 +791,27 @@ int trlhip_csr_create(trlhip_csr **out, int n, long nnz, const int *row_ptr,
     A->n = n;
     A->nnz = nnz;
     A->vtype = vtype;
-    A->esz = vtype == TRLHIP_C128 ? 2 : 1;
-    A->sw = pick_sw(nnz, n);
+    A->E = vtype == TRLHIP_C128 ? 2 : 1;
+    const std::vector<int> rb = row_blocks(n, row_ptr);
+    A->nb = (int)rb.size() - 1;
+    const size_t nz1 = (size_t)(nnz > 0 ? nnz : 1);
     int rc = 0;
     if (hipStreamCreate(&A->s) != hipSuccess) rc = TRLHIP_EHIP;
+    if (!rc && hipMalloc((void **)&A->rb, sizeof(int) * rb.size()) != hipSuccess) rc = TRLHIP_ENOMEM;
     if (!rc && hipMalloc((void **)&A->rp, sizeof(int) * (size_t)(n + 1)) != hipSuccess) rc = TRLHIP_ENOMEM;
-    if (!rc && hipMalloc((void **)&A->ci, sizeof(int) * (size_t)(nnz > 0 ? nnz : 1)) != hipSuccess) rc = TRLHIP_ENOMEM;
-    if (!rc && hipMalloc((void **)&A->val, sizeof(double) * A->esz * (size_t)(nnz > 0 ? nnz : 1)) != hipSuccess) rc = TRLHIP_ENOMEM;
+    if (!rc && hipMalloc((void **)&A->ci, sizeof(int) * nz1) != hipSuccess) rc = TRLHIP_ENOMEM;
+    if (!rc && hipMalloc((void **)&A->val, sizeof(double) * A->E * nz1) != hipSuccess) rc = TRLHIP_ENOMEM;
+    if (!rc && hipMemcpy(A->rb, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipMemcpy(A->rp, row_ptr, sizeof(int) * (size_t)(n + 1), hipMemcpyHostToDevice) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && nnz > 0 && hipMemcpy(A->ci, col_idx, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess) rc = TRLHIP_EHIP;
-    if (!rc && nnz > 0 && hipMemcpy(A->val, values, sizeof(double) * A->esz * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess) rc = TRLHIP_EHIP;
+    if (!rc && nnz > 0 && hipMemcpy(A->val, values, sizeof(double) * A->E * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess) rc = TRLHIP_EHIP;
+    if (!rc && vtype == TRLHIP_R32) {
+        std::vector<float> v32((size_t)nnz);
+        const double *vd = (const double *)values;
+        for (long k = 0; k < nnz; ++k) v32[(size_t)k] = (float)vd[k];
+        if (hipMalloc((void **)&A->val32, sizeof(float) * nz1) != hipSuccess) rc = TRLHIP_ENOMEM;
+        if (!rc && nnz > 0 && hipMemcpy(A->val32, v32.data(), sizeof(float) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess) rc = TRLHIP_EHIP;
+    }
     if (rc) {
         trlhip_csr_destroy(A);
         return rc;
@@ -684,9 +823,12 @@ int trlhip_csr_create(trlhip_csr **out, int n, long nnz, const int *row_ptr,
 int trlhip_csr_destroy(trlhip_csr *A)
 {
     if (!A) return 0;
+    if (A->s) (void)hipStreamSynchronize(A->s);
+    if (A->rb) (void)hipFree(A->rb);
     if (A->rp) (void)hipFree(A->rp);
     if (A->ci) (void)hipFree(A->ci);
     if (A->val) (void)hipFree(A->val);
+    if (A->val32) (void)hipFree(A->val32);
     if (A->dx) (void)hipFree(A->dx);
     if (A->dy) (void)hipFree(A->dy);
     if (A->part) (void)hipFree(A->part);
@@ -698,15 +840,17 @@ int trlhip_csr_destroy(trlhip_csr *A)
     return 0;
 }
 
-// Host-callable operator: stage x in, y = A x on the device, stage y out.
+// Host-callable operator: stage x in, y = A x (FP64 values) on the device,
+// stage y out.
 static int csr_matvec_host(trlhip_csr *A, int nrow, int ncol, const double *x, int ldx,
                            double *y, int ldy)
 {
-    const int E = A->esz;
+    const int E = A->E;
+    const int vt = A->vtype == TRLHIP_C128 ? TRLHIP_C128 : TRLHIP_R64;
     if (!A->dx) {
         if (hipMalloc((void **)&A->dx, sizeof(double) * E * (size_t)A->n) != hipSuccess) return TRLHIP_ENOMEM;
         if (hipMalloc((void **)&A->dy, sizeof(double) * E * (size_t)A->n) != hipSuccess) return TRLHIP_ENOMEM;
-        if (hipMalloc((void **)&A->part, sizeof(double) * kMaxGrid) != hipSuccess) return TRLHIP_ENOMEM;
+        if (hipMalloc((void **)&A->part, sizeof(double) * (size_t)A->nb) != hipSuccess) return TRLHIP_ENOMEM;
         if (hipMalloc((void **)&A->tick, sizeof(unsigned) * 4) != hipSuccess) return TRLHIP_ENOMEM;
         if (hipMalloc((void **)&A->dscal, sizeof(double) * 4) != hipSuccess) return TRLHIP_ENOMEM;
         TRY(hipMemset(A->tick, 0, sizeof(unsigned) * 4));
@@ -714,10 +858,7 @@ static int csr_matvec_host(trlhip_csr *A, int nrow, int ncol, const double *x, i
     for (int j = 0; j < ncol; ++j) {
         TRY(hipMemcpyAsync(A->dx, x + (size_t)j * ldx * E, sizeof(double) * E * (size_t)nrow,
                            hipMemcpyHostToDevice, A->s));
-        hipError_t e = (E == 1)
-            ? launch_spmv<1>(A->sw, nrow, A, A->dx, A->dy, A->part, A->tick, A->dscal, nullptr, A->s)
-            : launch_spmv<2>(A->sw, nrow, A, A->dx, A->dy, A->part, A->tick, A->dscal, nullptr, A->s);
-        if (e != hipSuccess) return TRLHIP_EHIP;
+        if (launch_spmv(A, vt, A->dx, A->dy, A->part, A->tick, A->dscal, nullptr, A->s)) return TRLHIP_EHIP;
         TRY(hipMemcpyAsync(y + (size_t)j * ldy * E, A->dy, sizeof(double) * E * (size_t)nrow,
                            hipMemcpyDeviceToHost, A->s));
     }
@@ -725,27 +866,23 @@ static int csr_matvec_host(trlhip_csr *A, int nrow, int ncol, const double *x, i
     return 0;
 }
 
-void trlhip_csr_matvec(const int nrow, const int ncol, const double *x, const int ldx,
-                       double *y, const int ldy, void *mvparam)
+void trlhip_csr_matvec(const int nrow, const int ncol, const double *x, const int ldx, double *y,
+                       const int ldy, void *mvparam)
 {
     trlhip_csr *A = (trlhip_csr *)mvparam;
-    if (!A || A->magic != kCsrMagic || A->esz != 1 || nrow != A->n) {
-        for (int j = 0; j < ncol; ++j)
-            for (int i = 0; i < nrow; ++i) y[(size_t)j * ldy + i] = __builtin_nan("");
-        return;
-    }
-    if (csr_matvec_host(A, nrow, ncol, x, ldx, y, ldy) != 0) {
+    if (!A || A->magic != kCsrMagic || A->E != 1 || nrow != A->n ||
+        csr_matvec_host(A, nrow, ncol, x, ldx, y, ldy) != 0) {
         for (int j = 0; j < ncol; ++j)
             for (int i = 0; i < nrow; ++i) y[(size_t)j * ldy + i] = __builtin_nan("");
     }
 }
 
-void trlhip_csr_zmatvec(const int nrow, const int ncol, const void *x, const int ldx,
-                        void *y, const int ldy, void *mvparam)
+void trlhip_csr_zmatvec(const int nrow, const int ncol, const void *x, const int ldx, void *y,
+                        const int ldy, void *mvparam)
 {
     trlhip_csr *A = (trlhip_csr *)mvparam;
     double *yd = (double *)y;
-    if (!A || A->magic != kCsrMagic || A->esz != 2 || nrow != A->n ||
+    if (!A || A->magic != kCsrMagic || A->E != 2 || nrow != A->n ||
         csr_matvec_host(A, nrow, ncol, (const double *)x, ldx, yd, ldy) != 0) {
         for (int j = 0; j < ncol; ++j)
             for (int i = 0; i < 2 * nrow; ++i) yd[(size_t)j * ldy * 2 + i] = __builtin_nan("");
@@ -773,7 +910,7 @@ int trlhip_ctx_destroy(trlhip_ctx *c)
 int trlhip_ctx_create(trlhip_ctx **out, int n, int ncols, int vtype)
 {
     if (!out || n <= 0 || ncols <= 0) return TRLHIP_EINVAL;
-    if (vtype != TRLHIP_R64 && vtype != TRLHIP_C128) return TRLHIP_EINVAL;
+    if (vtype != TRLHIP_R64 && vtype != TRLHIP_C128 && vtype != TRLHIP_R32) return TRLHIP_EINVAL;
     if (trlhip_device_count() <= 0) return TRLHIP_ENODEV;
     trlhip_ctx *c = (trlhip_ctx *)calloc(1, sizeof(trlhip_ctx));
     if (!c) return TRLHIP_ENOMEM;
@@ -782,19 +919,21 @@ int trlhip_ctx_create(trlhip_ctx **out, int n, int ncols, int vtype)
     c->ncols = ncols;
     c->vtype = vtype;
     c->E = vtype == TRLHIP_C128 ? 2 : 1;
+    c->esz = vtype == TRLHIP_R32 ? 4 : 8 * (size_t)c->E;
     const size_t ncoef = 2 * (size_t)(ncols + 2);
+    const size_t vbytes = c->esz * (size_t)c->ld * (ncols + 1);  // + scratch column
     int rc = 0;
     if (hipStreamCreate(&c->s) != hipSuccess) rc = TRLHIP_EHIP;
-    if (!rc && hipMalloc((void **)&c->V, sizeof(double) * c->E * (size_t)c->ld * ncols) != hipSuccess) rc = TRLHIP_ENOMEM;
-    if (!rc && hipMalloc((void **)&c->part, sizeof(double) * kMaxGrid * ((size_t)c->E * ncols + 64)) != hipSuccess) rc = TRLHIP_ENOMEM;
-    if (!rc && hipMalloc((void **)&c->tick, sizeof(unsigned) * 64) != hipSuccess) rc = TRLHIP_ENOMEM;
+    if (!rc && hipMalloc((void **)&c->V, vbytes) != hipSuccess) rc = TRLHIP_ENOMEM;
+    if (!rc) rc = ensure_part(c, (size_t)kMaxGrid * ((size_t)c->E * (ncols + 1) + 64));
+    if (!rc && hipMalloc((void **)&c->tick, sizeof(unsigned) * T_COUNT) != hipSuccess) rc = TRLHIP_ENOMEM;
     if (!rc && hipMalloc((void **)&c->dscal, sizeof(double) * 16) != hipSuccess) rc = TRLHIP_ENOMEM;
     if (!rc && hipMalloc((void **)&c->dcoef, sizeof(double) * ncoef) != hipSuccess) rc = TRLHIP_ENOMEM;
     if (!rc && hipMalloc((void **)&c->dh, sizeof(double) * ncoef) != hipSuccess) rc = TRLHIP_ENOMEM;
     c->hcap = 16 + 2 * ncoef;
     if (!rc && hipHostMalloc((void **)&c->hbuf, sizeof(double) * c->hcap, 0) != hipSuccess) rc = TRLHIP_ENOMEM;
-    if (!rc && hipMemsetAsync(c->tick, 0, sizeof(unsigned) * 64, c->s) != hipSuccess) rc = TRLHIP_EHIP;
-    if (!rc && hipMemsetAsync(c->V, 0, sizeof(double) * c->E * (size_t)c->ld * ncols, c->s) != hipSuccess) rc = TRLHIP_EHIP;
+    if (!rc && hipMemsetAsync(c->tick, 0, sizeof(unsigned) * T_COUNT, c->s) != hipSuccess) rc = TRLHIP_EHIP;
+    if (!rc && hipMemsetAsync(c->V, 0, vbytes, c->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipMemsetAsync(c->dscal, 0, sizeof(double) * 16, c->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipStreamSynchronize(c->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (rc) {
@@ -809,9 +948,8 @@ int trlhip_ctx_ld(const trlhip_ctx *c) { return c ? c->ld : 0; }
 
 int trlhip_col_set(trlhip_ctx *c, int col, const void *x)
 {
-    if (!c || !x || col < 0 || col >= c->ncols) return TRLHIP_EINVAL;
-    const size_t cs = (size_t)c->ld * c->E;
-    TRY(hipMemcpyAsync(c->V + (size_t)col * cs, x, sizeof(double) * c->E * (size_t)c->n,
+    if (!c || !x || !valid_col(c, col)) return TRLHIP_EINVAL;
+    TRY(hipMemcpyAsync(c->V + (size_t)col * c->ld * c->esz, x, c->esz * (size_t)c->n,
                        hipMemcpyHostToDevice, c->s));
     TRY(hipStreamSynchronize(c->s));
     return 0;
@@ -819,9 +957,8 @@ int trlhip_col_set(trlhip_ctx *c, int col, const void *x)
 
 int trlhip_col_get(trlhip_ctx *c, int col, void *x)
 {
-    if (!c || !x || col < 0 || col >= c->ncols) return TRLHIP_EINVAL;
-    const size_t cs = (size_t)c->ld * c->E;
-    TRY(hipMemcpyAsync(x, c->V + (size_t)col * cs, sizeof(double) * c->E * (size_t)c->n,
+    if (!c || !x || !valid_col(c, col)) return TRLHIP_EINVAL;
+    TRY(hipMemcpyAsync(x, c->V + (size_t)col * c->ld * c->esz, c->esz * (size_t)c->n,
                        hipMemcpyDeviceToHost, c->s));
     TRY(hipStreamSynchronize(c->s));
     stats_resolve();
@@ -830,30 +967,27 @@ int trlhip_col_get(trlhip_ctx *c, int col, void *x)
 
 int trlhip_col_copy(trlhip_ctx *c, int dst, int src)
 {
-    if (!c || dst < 0 || src < 0 || dst >= c->ncols || src >= c->ncols) return TRLHIP_EINVAL;
+    if (!c || !valid_col(c, dst) || !valid_col(c, src)) return TRLHIP_EINVAL;
     if (dst == src) return 0;
-    const size_t cs = (size_t)c->ld * c->E;
-    Timed t(TRLHIP_K_MISC, c->s, 2.0 * 8.0 * c->E * c->n);
-    TRY(hipMemcpyAsync(c->V + (size_t)dst * cs, c->V + (size_t)src * cs,
-                       sizeof(double) * c->E * (size_t)c->n, hipMemcpyDeviceToDevice, c->s));
-    return 0;
+    return run_scale_copy(c, dst, src, 1.0);
 }
 
 int trlhip_col_scale(trlhip_ctx *c, int col, double s)
 {
-    if (!c || col < 0 || col >= c->ncols) return TRLHIP_EINVAL;
-    const size_t cs = (size_t)c->ld * c->E;
-    const long ne = (long)c->n * c->E;
-    const int g = grid_for(ne, kBlock * 8);
-    Timed t(TRLHIP_K_MISC, c->s, 2.0 * 8.0 * ne);
-    hipLaunchKernelGGL(k_scale, dim3(g), dim3(kBlock), 0, c->s, ne, c->V + (size_t)col * cs, s);
-    return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
+    if (!c || !valid_col(c, col)) return TRLHIP_EINVAL;
+    return run_scale_copy(c, col, col, s);
+}
+
+int trlhip_col_scale_copy(trlhip_ctx *c, int dst, int src, double s)
+{
+    if (!c || !valid_col(c, dst) || src < 0 || src > c->ncols) return TRLHIP_EINVAL;
+    return run_scale_copy(c, dst, src, s);
 }
 
 int trlhip_spmv(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol, double *xdoty)
 {
-    if (!c || !A || A->magic != kCsrMagic || A->n != c->n || A->esz != c->E) return TRLHIP_EINVAL;
-    if (xcol < 0 || ycol < 0 || xcol >= c->ncols || ycol >= c->ncols || xcol == ycol) return TRLHIP_EINVAL;
+    if (!c || !csr_fits(c, A)) return TRLHIP_EINVAL;
+    if (!valid_col(c, xcol) || !valid_col(c, ycol) || xcol == ycol) return TRLHIP_EINVAL;
     int rc = run_spmv(c, A, xcol, ycol, nullptr);
     if (rc) return rc;
     rc = fetch(c, c->dscal + S_ALPHA, c->hbuf, 1);
@@ -862,24 +996,51 @@ int trlhip_spmv(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol, double *
     return 0;
 }
 
-int trlhip_lanczos_step(trlhip_ctx *c, const trlhip_csr *A, int qa, int qb, double beta_prev,
-                        int rcol, double *alpha, double *rr2)
+// y = A q_a (if A), alpha, and r = y - coef*V: shared by the step entries.
+// coefficients are set for columns [c0, c0+m) with the alpha slot last.
+static int form_residual(trlhip_ctx *c, const trlhip_csr *A, int qa, int first, int qb,
+                         double beta_prev, const double *beta, int jnd, int rcol)
 {
-    if (!c || qa < 0 || qa >= c->ncols || rcol < 0 || rcol >= c->ncols || qa == rcol) return TRLHIP_EINVAL;
-    if (qb >= 0 && (qb != qa - 1 || qb >= c->ncols)) return TRLHIP_EINVAL;
-    if (A && (A->magic != kCsrMagic || A->n != c->n || A->esz != c->E)) return TRLHIP_EINVAL;
     const int E = c->E;
-    // coefficient slots: [beta_prev, alpha] for columns (qb, qa)
-    double *slot_alpha = c->dcoef + E;
-    if (qb >= 0) {
+    int c0, m;
+    double *slot_alpha;
+    if (first) {
+        double *h = c->hbuf + 16;
+        for (int i = 0; i < jnd - 1; ++i) {
+            h[E * i] = beta[i];
+            if (E == 2) h[E * i + 1] = 0.0;
+        }
+        if (jnd > 1)
+            TRY(hipMemcpyAsync(c->dcoef, h, sizeof(double) * E * (size_t)(jnd - 1),
+                               hipMemcpyHostToDevice, c->s));
+        slot_alpha = c->dcoef + (size_t)E * (jnd - 1);
+        c0 = 0;
+        m = jnd;
+    } else if (qb >= 0) {
         c->hbuf[0] = beta_prev;
         c->hbuf[1] = 0.0;
         TRY(hipMemcpyAsync(c->dcoef, c->hbuf, sizeof(double) * E, hipMemcpyHostToDevice, c->s));
+        slot_alpha = c->dcoef + E;
+        c0 = qb;
+        m = 2;
+    } else {
+        slot_alpha = c->dcoef;
+        c0 = qa;
+        m = 1;
     }
     int rc = A ? run_spmv(c, A, qa, rcol, slot_alpha) : run_dot(c, qa, rcol, slot_alpha);
     if (rc) return rc;
-    rc = (qb >= 0) ? run_update(c, qb, 2, c->dcoef, rcol, c->dscal + S_RR2, nullptr, TRLHIP_K_RECUR)
-                   : run_update(c, qa, 1, slot_alpha, rcol, c->dscal + S_RR2, nullptr, TRLHIP_K_RECUR);
+    return run_update(c, c0, m, first ? c->dcoef : (qb >= 0 ? c->dcoef : slot_alpha), rcol, rcol,
+                      c->dscal + S_RR2, nullptr, TRLHIP_K_RECUR);
+}
+
+int trlhip_lanczos_step(trlhip_ctx *c, const trlhip_csr *A, int qa, int qb, double beta_prev,
+                        int rcol, double *alpha, double *rr2)
+{
+    if (!c || !valid_col(c, qa) || !valid_col(c, rcol) || qa == rcol) return TRLHIP_EINVAL;
+    if (qb >= 0 && qb != qa - 1) return TRLHIP_EINVAL;
+    if (A && !csr_fits(c, A)) return TRLHIP_EINVAL;
+    int rc = form_residual(c, A, qa, 0, qb, beta_prev, nullptr, 0, rcol);
     if (rc) return rc;
     rc = fetch(c, c->dscal, c->hbuf + 4, 2);
     if (rc) return rc;
@@ -891,21 +1052,10 @@ int trlhip_lanczos_step(trlhip_ctx *c, const trlhip_csr *A, int qa, int qb, doub
 int trlhip_restart_step(trlhip_ctx *c, const trlhip_csr *A, int qa, const double *beta, int jnd,
                         int rcol, double *alpha, double *rr2)
 {
-    if (!c || jnd < 1 || jnd > c->ncols || qa != jnd - 1 || rcol < 0 || rcol >= c->ncols || rcol == qa)
+    if (!c || jnd < 1 || jnd > c->ncols || qa != jnd - 1 || !valid_col(c, rcol) || rcol == qa)
         return TRLHIP_EINVAL;
-    if (A && (A->magic != kCsrMagic || A->n != c->n || A->esz != c->E)) return TRLHIP_EINVAL;
-    const int E = c->E;
-    double *h = c->hbuf + 16;
-    for (int i = 0; i < jnd - 1; ++i) {
-        h[E * i] = beta[i];
-        if (E == 2) h[E * i + 1] = 0.0;
-    }
-    if (jnd > 1)
-        TRY(hipMemcpyAsync(c->dcoef, h, sizeof(double) * E * (size_t)(jnd - 1), hipMemcpyHostToDevice, c->s));
-    double *slot_alpha = c->dcoef + (size_t)E * (jnd - 1);
-    int rc = A ? run_spmv(c, A, qa, rcol, slot_alpha) : run_dot(c, qa, rcol, slot_alpha);
-    if (rc) return rc;
-    rc = run_update(c, 0, jnd, c->dcoef, rcol, c->dscal + S_RR2, nullptr, TRLHIP_K_RECUR);
+    if (A && !csr_fits(c, A)) return TRLHIP_EINVAL;
+    int rc = form_residual(c, A, qa, 1, -1, 0.0, beta, jnd, rcol);
     if (rc) return rc;
     rc = fetch(c, c->dscal, c->hbuf + 4, 2);
     if (rc) return rc;
@@ -914,18 +1064,43 @@ int trlhip_restart_step(trlhip_ctx *c, const trlhip_csr *A, int qa, const double
     return 0;
 }
 
+int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int qb,
+                    double beta_prev, const double *beta, int jnd, int rcol, double out[4])
+{
+    if (!c || !out || !valid_col(c, qa) || !valid_col(c, rcol) || qa == rcol) return TRLHIP_EINVAL;
+    if (first && (jnd < 1 || qa != jnd - 1 || !beta)) return TRLHIP_EINVAL;
+    if (!first && qb >= 0 && qb != qa - 1) return TRLHIP_EINVAL;
+    if (jnd < 1 || jnd > c->ncols || (rcol >= 0 && rcol < jnd)) return TRLHIP_EINVAL;
+    if (A && !csr_fits(c, A)) return TRLHIP_EINVAL;
+    int rc = form_residual(c, A, qa, first, qb, beta_prev, beta, jnd, rcol);
+    if (rc) return rc;
+    // speculative first Gram-Schmidt pass into the scratch column
+    rc = run_proj(c, 0, jnd, rcol, c->dh);
+    if (rc) return rc;
+    rc = run_update(c, 0, jnd, c->dh, rcol, c->ncols, c->dscal + S_RR2B, c->dscal + S_HH,
+                    TRLHIP_K_UPDATE);
+    if (rc) return rc;
+    rc = fetch(c, c->dscal, c->hbuf + 4, 4);
+    if (rc) return rc;
+    out[0] = c->hbuf[4 + S_ALPHA];
+    out[1] = c->hbuf[4 + S_RR2];
+    out[2] = c->hbuf[4 + S_HH];
+    out[3] = c->hbuf[4 + S_RR2B];
+    return 0;
+}
+
 int trlhip_cgs_pass(trlhip_ctx *c, int c0, int m, int rcol, double *hh, double *rr2, double *h_out)
 {
-    if (!c || m < 1 || !valid_cols(c, c0, m) || rcol < 0 || rcol >= c->ncols || (rcol >= c0 && rcol < c0 + m))
+    if (!c || m < 1 || !valid_cols(c, c0, m) || !valid_col(c, rcol) || (rcol >= c0 && rcol < c0 + m))
         return TRLHIP_EINVAL;
     int rc = run_proj(c, c0, m, rcol, c->dh);
     if (rc) return rc;
-    rc = run_update(c, c0, m, c->dh, rcol, c->dscal + S_RR2, c->dscal + S_HH, TRLHIP_K_UPDATE);
+    rc = run_update(c, c0, m, c->dh, rcol, rcol, c->dscal + S_RR2, c->dscal + S_HH,
+                    TRLHIP_K_UPDATE);
     if (rc) return rc;
     const int E = c->E;
-    if (h_out) {
+    if (h_out)
         TRY(hipMemcpyAsync(c->hbuf + 16, c->dh, sizeof(double) * E * (size_t)m, hipMemcpyDeviceToHost, c->s));
-    }
     rc = fetch(c, c->dscal, c->hbuf + 4, 3);
     if (rc) return rc;
     if (hh) *hh = c->hbuf[4 + S_HH];
@@ -936,7 +1111,7 @@ int trlhip_cgs_pass(trlhip_ctx *c, int c0, int m, int rcol, double *hh, double *
 
 int trlhip_proj(trlhip_ctx *c, int c0, int m, int rcol, double *h)
 {
-    if (!c || !h || m < 1 || !valid_cols(c, c0, m) || rcol < 0 || rcol >= c->ncols) return TRLHIP_EINVAL;
+    if (!c || !h || m < 1 || !valid_cols(c, c0, m) || !valid_col(c, rcol)) return TRLHIP_EINVAL;
     int rc = run_proj(c, c0, m, rcol, c->dh);
     if (rc) return rc;
     rc = fetch(c, c->dh, c->hbuf + 16, (size_t)c->E * m);
@@ -947,14 +1122,14 @@ int trlhip_proj(trlhip_ctx *c, int c0, int m, int rcol, double *h)
 
 int trlhip_update(trlhip_ctx *c, int c0, int m, const double *h, int rcol, double *rr2)
 {
-    if (!c || !h || m < 0 || !valid_cols(c, c0, m) || rcol < 0 || rcol >= c->ncols ||
+    if (!c || !h || m < 0 || !valid_cols(c, c0, m) || !valid_col(c, rcol) ||
         (m > 0 && rcol >= c0 && rcol < c0 + m))
         return TRLHIP_EINVAL;
     const int E = c->E;
     memcpy(c->hbuf + 16, h, sizeof(double) * E * (size_t)m);
     if (m > 0)
         TRY(hipMemcpyAsync(c->dh, c->hbuf + 16, sizeof(double) * E * (size_t)m, hipMemcpyHostToDevice, c->s));
-    int rc = run_update(c, c0, m, c->dh, rcol, c->dscal + S_RR2, nullptr, TRLHIP_K_UPDATE);
+    int rc = run_update(c, c0, m, c->dh, rcol, rcol, c->dscal + S_RR2, nullptr, TRLHIP_K_UPDATE);
     if (rc) return rc;
     rc = fetch(c, c->dscal + S_RR2, c->hbuf + 4, 1);
     if (rc) return rc;
@@ -989,25 +1164,25 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y, int ldy)
     const int g = (int)((nrows + rows_per_block - 1) / rows_per_block);
     const int jl4 = (jl + 3) & ~3;
     const size_t shm = sizeof(double) * (size_t)jl4 * 16 * KT;
-    if (shm > 160 * 1024) return TRLHIP_EINVAL;
-    Timed t(TRLHIP_K_RITZ, c->s, 8.0 * (double)nrows * (jl + k));
-    double *Vr = c->V;
-    switch (KT) {
-#define RITZ_CASE(K)                                                                        \
-    case K:                                                                                 \
-        hipLaunchKernelGGL((k_ritz_mfma<K>), dim3(g), dim3(kBlock), shm, c->s, nrows, Vr, \
-                           ldr, lck, jl, k, c->dY, ldy, tpw);                               \
+    if (shm > 64 * 1024) return TRLHIP_EINVAL;
+    Timed t(TRLHIP_K_RITZ, c->s, (double)c->esz * c->n * (jl + k));
+#define RITZ_CASE(S, K)                                                                      \
+    case K:                                                                                  \
+        hipLaunchKernelGGL((k_ritz_mfma<S, K>), dim3(g), dim3(kBlock), shm, c->s, nrows,     \
+                           reinterpret_cast<S *>(c->V), ldr, lck, jl, k, c->dY, ldy, tpw); \
         break;
-        RITZ_CASE(1)
-        RITZ_CASE(2)
-        RITZ_CASE(3)
-        RITZ_CASE(4)
-        RITZ_CASE(5)
-        RITZ_CASE(6)
-        RITZ_CASE(7)
-        RITZ_CASE(8)
-#undef RITZ_CASE
+    if (is_f32(c)) {
+        switch (KT) {
+            RITZ_CASE(float, 1) RITZ_CASE(float, 2) RITZ_CASE(float, 3) RITZ_CASE(float, 4)
+            RITZ_CASE(float, 5) RITZ_CASE(float, 6) RITZ_CASE(float, 7) RITZ_CASE(float, 8)
+        }
+    } else {
+        switch (KT) {
+            RITZ_CASE(double, 1) RITZ_CASE(double, 2) RITZ_CASE(double, 3) RITZ_CASE(double, 4)
+            RITZ_CASE(double, 5) RITZ_CASE(double, 6) RITZ_CASE(double, 7) RITZ_CASE(double, 8)
+        }
     }
+#undef RITZ_CASE
     if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
     TRY(hipStreamSynchronize(c->s));
     stats_resolve();

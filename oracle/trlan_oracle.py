@@ -189,13 +189,17 @@ def smooth_rr(rr):
 
 
 class _State:
-    """Per-solve arrays of trlanczos (SRC/trlcore.c:756-769)."""
+    """Per-solve arrays of trlanczos (SRC/trlcore.c:756-769); a complex
+    basis for ztrlanczos (SRC/ztrlcore.c:556-584)."""
 
-    def __init__(self, n, mev, maxlan, rng):
+    def __init__(self, n, mev, maxlan, rng, complex_=False, gdot_quirk=True):
         self.n = n
         self.mev = mev
+        self.complex = complex_
+        self.gdot_quirk = gdot_quirk
         self.nbas = max(1, maxlan - mev + 1)
-        self.V = np.zeros((n, mev + self.nbas), order="F")
+        self.V = np.zeros((n, mev + self.nbas), order="F",
+                          dtype=np.complex128 if complex_ else np.float64)
         self.alpha = np.zeros(maxlan)
         self.beta = np.zeros(maxlan)
         self.alfrot = np.zeros(maxlan)
@@ -207,13 +211,60 @@ class _State:
         """Logical basis columns [evec(:,0:j1), base(:,0:j2)]."""
         return list(range(j1)) + [self.mev + k for k in range(j2)]
 
+    def random_vector(self):
+        if self.complex:
+            v = self.rng.random(self.n) + 1j * self.rng.random(self.n)
+            smooth_zz(v)
+        else:
+            v = self.rng.random(self.n)
+            smooth_rr(v)
+        return v
+
+
+def smooth_zz(z):
+    """trl_smooth_zz (SRC/ztrlcore.c:1289-1332), including the last
+    element's imaginary part taking its own real part."""
+    n = len(z)
+    if n <= 0:
+        return
+    r2, i2 = z[0].real, z[0].imag
+    z0 = complex(2 * z[0].real + z[2 % n].real + z[n - 1].real,
+                 2 * z[0].imag + z[2 % n].imag + z[n - 1].imag)
+    z[0] = z0
+    for i in range(1, n - 1):
+        r1, i1 = z[i].real, z[i].imag
+        z[i] = complex(2 * z[i].real + z[i + 1].real + r2, 2 * z[i].imag + z[i + 1].imag + i2)
+        r2, i2 = r1, i1
+    re = z[n - 1].real
+    z[n - 1] = complex(2 * re + z[1 % n].real + r2, 2 * re + z[1 % n].imag + i2)
+
+
+def _gdot(st, c1, c2, rr):
+    """[V1 V2]^H r.  With st.gdot_quirk the reference's two-column branches
+    of ztrl_g_dot_ are reproduced (SRC/ztrl_comm_none.c:88-101 drop the
+    imaginary part of the second value; 109-120 add it into the real part
+    of the last value); otherwise the exact product."""
+    V = st.V
+    h = V[:, c1 + c2].conj().T @ rr
+    if st.complex and st.gdot_quirk:
+        m1 = len(c1)
+        if m1 == 2:
+            h[1] = h[1].real
+        if len(c2) == 2:
+            v = V[:, c2[1]]
+            t = float(np.sum(v.real * rr.real + v.imag * rr.imag)
+                      + np.sum(v.real * rr.imag - v.imag * rr.real))
+            h[m1 + 1] = t
+    return h
+
 
 def cgs(info, st, cols, rr_col, rnrm, counter):
-    """trl_cgs (SRC/trlcore.c:2913-3117).  Returns (stat, rnrm).
+    """trl_cgs (SRC/trlcore.c:2913-3117) / ztrl_cgs (SRC/ztrlcore.c:
+    1977-2203).  Returns (stat, rnrm).
 
     ``counter`` is a one-element list standing in for the `north` pointer.
     The reference subtracts the evec part and the base part with two separate
-    dgemv calls (2996-3011); here the two parts are subtracted in that order.
+    gemv calls (2996-3011); here the two parts are subtracted in that order.
     """
     V = st.V
     n = st.n
@@ -221,6 +272,7 @@ def cgs(info, st, cols, rr_col, rnrm, counter):
     nold = len(cols)
     maxorth = 3
     irnd = 0
+    ierr = 0
     normrr2 = 0.0
     normDiff2 = 0.0
     if nold > 0:
@@ -228,7 +280,7 @@ def cgs(info, st, cols, rr_col, rnrm, counter):
         c2 = [c for c in cols if c >= st.mev]
         cnt = 0
         while cnt <= maxorth:
-            h = V[:, cols].T @ rr
+            h = _gdot(st, c1, c2, rr)
             h1 = h[: len(c1)]
             h2 = h[len(c1):]
             if len(c1) > 0:
@@ -237,9 +289,21 @@ def cgs(info, st, cols, rr_col, rnrm, counter):
                 rr -= V[:, c2] @ h2
             counter[0] += 1
             cnt += 1
-            normDiff2 = float(h @ h)
-            normrr2 = float(rr @ rr)
+            normDiff2 = float(np.vdot(h, h).real)
+            normrr2 = float(np.vdot(rr, rr).real)
             rnrm = math.sqrt(normrr2)
+            if st.complex:
+                old_rnrm = math.sqrt(normrr2 + normDiff2)
+                if DBL_EPSILON * normrr2 > normDiff2:
+                    break
+                if (((cnt > 1 and not (normDiff2 <= info.ntot * DBL_EPSILON * DBL_EPSILON
+                                       * (normrr2 + normDiff2)))
+                     or not (normrr2 > DBL_MIN)) and irnd < maxorth):
+                    cnt = 0
+                    irnd += 1
+                    info.nrand += 1
+                    _perturb_complex(st, rr, irnd, rnrm, old_rnrm)
+                continue
             if DBL_EPSILON * normrr2 > normDiff2:
                 break
             elif (cnt > 1 and nold * DBL_EPSILON * DBL_EPSILON * (normrr2 + normDiff2)
@@ -249,25 +313,48 @@ def cgs(info, st, cols, rr_col, rnrm, counter):
                 cnt = 0
                 irnd += 1
                 info.nrand += 1
-                rr[:] = st.rng.random(n)
-                smooth_rr(rr)
-        if normrr2 < normDiff2:
+                rr[:] = st.random_vector()
+        if st.complex:
+            if not (normrr2 >= normDiff2):
+                ierr = -203
+        elif normrr2 < normDiff2:
             return -203, rnrm
-    if rnrm > DBL_MIN:
-        rr *= 1.0 / rnrm
-    else:
-        return -204, rnrm
+    if ierr == 0:
+        if rnrm > DBL_MIN:
+            rr *= 1.0 / rnrm
+        else:
+            return -204, rnrm
     if irnd > 0:
         rnrm = 0.0
-    return 0, rnrm
+    return ierr, rnrm
+
+
+def _perturb_complex(st, rr, irnd, rnrm, old_rnrm):
+    """Random retry of ztrl_cgs (SRC/ztrlcore.c:2117-2178), numpy draws
+    (parity unpinned: the reference draws from drand48)."""
+    n = st.n
+    rng = st.rng
+    if irnd == 1 and rnrm > 0.0 and rnrm > DBL_EPSILON * old_rnrm:
+        i = int(n * rng.random())
+        k = i + int(max(1.0, n * math.sqrt(DBL_EPSILON * old_rnrm / rnrm)))
+        for j in range(i, min(k, n)):
+            t = rng.random()
+            rr[j] += rnrm * (t - 0.5) * (1 + 1j)
+    else:
+        i = int(n * rng.random())
+        j = int(n * rng.random())
+        lo, hi = (0, n - 1) if i == j else (min(i, j), max(i, j))
+        rr[lo:hi + 1] = rng.random(hi + 1 - lo) + 1j * rng.random(hi + 1 - lo)
+    smooth_zz(rr)
 
 
 def initial_guess(info, st, start):
-    """trl_initial_guess (SRC/trlcore.c:1577-1763) for guess <= 1.
+    """trl_initial_guess (SRC/trlcore.c:1577-1763) / ztrl_initial_guess
+    (SRC/ztrlcore.c:1335-1567) for guess <= 1.
 
     Returns (j1, j2).  guess == 1 takes evec[:,nec] as given (the path every
-    parity test uses); guess <= 0 uses [1,...,1] plus the reference's
-    perturbation count (drawn from numpy, parity unpinned).
+    parity test uses); guess <= 0 uses [1,...,1] ((1,0) or (1,1) for complex)
+    plus the reference's perturbation count (numpy draws, parity unpinned).
     """
     n = st.n
     j = info.nec
@@ -276,6 +363,18 @@ def initial_guess(info, st, start):
     col = st.V[:, j]
     if info.guess == 1:
         col[:] = start
+    elif st.complex:
+        col[:] = 1.0 if info.guess == 0 else 1.0 + 1.0j
+        if info.guess < 0:
+            nran = 2 * (min(-info.guess, 4 * info.maxlan) // 2)
+            if nran < n:
+                w = st.rng.random(nran) + 1j * st.rng.random(nran)
+                for i in range(0, nran - 1, 2):
+                    ii = int(n * w[i].real)
+                    col[ii] += w[i + 1] - (0.5 + 0.5j)
+            else:
+                col[:] = st.random_vector()
+                info.nrand += 1
     else:
         col[:] = 1.0
         nran = min(1 - info.guess, 4 * info.maxlan)
@@ -286,17 +385,15 @@ def initial_guess(info, st, start):
                 ii = int(n * w[i])
                 col[ii] += w[i + 1] - 0.5
         elif nran >= n:
-            col[:] = st.rng.random(n)
-            smooth_rr(col)
+            col[:] = st.random_vector()
             info.nrand += 1
     j1, j2 = info.nec, 0
-    normr2 = float(col @ col)
+    normr2 = float(np.vdot(col, col).real)
     rnrm = 0.0
     if DBL_MIN <= normr2 <= DBL_MAX:
         rnrm = math.sqrt(normr2)
     else:
-        col[:] = st.rng.random(n)
-        smooth_rr(col)
+        col[:] = st.random_vector()
         info.nrand += 1
     local = [0]
     if j1 < st.mev:
@@ -307,13 +404,14 @@ def initial_guess(info, st, start):
 
 
 def orth(info, st, j1, j2, rr_col, kept):
-    """trl_orth (SRC/trlcore.c:1766-1975)."""
+    """trl_orth (SRC/trlcore.c:1766-1975) / ztrl_orth (SRC/ztrlcore.c:
+    1754-1974)."""
     V = st.V
     alpha, beta = st.alpha, st.beta
     jnd = j1 + j2
     jm1 = jnd - 1
     rr = V[:, rr_col]
-    normrr2 = float(rr @ rr)
+    normrr2 = float(np.vdot(rr, rr).real)
     if not (normrr2 >= 0.0) or not (normrr2 <= DBL_MAX):
         info.stat = -102
         return
@@ -345,16 +443,16 @@ def orth(info, st, j1, j2, rr_col, kept):
     elif jnd > 1:
         qa = V[:, cols[-1]]
         qb = V[:, cols[-2]]
-        w0 = float(qa @ rr)
-        w1 = float(qb @ rr)
-        alpha[jnd - 1] += w0
+        w0 = np.vdot(qa, rr)
+        w1 = np.vdot(qb, rr)
+        alpha[jnd - 1] += w0.real
         rr -= w0 * qa
         rr -= w1 * qb
         rr *= 1.0 / beta[jnd - 1]
     else:
         qa = V[:, cols[-1]]
-        w0 = float(qa @ rr)
-        alpha[jnd - 1] += w0
+        w0 = np.vdot(qa, rr)
+        alpha[jnd - 1] += w0.real
         rr -= w0 * qa
         rr *= 1.0 / beta[jnd - 1]
     if info.stat == 0:
@@ -673,11 +771,15 @@ def ritz_vectors(st, lck, ny, yy, jl):
     st.V[:, lck:lck + ny] = out
 
 
-def trlan(matvec, info, nrow, mev, start, seed=12345):
-    """trlan (SRC/trlan.c:21-301) + trlanczos (SRC/trlcore.c:723-1532).
+def trlan(matvec, info, nrow, mev, start, seed=12345, complex_=False, gdot_quirk=True):
+    """trlan (SRC/trlan.c:21-301) + trlanczos (SRC/trlcore.c:723-1532), or
+    with complex_=True ztrlan (SRC/ztrlan.c:26-221) + ztrlanczos
+    (SRC/ztrlcore.c:436-1286).
 
     matvec(x) -> A x (numpy).  start: the start vector (used when
-    info.guess == 1).  Returns dict(eval, evec, wrk, info).
+    info.guess == 1).  gdot_quirk (complex only) reproduces the reference's
+    two-column ztrl_g_dot_ branches; False gives the exact V^H r.
+    Returns dict(eval, evec, wrk, info).
     """
     rng = np.random.default_rng(seed)
     out_eval = np.zeros(mev)
@@ -689,7 +791,7 @@ def trlan(matvec, info, nrow, mev, start, seed=12345):
     clear_counter(info, nrow, mev, nrow)
     if info.stat != 0:
         return dict(eval=out_eval, evec=np.zeros((nrow, mev)), wrk=np.zeros(mev), info=info)
-    st = _State(nrow, mev, info.maxlan, rng)
+    st = _State(nrow, mev, info.maxlan, rng, complex_, gdot_quirk)
     wrk_out = _trlanczos(matvec, info, st, mev, out_eval, start)
     evec = st.V[:, :mev].copy()
     return dict(eval=out_eval, evec=evec, wrk=wrk_out, info=info)
@@ -736,7 +838,7 @@ def _trlanczos(matvec, info, st, mev, eval_, start):
             rr = mev + j2n - 1
         V[:, rr] = matvec(V[:, qa])
         info.matvec += 1
-        alpha[jnd - 1] = float(V[:, qa] @ V[:, rr])
+        alpha[jnd - 1] = float(np.vdot(V[:, qa], V[:, rr]).real)
         beta[jnd - 1] = alpha[jnd - 1]
         if j1 > 2:
             V[:, rr] -= V[:, 0:j1] @ beta[0:j1]
@@ -783,7 +885,7 @@ def _trlanczos(matvec, info, st, mev, eval_, start):
                     break
             V[:, rr] = matvec(V[:, qa])
             info.matvec += 1
-            alpha[jnd - 1] = float(V[:, qa] @ V[:, rr])
+            alpha[jnd - 1] = float(np.vdot(V[:, qa], V[:, rr]).real)
             V[:, rr] -= alpha[jnd - 1] * V[:, qa]
             V[:, rr] -= beta[jnd - 2] * V[:, qb]
             orth(info, st, j1, j2, rr, kept)

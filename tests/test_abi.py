@@ -1,0 +1,124 @@
+"""The C-ABI boundary, checked without a GPU.
+
+* libnutrlan_amd.so loads and exports every function include/*.h declares.
+* trl_info has the reference's layout (size and field offsets, as the
+  compiled reference reported them) in both the C header and the Python
+  mirror.
+* trl_init_info applies the reference's defaults/clamps (oracle restatement).
+* Device entry points fail cleanly (no crash) when no GPU is visible.
+"""
+import ctypes
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from nutrlan_amd import trlan as T
+from oracle import trlan_oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "ref_cases.json")))
+
+
+def declared_functions():
+    names = set()
+    for h in ("trlan.h", "ztrlan.h", "trlhip.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(",
+                             txt, flags=re.M):
+            if m.group(0).lstrip().startswith("typedef"):
+                continue
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    T.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", T.LIBPATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    decl = declared_functions()
+    assert len(decl) >= 30
+    missing = [d for d in decl if d not in exported]
+    assert not missing, missing
+
+
+def test_info_layout_matches_reference():
+    lay = GOLD["info_layout"]  # as the compiled reference reported it
+    fields = ["stat", "lohi", "ned", "nec", "tol", "mpicomp", "maxlan", "klan", "maxmv",
+              "restart", "locked", "guess", "matvec", "nloop", "north", "nrand", "anrm",
+              "nloc", "ntot", "verbose", "trgt", "tres", "ref", "log_fp", "log_file", "cpfile",
+              "oldcpf", "mvparam"]
+    assert ctypes.sizeof(T.TrlInfo) == lay[0]
+    for f, off in zip(fields, lay[1:]):
+        assert getattr(T.TrlInfo, f).offset == off, f
+    # the C header itself, compiled here
+    src = os.path.join(ROOT, "_work", "layout_probe.c")
+    os.makedirs(os.path.dirname(src), exist_ok=True)
+    with open(src, "w") as fh:
+        fh.write('#include <stddef.h>\n#include <stdio.h>\n#include "trlan.h"\n'
+                 'int main(void){printf("%zu %zu %zu %zu\\n", sizeof(trl_info),'
+                 ' offsetof(trl_info, guess), offsetof(trl_info, anrm),'
+                 ' offsetof(trl_info, mvparam)); return 0;}\n')
+    exe = src[:-2]
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
+    got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    assert got == [lay[0], lay[12], lay[17], lay[-1]]
+
+
+@pytest.mark.parametrize("args", [
+    (1000, 40, 1, 5, 1.4e-8, 1, 5000),
+    (500, 5, -1, 8, 0.0, 0, 0),         # maxlan <= ned, default tol and maxmv
+    (2000, 100, 0, 10, 2.0, 9, 100),    # tol > 1 -> min(0.1, 1/tol)
+    (300, 20, -1, 4, 1e-320, -3, 7),    # tol <= DBL_MIN -> eps, restart <= 0 -> 0
+])
+def test_init_info_defaults(args):
+    n, maxlan, lohi, ned, tol, restart, maxmv = args
+    i = T.init_info(n, maxlan, lohi, ned, tol, restart, maxmv)
+    o = O.init_info(n, maxlan, lohi, ned, tol, restart, maxmv)
+    for f in ["tol", "restart", "maxmv", "maxlan", "lohi", "ned", "nloc", "ntot", "tmv",
+              "trgt", "tres", "crat"]:
+        assert getattr(i, f) == getattr(o, f), f
+    assert i.guess == 0 and i.nec == 0 and i.mvparam is None
+
+
+def test_trlan_validation_errors_without_gpu():
+    """Argument errors are reported in info->stat before any device use
+    (reference trl_clear_counter codes -1..-4)."""
+    L = T.lib()
+    n = 100
+    ev = np.zeros(5)
+    evec = np.zeros(5 * n)
+    wrk = np.zeros(5)
+    P = ctypes.c_void_p
+
+    def run(info, nrow, mev, lde):
+        L.trlan(P(T.fn_addr("trlhip_csr_matvec")), None, ctypes.byref(info), nrow, mev,
+                ev.ctypes.data_as(P), evec.ctypes.data_as(P), lde, 5, wrk.ctypes.data_as(P))
+        return info.stat
+
+    assert run(T.init_info(n, 20, 1, 3, 1e-8, 1, 100), n + 1, 5, n + 1) == -1   # nrow != nloc
+    assert run(T.init_info(n, 20, 1, 3, 1e-8, 1, 100), n, 5, n - 1) == -2       # lde < nrow
+    assert run(T.init_info(8, 20, 1, 3, 1e-8, 1, 100), 8, 5, 8) == -4           # ntot < 10
+    if T.device_count() == 0:
+        # valid arguments but no device: a clean device-failure status
+        assert run(T.init_info(n, 20, 1, 3, 1e-8, 1, 100), n, 5, n) == T.TRLHIP_STAT_DEVICE
+
+
+def test_device_entry_points_fail_cleanly_without_gpu():
+    if T.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    h = ctypes.c_void_p()
+    rp = np.arange(11, dtype=np.int32)
+    ci = np.arange(10, dtype=np.int32)
+    v = np.ones(10)
+    L = T.lib()
+    rc = L.trlhip_csr_create(ctypes.byref(h), 10, 10, rp.ctypes.data_as(ctypes.c_void_p),
+                             ci.ctypes.data_as(ctypes.c_void_p),
+                             v.ctypes.data_as(ctypes.c_void_p), T.TRLHIP_R64)
+    assert rc == -4  # TRLHIP_ENODEV

@@ -1,0 +1,230 @@
+"""Host FP64 kit of the driver (trlkit_* in libnutrlan_amd.so) against the
+oracle and LAPACK, on the CPU.
+
+The kit's dense eigen-solvers are this project's own (implicit QL,
+bottom-up Householder); they are checked against scipy's LAPACK.  Its
+decision logic (paired sorts, convergence test, restart schemes, locking)
+restates the reference and is checked against the oracle's restatement on
+randomized inputs, including ties and every restart scheme.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from nutrlan_amd import trlan as T
+from oracle import restart_oracle, sorts
+from oracle import trlan_oracle as O
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+D = ctypes.c_double
+
+
+@pytest.fixture(scope="module")
+def kit():
+    L = T.lib()
+    L.trlkit_sort2.argtypes = [I, I, D, P, P, P]
+    L.trlkit_tridiag_ql.argtypes = [I, P, P, P, I]
+    L.trlkit_tridiag_ql.restype = I
+    L.trlkit_householder.argtypes = [I, P, P, P]
+    L.trlkit_syev.argtypes = [I, P, P]
+    L.trlkit_syev.restype = I
+    L.trlkit_convergence_test.argtypes = [I, P, P, P, ctypes.POINTER(T.TrlInfo)]
+    L.trlkit_shuffle_eig.argtypes = [I, I, P, P, P, ctypes.POINTER(T.TrlInfo), I]
+    L.trlkit_shuffle_eig.restype = I
+    L.trlkit_set_locking.argtypes = [I, I, P, P, P, D]
+    L.trlkit_set_locking.restype = I
+    L.trlkit_sort_eig.argtypes = [I, I, I, D, P, P, P]
+    L.trlkit_smooth_rr.argtypes = [I, P]
+    return L
+
+
+def p(a):
+    return a.ctypes.data_as(P)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 40, 129])
+def test_tridiag_ql_vs_lapack(kit, n):
+    rng = np.random.default_rng(n)
+    d = rng.standard_normal(n)
+    e = rng.standard_normal(max(n - 1, 0))
+    w_ref, Z_ref = sla.eigh_tridiagonal(d, e)
+    dd, ee, z = d.copy(), np.concatenate([e, [0.0]]), np.zeros(n)
+    assert kit.trlkit_tridiag_ql(n, p(dd), p(ee), p(z), 1) == 0
+    scale = max(abs(w_ref).max(), 1.0)
+    assert np.abs(dd - w_ref).max() <= 1e-13 * scale
+    assert np.abs(np.abs(z) - np.abs(Z_ref[-1])).max() <= 1e-12
+    # full vectors from identity: A Z = Z diag(w)
+    dd, ee = d.copy(), np.concatenate([e, [0.0]])
+    Z = np.eye(n).copy(order="F")
+    assert kit.trlkit_tridiag_ql(n, p(dd), p(ee), p(Z), 2) == 0
+    A = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    Zc = Z  # Fortran order: column j = eigenvector j
+    assert np.abs(A @ Zc - Zc * dd).max() <= 1e-12 * scale
+    assert np.abs(Zc.T @ Zc - np.eye(n)).max() <= 1e-13 * max(n, 1)
+
+
+@pytest.mark.parametrize("n", [2, 5, 33, 64])
+def test_householder_arrowhead(kit, n):
+    """Arrowhead -> tridiagonal with Q fixing the last coordinate (the
+    dsytrd('U') + dorgtr convention): A = Q T Q^T."""
+    rng = np.random.default_rng(100 + n)
+    alpha = rng.standard_normal(n)
+    beta = rng.standard_normal(n)
+    A = np.diag(alpha)
+    A[:-1, -1] = beta[:-1]
+    A[-1, :-1] = beta[:-1]
+    a = np.asfortranarray(A.copy())
+    d, e = np.zeros(n), np.zeros(n)
+    kit.trlkit_householder(n, p(a), p(d), p(e))
+    Q = a  # column-major n x n in Fortran order
+    Tm = np.diag(d) + np.diag(e[:-1], 1) + np.diag(e[:-1], -1)
+    assert np.abs(Q @ Tm @ Q.T - A).max() <= 1e-13 * n * max(abs(A).max(), 1)
+    assert np.abs(Q.T @ Q - np.eye(n)).max() <= 1e-13 * n
+    assert np.allclose(Q[:, -1], np.eye(n)[:, -1]) and np.allclose(Q[-1], np.eye(n)[-1])
+    assert abs(d[-1] - alpha[-1]) == 0.0
+
+
+@pytest.mark.parametrize("n", [1, 4, 31, 100])
+def test_syev_vs_lapack(kit, n):
+    rng = np.random.default_rng(200 + n)
+    B = rng.standard_normal((n, n))
+    A = np.asfortranarray(B + B.T)
+    a = A.copy(order="F")
+    w = np.zeros(n)
+    assert kit.trlkit_syev(n, p(a), p(w)) == 0
+    w_ref = np.linalg.eigvalsh(A)
+    assert np.abs(w - w_ref).max() <= 1e-12 * abs(w_ref).max()
+    assert np.abs(A @ a - a * w).max() <= 1e-11 * abs(w_ref).max()
+
+
+@pytest.mark.parametrize("key", [0, 1, 2, 3, 4])
+def test_sort2_matches_reference_shell_sort(kit, key):
+    rng = np.random.default_rng(key)
+    for n in [0, 1, 2, 9, 50]:
+        a1 = np.round(rng.standard_normal(n) * 3) / 2  # ties
+        a2 = np.arange(n, dtype=np.float64)
+        s = 0.7
+        x1, x2 = a1.copy(), a2.copy()
+        tag = np.arange(n, dtype=np.int32)
+        kit.trlkit_sort2(n, key, s, p(x1), p(x2), p(tag))
+        y1, y2 = list(a1), list(a2)
+        fn = [lambda: sorts.dsort2(n, y1, y2), lambda: sorts.dsort2a(n, y1, y2),
+              lambda: sorts.dsort2s(n, s, y1, y2), lambda: sorts.dsort2su(n, s, y1, y2),
+              lambda: sorts.dsort2sd(n, s, y1, y2)][key]
+        fn()
+        assert list(x1) == y1 and list(x2) == y2
+        assert list(tag) == [int(v) for v in y2]
+
+
+def mk_info(o):
+    """TrlInfo with the fields of an oracle Info."""
+    i = T.TrlInfo()
+    for f, _ in T.TrlInfo._fields_:
+        if hasattr(o, f) and f not in ("mpicomp", "log_fp", "cpt_fp", "mvparam"):
+            v = getattr(o, f)
+            if isinstance(v, (int, float, np.floating, np.integer)):
+                setattr(i, f, type(getattr(i, f))(v) if not isinstance(getattr(i, f), bytes) else v)
+    return i
+
+
+def rand_state(seed, nd, lohi, restart, ned):
+    rng = np.random.default_rng(seed)
+    o = O.Info()
+    o.lohi, o.ned, o.restart = lohi, ned, restart
+    o.tol, o.anrm = 1e-8, 10.0
+    o.maxlan = nd + 4
+    o.klan = nd
+    o.maxmv = 5000
+    o.matvec = int(rng.integers(nd, 2000))
+    o.nloop = int(rng.integers(0, 5))
+    o.nec = int(rng.integers(0, ned))
+    o.locked = int(rng.integers(0, 3))
+    o.crat = float(rng.uniform(0.5, 0.99))
+    o.cfac = float(rng.uniform(0.01, 0.9))
+    o.ptres = float(rng.uniform(1e-6, 1e-2))
+    o.avgm = float(rng.uniform(5, 20))
+    o.k = int(rng.integers(0, nd // 2))
+    o.rfact = 1.5
+    lam = np.sort(rng.standard_normal(nd) * 3)
+    res = 10.0 ** rng.uniform(-14, -1, nd)
+    o.trgt = float(lam[int(rng.integers(0, nd))])
+    return o, lam, res
+
+
+@pytest.mark.parametrize("restart", [0, 1, 2, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("lohi", [-1, 0, 1])
+def test_shuffle_eig_vs_oracle(kit, restart, lohi):
+    for seed in range(25):
+        nd = 8 + seed
+        o, lam, res = rand_state(1000 * restart + 10 * (lohi + 1) + seed, nd, lohi, restart, 6)
+        info = mk_info(o)
+        # one-past-the-end reads see the same (reference-like) neighbours
+        l1 = np.concatenate([lam, [res[0]]])
+        r1 = np.concatenate([res, [0.5, 0.5]])
+        tag = np.arange(nd + 2, dtype=np.int32)
+        l2 = l1.copy()
+        r2 = r1.copy()
+        with np.errstate(all="ignore"):
+            kept_o = restart_oracle.shuffle_eig(nd, nd, l2, r2, o, 3, 0)
+        kept_c = kit.trlkit_shuffle_eig(nd, nd, p(l1), p(r1), p(tag), ctypes.byref(info), 3)
+        assert kept_c == kept_o, (seed, nd)
+        assert np.array_equal(l1[:kept_c], l2[:kept_c]) and np.array_equal(r1[:kept_c], r2[:kept_c])
+        assert all(l1[k] == lam[tag[k]] for k in range(kept_c))
+        assert info.klan == o.klan and info.lohi == o.lohi
+        assert info.avgm == o.avgm and info.k == o.k
+
+
+@pytest.mark.parametrize("lohi", [-4, -3, -2, -1, 0, 1])
+def test_convergence_test_vs_oracle(kit, lohi):
+    for seed in range(20):
+        nd = 5 + seed
+        o, lam, res = rand_state(77 + seed, nd, lohi, 1, 4)
+        o.tmv = 3 if seed % 2 else -1
+        o.tres = 1e-3
+        o.ref = 0.3
+        res[: seed % 4] = 1e-13
+        info = mk_info(o)
+        l1, r1 = lam.copy(), res.copy()
+        tag = np.arange(nd, dtype=np.int32)
+        kit.trlkit_convergence_test(nd, p(l1), p(r1), p(tag), ctypes.byref(info))
+        l2, r2 = lam.copy(), res.copy()
+        O.convergence_test(nd, l2, r2, o)
+        assert np.array_equal(l1, l2) and np.array_equal(r1, r2)
+        for f in ["nec", "trgt", "tres", "anrm", "crat", "cfac", "tmv", "ptres"]:
+            assert getattr(info, f) == getattr(o, f), f
+
+
+def test_set_locking_and_sort_eig_vs_oracle(kit):
+    rng = np.random.default_rng(5)
+    for trial in range(30):
+        nlam, jnd = 6 + trial % 5, 20
+        lam = np.sort(rng.standard_normal(nlam))
+        res = 10.0 ** rng.uniform(-18, -8, nlam)
+        yy = np.asfortranarray(rng.standard_normal((jnd, nlam)))
+        a1, b1, y1 = lam.copy(), res.copy(), yy.copy(order="F")
+        k1 = kit.trlkit_set_locking(jnd, nlam, p(a1), p(b1), p(y1), 3.0)
+        a2, b2, y2 = lam.copy(), res.copy(), yy.copy(order="F")
+        k2 = O.set_locking(jnd, nlam, a2, b2, y2, 3.0)
+        assert k1 == k2 and np.array_equal(a1, a2) and np.array_equal(b1, b2)
+        assert np.array_equal(y1, y2)
+    for lohi in [-4, -3, -2, -1, 0, 1]:
+        lam = rng.standard_normal(12)
+        res = rng.uniform(0, 1, 12)
+        a1, b1, tag = lam.copy(), res.copy(), np.arange(12, dtype=np.int32)
+        kit.trlkit_sort_eig(12, lohi, 5, 0.2, p(a1), p(b1), p(tag))
+        a2, b2 = lam.copy(), res.copy()
+        O.sort_eig(12, lohi, 5, 0.2, a2, b2)
+        assert np.array_equal(a1[:5], a2[:5]) and np.array_equal(b1[:5], b2[:5])
+        assert all(a1[k] == lam[tag[k]] for k in range(5))
+
+
+def test_smooth_rr_quirks(kit):
+    x = np.arange(1.0, 8.0)
+    y = x.copy()
+    kit.trlkit_smooth_rr(len(x), p(y))
+    z = list(x)
+    O.smooth_rr(z := np.array(z))
+    assert np.array_equal(y, z)
