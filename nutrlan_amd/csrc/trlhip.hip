@@ -9,7 +9,7 @@
 //
 // Reductions (alpha, ||r||^2, V^H r) never leave the device until the final
 // scalar: each block reduces its rows (wave shuffles, then LDS) into a
-// partial slot, and the last block to finish (atomic ticket) sums the
+// partial slot, and a finalize kernel (or the consuming kernel) sums the
 // partials in block order, so results are deterministic run to run.
 #include <hip/hip_runtime.h>
 
@@ -30,8 +30,6 @@ constexpr int kNnzBlk = 2048;          // nonzeros staged per SpMV row block
 constexpr int kRowsBlk = kBlock;       // rows per SpMV row block (at most)
 constexpr uint64_t kCsrMagic = 0x7472686970637372ull;  // "trhipcsr"
 
-// ticket slots of a context (one per concurrent reduction site)
-enum { T_PROJ = 0, T_SPMV = 61, T_DOT = 62, T_UPDATE = 63, T_COUNT = 64 };
 
 #define TRY(expr)                                  \
     do {                                           \
@@ -88,22 +86,11 @@ __device__ __forceinline__ double block_sum(double v)
     return t;
 }
 
-// Release this block's partial writes, take a ticket; true in every thread
-// of the block that finished last.  That block resets the ticket.
-__device__ __forceinline__ bool last_block(unsigned *ticket, unsigned nblocks)
-{
-    __shared__ int amlast;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned t = atomicAdd(ticket, 1u);
-        amlast = (t == nblocks - 1) ? 1 : 0;
-        if (amlast) atomicExch(ticket, 0u);
-    }
-    __syncthreads();
-    if (amlast) __threadfence();
-    return amlast != 0;
-}
+// Cross-block reductions never synchronize blocks inside a kernel (no
+// device-scope fences or tickets, which are costly on this memory model):
+// every block writes its partial, and a later kernel -- the finalize
+// kernels below, or the consumer itself -- sums the partials in block
+// order after the kernel boundary has made them visible.
 
 // Sum of part[b * stride + p] over b < nb, four independent accumulators
 // so the loads overlap; the order depends only on nb.
@@ -122,121 +109,130 @@ __device__ __forceinline__ double strided_sum(const double *part, int nb, size_t
     return (s0 + s1) + (s2 + s3);
 }
 
-// Ordered sum of nb scalar partials by the last block; result in thread 0.
+// Ordered sum of nb scalar partials by one block; result in thread 0.
 __device__ __forceinline__ double ordered_sum(const double *part, int nb)
 {
     return block_sum(strided_sum(part, nb, 1, 0, threadIdx.x, kBlock));
 }
 
-__device__ __forceinline__ void write_scalar(double t, double *out0, double *out1, int E)
+// Finalize a scalar reduction: out[0] = sum of nb partials (one block).
+__global__ __launch_bounds__(kBlock) void k_sum(const double *__restrict__ part, int nb,
+                                                double *__restrict__ out)
 {
-    out0[0] = t;
-    if (out1) {
-        out1[0] = t;
-        if (E == 2) out1[1] = 0.0;
-    }
+    const double t = ordered_sum(part, nb);
+    if (threadIdx.x == 0) out[0] = t;
+}
+
+// Finalize a vector reduction: hout[p] = sum_b part[b * np + p], p < np.
+// Block x handles 64 consecutive p; its 4 waves split b and combine in LDS.
+__global__ __launch_bounds__(kBlock) void k_colsum(const double *__restrict__ part, int nb, int np,
+                                                   double *__restrict__ hout)
+{
+    __shared__ double red[kBlock];
+    const int p = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int q = threadIdx.x >> 6;
+    red[threadIdx.x] = p < np ? strided_sum(part, nb, (size_t)np, (size_t)p, q, kBlock / 64) : 0.0;
+    __syncthreads();
+    if (threadIdx.x < 64 && p < np)
+        hout[p] = (red[threadIdx.x] + red[threadIdx.x + 64]) +
+                  (red[threadIdx.x + 128] + red[threadIdx.x + 192]);
 }
 
 // ---------------------------------------------------------------- SpMV
 
-// y = A x, CSR-stream: a block owns a row block [rb[b], rb[b+1]) whose
-// nonzeros (at most kNnzBlk) it stages as products val*x[col] in LDS with
-// fully coalesced loads, then each thread sums one row from LDS.  A row
-// block holding one very long row is reduced by the whole block.
-// Epilogue: Re(x^H y) into out0 (and out1 as a coefficient slot).
+// y = A x, CSR-stream: a block walks row blocks [rb[b], rb[b+1]) (grid
+// stride); it stages each row block's nonzeros (at most kNnzBlk) as
+// products val*x[col] in LDS with fully coalesced loads, then each thread
+// sums one row from LDS.  A row block holding one very long row is reduced
+// by the whole block.  Epilogue: the block's partial of Re(x^H y) in
+// part[blockIdx.x] (summed by the consumer).
 // S = vector storage, SV = matrix value storage, E = doubles per element.
 template <typename S, typename SV, int E>
 __global__ __launch_bounds__(kBlock) void k_spmv(
-    const int *__restrict__ rb, const int *__restrict__ rp, const int *__restrict__ ci,
+    int nb, const int *__restrict__ rb, const int *__restrict__ rp, const int *__restrict__ ci,
     const SV *__restrict__ val, const S *__restrict__ x, S *__restrict__ y,
-    double *__restrict__ part, unsigned *__restrict__ ticket, double *__restrict__ out0,
-    double *__restrict__ out1)
+    double *__restrict__ part)
 {
     __shared__ double prod[kNnzBlk * E];
-    const int r0 = rb[blockIdx.x], r1 = rb[blockIdx.x + 1];
-    const int k0 = rp[r0], nz = rp[r1] - k0;
     double acc = 0.0;
-    if (r1 - r0 > 1 || nz <= kNnzBlk) {
-        for (int k = threadIdx.x; k < nz; k += kBlock) {
-            const int c = ci[k0 + k];
-            if constexpr (E == 1) {
-                prod[k] = (double)val[k0 + k] * (double)x[c];
-            } else {
-                const double ar = val[2 * (k0 + k)], ai = val[2 * (k0 + k) + 1];
-                const double xr = x[2 * c], xi = x[2 * c + 1];
-                prod[2 * k] = ar * xr - ai * xi;
-                prod[2 * k + 1] = ar * xi + ai * xr;
-            }
-        }
-        __syncthreads();
-        const int t = threadIdx.x;
-        if (t < r1 - r0) {
-            const int row = r0 + t;
-            const int a = rp[row] - k0, b = rp[row + 1] - k0;
-            if constexpr (E == 1) {
-                double s = 0.0;
-                for (int k = a; k < b; ++k) s += prod[k];
-                y[row] = (S)s;
-                acc = (double)x[row] * (double)(S)s;
-            } else {
-                double sr = 0.0, si = 0.0;
-                for (int k = a; k < b; ++k) {
-                    sr += prod[2 * k];
-                    si += prod[2 * k + 1];
+    for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+        const int r0 = rb[blk], r1 = rb[blk + 1];
+        const int k0 = rp[r0], nz = rp[r1] - k0;
+        if (r1 - r0 > 1 || nz <= kNnzBlk) {
+#pragma unroll 4
+            for (int k = threadIdx.x; k < nz; k += kBlock) {
+                const int c = ci[k0 + k];
+                if constexpr (E == 1) {
+                    prod[k] = (double)val[k0 + k] * (double)x[c];
+                } else {
+                    const double ar = val[2 * (k0 + k)], ai = val[2 * (k0 + k) + 1];
+                    const double xr = x[2 * c], xi = x[2 * c + 1];
+                    prod[2 * k] = ar * xr - ai * xi;
+                    prod[2 * k + 1] = ar * xi + ai * xr;
                 }
-                y[2 * row] = sr;
-                y[2 * row + 1] = si;
-                acc = x[2 * row] * sr + x[2 * row + 1] * si;
             }
-        }
-    } else {
-        double sr = 0.0, si = 0.0;
-        for (int k = threadIdx.x; k < nz; k += kBlock) {
-            const int c = ci[k0 + k];
-            if constexpr (E == 1) {
-                sr += (double)val[k0 + k] * (double)x[c];
-            } else {
-                const double ar = val[2 * (k0 + k)], ai = val[2 * (k0 + k) + 1];
-                sr += ar * x[2 * c] - ai * x[2 * c + 1];
-                si += ar * x[2 * c + 1] + ai * x[2 * c];
+            __syncthreads();
+            const int t = threadIdx.x;
+            if (t < r1 - r0) {
+                const int row = r0 + t;
+                const int a = rp[row] - k0, b = rp[row + 1] - k0;
+                if constexpr (E == 1) {
+                    double s = 0.0;
+                    for (int k = a; k < b; ++k) s += prod[k];
+                    y[row] = (S)s;
+                    acc += (double)x[row] * (double)(S)s;
+                } else {
+                    double sr = 0.0, si = 0.0;
+                    for (int k = a; k < b; ++k) {
+                        sr += prod[2 * k];
+                        si += prod[2 * k + 1];
+                    }
+                    y[2 * row] = sr;
+                    y[2 * row + 1] = si;
+                    acc += x[2 * row] * sr + x[2 * row + 1] * si;
+                }
             }
-        }
-        sr = block_sum(sr);
-        if (E == 2) si = block_sum(si);
-        if (threadIdx.x == 0) {
-            if constexpr (E == 1) {
-                y[r0] = (S)sr;
-                acc = (double)x[r0] * (double)(S)sr;
-            } else {
-                y[2 * r0] = sr;
-                y[2 * r0 + 1] = si;
-                acc = x[2 * r0] * sr + x[2 * r0 + 1] * si;
+            __syncthreads();
+        } else {
+            double sr = 0.0, si = 0.0;
+            for (int k = threadIdx.x; k < nz; k += kBlock) {
+                const int c = ci[k0 + k];
+                if constexpr (E == 1) {
+                    sr += (double)val[k0 + k] * (double)x[c];
+                } else {
+                    const double ar = val[2 * (k0 + k)], ai = val[2 * (k0 + k) + 1];
+                    sr += ar * x[2 * c] - ai * x[2 * c + 1];
+                    si += ar * x[2 * c + 1] + ai * x[2 * c];
+                }
+            }
+            sr = block_sum(sr);
+            if (E == 2) si = block_sum(si);
+            if (threadIdx.x == 0) {
+                if constexpr (E == 1) {
+                    y[r0] = (S)sr;
+                    acc += (double)x[r0] * (double)(S)sr;
+                } else {
+                    y[2 * r0] = sr;
+                    y[2 * r0 + 1] = si;
+                    acc += x[2 * r0] * sr + x[2 * r0 + 1] * si;
+                }
             }
         }
     }
     const double bs = block_sum(acc);
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
-    if (last_block(ticket, gridDim.x)) {
-        const double t = ordered_sum(part, gridDim.x);
-        if (threadIdx.x == 0) write_scalar(t, out0, out1, E);
-    }
 }
 
-// Re(x^H y) over n elements, outputs as k_spmv's epilogue.
-template <typename S, int E>
-__global__ __launch_bounds__(kBlock) void k_dot(
-    long ne, const S *__restrict__ x, const S *__restrict__ y, double *__restrict__ part,
-    unsigned *__restrict__ ticket, double *__restrict__ out0, double *__restrict__ out1)
+// Re(x^H y) over ne doubles: per-block partials in part[blockIdx.x].
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_dot(long ne, const S *__restrict__ x,
+                                                const S *__restrict__ y, double *__restrict__ part)
 {
     double acc = 0.0;
     for (long i = blockIdx.x * kBlock + threadIdx.x; i < ne; i += (long)gridDim.x * kBlock)
         acc += (double)x[i] * (double)y[i];
     const double bs = block_sum(acc);
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
-    if (last_block(ticket, gridDim.x)) {
-        const double t = ordered_sum(part, gridDim.x);
-        if (threadIdx.x == 0) write_scalar(t, out0, out1, E);
-    }
 }
 
 // --------------------------------------------------------- Gram-Schmidt
@@ -246,12 +242,12 @@ __global__ __launch_bounds__(kBlock) void k_dot(
 // before the FMAs (column indices past the chunk's end are clamped to its
 // last column, so the extra loads hit cache and are discarded), keeps CB*E
 // running sums in registers, then transposes-and-reduces them over the
-// wave, combines waves in LDS and writes its partial row.  The last block
-// sums the partial rows in block order.
+// wave, combines waves in LDS and writes its partial row part[b*m*E ...]
+// (k_colsum sums the rows in block order).
 template <typename S, int E, int CB>
 __global__ __launch_bounds__(kBlock) void k_proj(
     int n, const S *__restrict__ V, long ld, int c0, int m, const S *__restrict__ r,
-    double *__restrict__ part, unsigned *__restrict__ ticket, double *__restrict__ hout)
+    double *__restrict__ part)
 {
     constexpr int N = CB * E;
     static_assert(N <= 64, "chunk too wide");
@@ -299,23 +295,31 @@ __global__ __launch_bounds__(kBlock) void k_proj(
         }
         __syncthreads();
     }
-    if (last_block(ticket, G)) {
-        for (int p = threadIdx.x; p < mE; p += kBlock)
-            hout[p] = strided_sum(part, G, (size_t)mE, (size_t)p, 0, 1);
-    }
 }
 
-// rout = rin - V(:, c0:c0+m) h ; ||rout||^2 -> out_rr2 ; ||h||^2 -> out_hh.
-// h (E*m doubles, FP64) is staged in LDS; four independent partial sums
-// per row keep the FMA chains short.  rin may equal rout.
+// rout = rin - V(:, c0:c0+m) h ; per-block partials of ||rout||^2 in
+// part[blockIdx.x] ; ||h||^2 -> out_hh.  h (E*m doubles, FP64) is staged in
+// LDS.  When cpart is given, coefficient `cslot` is not read from h but is
+// the sum of the cnp partials in cpart (the alpha that the preceding
+// SpMV/dot left as per-block partials); every block forms that sum in the
+// same order, and block 0 also stores it to out_coef.  Four independent
+// partial sums per row keep the FMA chains short.  rin may equal rout.
 template <typename S, int E>
 __global__ __launch_bounds__(kBlock) void k_update(
     int n, const S *__restrict__ V, long ld, int c0, int m, const double *__restrict__ h,
-    const S *rin, S *rout, double *__restrict__ part, unsigned *__restrict__ ticket,
-    double *__restrict__ out_rr2, double *__restrict__ out_hh)
+    const double *__restrict__ cpart, int cnp, int cslot, const S *rin, S *rout,
+    double *__restrict__ part, double *__restrict__ out_hh, double *__restrict__ out_coef)
 {
     extern __shared__ double hs[];
     for (int i = threadIdx.x; i < E * m; i += kBlock) hs[i] = h[i];
+    if (cpart) {
+        const double a = ordered_sum(cpart, cnp);  // contains __syncthreads
+        if (threadIdx.x == 0) {
+            hs[E * cslot] = a;
+            if (E == 2) hs[E * cslot + 1] = 0.0;
+            if (blockIdx.x == 0 && out_coef) out_coef[0] = a;
+        }
+    }
     __syncthreads();
     const S *Vc = V + (size_t)c0 * ld * E;
     double acc = 0.0;
@@ -375,10 +379,6 @@ __global__ __launch_bounds__(kBlock) void k_update(
         double t = 0.0;
         for (int i = 0; i < E * m; ++i) t += hs[i] * hs[i];
         out_hh[0] = t;
-    }
-    if (last_block(ticket, gridDim.x)) {
-        const double t = ordered_sum(part, gridDim.x);
-        if (threadIdx.x == 0 && out_rr2) out_rr2[0] = t;
     }
 }
 
@@ -527,7 +527,6 @@ struct trlhip_csr {
     float *val32;   // FP32 copy for a mixed-precision basis (R32 only)
     // staging for host-called matvec
     double *dx, *dy, *part, *dscal;
-    unsigned *tick;
     hipStream_t s;
 };
 
@@ -541,7 +540,7 @@ struct trlhip_ctx {
     char *V;
     double *part;
     size_t partcap; // doubles
-    unsigned *tick; // T_COUNT tickets
+    int np_alpha;   // partial count of the last SpMV / dot
     double *dscal;  // 16 device scalars
     double *dcoef;  // 2*(ncols+2) recurrence / restart-step coefficients
     double *dh;     // 2*(ncols+2) CGS projection
@@ -587,26 +586,40 @@ int ensure_part(trlhip_ctx *c, size_t need)
     return 0;
 }
 
+// Partial-sum regions of c->part: alpha partials (SpMV / dot), update
+// partials (||r||^2), projection partial rows.
+double *part_alpha(trlhip_ctx *c) { return c->part; }
+double *part_update(trlhip_ctx *c) { return c->part + kMaxGrid; }
+double *part_proj(trlhip_ctx *c) { return c->part + 2 * kMaxGrid; }
+size_t part_need(int E, int ncols) { return 2 * (size_t)kMaxGrid + (size_t)kProjGrid * E * (ncols + 1) + 64; }
+
 double spmv_bytes(const trlhip_csr *A, int E, size_t vsz, size_t xsz)
 {
     // values + col_idx + row_ptr + x read once + y written once
     return (double)A->nnz * ((double)vsz * E + 4.0) + 4.0 * (A->n + 1) + 2.0 * (double)xsz * E * A->n;
 }
 
-// y = A x on stream s with the row-block kernel for the context's storage.
-int launch_spmv(const trlhip_csr *A, int vtype, const void *x, void *y, double *part,
-                unsigned *tick, double *o0, double *o1, hipStream_t s)
+int spmv_grid(const trlhip_csr *A) { return A->nb < kMaxGrid ? A->nb : kMaxGrid; }
+
+// y = A x on stream s (row-block kernel for the storage); partials in part.
+int launch_spmv(const trlhip_csr *A, int vtype, const void *x, void *y, double *part, hipStream_t s)
 {
-    const dim3 g(A->nb), b(kBlock);
+    const dim3 g(spmv_grid(A)), b(kBlock);
     if (vtype == TRLHIP_R64)
-        hipLaunchKernelGGL((k_spmv<double, double, 1>), g, b, 0, s, A->rb, A->rp, A->ci, A->val,
-                           (const double *)x, (double *)y, part, tick, o0, o1);
+        hipLaunchKernelGGL((k_spmv<double, double, 1>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
+                           A->val, (const double *)x, (double *)y, part);
     else if (vtype == TRLHIP_C128)
-        hipLaunchKernelGGL((k_spmv<double, double, 2>), g, b, 0, s, A->rb, A->rp, A->ci, A->val,
-                           (const double *)x, (double *)y, part, tick, o0, o1);
+        hipLaunchKernelGGL((k_spmv<double, double, 2>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
+                           A->val, (const double *)x, (double *)y, part);
     else
-        hipLaunchKernelGGL((k_spmv<float, float, 1>), g, b, 0, s, A->rb, A->rp, A->ci, A->val32,
-                           (const float *)x, (float *)y, part, tick, o0, o1);
+        hipLaunchKernelGGL((k_spmv<float, float, 1>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
+                           A->val32, (const float *)x, (float *)y, part);
+    return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
+}
+
+int launch_sum(const double *part, int np, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(kBlock), 0, s, part, np, out);
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
@@ -618,83 +631,91 @@ bool csr_fits(const trlhip_ctx *c, const trlhip_csr *A)
     return A->vtype == TRLHIP_R64 || A->vtype == TRLHIP_R32;
 }
 
-int run_spmv(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol, double *o1)
+// y col = A x col; Re(x^H y) left as partials in part_alpha (c->np_alpha).
+int run_spmv(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol)
 {
-    int rc = ensure_part(c, (size_t)A->nb);
-    if (rc) return rc;
     const size_t vsz = is_f32(c) ? 4 : 8;
+    c->np_alpha = spmv_grid(A);
     Timed t(TRLHIP_K_SPMV, c->s, spmv_bytes(A, c->E, vsz, vsz));
     return launch_spmv(A, c->vtype, c->V + (size_t)xcol * c->ld * c->esz,
-                       c->V + (size_t)ycol * c->ld * c->esz, c->part, c->tick + T_SPMV,
-                       c->dscal + S_ALPHA, o1, c->s);
+                       c->V + (size_t)ycol * c->ld * c->esz, part_alpha(c), c->s);
 }
 
-int run_dot(trlhip_ctx *c, int xcol, int ycol, double *o1)
+// Re(x^H y) of two columns as partials in part_alpha.
+int run_dot(trlhip_ctx *c, int xcol, int ycol)
 {
     const long ne = (long)c->n * c->E;
     const int g = grid_for(ne, kBlock * 8, kMaxGrid);
+    c->np_alpha = g;
     Timed t(TRLHIP_K_MISC, c->s, 2.0 * (double)c->esz * c->n);
     if (is_f32(c))
-        hipLaunchKernelGGL((k_dot<float, 1>), dim3(g), dim3(kBlock), 0, c->s, ne,
-                           colp<float>(c, xcol), colp<float>(c, ycol), c->part, c->tick + T_DOT,
-                           c->dscal + S_ALPHA, o1);
-    else if (c->E == 1)
-        hipLaunchKernelGGL((k_dot<double, 1>), dim3(g), dim3(kBlock), 0, c->s, ne,
-                           colp<double>(c, xcol), colp<double>(c, ycol), c->part,
-                           c->tick + T_DOT, c->dscal + S_ALPHA, o1);
+        hipLaunchKernelGGL((k_dot<float>), dim3(g), dim3(kBlock), 0, c->s, ne, colp<float>(c, xcol),
+                           colp<float>(c, ycol), part_alpha(c));
     else
-        hipLaunchKernelGGL((k_dot<double, 2>), dim3(g), dim3(kBlock), 0, c->s, ne,
-                           colp<double>(c, xcol), colp<double>(c, ycol), c->part,
-                           c->tick + T_DOT, c->dscal + S_ALPHA, o1);
+        hipLaunchKernelGGL((k_dot<double>), dim3(g), dim3(kBlock), 0, c->s, ne,
+                           colp<double>(c, xcol), colp<double>(c, ycol), part_alpha(c));
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
-// rout = rin - V(:, c0:c0+m) dh
-int run_update(trlhip_ctx *c, int c0, int m, const double *dh, int rin, int rout, double *o_rr2,
-               double *o_hh, int kind)
+// rout = rin - V(:, c0:c0+m) dh; coefficient aslot (if >= 0) is the alpha
+// left in part_alpha (also stored to dscal[S_ALPHA]); ||rout||^2 -> o_rr2
+// and ||dh||^2 -> o_hh when requested.
+int run_update(trlhip_ctx *c, int c0, int m, const double *dh, int aslot, int rin, int rout,
+               double *o_rr2, double *o_hh, int kind)
 {
     const int g = grid_for(c->n, kBlock * 4, kMaxGrid);
     const size_t shm = sizeof(double) * (size_t)c->E * (m > 0 ? m : 1);
-    Timed t(kind, c->s, (double)c->esz * c->n * (m + 2));
-    if (is_f32(c))
-        hipLaunchKernelGGL((k_update<float, 1>), dim3(g), dim3(kBlock), shm, c->s, c->n,
-                           colp<float>(c, 0), (long)c->ld, c0, m, dh, colp<float>(c, rin),
-                           colp<float>(c, rout), c->part, c->tick + T_UPDATE, o_rr2, o_hh);
-    else if (c->E == 1)
-        hipLaunchKernelGGL((k_update<double, 1>), dim3(g), dim3(kBlock), shm, c->s, c->n,
-                           colp<double>(c, 0), (long)c->ld, c0, m, dh, colp<double>(c, rin),
-                           colp<double>(c, rout), c->part, c->tick + T_UPDATE, o_rr2, o_hh);
-    else
-        hipLaunchKernelGGL((k_update<double, 2>), dim3(g), dim3(kBlock), shm, c->s, c->n,
-                           colp<double>(c, 0), (long)c->ld, c0, m, dh, colp<double>(c, rin),
-                           colp<double>(c, rout), c->part, c->tick + T_UPDATE, o_rr2, o_hh);
-    return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
+    const double *cp = aslot >= 0 ? part_alpha(c) : nullptr;
+    double *oc = c->dscal + S_ALPHA;
+    {
+        Timed t(kind, c->s, (double)c->esz * c->n * (m + 2));
+        if (is_f32(c))
+            hipLaunchKernelGGL((k_update<float, 1>), dim3(g), dim3(kBlock), shm, c->s, c->n,
+                               colp<float>(c, 0), (long)c->ld, c0, m, dh, cp, c->np_alpha, aslot,
+                               colp<float>(c, rin), colp<float>(c, rout), part_update(c), o_hh, oc);
+        else if (c->E == 1)
+            hipLaunchKernelGGL((k_update<double, 1>), dim3(g), dim3(kBlock), shm, c->s, c->n,
+                               colp<double>(c, 0), (long)c->ld, c0, m, dh, cp, c->np_alpha, aslot,
+                               colp<double>(c, rin), colp<double>(c, rout), part_update(c), o_hh, oc);
+        else
+            hipLaunchKernelGGL((k_update<double, 2>), dim3(g), dim3(kBlock), shm, c->s, c->n,
+                               colp<double>(c, 0), (long)c->ld, c0, m, dh, cp, c->np_alpha, aslot,
+                               colp<double>(c, rin), colp<double>(c, rout), part_update(c), o_hh, oc);
+        if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
+    }
+    return o_rr2 ? launch_sum(part_update(c), g, o_rr2, c->s) : 0;
 }
 
 template <typename S, int E, int CB>
-void launch_proj(trlhip_ctx *c, int g, int c0, int m, int rcol, double *hout)
+void launch_proj(trlhip_ctx *c, int g, int c0, int m, int rcol)
 {
     hipLaunchKernelGGL((k_proj<S, E, CB>), dim3(g), dim3(kBlock), 0, c->s, c->n, colp<S>(c, 0),
-                       (long)c->ld, c0, m, colp<S>(c, rcol), c->part, c->tick + T_PROJ, hout);
+                       (long)c->ld, c0, m, colp<S>(c, rcol), part_proj(c));
 }
 
+// hout = V(:, c0:c0+m)^H r (device, E*m doubles)
 int run_proj(trlhip_ctx *c, int c0, int m, int rcol, double *hout)
 {
     if (m <= 0) return 0;
     const int g = grid_for(c->n, kBlock * 4, kProjGrid);
-    Timed t(TRLHIP_K_PROJ, c->s, (double)c->esz * c->n * (m + 1));
-    if (is_f32(c)) {
-        if (m <= 8) launch_proj<float, 1, 8>(c, g, c0, m, rcol, hout);
-        else if (m <= 16) launch_proj<float, 1, 16>(c, g, c0, m, rcol, hout);
-        else launch_proj<float, 1, 32>(c, g, c0, m, rcol, hout);
-    } else if (c->E == 1) {
-        if (m <= 8) launch_proj<double, 1, 8>(c, g, c0, m, rcol, hout);
-        else if (m <= 16) launch_proj<double, 1, 16>(c, g, c0, m, rcol, hout);
-        else launch_proj<double, 1, 32>(c, g, c0, m, rcol, hout);
-    } else {
-        if (m <= 8) launch_proj<double, 2, 8>(c, g, c0, m, rcol, hout);
-        else launch_proj<double, 2, 16>(c, g, c0, m, rcol, hout);
+    {
+        Timed t(TRLHIP_K_PROJ, c->s, (double)c->esz * c->n * (m + 1));
+        if (is_f32(c)) {
+            if (m <= 8) launch_proj<float, 1, 8>(c, g, c0, m, rcol);
+            else if (m <= 16) launch_proj<float, 1, 16>(c, g, c0, m, rcol);
+            else launch_proj<float, 1, 32>(c, g, c0, m, rcol);
+        } else if (c->E == 1) {
+            if (m <= 8) launch_proj<double, 1, 8>(c, g, c0, m, rcol);
+            else if (m <= 16) launch_proj<double, 1, 16>(c, g, c0, m, rcol);
+            else launch_proj<double, 1, 32>(c, g, c0, m, rcol);
+        } else {
+            if (m <= 8) launch_proj<double, 2, 8>(c, g, c0, m, rcol);
+            else launch_proj<double, 2, 16>(c, g, c0, m, rcol);
+        }
+        if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
     }
+    const int np = m * c->E;
+    hipLaunchKernelGGL(k_colsum, dim3((np + 63) / 64), dim3(kBlock), 0, c->s, part_proj(c), g, np, hout);
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
@@ -832,7 +853,6 @@ int trlhip_csr_destroy(trlhip_csr *A)
     if (A->dx) (void)hipFree(A->dx);
     if (A->dy) (void)hipFree(A->dy);
     if (A->part) (void)hipFree(A->part);
-    if (A->tick) (void)hipFree(A->tick);
     if (A->dscal) (void)hipFree(A->dscal);
     if (A->s) (void)hipStreamDestroy(A->s);
     A->magic = 0;
@@ -850,15 +870,13 @@ static int csr_matvec_host(trlhip_csr *A, int nrow, int ncol, const double *x, i
     if (!A->dx) {
         if (hipMalloc((void **)&A->dx, sizeof(double) * E * (size_t)A->n) != hipSuccess) return TRLHIP_ENOMEM;
         if (hipMalloc((void **)&A->dy, sizeof(double) * E * (size_t)A->n) != hipSuccess) return TRLHIP_ENOMEM;
-        if (hipMalloc((void **)&A->part, sizeof(double) * (size_t)A->nb) != hipSuccess) return TRLHIP_ENOMEM;
-        if (hipMalloc((void **)&A->tick, sizeof(unsigned) * 4) != hipSuccess) return TRLHIP_ENOMEM;
+        if (hipMalloc((void **)&A->part, sizeof(double) * (size_t)kMaxGrid) != hipSuccess) return TRLHIP_ENOMEM;
         if (hipMalloc((void **)&A->dscal, sizeof(double) * 4) != hipSuccess) return TRLHIP_ENOMEM;
-        TRY(hipMemset(A->tick, 0, sizeof(unsigned) * 4));
     }
     for (int j = 0; j < ncol; ++j) {
         TRY(hipMemcpyAsync(A->dx, x + (size_t)j * ldx * E, sizeof(double) * E * (size_t)nrow,
                            hipMemcpyHostToDevice, A->s));
-        if (launch_spmv(A, vt, A->dx, A->dy, A->part, A->tick, A->dscal, nullptr, A->s)) return TRLHIP_EHIP;
+        if (launch_spmv(A, vt, A->dx, A->dy, A->part, A->s)) return TRLHIP_EHIP;
         TRY(hipMemcpyAsync(y + (size_t)j * ldy * E, A->dy, sizeof(double) * E * (size_t)nrow,
                            hipMemcpyDeviceToHost, A->s));
     }
@@ -895,7 +913,6 @@ int trlhip_ctx_destroy(trlhip_ctx *c)
     if (c->s) (void)hipStreamSynchronize(c->s);
     if (c->V) (void)hipFree(c->V);
     if (c->part) (void)hipFree(c->part);
-    if (c->tick) (void)hipFree(c->tick);
     if (c->dscal) (void)hipFree(c->dscal);
     if (c->dcoef) (void)hipFree(c->dcoef);
     if (c->dh) (void)hipFree(c->dh);
@@ -925,14 +942,12 @@ int trlhip_ctx_create(trlhip_ctx **out, int n, int ncols, int vtype)
     int rc = 0;
     if (hipStreamCreate(&c->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipMalloc((void **)&c->V, vbytes) != hipSuccess) rc = TRLHIP_ENOMEM;
-    if (!rc) rc = ensure_part(c, (size_t)kMaxGrid * ((size_t)c->E * (ncols + 1) + 64));
-    if (!rc && hipMalloc((void **)&c->tick, sizeof(unsigned) * T_COUNT) != hipSuccess) rc = TRLHIP_ENOMEM;
+    if (!rc) rc = ensure_part(c, part_need(c->E, ncols));
     if (!rc && hipMalloc((void **)&c->dscal, sizeof(double) * 16) != hipSuccess) rc = TRLHIP_ENOMEM;
     if (!rc && hipMalloc((void **)&c->dcoef, sizeof(double) * ncoef) != hipSuccess) rc = TRLHIP_ENOMEM;
     if (!rc && hipMalloc((void **)&c->dh, sizeof(double) * ncoef) != hipSuccess) rc = TRLHIP_ENOMEM;
     c->hcap = 16 + 2 * ncoef;
     if (!rc && hipHostMalloc((void **)&c->hbuf, sizeof(double) * c->hcap, 0) != hipSuccess) rc = TRLHIP_ENOMEM;
-    if (!rc && hipMemsetAsync(c->tick, 0, sizeof(unsigned) * T_COUNT, c->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipMemsetAsync(c->V, 0, vbytes, c->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipMemsetAsync(c->dscal, 0, sizeof(double) * 16, c->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipStreamSynchronize(c->s) != hipSuccess) rc = TRLHIP_EHIP;
@@ -988,22 +1003,23 @@ int trlhip_spmv(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol, double *
 {
     if (!c || !csr_fits(c, A)) return TRLHIP_EINVAL;
     if (!valid_col(c, xcol) || !valid_col(c, ycol) || xcol == ycol) return TRLHIP_EINVAL;
-    int rc = run_spmv(c, A, xcol, ycol, nullptr);
-    if (rc) return rc;
-    rc = fetch(c, c->dscal + S_ALPHA, c->hbuf, 1);
+    int rc = run_spmv(c, A, xcol, ycol);
+    if (!rc) rc = launch_sum(part_alpha(c), c->np_alpha, c->dscal + S_ALPHA, c->s);
+    if (!rc) rc = fetch(c, c->dscal + S_ALPHA, c->hbuf, 1);
     if (rc) return rc;
     if (xdoty) *xdoty = c->hbuf[0];
     return 0;
 }
 
-// y = A q_a (if A), alpha, and r = y - coef*V: shared by the step entries.
-// coefficients are set for columns [c0, c0+m) with the alpha slot last.
+// y = A q_a (if A) or the host-written column, alpha = Re(q_a^H y), and
+// r = y - coef*V over columns [c0, c0+m) with alpha as the last
+// coefficient (filled on the device from the partials).  ||r||^2 ->
+// dscal[S_RR2], alpha -> dscal[S_ALPHA].
 static int form_residual(trlhip_ctx *c, const trlhip_csr *A, int qa, int first, int qb,
                          double beta_prev, const double *beta, int jnd, int rcol)
 {
     const int E = c->E;
     int c0, m;
-    double *slot_alpha;
     if (first) {
         double *h = c->hbuf + 16;
         for (int i = 0; i < jnd - 1; ++i) {
@@ -1013,25 +1029,22 @@ static int form_residual(trlhip_ctx *c, const trlhip_csr *A, int qa, int first, 
         if (jnd > 1)
             TRY(hipMemcpyAsync(c->dcoef, h, sizeof(double) * E * (size_t)(jnd - 1),
                                hipMemcpyHostToDevice, c->s));
-        slot_alpha = c->dcoef + (size_t)E * (jnd - 1);
         c0 = 0;
         m = jnd;
     } else if (qb >= 0) {
         c->hbuf[0] = beta_prev;
         c->hbuf[1] = 0.0;
         TRY(hipMemcpyAsync(c->dcoef, c->hbuf, sizeof(double) * E, hipMemcpyHostToDevice, c->s));
-        slot_alpha = c->dcoef + E;
         c0 = qb;
         m = 2;
     } else {
-        slot_alpha = c->dcoef;
         c0 = qa;
         m = 1;
     }
-    int rc = A ? run_spmv(c, A, qa, rcol, slot_alpha) : run_dot(c, qa, rcol, slot_alpha);
+    int rc = A ? run_spmv(c, A, qa, rcol) : run_dot(c, qa, rcol);
     if (rc) return rc;
-    return run_update(c, c0, m, first ? c->dcoef : (qb >= 0 ? c->dcoef : slot_alpha), rcol, rcol,
-                      c->dscal + S_RR2, nullptr, TRLHIP_K_RECUR);
+    return run_update(c, c0, m, c->dcoef, m - 1, rcol, rcol, c->dscal + S_RR2, nullptr,
+                      TRLHIP_K_RECUR);
 }
 
 int trlhip_lanczos_step(trlhip_ctx *c, const trlhip_csr *A, int qa, int qb, double beta_prev,
@@ -1077,7 +1090,7 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
     // speculative first Gram-Schmidt pass into the scratch column
     rc = run_proj(c, 0, jnd, rcol, c->dh);
     if (rc) return rc;
-    rc = run_update(c, 0, jnd, c->dh, rcol, c->ncols, c->dscal + S_RR2B, c->dscal + S_HH,
+    rc = run_update(c, 0, jnd, c->dh, -1, rcol, c->ncols, c->dscal + S_RR2B, c->dscal + S_HH,
                     TRLHIP_K_UPDATE);
     if (rc) return rc;
     rc = fetch(c, c->dscal, c->hbuf + 4, 4);
@@ -1095,7 +1108,7 @@ int trlhip_cgs_pass(trlhip_ctx *c, int c0, int m, int rcol, double *hh, double *
         return TRLHIP_EINVAL;
     int rc = run_proj(c, c0, m, rcol, c->dh);
     if (rc) return rc;
-    rc = run_update(c, c0, m, c->dh, rcol, rcol, c->dscal + S_RR2, c->dscal + S_HH,
+    rc = run_update(c, c0, m, c->dh, -1, rcol, rcol, c->dscal + S_RR2, c->dscal + S_HH,
                     TRLHIP_K_UPDATE);
     if (rc) return rc;
     const int E = c->E;
@@ -1129,7 +1142,7 @@ int trlhip_update(trlhip_ctx *c, int c0, int m, const double *h, int rcol, doubl
     memcpy(c->hbuf + 16, h, sizeof(double) * E * (size_t)m);
     if (m > 0)
         TRY(hipMemcpyAsync(c->dh, c->hbuf + 16, sizeof(double) * E * (size_t)m, hipMemcpyHostToDevice, c->s));
-    int rc = run_update(c, c0, m, c->dh, rcol, rcol, c->dscal + S_RR2, nullptr, TRLHIP_K_UPDATE);
+    int rc = run_update(c, c0, m, c->dh, -1, rcol, rcol, c->dscal + S_RR2, nullptr, TRLHIP_K_UPDATE);
     if (rc) return rc;
     rc = fetch(c, c->dscal + S_RR2, c->hbuf + 4, 1);
     if (rc) return rc;
@@ -1164,7 +1177,7 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y, int ldy)
     const int g = (int)((nrows + rows_per_block - 1) / rows_per_block);
     const int jl4 = (jl + 3) & ~3;
     const size_t shm = sizeof(double) * (size_t)jl4 * 16 * KT;
-    if (shm > 64 * 1024) return TRLHIP_EINVAL;
+    if (shm > 160 * 1024) return TRLHIP_EINVAL;  // gfx950: 160 KB of LDS per CU
     Timed t(TRLHIP_K_RITZ, c->s, (double)c->esz * c->n * (jl + k));
 #define RITZ_CASE(S, K)                                                                      \
     case K:                                                                                  \
