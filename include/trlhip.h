@@ -160,10 +160,12 @@ enum {
     TRLHIP_K_COUNT = 6
 };
 
-/* Enable (1) / disable (0) per-kernel event timing; resets counters. */
+/* Per-kernel HIP-event timing: on = 0 disables, on = N > 0 times every
+ * N-th launch of each kernel kind (1: every launch).  Resets counters. */
 void trlhip_stats_enable(int on);
-/* out[k*3 + 0] = launches, out[k*3+1] = total ms, out[k*3+2] = algorithmic
- * bytes, for k < TRLHIP_K_COUNT.  Returns TRLHIP_K_COUNT. */
+/* out[k*3 + 0] = timed launches, out[k*3+1] = their total ms,
+ * out[k*3+2] = their algorithmic bytes, for k < TRLHIP_K_COUNT.  Returns
+ * TRLHIP_K_COUNT. */
 int trlhip_stats_get(double *out, int nout);
 
 /* Number of visible GPUs (0 when none); never aborts. */

@@ -25,7 +25,8 @@ namespace {
 
 constexpr int kBlock = 256;            // 4 waves of 64
 constexpr int kMaxGrid = 1024;         // partial slots per streaming reduction
-constexpr int kProjGrid = 512;         // blocks of the projection kernel
+constexpr int kProjGrid = 2048;        // row blocks (partial rows) of the projection
+constexpr int kProjBlocks = 4096;      // projection blocks wanted over all column chunks
 constexpr int kNnzBlk = 2048;          // nonzeros staged per SpMV row block
 constexpr int kRowsBlk = kBlock;       // rows per SpMV row block (at most)
 constexpr uint64_t kCsrMagic = 0x7472686970637372ull;  // "trhipcsr"
@@ -59,8 +60,11 @@ __device__ __forceinline__ double wave_transpose_sum(double (&v)[N], int lane)
         const bool hi = (lane & o) != 0;
 #pragma unroll
         for (int i = 0; i < cnt / 2; ++i) {
-            const double send = hi ? v[i] : v[i + cnt / 2];
-            const double keep = hi ? v[i + cnt / 2] : v[i];
+            // select values, not array elements: a conditional lvalue
+            // would become a dynamically indexed (scratch) array access
+            const double lo_v = v[i], hi_v = v[i + cnt / 2];
+            const double send = hi ? lo_v : hi_v;
+            const double keep = hi ? hi_v : lo_v;
             v[i] = keep + __shfl_xor(send, o);
         }
     }
@@ -123,19 +127,24 @@ __global__ __launch_bounds__(kBlock) void k_sum(const double *__restrict__ part,
     if (threadIdx.x == 0) out[0] = t;
 }
 
-// Finalize a vector reduction: hout[p] = sum_b part[b * np + p], p < np.
-// Block x handles 64 consecutive p; its 4 waves split b and combine in LDS.
-__global__ __launch_bounds__(kBlock) void k_colsum(const double *__restrict__ part, int nb, int np,
+// Two scalar reductions at once: block 0 sums p0 into o0, block 1 p1 into o1.
+__global__ __launch_bounds__(kBlock) void k_sum2(const double *__restrict__ p0, int n0,
+                                                 double *__restrict__ o0,
+                                                 const double *__restrict__ p1, int n1,
+                                                 double *__restrict__ o1)
+{
+    const bool second = blockIdx.x == 1;
+    const double t = ordered_sum(second ? p1 : p0, second ? n1 : n0);
+    if (threadIdx.x == 0) *(second ? o1 : o0) = t;
+}
+
+// Finalize a vector reduction: hout[p] = sum_b part[p * nb + b], p < np
+// (each coefficient's nb partials are contiguous).  One block per p.
+__global__ __launch_bounds__(kBlock) void k_colsum(const double *__restrict__ part, int nb,
                                                    double *__restrict__ hout)
 {
-    __shared__ double red[kBlock];
-    const int p = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int q = threadIdx.x >> 6;
-    red[threadIdx.x] = p < np ? strided_sum(part, nb, (size_t)np, (size_t)p, q, kBlock / 64) : 0.0;
-    __syncthreads();
-    if (threadIdx.x < 64 && p < np)
-        hout[p] = (red[threadIdx.x] + red[threadIdx.x + 64]) +
-                  (red[threadIdx.x + 128] + red[threadIdx.x + 192]);
+    const double t = ordered_sum(part + (size_t)blockIdx.x * nb, nb);
+    if (threadIdx.x == 0) hout[blockIdx.x] = t;
 }
 
 // ---------------------------------------------------------------- SpMV
@@ -237,14 +246,17 @@ __global__ __launch_bounds__(kBlock) void k_dot(long ne, const S *__restrict__ x
 
 // --------------------------------------------------------- Gram-Schmidt
 
-// h = V(:, c0:c0+m)^H r.  A block owns a grid-stride set of rows and walks
-// the m columns in chunks of CB: per row it issues all CB column loads
-// before the FMAs (column indices past the chunk's end are clamped to its
-// last column, so the extra loads hit cache and are discarded), keeps CB*E
-// running sums in registers, then transposes-and-reduces them over the
-// wave, combines waves in LDS and writes its partial row part[b*m*E ...]
-// (k_colsum sums the rows in block order).
-template <typename S, int E, int CB>
+// h = V(:, c0:c0+m)^H r.  Grid: blockIdx.y picks a chunk of CB columns,
+// blockIdx.x a grid-stride set of rows (RU rows of the stride per
+// iteration, so RU*CB loads are in flight per thread).  Column indices past
+// the chunk's end are clamped to its last column (the extra loads hit
+// cache and are discarded).  Each thread keeps CB*E running sums, which
+// are transposed-and-reduced over the wave and combined over waves in LDS;
+// the block writes its partial of coefficient p to part[p*gridDim.x +
+// blockIdx.x] (k_colsum sums each coefficient's partials in block order).
+// Chunks of one row range read the same stretch of r, which the cache
+// serves after the first chunk.
+template <typename S, int E, int CB, int RU>
 __global__ __launch_bounds__(kBlock) void k_proj(
     int n, const S *__restrict__ V, long ld, int c0, int m, const S *__restrict__ r,
     double *__restrict__ part)
@@ -253,48 +265,164 @@ __global__ __launch_bounds__(kBlock) void k_proj(
     static_assert(N <= 64, "chunk too wide");
     __shared__ double sm[kBlock / 64][N];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int G = gridDim.x;
-    const int mE = m * E;
-    double *prow = part + (size_t)blockIdx.x * mE;
-    for (int cb = 0; cb < m; cb += CB) {
-        const int last = min(CB, m - cb) - 1;
-        const S *Vc = V + (size_t)(c0 + cb) * ld * E;
-        double acc[N];
+    const int cb = blockIdx.y * CB;
+    const int last = min(CB, m - cb) - 1;
+    // wave-uniform column bases; rows index them
+    const S *col[CB];
 #pragma unroll
-        for (int i = 0; i < N; ++i) acc[i] = 0.0;
-        for (int row = blockIdx.x * kBlock + threadIdx.x; row < n; row += G * kBlock) {
-            if constexpr (E == 1) {
-                double vv[CB];
+    for (int c = 0; c < CB; ++c) col[c] = V + ((size_t)(c0 + cb + min(c, last)) * ld) * E;
+    double acc[N];
 #pragma unroll
-                for (int c = 0; c < CB; ++c) vv[c] = (double)Vc[(size_t)min(c, last) * ld + row];
-                const double rv = (double)r[row];
+    for (int i = 0; i < N; ++i) acc[i] = 0.0;
+    const int stride = gridDim.x * kBlock * RU;
+    int row = blockIdx.x * kBlock * RU + threadIdx.x;
+    if constexpr (E == 1) {
+        for (; row + (RU - 1) * kBlock < n; row += stride) {
+            double rv[RU], vv[RU][CB];
 #pragma unroll
-                for (int c = 0; c < CB; ++c) acc[c] += vv[c] * rv;
-            } else {
-                double2 vv[CB];
-                const double2 *V2 = reinterpret_cast<const double2 *>(Vc);
+            for (int j = 0; j < RU; ++j) {
+                rv[j] = (double)r[row + j * kBlock];
 #pragma unroll
-                for (int c = 0; c < CB; ++c) vv[c] = V2[(size_t)min(c, last) * ld + row];
-                const double2 rv = reinterpret_cast<const double2 *>(r)[row];
+                for (int c = 0; c < CB; ++c) vv[j][c] = (double)col[c][row + j * kBlock];
+            }
+#pragma unroll
+            for (int j = 0; j < RU; ++j)
+#pragma unroll
+                for (int c = 0; c < CB; ++c) acc[c] += vv[j][c] * rv[j];
+        }
+#pragma unroll
+        for (int j = 0; j < RU; ++j) {
+            const int rr = row + j * kBlock;
+            if (rr < n) {
+                const double rv = (double)r[rr];
+#pragma unroll
+                for (int c = 0; c < CB; ++c) acc[c] += (double)col[c][rr] * rv;
+            }
+        }
+    } else {
+        const double2 *r2 = reinterpret_cast<const double2 *>(r);
+        for (; row + (RU - 1) * kBlock < n; row += stride) {
+            double2 rv[RU], vv[RU][CB];
+#pragma unroll
+            for (int j = 0; j < RU; ++j) {
+                rv[j] = r2[row + j * kBlock];
+#pragma unroll
+                for (int c = 0; c < CB; ++c)
+                    vv[j][c] = reinterpret_cast<const double2 *>(col[c])[row + j * kBlock];
+            }
+#pragma unroll
+            for (int j = 0; j < RU; ++j)
 #pragma unroll
                 for (int c = 0; c < CB; ++c) {
-                    acc[2 * c] += vv[c].x * rv.x + vv[c].y * rv.y;  // conj(v) * r
-                    acc[2 * c + 1] += vv[c].x * rv.y - vv[c].y * rv.x;
+                    acc[2 * c] += vv[j][c].x * rv[j].x + vv[j][c].y * rv[j].y;  // conj(v) * r
+                    acc[2 * c + 1] += vv[j][c].x * rv[j].y - vv[j][c].y * rv[j].x;
+                }
+        }
+#pragma unroll
+        for (int j = 0; j < RU; ++j) {
+            const int rr = row + j * kBlock;
+            if (rr < n) {
+                const double2 rv = r2[rr];
+#pragma unroll
+                for (int c = 0; c < CB; ++c) {
+                    const double2 v = reinterpret_cast<const double2 *>(col[c])[rr];
+                    acc[2 * c] += v.x * rv.x + v.y * rv.y;
+                    acc[2 * c + 1] += v.x * rv.y - v.y * rv.x;
                 }
             }
         }
-        const double ws = wave_transpose_sum<N>(acc, lane);
-        constexpr int grp = 64 / N;
-        if (lane % grp == 0) sm[w][lane / grp] = ws;
-        __syncthreads();
-        if (threadIdx.x < (last + 1) * E) {
-            double t = 0.0;
-#pragma unroll
-            for (int i = 0; i < kBlock / 64; ++i) t += sm[i][threadIdx.x];
-            prow[cb * E + threadIdx.x] = t;
-        }
-        __syncthreads();
     }
+    const double ws = wave_transpose_sum<N>(acc, lane);
+    constexpr int grp = 64 / N;
+    if (lane % grp == 0) sm[w][lane / grp] = ws;
+    __syncthreads();
+    if (threadIdx.x < (last + 1) * E) {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < kBlock / 64; ++i) t += sm[i][threadIdx.x];
+        part[(size_t)(cb * E + threadIdx.x) * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+// Rows row + j*kBlock (j < R) of rout = rin - Vc hs; returns the sum of
+// their |rout|^2.  Per row the products are summed into four interleaved
+// partial sums (column c into sum c mod 4 within each group of 8 columns,
+// E == 1; pairs of columns, E == 2), the same order for any R; R rows at a
+// time put R*8 (R*4) column loads in flight per thread.
+template <typename S, int E, int R>
+__device__ __forceinline__ double update_rows(int row, const S *__restrict__ Vc, long ld, int m,
+                                              const double *hs, const S *rin, S *rout)
+{
+    double nrm = 0.0;
+    if constexpr (E == 1) {
+        double s[R][4];
+#pragma unroll
+        for (int j = 0; j < R; ++j) s[j][0] = s[j][1] = s[j][2] = s[j][3] = 0.0;
+        int c = 0;
+        for (; c + 8 <= m; c += 8) {
+            double v[R][8];
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+#pragma unroll
+                for (int q = 0; q < 8; ++q) v[j][q] = (double)Vc[(size_t)(c + q) * ld + row + j * kBlock];
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+#pragma unroll
+                for (int q = 0; q < 8; ++q) s[j][q & 3] += v[j][q] * hs[c + q];
+        }
+        for (; c < m; ++c)
+#pragma unroll
+            for (int j = 0; j < R; ++j) s[j][0] += (double)Vc[(size_t)c * ld + row + j * kBlock] * hs[c];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int rr = row + j * kBlock;
+            const S rn = (S)((double)rin[rr] - ((s[j][0] + s[j][1]) + (s[j][2] + s[j][3])));
+            rout[rr] = rn;
+            nrm += (double)rn * (double)rn;
+        }
+    } else {
+        const double2 *V2 = reinterpret_cast<const double2 *>(Vc);
+        double ar[R], ai[R], br[R], bi[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) ar[j] = ai[j] = br[j] = bi[j] = 0.0;
+        int c = 0;
+        for (; c + 4 <= m; c += 4) {
+            double2 v[R][4];
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[j][q] = V2[(size_t)(c + q) * ld + row + j * kBlock];
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; q += 2) {
+                    const double h0r = hs[2 * (c + q)], h0i = hs[2 * (c + q) + 1];
+                    const double h1r = hs[2 * (c + q + 1)], h1i = hs[2 * (c + q + 1) + 1];
+                    ar[j] += v[j][q].x * h0r - v[j][q].y * h0i;
+                    ai[j] += v[j][q].x * h0i + v[j][q].y * h0r;
+                    br[j] += v[j][q + 1].x * h1r - v[j][q + 1].y * h1i;
+                    bi[j] += v[j][q + 1].x * h1i + v[j][q + 1].y * h1r;
+                }
+        }
+        for (; c < m; ++c)
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const double2 v = V2[(size_t)c * ld + row + j * kBlock];
+                ar[j] += v.x * hs[2 * c] - v.y * hs[2 * c + 1];
+                ai[j] += v.x * hs[2 * c + 1] + v.y * hs[2 * c];
+            }
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int rr = row + j * kBlock;
+            const double2 ri = reinterpret_cast<const double2 *>(rin)[rr];
+            double2 ro;
+            ro.x = ri.x - (ar[j] + br[j]);
+            ro.y = ri.y - (ai[j] + bi[j]);
+            reinterpret_cast<double2 *>(rout)[rr] = ro;
+            nrm += ro.x * ro.x + ro.y * ro.y;
+        }
+    }
+    return nrm;
 }
 
 // rout = rin - V(:, c0:c0+m) h ; per-block partials of ||rout||^2 in
@@ -302,16 +430,17 @@ __global__ __launch_bounds__(kBlock) void k_proj(
 // LDS.  When cpart is given, coefficient `cslot` is not read from h but is
 // the sum of the cnp partials in cpart (the alpha that the preceding
 // SpMV/dot left as per-block partials); every block forms that sum in the
-// same order, and block 0 also stores it to out_coef.  Four independent
-// partial sums per row keep the FMA chains short.  rin may equal rout.
-template <typename S, int E>
+// same order, and block 0 also stores it to out_coef.  Without h the
+// coefficients are (h0, 0, ..., 0) apart from cslot.  A block takes RU
+// rows of its grid stride per iteration.  rin may equal rout.
+template <typename S, int E, int RU>
 __global__ __launch_bounds__(kBlock) void k_update(
-    int n, const S *__restrict__ V, long ld, int c0, int m, const double *__restrict__ h,
+    int n, const S *__restrict__ V, long ld, int c0, int m, const double *__restrict__ h, double h0,
     const double *__restrict__ cpart, int cnp, int cslot, const S *rin, S *rout,
     double *__restrict__ part, double *__restrict__ out_hh, double *__restrict__ out_coef)
 {
     extern __shared__ double hs[];
-    for (int i = threadIdx.x; i < E * m; i += kBlock) hs[i] = h[i];
+    for (int i = threadIdx.x; i < E * m; i += kBlock) hs[i] = h ? h[i] : (i == 0 ? h0 : 0.0);
     if (cpart) {
         const double a = ordered_sum(cpart, cnp);  // contains __syncthreads
         if (threadIdx.x == 0) {
@@ -323,56 +452,13 @@ __global__ __launch_bounds__(kBlock) void k_update(
     __syncthreads();
     const S *Vc = V + (size_t)c0 * ld * E;
     double acc = 0.0;
-    for (int row = blockIdx.x * kBlock + threadIdx.x; row < n; row += gridDim.x * kBlock) {
-        if constexpr (E == 1) {
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-            int c = 0;
-            for (; c + 8 <= m; c += 8) {
-                double v[8];
+    const int stride = gridDim.x * kBlock * RU;
+    int row = blockIdx.x * kBlock * RU + threadIdx.x;
+    for (; row + (RU - 1) * kBlock < n; row += stride)
+        acc += update_rows<S, E, RU>(row, Vc, ld, m, hs, rin, rout);
 #pragma unroll
-                for (int q = 0; q < 8; ++q) v[q] = (double)Vc[(size_t)(c + q) * ld + row];
-                s0 += v[0] * hs[c];
-                s1 += v[1] * hs[c + 1];
-                s2 += v[2] * hs[c + 2];
-                s3 += v[3] * hs[c + 3];
-                s0 += v[4] * hs[c + 4];
-                s1 += v[5] * hs[c + 5];
-                s2 += v[6] * hs[c + 6];
-                s3 += v[7] * hs[c + 7];
-            }
-            for (; c < m; ++c) s0 += (double)Vc[(size_t)c * ld + row] * hs[c];
-            const S rn = (S)((double)rin[row] - ((s0 + s1) + (s2 + s3)));
-            rout[row] = rn;
-            acc += (double)rn * (double)rn;
-        } else {
-            const double2 *V2 = reinterpret_cast<const double2 *>(Vc);
-            double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
-            int c = 0;
-            for (; c + 4 <= m; c += 4) {
-                double2 v[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) v[q] = V2[(size_t)(c + q) * ld + row];
-#pragma unroll
-                for (int q = 0; q < 4; q += 2) {
-                    ar += v[q].x * hs[2 * (c + q)] - v[q].y * hs[2 * (c + q) + 1];
-                    ai += v[q].x * hs[2 * (c + q) + 1] + v[q].y * hs[2 * (c + q)];
-                    br += v[q + 1].x * hs[2 * (c + q + 1)] - v[q + 1].y * hs[2 * (c + q + 1) + 1];
-                    bi += v[q + 1].x * hs[2 * (c + q + 1) + 1] + v[q + 1].y * hs[2 * (c + q + 1)];
-                }
-            }
-            for (; c < m; ++c) {
-                const double2 v = V2[(size_t)c * ld + row];
-                ar += v.x * hs[2 * c] - v.y * hs[2 * c + 1];
-                ai += v.x * hs[2 * c + 1] + v.y * hs[2 * c];
-            }
-            const double2 ri = reinterpret_cast<const double2 *>(rin)[row];
-            double2 ro;
-            ro.x = ri.x - (ar + br);
-            ro.y = ri.y - (ai + bi);
-            reinterpret_cast<double2 *>(rout)[row] = ro;
-            acc += ro.x * ro.x + ro.y * ro.y;
-        }
-    }
+    for (int j = 0; j < RU - 1; ++j)
+        if (row + j * kBlock < n) acc += update_rows<S, E, 1>(row + j * kBlock, Vc, ld, m, hs, rin, rout);
     const double bs = block_sum(acc);
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
     if (out_hh && blockIdx.x == 0 && threadIdx.x == 0) {
@@ -450,7 +536,8 @@ __global__ __launch_bounds__(kBlock) void k_ritz_mfma(
 // ---------------------------------------------------------------- stats
 
 struct Stats {
-    int on = 0;
+    int on = 0;      // 0 off, else time every on-th launch of each kind
+    long seen[TRLHIP_K_COUNT] = {};
     double launches[TRLHIP_K_COUNT] = {};
     double ms[TRLHIP_K_COUNT] = {};
     double bytes[TRLHIP_K_COUNT] = {};
@@ -487,14 +574,16 @@ void stats_resolve()
     g_stats.pending.clear();
 }
 
-// Brackets one kernel launch with HIP events on its stream when enabled.
+// Brackets one kernel launch with HIP events on its stream when enabled
+// (every g_stats.on-th launch of its kind; launches and bytes count the
+// timed launches only, so bytes/ms stays the timed launches' rate).
 struct Timed {
     hipEvent_t a = nullptr, b = nullptr;
     int k;
     hipStream_t s;
     Timed(int kind, hipStream_t st, double bytes) : k(kind), s(st)
     {
-        if (!g_stats.on) return;
+        if (!g_stats.on || g_stats.seen[k]++ % g_stats.on != 0) return;
         g_stats.launches[k] += 1.0;
         g_stats.bytes[k] += bytes;
         a = ev_get();
@@ -503,7 +592,7 @@ struct Timed {
     }
     ~Timed()
     {
-        if (!g_stats.on || !a || !b) return;
+        if (!a || !b) return;
         (void)hipEventRecord(b, s);
         g_stats.pending.push_back({a, b, k});
     }
@@ -541,6 +630,7 @@ struct trlhip_ctx {
     double *part;
     size_t partcap; // doubles
     int np_alpha;   // partial count of the last SpMV / dot
+    int np_recur;   // partial count of a deferred recurrence norm
     double *dscal;  // 16 device scalars
     double *dcoef;  // 2*(ncols+2) recurrence / restart-step coefficients
     double *dh;     // 2*(ncols+2) CGS projection
@@ -590,8 +680,9 @@ int ensure_part(trlhip_ctx *c, size_t need)
 // partials (||r||^2), projection partial rows.
 double *part_alpha(trlhip_ctx *c) { return c->part; }
 double *part_update(trlhip_ctx *c) { return c->part + kMaxGrid; }
-double *part_proj(trlhip_ctx *c) { return c->part + 2 * kMaxGrid; }
-size_t part_need(int E, int ncols) { return 2 * (size_t)kMaxGrid + (size_t)kProjGrid * E * (ncols + 1) + 64; }
+double *part_recur(trlhip_ctx *c) { return c->part + 2 * kMaxGrid; }
+double *part_proj(trlhip_ctx *c) { return c->part + 3 * kMaxGrid; }
+size_t part_need(int E, int ncols) { return 3 * (size_t)kMaxGrid + (size_t)kProjGrid * E * (ncols + 1) + 64; }
 
 double spmv_bytes(const trlhip_csr *A, int E, size_t vsz, size_t xsz)
 {
@@ -657,65 +748,82 @@ int run_dot(trlhip_ctx *c, int xcol, int ycol)
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
-// rout = rin - V(:, c0:c0+m) dh; coefficient aslot (if >= 0) is the alpha
-// left in part_alpha (also stored to dscal[S_ALPHA]); ||rout||^2 -> o_rr2
-// and ||dh||^2 -> o_hh when requested.
-int run_update(trlhip_ctx *c, int c0, int m, const double *dh, int aslot, int rin, int rout,
-               double *o_rr2, double *o_hh, int kind)
+template <typename S, int E, int RU>
+void launch_update(trlhip_ctx *c, int g, int c0, int m, const double *dh, double h0, int aslot,
+                   int rin, int rout, double *o_hh, double *pdst)
 {
-    const int g = grid_for(c->n, kBlock * 4, kMaxGrid);
     const size_t shm = sizeof(double) * (size_t)c->E * (m > 0 ? m : 1);
     const double *cp = aslot >= 0 ? part_alpha(c) : nullptr;
-    double *oc = c->dscal + S_ALPHA;
+    hipLaunchKernelGGL((k_update<S, E, RU>), dim3(g), dim3(kBlock), shm, c->s, c->n, colp<S>(c, 0),
+                       (long)c->ld, c0, m, dh, h0, cp, c->np_alpha, aslot, colp<S>(c, rin),
+                       colp<S>(c, rout), pdst, o_hh, c->dscal + S_ALPHA);
+}
+
+// rout = rin - V(:, c0:c0+m) dh; coefficient aslot (if >= 0) is the alpha
+// left in part_alpha (also stored to dscal[S_ALPHA]); ||rout||^2 -> o_rr2
+// and ||dh||^2 -> o_hh when requested.  dh == nullptr: coefficients
+// (h0, 0, ...) apart from aslot.  The per-block partials of ||rout||^2 go
+// to pdst (default part_update) and their count to *g_out when given.
+int run_update(trlhip_ctx *c, int c0, int m, const double *dh, int aslot, int rin, int rout,
+               double *o_rr2, double *o_hh, int kind, double h0 = 0.0, double *pdst = nullptr,
+               int *g_out = nullptr)
+{
+    const int g = grid_for(c->n, kBlock * 4, kMaxGrid);
+    if (!pdst) pdst = part_update(c);
+    if (g_out) *g_out = g;
     {
         Timed t(kind, c->s, (double)c->esz * c->n * (m + 2));
         if (is_f32(c))
-            hipLaunchKernelGGL((k_update<float, 1>), dim3(g), dim3(kBlock), shm, c->s, c->n,
-                               colp<float>(c, 0), (long)c->ld, c0, m, dh, cp, c->np_alpha, aslot,
-                               colp<float>(c, rin), colp<float>(c, rout), part_update(c), o_hh, oc);
+            launch_update<float, 1, 4>(c, g, c0, m, dh, h0, aslot, rin, rout, o_hh, pdst);
         else if (c->E == 1)
-            hipLaunchKernelGGL((k_update<double, 1>), dim3(g), dim3(kBlock), shm, c->s, c->n,
-                               colp<double>(c, 0), (long)c->ld, c0, m, dh, cp, c->np_alpha, aslot,
-                               colp<double>(c, rin), colp<double>(c, rout), part_update(c), o_hh, oc);
+            launch_update<double, 1, 4>(c, g, c0, m, dh, h0, aslot, rin, rout, o_hh, pdst);
         else
-            hipLaunchKernelGGL((k_update<double, 2>), dim3(g), dim3(kBlock), shm, c->s, c->n,
-                               colp<double>(c, 0), (long)c->ld, c0, m, dh, cp, c->np_alpha, aslot,
-                               colp<double>(c, rin), colp<double>(c, rout), part_update(c), o_hh, oc);
+            launch_update<double, 2, 4>(c, g, c0, m, dh, h0, aslot, rin, rout, o_hh, pdst);
         if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
     }
-    return o_rr2 ? launch_sum(part_update(c), g, o_rr2, c->s) : 0;
+    return o_rr2 ? launch_sum(pdst, g, o_rr2, c->s) : 0;
 }
 
-template <typename S, int E, int CB>
-void launch_proj(trlhip_ctx *c, int g, int c0, int m, int rcol)
+template <typename S, int E, int CB, int RU>
+void launch_proj(trlhip_ctx *c, int gx, int c0, int m, int rcol)
 {
-    hipLaunchKernelGGL((k_proj<S, E, CB>), dim3(g), dim3(kBlock), 0, c->s, c->n, colp<S>(c, 0),
+    const dim3 grid(gx, (m + CB - 1) / CB);
+    hipLaunchKernelGGL((k_proj<S, E, CB, RU>), grid, dim3(kBlock), 0, c->s, c->n, colp<S>(c, 0),
                        (long)c->ld, c0, m, colp<S>(c, rcol), part_proj(c));
+}
+
+// Row blocks of the projection: about kProjBlocks blocks over all column
+// chunks, at most kProjGrid (the partial rows).
+int proj_gx(const trlhip_ctx *c, int m, int cb)
+{
+    const int chunks = (m + cb - 1) / cb;
+    const int want = (kProjBlocks + chunks - 1) / chunks;
+    return grid_for(c->n, kBlock * 4, want < kProjGrid ? want : kProjGrid);
 }
 
 // hout = V(:, c0:c0+m)^H r (device, E*m doubles)
 int run_proj(trlhip_ctx *c, int c0, int m, int rcol, double *hout)
 {
     if (m <= 0) return 0;
-    const int g = grid_for(c->n, kBlock * 4, kProjGrid);
+    int g;
     {
         Timed t(TRLHIP_K_PROJ, c->s, (double)c->esz * c->n * (m + 1));
+        // chunk widths and row unrolling from a sweep at n = 2^21, m = 16..64
+        // (about 5.7 TB/s for both element types)
         if (is_f32(c)) {
-            if (m <= 8) launch_proj<float, 1, 8>(c, g, c0, m, rcol);
-            else if (m <= 16) launch_proj<float, 1, 16>(c, g, c0, m, rcol);
-            else launch_proj<float, 1, 32>(c, g, c0, m, rcol);
+            g = proj_gx(c, m, 8);
+            launch_proj<float, 1, 8, 4>(c, g, c0, m, rcol);
         } else if (c->E == 1) {
-            if (m <= 8) launch_proj<double, 1, 8>(c, g, c0, m, rcol);
-            else if (m <= 16) launch_proj<double, 1, 16>(c, g, c0, m, rcol);
-            else launch_proj<double, 1, 32>(c, g, c0, m, rcol);
+            g = proj_gx(c, m, 8);
+            launch_proj<double, 1, 8, 4>(c, g, c0, m, rcol);
         } else {
-            if (m <= 8) launch_proj<double, 2, 8>(c, g, c0, m, rcol);
-            else launch_proj<double, 2, 16>(c, g, c0, m, rcol);
+            g = proj_gx(c, m, 4);
+            launch_proj<double, 2, 4, 4>(c, g, c0, m, rcol);
         }
         if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
     }
     const int np = m * c->E;
-    hipLaunchKernelGGL(k_colsum, dim3((np + 63) / 64), dim3(kBlock), 0, c->s, part_proj(c), g, np, hout);
+    hipLaunchKernelGGL(k_colsum, dim3(np), dim3(kBlock), 0, c->s, part_proj(c), g, hout);
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
@@ -1014,12 +1122,16 @@ int trlhip_spmv(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol, double *
 // y = A q_a (if A) or the host-written column, alpha = Re(q_a^H y), and
 // r = y - coef*V over columns [c0, c0+m) with alpha as the last
 // coefficient (filled on the device from the partials).  ||r||^2 ->
-// dscal[S_RR2], alpha -> dscal[S_ALPHA].
+// dscal[S_RR2] (defer: its partials stay in part_recur, np_recur of them),
+// alpha -> dscal[S_ALPHA].
 static int form_residual(trlhip_ctx *c, const trlhip_csr *A, int qa, int first, int qb,
-                         double beta_prev, const double *beta, int jnd, int rcol)
+                         double beta_prev, const double *beta, int jnd, int rcol,
+                         bool defer = false)
 {
     const int E = c->E;
     int c0, m;
+    const double *coef = c->dcoef;
+    double h0 = 0.0;
     if (first) {
         double *h = c->hbuf + 16;
         for (int i = 0; i < jnd - 1; ++i) {
@@ -1032,9 +1144,8 @@ static int form_residual(trlhip_ctx *c, const trlhip_csr *A, int qa, int first, 
         c0 = 0;
         m = jnd;
     } else if (qb >= 0) {
-        c->hbuf[0] = beta_prev;
-        c->hbuf[1] = 0.0;
-        TRY(hipMemcpyAsync(c->dcoef, c->hbuf, sizeof(double) * E, hipMemcpyHostToDevice, c->s));
+        coef = nullptr;  // (beta_prev, alpha) passed by value
+        h0 = beta_prev;
         c0 = qb;
         m = 2;
     } else {
@@ -1043,8 +1154,11 @@ static int form_residual(trlhip_ctx *c, const trlhip_csr *A, int qa, int first, 
     }
     int rc = A ? run_spmv(c, A, qa, rcol) : run_dot(c, qa, rcol);
     if (rc) return rc;
-    return run_update(c, c0, m, c->dcoef, m - 1, rcol, rcol, c->dscal + S_RR2, nullptr,
-                      TRLHIP_K_RECUR);
+    if (defer)
+        return run_update(c, c0, m, coef, m - 1, rcol, rcol, nullptr, nullptr, TRLHIP_K_RECUR, h0,
+                          part_recur(c), &c->np_recur);
+    return run_update(c, c0, m, coef, m - 1, rcol, rcol, c->dscal + S_RR2, nullptr,
+                      TRLHIP_K_RECUR, h0);
 }
 
 int trlhip_lanczos_step(trlhip_ctx *c, const trlhip_csr *A, int qa, int qb, double beta_prev,
@@ -1085,14 +1199,19 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
     if (!first && qb >= 0 && qb != qa - 1) return TRLHIP_EINVAL;
     if (jnd < 1 || jnd > c->ncols || (rcol >= 0 && rcol < jnd)) return TRLHIP_EINVAL;
     if (A && !csr_fits(c, A)) return TRLHIP_EINVAL;
-    int rc = form_residual(c, A, qa, first, qb, beta_prev, beta, jnd, rcol);
+    int rc = form_residual(c, A, qa, first, qb, beta_prev, beta, jnd, rcol, true);
     if (rc) return rc;
     // speculative first Gram-Schmidt pass into the scratch column
     rc = run_proj(c, 0, jnd, rcol, c->dh);
     if (rc) return rc;
-    rc = run_update(c, 0, jnd, c->dh, -1, rcol, c->ncols, c->dscal + S_RR2B, c->dscal + S_HH,
-                    TRLHIP_K_UPDATE);
+    int g2 = 0;
+    rc = run_update(c, 0, jnd, c->dh, -1, rcol, c->ncols, nullptr, c->dscal + S_HH,
+                    TRLHIP_K_UPDATE, 0.0, nullptr, &g2);
     if (rc) return rc;
+    // both norms (before and after the pass) in one finalize launch
+    hipLaunchKernelGGL(k_sum2, dim3(2), dim3(kBlock), 0, c->s, part_recur(c), c->np_recur,
+                       c->dscal + S_RR2, part_update(c), g2, c->dscal + S_RR2B);
+    if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
     rc = fetch(c, c->dscal, c->hbuf + 4, 4);
     if (rc) return rc;
     out[0] = c->hbuf[4 + S_ALPHA];
@@ -1205,8 +1324,9 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y, int ldy)
 void trlhip_stats_enable(int on)
 {
     stats_resolve();
-    g_stats.on = on ? 1 : 0;
+    g_stats.on = on > 0 ? on : 0;
     for (int k = 0; k < TRLHIP_K_COUNT; ++k) {
+        g_stats.seen[k] = 0;
         g_stats.launches[k] = 0;
         g_stats.ms[k] = 0;
         g_stats.bytes[k] = 0;

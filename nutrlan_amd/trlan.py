@@ -224,12 +224,14 @@ def solve_hostop(matvec, n, lohi, ned, maxlan, tol, restart, maxmv, mev, start):
     return _result(info, ev, evec.T, wrk, wall)
 
 
-def stats_enable(on=True):
-    lib().trlhip_stats_enable(1 if on else 0)
+def stats_enable(on=True, every=1):
+    """Per-kernel event timing of every `every`-th launch of each kind."""
+    lib().trlhip_stats_enable(int(every) if on else 0)
 
 
 def stats_get():
-    """{kernel: (launches, total_ms, algorithmic_bytes)} since stats_enable."""
+    """{kernel: (timed launches, their total ms, their algorithmic bytes)}
+    since stats_enable."""
     out = np.zeros(3 * len(KERNELS))
     lib().trlhip_stats_get(_ptr(out), len(out))
     return {k: (int(out[3 * i]), float(out[3 * i + 1]), float(out[3 * i + 2]))
