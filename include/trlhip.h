@@ -120,7 +120,11 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
                     double beta_prev, const double *beta, int jnd, int rcol,
                     double out[4]);
 
-/* dst = s * src; src may be the scratch column (index ncols). */
+/* dst = s * src; src may be the scratch column (index ncols).  From the
+ * scratch column the copy is deferred: the next trlhip_step_cgs whose q_a
+ * is dst forms s * src inside its SpMV and stores it to dst; any other
+ * entry point on the context applies the copy first.  Replaces the final
+ * dscal of trl_cgs (SRC/trlcore.c:3104-3109). */
 int trlhip_col_scale_copy(trlhip_ctx *c, int dst, int src, double s);
 
 /* One classical Gram-Schmidt pass against columns [c0, c0+m):

@@ -11,8 +11,10 @@
 // scalar: each block reduces its rows (wave shuffles, then LDS) into a
 // partial slot, and a finalize kernel (or the consuming kernel) sums the
 // partials in block order, so results are deterministic run to run.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -127,15 +129,31 @@ __global__ __launch_bounds__(kBlock) void k_sum(const double *__restrict__ part,
     if (threadIdx.x == 0) out[0] = t;
 }
 
-// Two scalar reductions at once: block 0 sums p0 into o0, block 1 p1 into o1.
+// Two scalar reductions at once: block 0 sums p0 into o0, block 1 p1 into
+// o1.  With a host mirror (pinned, host-coherent memory), the results and
+// the scalars `dscal` already holds are also written there, so the host
+// reads them after a stream synchronize without a copy.
 __global__ __launch_bounds__(kBlock) void k_sum2(const double *__restrict__ p0, int n0,
                                                  double *__restrict__ o0,
                                                  const double *__restrict__ p1, int n1,
-                                                 double *__restrict__ o1)
+                                                 double *__restrict__ o1,
+                                                 const double *__restrict__ dscal,
+                                                 double *__restrict__ hmirror)
 {
     const bool second = blockIdx.x == 1;
     const double t = ordered_sum(second ? p1 : p0, second ? n1 : n0);
-    if (threadIdx.x == 0) *(second ? o1 : o0) = t;
+    if (threadIdx.x == 0) {
+        *(second ? o1 : o0) = t;
+        if (hmirror) {
+            if (second) {
+                hmirror[o1 - dscal] = t;
+            } else {
+                hmirror[o0 - dscal] = t;
+                for (int i = 0; i < 4; ++i)
+                    if (dscal + i != o0 && dscal + i != o1) hmirror[i] = dscal[i];
+            }
+        }
+    }
 }
 
 // Finalize a vector reduction: hout[p] = sum_b part[p * nb + b], p < np
@@ -232,6 +250,109 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
+// y = A x, vector CSR: LPR lanes per row (a power of two), 64/LPR rows per
+// wave and U rows per lane group in flight (their column-index, value and
+// x loads are issued before any product is used).  Lane `sub` of a row
+// sums nonzeros sub, sub+LPR, ... and the LPR partial sums are combined
+// with a butterfly over the lane group; no LDS and no barriers.  FUSE: x
+// is fl(xs * xscale) formed on the fly from the unscaled source xs, and the
+// kernel also stores that scaled vector to xq (the basis column it
+// normalises), so the separate scale-copy pass disappears.  Epilogue: the
+// block's partial of Re(x^H y) in part[blockIdx.x].
+template <typename S, typename SV, int E, int LPR, int U, bool FUSE>
+__global__ __launch_bounds__(kBlock) void k_spmv_vec(
+    int n, const int *__restrict__ rp, const int *__restrict__ ci, const SV *__restrict__ val,
+    const S *__restrict__ x, S *__restrict__ y, double *__restrict__ part, double xscale,
+    S *__restrict__ xq)
+{
+    static_assert(LPR >= 1 && LPR <= 64 && (LPR & (LPR - 1)) == 0, "LPR power of 2");
+    constexpr int RPW = 64 / LPR;
+    const int lane = threadIdx.x & 63;
+    const int sub = lane % LPR, grp = lane / LPR;
+    const long wave = (long)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const long nw = (long)gridDim.x * (kBlock / 64);
+    auto xval = [&](int c, int e) -> double {
+        if constexpr (FUSE)
+            return (double)(S)((double)x[(size_t)E * c + e] * xscale);
+        else
+            return (double)x[(size_t)E * c + e];
+    };
+    double acc = 0.0;
+    for (long base = wave * RPW * U; base < n; base += nw * RPW * U) {
+        int kb[U], ke[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long row = base + (long)u * RPW + grp;
+            kb[u] = row < n ? rp[row] + sub : 0;
+            ke[u] = row < n ? rp[row + 1] : 0;
+        }
+        double sr[U], si[U];
+        int cc[U];
+        SV vr[U], vi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            sr[u] = si[u] = 0.0;
+            if (kb[u] < ke[u]) {
+                cc[u] = ci[kb[u]];
+                vr[u] = val[(size_t)E * kb[u]];
+                if constexpr (E == 2) vi[u] = val[(size_t)E * kb[u] + 1];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (kb[u] < ke[u]) {
+                if constexpr (E == 1) {
+                    sr[u] = (double)vr[u] * xval(cc[u], 0);
+                } else {
+                    const double xr = xval(cc[u], 0), xi = xval(cc[u], 1);
+                    sr[u] = (double)vr[u] * xr - (double)vi[u] * xi;
+                    si[u] = (double)vr[u] * xi + (double)vi[u] * xr;
+                }
+            }
+            // rows longer than LPR
+            for (int k = kb[u] + LPR; k < ke[u]; k += LPR) {
+                const int c = ci[k];
+                if constexpr (E == 1) {
+                    sr[u] += (double)val[k] * xval(c, 0);
+                } else {
+                    const double ar = val[2 * (size_t)k], ai = val[2 * (size_t)k + 1];
+                    const double xr = xval(c, 0), xi = xval(c, 1);
+                    sr[u] += ar * xr - ai * xi;
+                    si[u] += ar * xi + ai * xr;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int o = LPR / 2; o > 0; o >>= 1) {
+                sr[u] += __shfl_xor(sr[u], o);
+                if constexpr (E == 2) si[u] += __shfl_xor(si[u], o);
+            }
+            const long row = base + (long)u * RPW + grp;
+            if (sub == 0 && row < n) {
+                if constexpr (E == 1) {
+                    const double xr = xval((int)row, 0);
+                    y[row] = (S)sr[u];
+                    if constexpr (FUSE) xq[row] = (S)xr;
+                    acc += xr * (double)(S)sr[u];
+                } else {
+                    const double xr = xval((int)row, 0), xi = xval((int)row, 1);
+                    y[2 * row] = sr[u];
+                    y[2 * row + 1] = si[u];
+                    if constexpr (FUSE) {
+                        xq[2 * row] = xr;
+                        xq[2 * row + 1] = xi;
+                    }
+                    acc += xr * sr[u] + xi * si[u];
+                }
+            }
+        }
+    }
+    const double bs = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
 // Re(x^H y) over ne doubles: per-block partials in part[blockIdx.x].
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_dot(long ne, const S *__restrict__ x,
@@ -246,26 +367,38 @@ __global__ __launch_bounds__(kBlock) void k_dot(long ne, const S *__restrict__ x
 
 // --------------------------------------------------------- Gram-Schmidt
 
-// h = V(:, c0:c0+m)^H r.  Grid: blockIdx.y picks a chunk of CB columns,
-// blockIdx.x a grid-stride set of rows (RU rows of the stride per
-// iteration, so RU*CB loads are in flight per thread).  Column indices past
-// the chunk's end are clamped to its last column (the extra loads hit
-// cache and are discarded).  Each thread keeps CB*E running sums, which
-// are transposed-and-reduced over the wave and combined over waves in LDS;
-// the block writes its partial of coefficient p to part[p*gridDim.x +
-// blockIdx.x] (k_colsum sums each coefficient's partials in block order).
-// Chunks of one row range read the same stretch of r, which the cache
-// serves after the first chunk.
+// h = V(:, c0:c0+m)^H r.  gx*gy blocks: each takes one chunk of CB columns
+// (of gy) and one row block rb (of gx), whose grid-stride rows it walks RU
+// rows of the stride per iteration, so RU*CB loads are in flight per
+// thread.  Column indices past the chunk's end are clamped to its last
+// column (the extra loads hit cache and are discarded).  Each thread keeps
+// CB*E running sums, which are transposed-and-reduced over the wave and
+// combined over waves in LDS; the block writes its partial of coefficient p
+// to part[p*gx + rb] (k_colsum sums each coefficient's partials in row
+// block order).
 template <typename S, int E, int CB, int RU>
 __global__ __launch_bounds__(kBlock) void k_proj(
     int n, const S *__restrict__ V, long ld, int c0, int m, const S *__restrict__ r,
-    double *__restrict__ part)
+    double *__restrict__ part, int gx, int gy)
 {
     constexpr int N = CB * E;
     static_assert(N <= 64, "chunk too wide");
     __shared__ double sm[kBlock / 64][N];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int cb = blockIdx.y * CB;
+    // Workgroups go to the 8 XCDs round-robin by linear index.  With gx a
+    // multiple of 8, XCD x runs row blocks x, x+8, ... and takes all gy
+    // chunks of one row block back to back, so r's stretch is read from
+    // HBM once and then served by that XCD's L2.
+    int rb, chunk;
+    if ((gx & 7) == 0) {
+        const int local = blockIdx.x >> 3;
+        rb = (local / gy) * 8 + (blockIdx.x & 7);
+        chunk = local % gy;
+    } else {
+        rb = blockIdx.x / gy;
+        chunk = blockIdx.x % gy;
+    }
+    const int cb = chunk * CB;
     const int last = min(CB, m - cb) - 1;
     // wave-uniform column bases; rows index them
     const S *col[CB];
@@ -274,8 +407,8 @@ __global__ __launch_bounds__(kBlock) void k_proj(
     double acc[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) acc[i] = 0.0;
-    const int stride = gridDim.x * kBlock * RU;
-    int row = blockIdx.x * kBlock * RU + threadIdx.x;
+    const int stride = gx * kBlock * RU;
+    int row = rb * kBlock * RU + threadIdx.x;
     if constexpr (E == 1) {
         for (; row + (RU - 1) * kBlock < n; row += stride) {
             double rv[RU], vv[RU][CB];
@@ -340,7 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_proj(
         double t = 0.0;
 #pragma unroll
         for (int i = 0; i < kBlock / 64; ++i) t += sm[i][threadIdx.x];
-        part[(size_t)(cb * E + threadIdx.x) * gridDim.x + blockIdx.x] = t;
+        part[(size_t)(cb * E + threadIdx.x) * gx + rb] = t;
     }
 }
 
@@ -574,26 +707,59 @@ void stats_resolve()
     g_stats.pending.clear();
 }
 
-// Brackets one kernel launch with HIP events on its stream when enabled
-// (every g_stats.on-th launch of its kind; launches and bytes count the
-// timed launches only, so bytes/ms stays the timed launches' rate).
+// Start/stop events armed for the next kernel launch.  KLAUNCH passes them
+// to hipExtLaunchKernelGGL, which stamps them from the kernel's own
+// dispatch (start and end of the kernel, no marker packets in the stream).
+struct LaunchEvents {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+LaunchEvents g_launch_ev;
+
+#define KLAUNCH(kernel, grid, block, shm, stream, ...)                                      \
+    do {                                                                                    \
+        const LaunchEvents ev_ = g_launch_ev;                                               \
+        g_launch_ev = LaunchEvents{};                                                       \
+        hipExtLaunchKernelGGL(kernel, grid, block, shm, stream, ev_.a, ev_.b, 0, __VA_ARGS__); \
+    } while (0)
+
+// Times the first kernel launched in its scope when enabled: one scope in
+// g_stats.on of each kind, picked by a multiplicative hash of its ordinal
+// (a sample that does not alias with the periodic growth of the basis
+// during a restart cycle).  Launches and bytes count the timed launches
+// only, so bytes/ms stays the timed launches' rate.
+bool timed_pick(int k)
+{
+    if (!g_stats.on) return false;
+    const uint64_t i = (uint64_t)g_stats.seen[k]++;
+    return g_stats.on == 1 || ((i * 0x9E3779B97F4A7C15ull) >> 40) % (uint64_t)g_stats.on == 0;
+}
+
 struct Timed {
     hipEvent_t a = nullptr, b = nullptr;
     int k;
-    hipStream_t s;
-    Timed(int kind, hipStream_t st, double bytes) : k(kind), s(st)
+    double bytes;
+    Timed(int kind, hipStream_t, double nbytes) : k(kind), bytes(nbytes)
     {
-        if (!g_stats.on || g_stats.seen[k]++ % g_stats.on != 0) return;
-        g_stats.launches[k] += 1.0;
-        g_stats.bytes[k] += bytes;
+        if (!timed_pick(k)) return;
         a = ev_get();
         b = ev_get();
-        if (a) (void)hipEventRecord(a, s);
+        if (a && b) g_launch_ev = LaunchEvents{a, b};
     }
     ~Timed()
     {
-        if (!a || !b) return;
-        (void)hipEventRecord(b, s);
+        if (!a || !b) {
+            if (a) g_stats.pool.push_back(a);
+            if (b) g_stats.pool.push_back(b);
+            return;
+        }
+        if (g_launch_ev.a == a) {  // nothing was launched in the scope
+            g_launch_ev = LaunchEvents{};
+            g_stats.pool.push_back(a);
+            g_stats.pool.push_back(b);
+            return;
+        }
+        g_stats.launches[k] += 1.0;
+        g_stats.bytes[k] += bytes;
         g_stats.pending.push_back({a, b, k});
     }
 };
@@ -608,7 +774,9 @@ struct trlhip_csr {
     long nnz;
     int vtype;
     int E;          // doubles per value (complex 2)
-    int nb;         // SpMV row blocks
+    int nb;         // SpMV row blocks (CSR-stream kernel)
+    int lpr;        // lanes per row of the vector-CSR kernel; 0: CSR-stream
+    int vgrid;      // blocks of the vector-CSR kernel
     int *rb;        // nb + 1 row-block starts
     int *rp;
     int *ci;
@@ -637,6 +805,10 @@ struct trlhip_ctx {
     double *dY;     // restart matrix
     size_t dYcap;
     double *hbuf;   // pinned host mirror: 16 scalars + coefficients
+    double *hres;   // pinned host-coherent results of a fused step (4 scalars)
+    double *dres;   // its device address
+    int pend_dst;   // column of a deferred scale-copy from the scratch column (-1: none)
+    double pend_s;  // its scale
     size_t hcap;
     double *hY;     // pinned staging for Y
     hipStream_t s;
@@ -690,27 +862,67 @@ double spmv_bytes(const trlhip_csr *A, int E, size_t vsz, size_t xsz)
     return (double)A->nnz * ((double)vsz * E + 4.0) + 4.0 * (A->n + 1) + 2.0 * (double)xsz * E * A->n;
 }
 
-int spmv_grid(const trlhip_csr *A) { return A->nb < kMaxGrid ? A->nb : kMaxGrid; }
-
-// y = A x on stream s (row-block kernel for the storage); partials in part.
-int launch_spmv(const trlhip_csr *A, int vtype, const void *x, void *y, double *part, hipStream_t s)
+int spmv_grid(const trlhip_csr *A)
 {
+    if (A->lpr) return A->vgrid;
+    return A->nb < kMaxGrid ? A->nb : kMaxGrid;
+}
+
+template <typename S, typename SV, int E, bool FUSE>
+void launch_vec(const trlhip_csr *A, const SV *val, const void *x, void *y, double *part,
+                hipStream_t s, double xscale, void *xq)
+{
+    constexpr int U = E == 1 ? 4 : 2;
+    const dim3 g(A->vgrid), b(kBlock);
+    const S *xs = (const S *)x;
+    S *ys = (S *)y, *q = (S *)xq;
+#define VEC_CASE(L)                                                                          \
+    case L:                                                                                  \
+        KLAUNCH((k_spmv_vec<S, SV, E, L, U, FUSE>), g, b, 0, s, A->n, A->rp, A->ci, val, xs, \
+                ys, part, xscale, q);                                                        \
+        break;
+    switch (A->lpr) {
+        VEC_CASE(2) VEC_CASE(4) VEC_CASE(8) VEC_CASE(16) VEC_CASE(32)
+    }
+#undef VEC_CASE
+}
+
+// y = A x on stream s with the kernel chosen for the matrix; partials in
+// part.  With xq (vector-CSR kernel only) x is an unscaled source: the
+// product uses fl(x * xscale), which is also stored to xq.
+int launch_spmv(const trlhip_csr *A, int vtype, const void *x, void *y, double *part, hipStream_t s,
+                double xscale = 1.0, void *xq = nullptr)
+{
+    if (A->lpr) {
+        if (vtype == TRLHIP_R64) {
+            if (xq) launch_vec<double, double, 1, true>(A, A->val, x, y, part, s, xscale, xq);
+            else launch_vec<double, double, 1, false>(A, A->val, x, y, part, s, xscale, xq);
+        } else if (vtype == TRLHIP_C128) {
+            if (xq) launch_vec<double, double, 2, true>(A, A->val, x, y, part, s, xscale, xq);
+            else launch_vec<double, double, 2, false>(A, A->val, x, y, part, s, xscale, xq);
+        } else {
+            if (xq) launch_vec<float, float, 1, true>(A, A->val32, x, y, part, s, xscale, xq);
+            else launch_vec<float, float, 1, false>(A, A->val32, x, y, part, s, xscale, xq);
+        }
+        return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
+    }
+    if (xq) return TRLHIP_EINVAL;
     const dim3 g(spmv_grid(A)), b(kBlock);
     if (vtype == TRLHIP_R64)
-        hipLaunchKernelGGL((k_spmv<double, double, 1>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
+        KLAUNCH((k_spmv<double, double, 1>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
                            A->val, (const double *)x, (double *)y, part);
     else if (vtype == TRLHIP_C128)
-        hipLaunchKernelGGL((k_spmv<double, double, 2>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
+        KLAUNCH((k_spmv<double, double, 2>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
                            A->val, (const double *)x, (double *)y, part);
     else
-        hipLaunchKernelGGL((k_spmv<float, float, 1>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
+        KLAUNCH((k_spmv<float, float, 1>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
                            A->val32, (const float *)x, (float *)y, part);
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
 int launch_sum(const double *part, int np, double *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_sum, dim3(1), dim3(kBlock), 0, s, part, np, out);
+    KLAUNCH(k_sum, dim3(1), dim3(kBlock), 0, s, part, np, out);
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
@@ -732,6 +944,20 @@ int run_spmv(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol)
                        c->V + (size_t)ycol * c->ld * c->esz, part_alpha(c), c->s);
 }
 
+// The deferred scale-copy into column xcol fused into the SpMV that uses
+// it: y = A (s * scratch), and s * scratch stored to column xcol.
+int run_spmv_fused(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol)
+{
+    const size_t vsz = is_f32(c) ? 4 : 8;
+    c->np_alpha = spmv_grid(A);
+    const double s = c->pend_s;
+    c->pend_dst = -1;
+    Timed t(TRLHIP_K_SPMV, c->s, spmv_bytes(A, c->E, vsz, vsz) + (double)c->esz * c->n);
+    return launch_spmv(A, c->vtype, c->V + (size_t)c->ncols * c->ld * c->esz,
+                       c->V + (size_t)ycol * c->ld * c->esz, part_alpha(c), c->s, s,
+                       c->V + (size_t)xcol * c->ld * c->esz);
+}
+
 // Re(x^H y) of two columns as partials in part_alpha.
 int run_dot(trlhip_ctx *c, int xcol, int ycol)
 {
@@ -740,10 +966,10 @@ int run_dot(trlhip_ctx *c, int xcol, int ycol)
     c->np_alpha = g;
     Timed t(TRLHIP_K_MISC, c->s, 2.0 * (double)c->esz * c->n);
     if (is_f32(c))
-        hipLaunchKernelGGL((k_dot<float>), dim3(g), dim3(kBlock), 0, c->s, ne, colp<float>(c, xcol),
+        KLAUNCH((k_dot<float>), dim3(g), dim3(kBlock), 0, c->s, ne, colp<float>(c, xcol),
                            colp<float>(c, ycol), part_alpha(c));
     else
-        hipLaunchKernelGGL((k_dot<double>), dim3(g), dim3(kBlock), 0, c->s, ne,
+        KLAUNCH((k_dot<double>), dim3(g), dim3(kBlock), 0, c->s, ne,
                            colp<double>(c, xcol), colp<double>(c, ycol), part_alpha(c));
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
@@ -754,7 +980,7 @@ void launch_update(trlhip_ctx *c, int g, int c0, int m, const double *dh, double
 {
     const size_t shm = sizeof(double) * (size_t)c->E * (m > 0 ? m : 1);
     const double *cp = aslot >= 0 ? part_alpha(c) : nullptr;
-    hipLaunchKernelGGL((k_update<S, E, RU>), dim3(g), dim3(kBlock), shm, c->s, c->n, colp<S>(c, 0),
+    KLAUNCH((k_update<S, E, RU>), dim3(g), dim3(kBlock), shm, c->s, c->n, colp<S>(c, 0),
                        (long)c->ld, c0, m, dh, h0, cp, c->np_alpha, aslot, colp<S>(c, rin),
                        colp<S>(c, rout), pdst, o_hh, c->dscal + S_ALPHA);
 }
@@ -787,18 +1013,20 @@ int run_update(trlhip_ctx *c, int c0, int m, const double *dh, int aslot, int ri
 template <typename S, int E, int CB, int RU>
 void launch_proj(trlhip_ctx *c, int gx, int c0, int m, int rcol)
 {
-    const dim3 grid(gx, (m + CB - 1) / CB);
-    hipLaunchKernelGGL((k_proj<S, E, CB, RU>), grid, dim3(kBlock), 0, c->s, c->n, colp<S>(c, 0),
-                       (long)c->ld, c0, m, colp<S>(c, rcol), part_proj(c));
+    const int gy = (m + CB - 1) / CB;
+    KLAUNCH((k_proj<S, E, CB, RU>), dim3(gx * gy), dim3(kBlock), 0, c->s, c->n, colp<S>(c, 0),
+            (long)c->ld, c0, m, colp<S>(c, rcol), part_proj(c), gx, gy);
 }
 
 // Row blocks of the projection: about kProjBlocks blocks over all column
-// chunks, at most kProjGrid (the partial rows).
+// chunks, at most kProjGrid (the partial rows), a multiple of 8 (the XCD
+// count) when there are at least 8.
 int proj_gx(const trlhip_ctx *c, int m, int cb)
 {
     const int chunks = (m + cb - 1) / cb;
     const int want = (kProjBlocks + chunks - 1) / chunks;
-    return grid_for(c->n, kBlock * 4, want < kProjGrid ? want : kProjGrid);
+    const int g = grid_for(c->n, kBlock * 4, want < kProjGrid ? want : kProjGrid);
+    return g >= 8 ? g & ~7 : g;
 }
 
 // hout = V(:, c0:c0+m)^H r (device, E*m doubles)
@@ -823,7 +1051,7 @@ int run_proj(trlhip_ctx *c, int c0, int m, int rcol, double *hout)
         if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
     }
     const int np = m * c->E;
-    hipLaunchKernelGGL(k_colsum, dim3(np), dim3(kBlock), 0, c->s, part_proj(c), g, hout);
+    KLAUNCH(k_colsum, dim3(np), dim3(kBlock), 0, c->s, part_proj(c), g, hout);
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
@@ -833,12 +1061,22 @@ int run_scale_copy(trlhip_ctx *c, int dst, int src, double s)
     const int g = grid_for(ne, kBlock * 8, kMaxGrid);
     Timed t(TRLHIP_K_MISC, c->s, 2.0 * (double)c->esz * c->n);
     if (is_f32(c))
-        hipLaunchKernelGGL((k_scale_copy<float>), dim3(g), dim3(kBlock), 0, c->s, ne,
+        KLAUNCH((k_scale_copy<float>), dim3(g), dim3(kBlock), 0, c->s, ne,
                            colp<float>(c, src), colp<float>(c, dst), s);
     else
-        hipLaunchKernelGGL((k_scale_copy<double>), dim3(g), dim3(kBlock), 0, c->s, ne,
+        KLAUNCH((k_scale_copy<double>), dim3(g), dim3(kBlock), 0, c->s, ne,
                            colp<double>(c, src), colp<double>(c, dst), s);
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
+}
+
+// Apply a deferred scale-copy (trlhip_col_scale_copy from the scratch
+// column records it; the next step's SpMV usually absorbs it).
+int flush_pending(trlhip_ctx *c)
+{
+    if (c->pend_dst < 0) return 0;
+    const int dst = c->pend_dst;
+    c->pend_dst = -1;
+    return run_scale_copy(c, dst, c->ncols, c->pend_s);
 }
 
 // copy scalars back and wait
@@ -902,6 +1140,26 @@ int trlhip_device_count(void)
     return n;
 }
 
+// SpMV kernel for a matrix: vector CSR with LPR = the power of two at or
+// above the mean row length (2..32) when no row is longer than 8*LPR
+// (balanced rows); otherwise CSR-stream, whose row blocks absorb uneven
+// rows.  A choice made from the structure alone, so a matrix always runs
+// the same kernel (and reduction order).
+static void choose_spmv(trlhip_csr *A, const int *row_ptr)
+{
+    const int n = A->n;
+    int maxlen = 0;
+    for (int i = 0; i < n; ++i) maxlen = std::max(maxlen, row_ptr[i + 1] - row_ptr[i]);
+    const double avg = (double)A->nnz / n;
+    int lpr = 2;
+    while (lpr < 32 && lpr < avg) lpr <<= 1;
+    A->lpr = maxlen <= 8 * lpr ? lpr : 0;
+    const int U = A->E == 1 ? 4 : 2;
+    const long rows_per_block = (long)(64 / lpr) * U * (kBlock / 64);
+    const long g = (n + rows_per_block - 1) / rows_per_block;
+    A->vgrid = (int)std::min<long>(std::max<long>(g, 1), kMaxGrid);
+}
+
 int trlhip_csr_create(trlhip_csr **out, int n, long nnz, const int *row_ptr, const int *col_idx,
                       const void *values, int vtype)
 {
@@ -923,6 +1181,7 @@ int trlhip_csr_create(trlhip_csr **out, int n, long nnz, const int *row_ptr, con
     A->E = vtype == TRLHIP_C128 ? 2 : 1;
     const std::vector<int> rb = row_blocks(n, row_ptr);
     A->nb = (int)rb.size() - 1;
+    choose_spmv(A, row_ptr);
     const size_t nz1 = (size_t)(nnz > 0 ? nnz : 1);
     int rc = 0;
     if (hipStreamCreate(&A->s) != hipSuccess) rc = TRLHIP_EHIP;
@@ -1026,6 +1285,7 @@ int trlhip_ctx_destroy(trlhip_ctx *c)
     if (c->dh) (void)hipFree(c->dh);
     if (c->dY) (void)hipFree(c->dY);
     if (c->hbuf) (void)hipHostFree(c->hbuf);
+    if (c->hres) (void)hipHostFree(c->hres);
     if (c->hY) (void)hipHostFree(c->hY);
     if (c->s) (void)hipStreamDestroy(c->s);
     free(c);
@@ -1055,7 +1315,12 @@ int trlhip_ctx_create(trlhip_ctx **out, int n, int ncols, int vtype)
     if (!rc && hipMalloc((void **)&c->dcoef, sizeof(double) * ncoef) != hipSuccess) rc = TRLHIP_ENOMEM;
     if (!rc && hipMalloc((void **)&c->dh, sizeof(double) * ncoef) != hipSuccess) rc = TRLHIP_ENOMEM;
     c->hcap = 16 + 2 * ncoef;
+    c->pend_dst = -1;
     if (!rc && hipHostMalloc((void **)&c->hbuf, sizeof(double) * c->hcap, 0) != hipSuccess) rc = TRLHIP_ENOMEM;
+    if (!rc && hipHostMalloc((void **)&c->hres, sizeof(double) * 8,
+                             hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
+        rc = TRLHIP_ENOMEM;
+    if (!rc && hipHostGetDevicePointer((void **)&c->dres, c->hres, 0) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipMemsetAsync(c->V, 0, vbytes, c->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipMemsetAsync(c->dscal, 0, sizeof(double) * 16, c->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipStreamSynchronize(c->s) != hipSuccess) rc = TRLHIP_EHIP;
@@ -1072,6 +1337,7 @@ int trlhip_ctx_ld(const trlhip_ctx *c) { return c ? c->ld : 0; }
 int trlhip_col_set(trlhip_ctx *c, int col, const void *x)
 {
     if (!c || !x || !valid_col(c, col)) return TRLHIP_EINVAL;
+    if (int rc = flush_pending(c)) return rc;
     TRY(hipMemcpyAsync(c->V + (size_t)col * c->ld * c->esz, x, c->esz * (size_t)c->n,
                        hipMemcpyHostToDevice, c->s));
     TRY(hipStreamSynchronize(c->s));
@@ -1081,6 +1347,7 @@ int trlhip_col_set(trlhip_ctx *c, int col, const void *x)
 int trlhip_col_get(trlhip_ctx *c, int col, void *x)
 {
     if (!c || !x || !valid_col(c, col)) return TRLHIP_EINVAL;
+    if (int rc = flush_pending(c)) return rc;
     TRY(hipMemcpyAsync(x, c->V + (size_t)col * c->ld * c->esz, c->esz * (size_t)c->n,
                        hipMemcpyDeviceToHost, c->s));
     TRY(hipStreamSynchronize(c->s));
@@ -1091,6 +1358,7 @@ int trlhip_col_get(trlhip_ctx *c, int col, void *x)
 int trlhip_col_copy(trlhip_ctx *c, int dst, int src)
 {
     if (!c || !valid_col(c, dst) || !valid_col(c, src)) return TRLHIP_EINVAL;
+    if (int rc = flush_pending(c)) return rc;
     if (dst == src) return 0;
     return run_scale_copy(c, dst, src, 1.0);
 }
@@ -1098,12 +1366,19 @@ int trlhip_col_copy(trlhip_ctx *c, int dst, int src)
 int trlhip_col_scale(trlhip_ctx *c, int col, double s)
 {
     if (!c || !valid_col(c, col)) return TRLHIP_EINVAL;
+    if (int rc = flush_pending(c)) return rc;
     return run_scale_copy(c, col, col, s);
 }
 
 int trlhip_col_scale_copy(trlhip_ctx *c, int dst, int src, double s)
 {
     if (!c || !valid_col(c, dst) || src < 0 || src > c->ncols) return TRLHIP_EINVAL;
+    if (int rc = flush_pending(c)) return rc;
+    if (src == c->ncols) {  // deferred: the next SpMV of column dst applies it
+        c->pend_dst = dst;
+        c->pend_s = s;
+        return 0;
+    }
     return run_scale_copy(c, dst, src, s);
 }
 
@@ -1111,7 +1386,8 @@ int trlhip_spmv(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol, double *
 {
     if (!c || !csr_fits(c, A)) return TRLHIP_EINVAL;
     if (!valid_col(c, xcol) || !valid_col(c, ycol) || xcol == ycol) return TRLHIP_EINVAL;
-    int rc = run_spmv(c, A, xcol, ycol);
+    int rc = flush_pending(c);
+    if (!rc) rc = run_spmv(c, A, xcol, ycol);
     if (!rc) rc = launch_sum(part_alpha(c), c->np_alpha, c->dscal + S_ALPHA, c->s);
     if (!rc) rc = fetch(c, c->dscal + S_ALPHA, c->hbuf, 1);
     if (rc) return rc;
@@ -1152,7 +1428,13 @@ static int form_residual(trlhip_ctx *c, const trlhip_csr *A, int qa, int first, 
         c0 = qa;
         m = 1;
     }
-    int rc = A ? run_spmv(c, A, qa, rcol) : run_dot(c, qa, rcol);
+    int rc;
+    if (A && A->lpr && c->pend_dst == qa) {
+        rc = run_spmv_fused(c, A, qa, rcol);
+    } else {
+        rc = flush_pending(c);
+        if (!rc) rc = A ? run_spmv(c, A, qa, rcol) : run_dot(c, qa, rcol);
+    }
     if (rc) return rc;
     if (defer)
         return run_update(c, c0, m, coef, m - 1, rcol, rcol, nullptr, nullptr, TRLHIP_K_RECUR, h0,
@@ -1167,6 +1449,7 @@ int trlhip_lanczos_step(trlhip_ctx *c, const trlhip_csr *A, int qa, int qb, doub
     if (!c || !valid_col(c, qa) || !valid_col(c, rcol) || qa == rcol) return TRLHIP_EINVAL;
     if (qb >= 0 && qb != qa - 1) return TRLHIP_EINVAL;
     if (A && !csr_fits(c, A)) return TRLHIP_EINVAL;
+    if (int rc = flush_pending(c)) return rc;
     int rc = form_residual(c, A, qa, 0, qb, beta_prev, nullptr, 0, rcol);
     if (rc) return rc;
     rc = fetch(c, c->dscal, c->hbuf + 4, 2);
@@ -1182,6 +1465,7 @@ int trlhip_restart_step(trlhip_ctx *c, const trlhip_csr *A, int qa, const double
     if (!c || jnd < 1 || jnd > c->ncols || qa != jnd - 1 || !valid_col(c, rcol) || rcol == qa)
         return TRLHIP_EINVAL;
     if (A && !csr_fits(c, A)) return TRLHIP_EINVAL;
+    if (int rc = flush_pending(c)) return rc;
     int rc = form_residual(c, A, qa, 1, -1, 0.0, beta, jnd, rcol);
     if (rc) return rc;
     rc = fetch(c, c->dscal, c->hbuf + 4, 2);
@@ -1208,16 +1492,18 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
     rc = run_update(c, 0, jnd, c->dh, -1, rcol, c->ncols, nullptr, c->dscal + S_HH,
                     TRLHIP_K_UPDATE, 0.0, nullptr, &g2);
     if (rc) return rc;
-    // both norms (before and after the pass) in one finalize launch
-    hipLaunchKernelGGL(k_sum2, dim3(2), dim3(kBlock), 0, c->s, part_recur(c), c->np_recur,
-                       c->dscal + S_RR2, part_update(c), g2, c->dscal + S_RR2B);
+    // both norms (before and after the pass) in one finalize launch, which
+    // also writes the four results to host-coherent memory
+    KLAUNCH(k_sum2, dim3(2), dim3(kBlock), 0, c->s, part_recur(c), c->np_recur,
+            c->dscal + S_RR2, part_update(c), g2, c->dscal + S_RR2B, c->dscal, c->dres);
     if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
-    rc = fetch(c, c->dscal, c->hbuf + 4, 4);
-    if (rc) return rc;
-    out[0] = c->hbuf[4 + S_ALPHA];
-    out[1] = c->hbuf[4 + S_RR2];
-    out[2] = c->hbuf[4 + S_HH];
-    out[3] = c->hbuf[4 + S_RR2B];
+    TRY(hipStreamSynchronize(c->s));
+    stats_resolve();
+    const volatile double *hr = c->hres;
+    out[0] = hr[S_ALPHA];
+    out[1] = hr[S_RR2];
+    out[2] = hr[S_HH];
+    out[3] = hr[S_RR2B];
     return 0;
 }
 
@@ -1225,7 +1511,8 @@ int trlhip_cgs_pass(trlhip_ctx *c, int c0, int m, int rcol, double *hh, double *
 {
     if (!c || m < 1 || !valid_cols(c, c0, m) || !valid_col(c, rcol) || (rcol >= c0 && rcol < c0 + m))
         return TRLHIP_EINVAL;
-    int rc = run_proj(c, c0, m, rcol, c->dh);
+    int rc = flush_pending(c);
+    if (!rc) rc = run_proj(c, c0, m, rcol, c->dh);
     if (rc) return rc;
     rc = run_update(c, c0, m, c->dh, -1, rcol, rcol, c->dscal + S_RR2, c->dscal + S_HH,
                     TRLHIP_K_UPDATE);
@@ -1244,7 +1531,8 @@ int trlhip_cgs_pass(trlhip_ctx *c, int c0, int m, int rcol, double *hh, double *
 int trlhip_proj(trlhip_ctx *c, int c0, int m, int rcol, double *h)
 {
     if (!c || !h || m < 1 || !valid_cols(c, c0, m) || !valid_col(c, rcol)) return TRLHIP_EINVAL;
-    int rc = run_proj(c, c0, m, rcol, c->dh);
+    int rc = flush_pending(c);
+    if (!rc) rc = run_proj(c, c0, m, rcol, c->dh);
     if (rc) return rc;
     rc = fetch(c, c->dh, c->hbuf + 16, (size_t)c->E * m);
     if (rc) return rc;
@@ -1257,6 +1545,7 @@ int trlhip_update(trlhip_ctx *c, int c0, int m, const double *h, int rcol, doubl
     if (!c || !h || m < 0 || !valid_cols(c, c0, m) || !valid_col(c, rcol) ||
         (m > 0 && rcol >= c0 && rcol < c0 + m))
         return TRLHIP_EINVAL;
+    if (int rc = flush_pending(c)) return rc;
     const int E = c->E;
     memcpy(c->hbuf + 16, h, sizeof(double) * E * (size_t)m);
     if (m > 0)
@@ -1273,6 +1562,7 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y, int ldy)
 {
     if (!c || !Y || lck < 0 || jl < 1 || k < 1 || k > jl || ldy < jl || lck + jl > c->ncols)
         return TRLHIP_EINVAL;
+    if (int rc = flush_pending(c)) return rc;
     const int KT = (k + 15) / 16;
     if (KT > 8) return TRLHIP_EINVAL;  // k <= 128
     const size_t ny = (size_t)ldy * k;
@@ -1300,7 +1590,7 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y, int ldy)
     Timed t(TRLHIP_K_RITZ, c->s, (double)c->esz * c->n * (jl + k));
 #define RITZ_CASE(S, K)                                                                      \
     case K:                                                                                  \
-        hipLaunchKernelGGL((k_ritz_mfma<S, K>), dim3(g), dim3(kBlock), shm, c->s, nrows,     \
+        KLAUNCH((k_ritz_mfma<S, K>), dim3(g), dim3(kBlock), shm, c->s, nrows,     \
                            reinterpret_cast<S *>(c->V), ldr, lck, jl, k, c->dY, ldy, tpw); \
         break;
     if (is_f32(c)) {

@@ -102,6 +102,11 @@ def lib():
         L.trl_check_ritz.restype = None
         L.trl_rayleigh_quotients.argtypes = [P, ctypes.POINTER(TrlInfo), I, P, I, P, P]
         L.trl_rayleigh_quotients.restype = None
+        L.trl_ritz_projection.argtypes = [P, ctypes.POINTER(TrlInfo), I, P, I, P, P, I, P]
+        L.trl_ritz_projection.restype = None
+        L.ztrl_check_ritz.argtypes = [P, ctypes.POINTER(TrlInfo), I, I, P, I, P,
+                                      ctypes.POINTER(I), P, P, P, I]
+        L.ztrl_check_ritz.restype = None
         _lib = L
     return _lib
 
@@ -262,6 +267,8 @@ class Ctx:
             ("trlhip_proj", [P, I, I, I, P]),
             ("trlhip_update", [P, I, I, P, I, P]),
             ("trlhip_ritz", [P, I, I, I, P, I]),
+            ("trlhip_col_scale_copy", [P, I, I, D]),
+            ("trlhip_step_cgs", [P, P, I, I, I, D, P, I, I, P]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -310,6 +317,20 @@ class Ctx:
         a = ctypes.c_double()
         self._chk(lib().trlhip_spmv(self.h, A.handle, xcol, ycol, ctypes.byref(a)), "spmv")
         return a.value
+
+    def scale_copy(self, dst, src, scale):
+        """dst = scale * src; src may be the scratch column (index ncols)."""
+        self._chk(lib().trlhip_col_scale_copy(self.h, dst, src, scale), "scale_copy")
+
+    def step_cgs(self, A, first, qa, qb, beta_prev, beta, jnd, rcol):
+        """One fused Lanczos step (trlhip_step_cgs); returns (alpha, ||r||^2,
+        ||h||^2, ||r - V h||^2) with r left in rcol and r - V h in the
+        scratch column."""
+        out = np.zeros(4)
+        b = np.ascontiguousarray(beta if beta is not None else np.zeros(1), dtype=np.float64)
+        self._chk(lib().trlhip_step_cgs(self.h, A.handle if A is not None else None, first, qa,
+                                        qb, beta_prev, _ptr(b), jnd, rcol, _ptr(out)), "step_cgs")
+        return out
 
     def ritz(self, lck, jl, k, Y):
         Y = np.asfortranarray(Y, dtype=np.float64)

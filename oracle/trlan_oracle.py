@@ -26,12 +26,19 @@ variant dstqrb) bound through oracle/lapack.py.
 The Lanczos basis is one n x (maxlan+1) array: logical column k is evec[:,k]
 for k < mev and base[:,k-mev] beyond, exactly how the reference splits it.
 
-Pinning: integer trajectory + eigenvalues are compared against the compiled
-reference (oracle/_ref) in tests/test_oracle_vs_ref.py, against closed forms
-(diagonal operators) and against LAPACK eigh of small dense matrices.
-Random-vector paths (guess <= 0 with perturbation, trl_cgs random injection)
-use numpy's generator instead of the reference's clock()-seeded drand48 and
-are parity unpinned.
+The complex Hermitian path (ztrlan, SRC/ztrlan.c:26-221, SRC/ztrlcore.c)
+runs through the same functions with complex_=True: conjugated dots,
+ztrl_cgs's ntot-scaled randomization threshold and perturbation retries,
+and optionally (gdot_quirk) the reference's two-column ztrl_g_dot_ result.
+
+Pinning (tests/test_oracle.py): the integer trajectory (matvec, nloop,
+north, nrand, locked) and eigenvalues agree exactly / to 1e-12 ||A|| with
+the compiled reference's recorded outputs (tests/golden/ref_cases.json) for
+the real cases; complex eigenvalues agree to 1e-12 ||A||; closed forms
+(diagonal operators) and LAPACK eigh of small dense matrices pin the rest;
+a planted mistake must break the agreement.  Random-vector paths (guess <= 0
+with perturbation, trl_cgs random injection) use numpy's generator instead
+of the reference's clock()-seeded drand48 and are parity unpinned.
 """
 import math
 

@@ -89,3 +89,69 @@ def test_spmv_alpha(gpu_lib, builder):
     assert abs(a - np.vdot(x, y).real) <= 1e-12 * abs(np.vdot(x, y))
     c.close()
     A.close()
+
+
+def _arrow_csr(n, seed):
+    """Symmetric arrowhead: dense first row/column plus a diagonal.  Its one
+    long row sends the SpMV to the row-block (CSR-stream) kernel."""
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal(n)
+    d = rng.standard_normal(n)
+    rows = [np.arange(n)] + [np.array([0, i]) for i in range(1, n)]
+    vals = [np.concatenate([[d[0]], a[1:]])] + [np.array([a[i], d[i]]) for i in range(1, n)]
+    rp = np.zeros(n + 1, dtype=np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    return rp, np.concatenate(rows).astype(np.int32), np.concatenate(vals)
+
+
+@pytest.mark.parametrize("builder", ["lap", "arrow", "anderson3d"])
+def test_fused_step_sequence(gpu_lib, builder):
+    """Three trlhip_step_cgs steps with the normalising scale-copy deferred
+    into the next step's SpMV (vector-CSR kernels) or applied before it
+    (CSR-stream kernel), against the same recurrence in numpy."""
+    T = gpu_lib
+    cplx = builder == "anderson3d"
+    if builder == "lap":
+        rp, ci, v = W.laplace3d_csr(16, seed=2)
+    elif builder == "arrow":
+        rp, ci, v = _arrow_csr(3000, seed=3)
+    else:
+        rp, ci, v = W.anderson3d_csr(12, seed=4)
+    n = len(rp) - 1
+    f = W.csr_matvec(rp, ci, v)
+    x = W.start_vector(n, seed=6, complex_=cplx)
+    x = x / np.linalg.norm(x)
+    A = T.Csr(rp, ci, v, complex_=cplx)
+    ncols = 6
+    c = T.Ctx(n, ncols, cplx)
+    c.set(0, x)
+    Q = [x]
+    beta_prev = 0.0
+    scale = 1.0
+    for j in range(1, 4):
+        qa, qb = j - 1, (j - 2 if j >= 2 else -1)
+        first = 1 if j == 1 else 0
+        out = c.step_cgs(A, first, qa, qb, beta_prev, None, j, j)
+        # numpy restatement of the step
+        q = Q[qa]
+        y = f(q)
+        alpha = np.vdot(q, y).real
+        r = y - alpha * q - (beta_prev * Q[qb] if qb >= 0 else 0.0)
+        Vm = np.stack(Q[:j], axis=1)
+        h = Vm.conj().T @ r
+        s = r - Vm @ h
+        ref = [alpha, np.vdot(r, r).real, np.vdot(h, h).real, np.vdot(s, s).real]
+        scl = max(1.0, abs(alpha))
+        assert abs(out[0] - ref[0]) <= 1e-12 * scl, (j, out, ref)
+        assert abs(out[1] - ref[1]) <= 1e-11 * max(ref[1], 1.0)
+        assert abs(out[3] - ref[3]) <= 1e-9 * max(ref[1], 1.0)
+        assert np.abs(c.get(j) - r).max() <= 1e-12 * max(np.abs(r).max(), 1.0)
+        beta = np.sqrt(out[3])
+        scale = 1.0 / beta
+        c.scale_copy(j, ncols, scale)   # deferred into the next step's SpMV
+        Q.append(s * scale)
+        beta_prev = beta
+    # the last deferred copy is applied by any other entry point (col_get)
+    assert np.abs(c.get(3) - Q[3]).max() <= 1e-10
+    c.close()
+    A.close()

@@ -81,7 +81,7 @@ def test_trlan_host_op_matches_device_op(gpu_lib):
     check_against_oracle(h, o, mv, 1e-8)
 
 
-@pytest.mark.parametrize("restart", [0, 2, 4, 5, 6, 8])
+@pytest.mark.parametrize("restart", [0, 2, 4, 5, 6, 7, 8])
 def test_restart_schemes_vs_oracle(gpu_lib, restart):
     T = gpu_lib
     rp, ci, v = W.diag_csr(2000)
@@ -91,3 +91,46 @@ def test_restart_schemes_vs_oracle(gpu_lib, restart):
     A.close()
     o = oracle_solve(rp, ci, v, -1, 10, 100, 1.4901e-8, restart, 2000, 10, x0)
     check_against_oracle(g, o, W.csr_matvec(rp, ci, v), 1.4901e-8)
+
+
+def _csr_norm_bound(rp, ci, v):
+    """max row sum of |a_ij| (Gershgorin bound on ||A||_2)."""
+    return float(np.add.reduceat(np.abs(v), rp[:-1]).max())
+
+
+@pytest.mark.parametrize("cplx", [False, True], ids=["lap128_lg16_m64", "zand96_sm24_m80"])
+def test_full_size_invariants(gpu_lib, cplx):
+    """The two full-size GPU workloads (128^3 Laplacian FP64, 96^3 complex
+    Anderson) where the oracle is too slow: size-independent properties.
+    Every returned pair has ||A v - lambda v|| <= tol ||A||; the Ritz
+    vectors are orthonormal to 1e-12; eigenvalues lie inside the Gershgorin
+    bound; a second solve from a different start vector returns the same
+    eigenvalues within 2 tol ||A|| (they are converged to tol)."""
+    T = gpu_lib
+    if cplx:
+        rp, ci, v = W.anderson3d_csr(96, seed=0)
+        lohi, ned, maxlan, mev = -1, 24, 80, 24
+    else:
+        rp, ci, v = W.laplace3d_csr(128, seed=0)
+        lohi, ned, maxlan, mev = 1, 16, 64, 16
+    tol = 1e-8
+    n = len(rp) - 1
+    print(f"full-size {'c128' if cplx else 'f64'} n={n}", flush=True)
+    A = T.Csr(rp, ci, v, complex_=cplx)
+    runs = []
+    for s in (1, 2):
+        x0 = W.start_vector(n, seed=s, complex_=cplx)
+        runs.append(T.solve_csr(A, lohi, ned, maxlan, tol, 1, 40000, mev, x0))
+    A.close()
+    f = W.csr_matvec(rp, ci, v)
+    bound = _csr_norm_bound(rp, ci, v)
+    for g in runs:
+        assert g["stat"] == 0 and g["nec"] == ned
+        X = g["evec"][:, :ned]
+        assert np.abs(X.conj().T @ X - np.eye(ned)).max() <= 1e-12
+        for i in range(ned):
+            r = f(X[:, i]) - g["eval"][i] * X[:, i]
+            assert np.linalg.norm(r) <= 1.01 * tol * g["anrm"]
+        assert np.abs(g["eval"][:ned]).max() <= bound * (1 + 1e-12)
+    e1, e2 = (np.sort(g["eval"][:ned]) for g in runs)
+    assert np.abs(e1 - e2).max() <= 2 * tol * runs[0]["anrm"]
