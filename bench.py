@@ -1,16 +1,22 @@
 """Throughput of the device-resident thick-restart Lanczos solver on one GPU.
 
-Workload (one step = one complete trlan() solve, inputs resident in HBM):
+Workload (one step = one complete trlan() solve, CSR resident in HBM):
   periodic 3D 7-point Laplacian on 128^3 (n = 2,097,152, nnz = 14,680,064),
   off-diagonal -1, diagonal 6 + U[0,1) (seed 0), FP64 CSR on the device;
   nev = 16 largest, maxlan = 64, tol = 1e-8, restart scheme 1,
-  start vector U[0,1) (seed 1), plain FP64 path.
-value = eigenpairs with ||A v - lambda v|| <= 1e-8 ||A|| per second.
+  start vector U[0,1) (seed 1).
+value = eigenpairs with ||A v - lambda v|| <= 1e-8 ||A|| (checked on the
+host after the timed solves) per second, for the path given by --path
+(default: mixed precision); the other path is timed the same way and
+reported beside it, with the ratio and the largest eigenvalue difference.
+Each solve is synchronous (trlan() returns after its last device
+synchronisation), so the wall clock around the K solves brackets all GPU
+work.
 
-Prints one JSON line (see README/DESIGN for the fields), including
+Prints one JSON line, including
   roofline: the dominant kernel's algorithmic bytes / HIP-event time vs the
             8 TB/s HBM peak, plus the SpMV + recurrence + Gram-Schmidt
-            aggregate;
+            aggregate, and the rocprofv3-measured HBM traffic per launch;
   cpu_baseline: the compiled reference trlan (oracle/_ref, OpenBLAS +
             OpenMP CSR operator) on a bounded sample of the same solve.
 """
@@ -33,9 +39,13 @@ STATS_EVERY = 7
 METRIC = "eigenpairs/sec at resid<=1e-8||A||: mixed+learned-start vs plain FP64, 1 GPU"
 
 
-# device kernel behind each timed kernel class of the FP64 path
-KERNEL_NAMES = {"proj": "k_proj<double, 1, 8, 4>", "update": "k_update<double, 1, 4>",
-                "spmv": "k_spmv<double, double, 1>"}
+# device kernel behind each timed kernel class, per solver path
+KERNEL_NAMES = {
+    "plain": {"proj": "k_proj<double, double, 1, 8, 4>", "update": "k_update<double, double, 1, 4>",
+              "spmv": "k_spmv_vec<double, double, 1, 8, 4, true>"},
+    "mixed": {"proj": "k_proj<float, double, 1, 8, 4>", "update": "k_update<float, double, 1, 4>",
+              "spmv": "k_spmv_vec<double, double, 1, 8, 4, true>"},
+}
 
 
 def measured_traffic(kernel):
@@ -100,11 +110,35 @@ def cpu_baseline(sample_matvec, full_matvec, nthreads):
         f"the full solve's {full_matvec} applications")
 
 
+def leg(T, A, x0, steps, rp, ci, v, stats_every=0):
+    """Time `steps` complete solves on operator A (inputs resident); returns
+    (seconds, last result, eigenpairs at resid <= 1e-8 ||A||, stats)."""
+    def solve():
+        return T.solve_csr(A, 1, NEV, MAXLAN, TOL, RESTART, MAXMV, NEV, x0)
+
+    if stats_every:
+        T.stats_enable(True, every=stats_every)
+    t0 = time.perf_counter()
+    outs = [solve() for _ in range(steps)]
+    t1 = time.perf_counter()
+    stats = T.stats_get() if stats_every else None
+    if stats_every:
+        T.stats_enable(False)
+    g = outs[-1]
+    assert g["stat"] == 0, g["stat"]
+    nec = g["nec"]
+    res = residuals(rp, ci, v, g["evec"][:, :nec], g["eval"][:nec])
+    good = int((res <= 1e-8 * g["anrm"]).sum())
+    return t1 - t0, g, good, stats
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--path", choices=["mixed", "plain"], default="mixed",
+                    help="solver path measured for `value` (the other is reported beside it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=96)
     args = ap.parse_args()
@@ -120,30 +154,22 @@ def main():
     assert T.device_count() > 0, "no GPU"
     rp, ci, v, x0 = build_problem()
     n = len(rp) - 1
-    A = T.Csr(rp, ci, v)
-
-    def solve():
-        return T.solve_csr(A, 1, NEV, MAXLAN, TOL, RESTART, MAXMV, NEV, x0)
-
+    ops = {"plain": T.Csr(rp, ci, v), "mixed": T.Csr(rp, ci, v, mixed=True)}
+    other = "plain" if args.path == "mixed" else "mixed"
     for _ in range(args.warmup):
-        solve()
-    # one launch in STATS_EVERY of each kernel kind (a hashed, non-periodic
-    # pick) carries HIP start/stop events stamped from its own dispatch on
-    # the solver's stream; timing every launch would cost ~10% of the solve.
-    T.stats_enable(True, every=STATS_EVERY)
-    t0 = time.perf_counter()
-    outs = [solve() for _ in range(args.steps)]
-    t1 = time.perf_counter()
-    stats = T.stats_get()
-    T.stats_enable(False)
-    g = outs[-1]
-    assert g["stat"] == 0, g["stat"]
-    nec = g["nec"]
-    res = residuals(rp, ci, v, g["evec"][:, :nec], g["eval"][:nec])
-    anrm = g["anrm"]
-    good = int((res <= 1e-8 * anrm).sum())
-    elapsed = t1 - t0
+        for k in ops:
+            T.solve_csr(ops[k], 1, NEV, MAXLAN, TOL, RESTART, MAXMV, NEV, x0)
+    # the path beside the headline one, timed on its own (no events)
+    o_el, o_g, o_good, _ = leg(T, ops[other], x0, args.steps, rp, ci, v)
+    # headline path: one launch in STATS_EVERY of each kernel kind (a hashed,
+    # non-periodic pick) carries HIP start/stop events stamped from its own
+    # dispatch on the solver's stream; timing every launch would cost ~10%.
+    elapsed, g, good, stats = leg(T, ops[args.path], x0, args.steps, rp, ci, v, STATS_EVERY)
     value = good * args.steps / elapsed
+    other_value = o_good * args.steps / o_el
+    nec = g["nec"]
+    dl = float(np.abs(np.sort(g["eval"][:nec]) - np.sort(o_g["eval"][:o_g["nec"]])).max()
+               / g["anrm"]) if o_g["nec"] == nec else None
 
     # roofline: dominant kernel by time, plus the path aggregate
     dom = max(stats, key=lambda k: stats[k][1])
@@ -156,7 +182,7 @@ def main():
     roofline = {
         "bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(KERNEL_NAMES.get(dom)),
+        "traffic": measured_traffic(KERNEL_NAMES[args.path].get(dom)),
         "avg_launch_us": round(ms * 1e3 / max(launches, 1), 2),
         "bytes_per_launch": round(byts / max(launches, 1)),
         "path_spmv_recur_cgs": {"achieved": round(path_gbs, 1),
@@ -168,10 +194,14 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         nthreads = min(os.cpu_count() or 1, 16)
-        cb = cpu_baseline(args.cpu_sample, g["matvec"], nthreads)
+        cb = cpu_baseline(args.cpu_sample, o_g["matvec"] if args.path == "mixed" else g["matvec"],
+                          nthreads)
         if cb is not None:
             cpu = {"value": round(cb[0], 4), "unit": "eigenpairs/s", "cores": nthreads,
                    "kind": "reference", "sample": cb[2]}
+    desc = {"mixed": "mixed: FP32 shadow basis for Gram-Schmidt, FP64 SpMV/recurrence/basis/"
+                     "restart GEMM, FP64 Rayleigh-quotient refinement",
+            "plain": "plain FP64 device basis"}
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -187,14 +217,23 @@ def main():
         "data": "synthetic",
         "config": {"workload": "laplace3d_128_periodic_fp64", "n": n, "nnz": int(len(ci)),
                    "nev": NEV, "lohi": 1, "maxlan": MAXLAN, "tol": TOL, "restart": RESTART,
-                   "path": "plain FP64 device basis", "matvec": g["matvec"],
-                   "nloop": g["nloop"], "north": g["north"], "converged": nec,
-                   "converged_at_1e-8": good},
+                   "path": desc[args.path], "learned_start": "none (random start; the learned "
+                   "start applies to the batched 2D Anderson models)",
+                   "matvec": g["matvec"], "nloop": g["nloop"], "north": g["north"],
+                   "converged": nec, "converged_at_1e-8": good},
+        other + "_path": {"value": round(other_value, 3),
+                          "ms_per_step": round(1e3 * o_el / args.steps, 2),
+                          "matvec": o_g["matvec"], "converged_at_1e-8": o_good,
+                          "path": desc[other]},
+        "mixed_vs_plain": round((value / other_value) if args.path == "mixed"
+                                else (other_value / value), 3),
+        "max_eigenvalue_diff_vs_" + other: dl,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    A.close()
+    for A in ops.values():
+        A.close()
 
 
 if __name__ == "__main__":

@@ -26,7 +26,9 @@ extern "C" {
 enum {
     TRLHIP_R64 = 0,   /* real FP64 values / basis       */
     TRLHIP_C128 = 1,  /* complex FP64 (interleaved)     */
-    TRLHIP_R32 = 2    /* real FP32 values / basis       */
+    TRLHIP_R32 = 2    /* real, mixed precision: FP64 values, basis and
+                         recurrence; FP32 shadow of the basis for
+                         Gram-Schmidt                    */
 };
 
 enum {
@@ -46,11 +48,14 @@ typedef struct trlhip_ctx trlhip_ctx;   /* device basis + workspace  */
 /* ---- device CSR operator ------------------------------------------ */
 
 /* Copy a CSR matrix (row_ptr[n+1], col_idx[nnz] int32 in [0, n)) to the
- * device.  values: nnz doubles for TRLHIP_R64, nnz interleaved complex
- * doubles for TRLHIP_C128; for TRLHIP_R32 nnz doubles that are kept in
- * FP64 and also rounded to an FP32 copy (the FP32 copy drives a
- * mixed-precision solve, the FP64 one its refinement).  Invalid structure
- * (decreasing row_ptr, column out of range) is rejected with EINVAL. */
+ * device.  values: nnz doubles for TRLHIP_R64 and TRLHIP_R32, nnz
+ * interleaved complex doubles for TRLHIP_C128.  TRLHIP_R32 marks the
+ * operator for the mixed-precision solver: trlan() then reorthogonalizes
+ * against an FP32 shadow of the basis (half the Gram-Schmidt traffic),
+ * keeps the recurrence, basis and restart in FP64, and refines each
+ * returned pair by an FP64 Rayleigh quotient whose true residual it
+ * reports in wrk.  Invalid structure (decreasing row_ptr, column out of
+ * range) is rejected with EINVAL. */
 int trlhip_csr_create(trlhip_csr **A, int n, long nnz, const int *row_ptr,
                       const int *col_idx, const void *values, int vtype);
 int trlhip_csr_destroy(trlhip_csr *A);

@@ -11,6 +11,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -427,4 +428,75 @@ void trlkit_smooth_rr(int n, double *rr)
         prev = t;
     }
     rr[n - 1] = 2 * rr[n - 1] + rr[1 < n ? 1 : 0] + prev;
+}
+
+/* --------------------------------------------------------- checkpoints */
+
+static int ck_close(FILE *f, int ok_code, int err_code)
+{
+    return fclose(f) == 0 ? ok_code : err_code;
+}
+
+int trlkit_write_checkpoint(const char *file, int nrow, int jnd, const double *alpha,
+                            const double *beta, int ncol,
+                            int (*col)(void *u, int i, double *buf), void *u, double *buf)
+{
+    FILE *f = fopen(file, "w");
+    if (!f) {
+        printf("TRL_WRITE_CHECKPOINT: failed to open file: %s.\n", file);
+        return -221;
+    }
+    if (fwrite(&nrow, sizeof nrow, 1, f) != 1 || fwrite(&jnd, sizeof jnd, 1, f) != 1 ||
+        (jnd > 0 && fwrite(alpha, sizeof(double), (size_t)jnd, f) != (size_t)jnd) ||
+        (jnd > 0 && fwrite(beta, sizeof(double), (size_t)jnd, f) != (size_t)jnd))
+        return ck_close(f, -223, -222);
+    for (int i = 0; i < ncol; ++i) {
+        if (col(u, i, buf) != 0 || fwrite(buf, sizeof(double), (size_t)nrow, f) != (size_t)nrow)
+            return ck_close(f, -223, -222);
+    }
+    return ck_close(f, 0, -223);
+}
+
+int trlkit_read_checkpoint(const char *file, int nrow, int mev, int maxjnd, int *jnd_out,
+                           double *alpha, double *beta,
+                           int (*col)(void *u, int i, double *buf), void *u, double *buf)
+{
+    FILE *f = fopen(file, "r");
+    if (!f) {
+        printf("TRL_READ_CHECKPOINT: failed to open check-point file %s.\n", file);
+        return -212;
+    }
+    int fn = 0, jnd = 0;
+    if (fread(&fn, sizeof fn, 1, f) != 1 || fread(&jnd, sizeof jnd, 1, f) != 1)
+        return ck_close(f, -215, -216);
+    if (fn != nrow) {
+        printf("TRL_READ_CHECKPOINT: Nrow mismatch.\n");
+        (void)fclose(f);
+        return -213;
+    }
+    if (jnd > maxjnd || jnd < 0) {
+        printf("TRL_READ_CHECKPOINT: MAXLAN too small.");
+        (void)fclose(f);
+        return -214;
+    }
+    if (jnd > 0 && (fread(alpha, sizeof(double), (size_t)jnd, f) != (size_t)jnd ||
+                    fread(beta, sizeof(double), (size_t)jnd, f) != (size_t)jnd))
+        return ck_close(f, -215, -216);
+    const int j1 = jnd < mev ? jnd : mev;
+    const int ncol = j1 < mev ? j1 + 1 : jnd;
+    for (int i = 0; i < ncol; ++i) {
+        if (fread(buf, sizeof(double), (size_t)nrow, f) != (size_t)nrow || col(u, i, buf) != 0)
+            return ck_close(f, -215, -216);
+    }
+    *jnd_out = jnd;
+    return ck_close(f, 0, -215);
+}
+
+void trlkit_pe_filename(char *out, size_t cap, const char *base)
+{
+    size_t len = strcspn(base, " ");
+    if (len + 2 > cap) len = cap > 2 ? cap - 2 : 0;
+    memcpy(out, base, len);
+    out[len] = '0';
+    out[len + 1] = '\0';
 }

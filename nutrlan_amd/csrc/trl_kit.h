@@ -10,6 +10,8 @@
 #ifndef NUTRLAN_AMD_TRL_KIT_H
 #define NUTRLAN_AMD_TRL_KIT_H
 
+#include <stddef.h>
+
 #include "trlan.h"
 
 #ifdef __cplusplus
@@ -60,6 +62,29 @@ void trlkit_sort_eig(int nd, int lohi, int nec, double ref, double *lambda,
 
 /* The reference's trl_smooth_rr (SRC/trlcore.c:1534-1575), end quirks kept. */
 void trlkit_smooth_rr(int n, double *rr);
+
+/* Checkpoint files, the reference's format (SRC/trlaux.c:780-1014): int
+ * nrow, int jnd, alpha[jnd], beta[jnd], then columns of nrow doubles,
+ * native byte order.  Columns are streamed through a callback so they can
+ * come from (or go to) device memory: col(u, i, buf) fills buf with column
+ * i for writing / stores buf as column i when reading, returning 0.
+ *
+ * Write ncol columns (the reference writes jnd + 1: the basis and the
+ * residual).  Returns 0, -221 (open), -222 (write) or -223 (close). */
+int trlkit_write_checkpoint(const char *file, int nrow, int jnd, const double *alpha,
+                            const double *beta, int ncol,
+                            int (*col)(void *u, int i, double *buf), void *u, double *buf);
+/* Read like the reference's trl_read_checkpoint: check nrow (-213) and
+ * jnd <= maxjnd (-214), read alpha/beta[jnd]; with j1 = min(mev, jnd) read
+ * j1 + 1 columns when j1 < mev, else jnd columns (the residual then is not
+ * in the file's prefix that is read).  *jnd_out = jnd.  Returns 0, -212
+ * (open), -213, -214, -215/-216 (short read). */
+int trlkit_read_checkpoint(const char *file, int nrow, int mev, int maxjnd, int *jnd_out,
+                           double *alpha, double *beta,
+                           int (*col)(void *u, int i, double *buf), void *u, double *buf);
+/* The reference's per-process file name (SRC/trlaux.c:1067-1104) for one
+ * process: base up to its first blank, then rank 0 as one digit. */
+void trlkit_pe_filename(char *out, size_t cap, const char *base);
 
 #ifdef __cplusplus
 }

@@ -20,9 +20,11 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <ucontext.h>
 
 #include "trl_kit.h"
 #include "trlan.h"
+#include "trlan_batch.h"
 #include "trlan_internal.h"
 #include "ztrlan.h"
 #include "trlhip.h"
@@ -63,6 +65,7 @@ typedef struct {
     ztrl_matvec zop;       /* complex Hermitian operator (ztrlan) */
     trl_uo uo;
     int cplx, E;           /* complex basis?  doubles per element */
+    int mixed;             /* FP32 shadow basis for Gram-Schmidt (TRLHIP_R32 operator) */
     int n, mev, nbas, ncols, maxlan;
     double *alpha, *beta, *alfrot, *betrot; /* maxlan each */
     double *rot;                            /* maxlan^2 */
@@ -190,15 +193,20 @@ static int lz_adopt(lz_t *z, int rcol, int *src)
     return 0;
 }
 
+/* The reference's trl_cgs (SRC/trlcore.c:2913-3117) on the device basis.
+ * Mixed precision (FP32 shadow basis): the projections are accurate to
+ * FP32 rounding of the basis, so the pass/stop test uses FLT_EPSILON and
+ * at most two passes run (CGS2). */
 static int lz_cgs(lz_t *z, int nold, int rcol, double *rnrm, int *north, const double *pass1)
 {
-    const double eps = DBL_EPSILON;
+    const double eps = z->mixed ? FLT_EPSILON : DBL_EPSILON;
+    const int maxpass = z->mixed ? 1 : 3;
     int irnd = 0, src = rcol;
     double rr2 = 0.0, hh = 0.0;
     if (z->cplx) return lz_cgs_complex(z, nold, rcol, rnrm, north, pass1);
     if (nold > 0) {
         int cnt = 0;
-        while (cnt <= 3) {
+        while (cnt <= maxpass) {
             if (pass1) {
                 hh = pass1[0];
                 rr2 = pass1[1];
@@ -508,6 +516,44 @@ static int lz_get_tvec_a(lz_t *z, int nd, int kept, double *alpha, const double 
 /* Initial vector in column nec (reference trl_initial_guess,
  * SRC/trlcore.c:1577-1763, guess <= 1).  Random numbers come from this
  * library's own generator (the reference seeds drand48 from clock()). */
+/* Column streaming between checkpoint files and the device basis. */
+typedef struct {
+    lz_t *z;
+    int base;   /* read: first basis column to fill */
+    int jnd;    /* write: columns 0..jnd-1, then column rcol */
+    int rcol;
+} lz_colio_t;
+
+static int lz_put_col(void *u, int i, double *buf)
+{
+    lz_colio_t *io = (lz_colio_t *)u;
+    return trlhip_col_set(io->z->dev, io->base + i, buf);
+}
+
+static int lz_get_col(void *u, int i, double *buf)
+{
+    lz_colio_t *io = (lz_colio_t *)u;
+    return trlhip_col_get(io->z->dev, i < io->jnd ? i : io->rcol, buf);
+}
+
+/* The reference's write_checkpoint (SRC/trlcore.c:434-460): alpha/beta of
+ * the jnd basis vectors, the vectors and the residual (column rcol) to
+ * cpfile + process number.  Real bases only. */
+static void lz_write_checkpoint(lz_t *z, int jnd, int rcol)
+{
+    trl_info *info = z->info;
+    char file[160];
+    trlkit_pe_filename(file, sizeof file, info->cpfile);
+    lz_colio_t io = {z, 0, jnd, rcol};
+    const clock_t c1 = clock();
+    const int ii = trlkit_write_checkpoint(file, z->n, jnd, z->alpha, z->beta, jnd + 1, lz_get_col,
+                                           &io, z->hx);
+    const clock_t c2 = clock();
+    info->clk_out += c2 >= c1 ? c2 - c1 : 0;
+    info->wrds_out += jnd * (z->n + z->n + 2) + z->n + 2;
+    info->stat = ii;
+}
+
 static int lz_initial_guess(lz_t *z, const double *evec, int lde, int *j1)
 {
     trl_info *info = z->info;
@@ -515,8 +561,26 @@ static int lz_initial_guess(lz_t *z, const double *evec, int lde, int *j1)
     double *x = z->hx;
     const int E = z->E;
     if (info->guess > 1) {
-        info->stat = TRLAN_STAT_NO_CHECKPOINT;
-        return info->stat;
+        /* restore a checkpoint (SRC/trlcore.c:1653-1679): oldcpf if set,
+         * else cpfile; the basis continues from its saved state */
+        if (z->cplx) {
+            info->stat = TRLAN_STAT_NO_CHECKPOINT;
+            return info->stat;
+        }
+        char file[160];
+        trlkit_pe_filename(file, sizeof file, strlen(info->oldcpf) > 0 ? info->oldcpf : info->cpfile);
+        lz_colio_t io = {z, j, 0, 0};
+        int jnd = 0;
+        const clock_t c1 = clock();
+        const int rc = trlkit_read_checkpoint(file, z->n, z->mev - j, z->ncols - 1 - j, &jnd,
+                                              z->alpha + j, z->beta + j, lz_put_col, &io, x);
+        const clock_t c2 = clock();
+        info->clk_in = c2 >= c1 ? c2 - c1 : 0;
+        info->wrds_in = jnd * (z->n + z->n + 2) + z->n + 2;
+        info->stat = rc;
+        if (rc) return rc;
+        *j1 = jnd + j;
+        return 0;
     }
     if (info->guess == 1) {
         memcpy(x, evec + (size_t)j * lde * E, sizeof(double) * E * (size_t)z->n);
@@ -648,6 +712,34 @@ static int lz_matvec_step(lz_t *z, int first, int qa, int qb, double beta_prev, 
     return 0;
 }
 
+/* Final FP64 Rayleigh-quotient refinement of a mixed-precision solve:
+ * for each of the k Ritz vectors v in device columns 0..k-1,
+ * eval = v'Av / v'v with the FP64 operator, and beta = ||A v - eval v|| /
+ * ||v|| (the true residual, reported in wrk instead of the estimate).
+ * Column `avoid` (the last residual) is kept; another free column holds A v.
+ * One operator application per pair, counted in matvec. */
+static int lz_refine(lz_t *z, int k, int avoid, double *eval, double *beta)
+{
+    int tmp = k;
+    while (tmp == avoid) ++tmp;
+    if (tmp >= z->ncols || !z->A) {
+        z->info->stat = TRLHIP_STAT_DEVICE;
+        return TRLHIP_STAT_DEVICE;
+    }
+    for (int i = 0; i < k; ++i) {
+        double vav = 0.0, vv = 0.0, rr2 = 0.0;
+        DEV(trlhip_spmv(z->dev, z->A, i, tmp, &vav));
+        DEV(trlhip_proj(z->dev, i, 1, i, &vv));
+        z->info->matvec++;
+        if (!(vv > 0.0)) continue;
+        const double rho = vav / vv;
+        DEV(trlhip_update(z->dev, i, 1, &rho, tmp, &rr2));
+        eval[i] = rho;
+        beta[i] = sqrt(rr2 / vv);
+    }
+    return 0;
+}
+
 /* The thick-restart iteration; mirrors the decision sequence of the
  * reference's trlanczos (SRC/trlcore.c:723-1532). */
 static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk, int lwrk)
@@ -670,6 +762,9 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
     int jnd = j1, kept = jnd;
     int i1 = info->ned - jnd;
     int next_test = i1 + DMIN(i1, DMIN(6, info->ned / 2));
+    /* checkpoints every maxmv/cpflag operator applications (real bases) */
+    const int ckpt = info->cpflag > 0 && !z->cplx;
+    int chkpnt = ckpt ? info->maxmv / info->cpflag : info->maxmv + info->maxlan;
     int rr = jnd, qa = -1, qb = -1, i2 = 0;
     while (info->matvec < info->maxmv && info->nec < info->ned) {
         /* ---- first step of a cycle: r = A q - sum beta_i q_i */
@@ -785,6 +880,11 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
                 return;
             }
             rr = jnd;
+            if (ckpt && info->matvec >= chkpnt) {
+                lz_write_checkpoint(z, jnd, rr);
+                if (info->stat != 0) return;
+                chkpnt += info->maxmv / info->cpflag;
+            }
         } else {
             kept = DMIN(info->nec, DMAX(info->ned, mev - 1));
             info->nec = kept;
@@ -808,6 +908,7 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
                 return;
             }
             info->flop += 2 * z->n * jml * kept;
+            if (z->mixed && lz_refine(z, kept, rr, eval, beta) != 0) return;
             jnd = kept;
             j1 = kept;
             /* Ritz vectors (and the last residual after them) to the caller */
@@ -819,6 +920,10 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
             if (kept < mev && trlhip_col_get(z->dev, rr, evec + (size_t)kept * lde * z->E)) {
                 info->stat = TRLHIP_STAT_DEVICE;
                 return;
+            }
+            if (ckpt) {
+                lz_write_checkpoint(z, kept, rr);
+                if (info->stat != 0) return;
             }
         }
         add_ticks(info, &info->clk_res, &info->tick_r, tr);
@@ -905,54 +1010,184 @@ static void check_and_clear(trl_info *info, int nrow, int mev, int lde)
 /* Shared entry of trlan()/ztrlan(): the reference's trlan()
  * (SRC/trlan.c:21-301) / ztrlan() (SRC/ztrlan.c:26-221) bookkeeping around
  * one solve on a fresh device basis. */
-void trl_solve_device(trl_matvec op, ztrl_matvec zop, trl_uo user_ortho, trl_info *info,
-                      int nrow, int mev, double *eval, double *evec, int lde, int lwrk,
-                      double *wrk, int cplx)
+/* The reference's trlan() bookkeeping before trlanczos (SRC/trlan.c:
+ * 21-301): argument checks, sizes, host state and a fresh device basis
+ * (in batch b when given).  Returns 1 when there is an iteration to run;
+ * 0 when done early or on error (info->stat). */
+static int lz_setup(lz_t *z, trl_matvec op, ztrl_matvec zop, trl_uo user_ortho, trl_info *info,
+                    int nrow, int mev, int lde, int cplx, trlhip_batch *b)
 {
-    const clock_t t_start = clock();
-    info->clk_tot = t_start;
+    memset(z, 0, sizeof *z);
+    info->clk_tot = clock();
     if (info->ned > mev) {
         printf("info->ned (%d) is larger than mev (%d) reducing info->ned to %d\n", info->ned,
                mev, mev);
         info->ned = mev;
     }
-    if (info->ned <= info->nec || info->ned <= 0) return;
+    if (info->ned <= info->nec || info->ned <= 0) return 0;
     info->stat = 0;
     check_and_clear(info, nrow, mev, lde);
-    if (info->stat != 0) return;
-
-    lz_t z;
-    memset(&z, 0, sizeof z);
-    z.info = info;
-    z.op = op;
-    z.zop = zop;
-    z.uo = user_ortho;
-    z.cplx = cplx;
-    z.E = cplx ? 2 : 1;
-    z.n = nrow;
-    z.mev = mev;
-    z.maxlan = info->maxlan;
-    z.nbas = DMAX(1, info->maxlan - mev + 1);
-    z.ncols = DMAX(info->maxlan, mev) + 1;
-    z.rng = 0x9E3779B97F4A7C15ull;
-    const int vt = cplx ? TRLHIP_C128 : TRLHIP_R64;
-    z.A = trlhip_csr_from_op(cplx ? (void (*)(void))zop : (void (*)(void))op, info->mvparam, vt);
-    if (lz_alloc(&z, DMAX(info->maxlan, mev)) != 0) {
+    if (info->stat != 0) return 0;
+    z->info = info;
+    z->op = op;
+    z->zop = zop;
+    z->uo = user_ortho;
+    z->cplx = cplx;
+    z->E = cplx ? 2 : 1;
+    z->n = nrow;
+    z->mev = mev;
+    z->maxlan = info->maxlan;
+    z->nbas = DMAX(1, info->maxlan - mev + 1);
+    z->ncols = DMAX(info->maxlan, mev) + 1;
+    z->rng = 0x9E3779B97F4A7C15ull;
+    z->A = trlhip_csr_from_op(cplx ? (void (*)(void))zop : (void (*)(void))op, info->mvparam,
+                              cplx ? TRLHIP_C128 : TRLHIP_R64);
+    z->mixed = !cplx && trlhip_csr_is_mixed(z->A);
+    const int vt = cplx ? TRLHIP_C128 : (z->mixed ? TRLHIP_R32 : TRLHIP_R64);
+    if (lz_alloc(z, DMAX(info->maxlan, mev)) != 0) {
         info->stat = -4;
-        lz_free(&z);
-        return;
+        lz_free(z);
+        return 0;
     }
-    if (trlhip_ctx_create(&z.dev, nrow, z.ncols, vt) != 0) {
+    if (trlhip_ctx_create_batched(&z->dev, nrow, z->ncols, vt, b) != 0) {
         info->stat = TRLHIP_STAT_DEVICE;
-        lz_free(&z);
-        return;
+        lz_free(z);
+        return 0;
     }
-    lz_iterate(&z, eval, evec, lde, wrk, lwrk);
-    trlhip_ctx_destroy(z.dev);
-    lz_free(&z);
+    return 1;
+}
+
+static void lz_teardown(lz_t *z)
+{
+    trl_info *info = z->info;
+    trlhip_ctx_destroy(z->dev);
+    lz_free(z);
     const clock_t t_end = clock();
-    info->tick_t += (double)(t_end - t_start) / (double)info->clk_rate;
+    info->tick_t += (double)(t_end - info->clk_tot) / (double)info->clk_rate;
     info->clk_tot = 0;
+}
+
+/* Shared entry of trlan()/ztrlan(): the reference's trlan()
+ * (SRC/trlan.c:21-301) / ztrlan() (SRC/ztrlan.c:26-221) bookkeeping around
+ * one solve on a fresh device basis. */
+void trl_solve_device(trl_matvec op, ztrl_matvec zop, trl_uo user_ortho, trl_info *info,
+                      int nrow, int mev, double *eval, double *evec, int lde, int lwrk,
+                      double *wrk, int cplx)
+{
+    lz_t z;
+    if (!lz_setup(&z, op, zop, user_ortho, info, nrow, mev, lde, cplx, NULL)) return;
+    lz_iterate(&z, eval, evec, lde, wrk, lwrk);
+    lz_teardown(&z);
+}
+
+/* ------------------------------------------------------------- batches
+ *
+ * trlan_batch runs many trlan() solves together.  Each solve is the same
+ * lz_iterate as above, run as a coroutine (ucontext) on its own stack;
+ * wherever it must wait for the device it yields to the scheduler below.
+ * Once every unfinished solve has yielded, the scheduler flushes the
+ * batch: the recorded Lanczos steps of all solves run as one launch per
+ * phase, the shared stream is synchronized once, and every solve resumes
+ * with its results.  Each solve makes exactly the decisions it would make
+ * alone, so its eigenpairs and counters match a single trlan() call. */
+
+enum { LZ_STACK = 256 * 1024 };
+
+typedef struct {
+    ucontext_t uc;
+    void *stack;
+    lz_t z;
+    trl_batch_job *job;
+    int done;
+} lz_co_t;
+
+typedef struct {
+    ucontext_t sched;
+    lz_co_t *cur;
+} lz_sched_t;
+
+static lz_sched_t *g_sched;  /* the running scheduler (one batch at a time per process) */
+
+static void lz_co_yield(void *user)
+{
+    lz_sched_t *sc = (lz_sched_t *)user;
+    lz_co_t *co = sc->cur;
+    swapcontext(&co->uc, &sc->sched);
+}
+
+static void lz_co_main(void)
+{
+    lz_co_t *co = g_sched->cur;
+    trl_batch_job *j = co->job;
+    lz_iterate(&co->z, j->eval, j->evec, j->lde, j->wrk, j->lwrk);
+    co->done = 1;
+    /* returning resumes uc_link, the scheduler */
+}
+
+int trlan_batch(trl_batch_job *jobs, int njobs)
+{
+    if (njobs <= 0) return 0;
+    if (!jobs || g_sched) return TRLHIP_EINVAL;
+    trlhip_batch *b = NULL;
+    int rc = trlhip_batch_create(&b);
+    if (rc) {
+        for (int i = 0; i < njobs; ++i) jobs[i].info->stat = TRLHIP_STAT_DEVICE;
+        return rc;
+    }
+    lz_sched_t sc;
+    memset(&sc, 0, sizeof sc);
+    trlhip_batch_set_yield(b, lz_co_yield, &sc);
+    lz_co_t *cos = (lz_co_t *)calloc((size_t)njobs, sizeof(lz_co_t));
+    if (!cos) {
+        trlhip_batch_destroy(b);
+        return TRLHIP_ENOMEM;
+    }
+    g_sched = &sc;
+    int live = 0;
+    for (int i = 0; i < njobs; ++i) {
+        trl_batch_job *j = &jobs[i];
+        lz_co_t *co = &cos[i];
+        co->job = j;
+        co->done = 1;
+        j->info->mvparam = j->A;
+        if (!lz_setup(&co->z, trlhip_csr_matvec, NULL, NULL, j->info, j->nrow, j->mev, j->lde, 0,
+                      b))
+            continue;
+        co->stack = malloc(LZ_STACK);
+        if (!co->stack || getcontext(&co->uc) != 0) {
+            j->info->stat = -4;
+            lz_teardown(&co->z);
+            continue;
+        }
+        co->uc.uc_stack.ss_sp = co->stack;
+        co->uc.uc_stack.ss_size = LZ_STACK;
+        co->uc.uc_link = &sc.sched;
+        makecontext(&co->uc, lz_co_main, 0);
+        co->done = 0;
+        ++live;
+    }
+    while (live > 0) {
+        for (int i = 0; i < njobs; ++i) {
+            lz_co_t *co = &cos[i];
+            if (co->done) continue;
+            sc.cur = co;
+            swapcontext(&sc.sched, &co->uc);
+            if (co->done) {
+                lz_teardown(&co->z);
+                free(co->stack);
+                co->stack = NULL;
+                --live;
+            }
+        }
+        if (live > 0) {
+            const int frc = trlhip_batch_flush(b);
+            if (frc && !rc) rc = frc;
+        }
+    }
+    g_sched = NULL;
+    free(cos);
+    trlhip_batch_destroy(b);
+    return rc;
 }
 
 void trlan(trl_matvec op, trl_uo user_ortho, trl_info *info, int nrow, int mev, double *eval,

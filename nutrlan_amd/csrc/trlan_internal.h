@@ -24,12 +24,22 @@ extern "C" {
  * handle of that type. */
 const trlhip_csr *trlhip_csr_from_op(void (*op)(void), void *mvparam, int vtype);
 
-/* 1 when A carries an FP32 value copy for a mixed-precision solve. */
+/* 1 when A was created for the mixed-precision solver (TRLHIP_R32). */
 int trlhip_csr_is_mixed(const trlhip_csr *A);
 
 /* Uniform [0,1) from a 64-bit xorshift state (the library's own generator;
  * the reference uses drand48 seeded from clock()). */
 double trl_rand_uniform(uint64_t *state);
+
+/* A batch of contexts that step together (trlhip.hip, batch section):
+ * their device waits yield through `yield(user)` to a scheduler, which
+ * calls trlhip_batch_flush once every waiting context has yielded. */
+typedef struct trlhip_batch trlhip_batch;
+int trlhip_batch_create(trlhip_batch **b);
+void trlhip_batch_destroy(trlhip_batch *b);
+void trlhip_batch_set_yield(trlhip_batch *b, void (*yield)(void *user), void *user);
+int trlhip_batch_flush(trlhip_batch *b);
+int trlhip_ctx_create_batched(trlhip_ctx **c, int n, int ncols, int vtype, trlhip_batch *b);
 
 /* One thick-restart solve on a device basis; cplx selects ztrlan (zop,
  * evec/wrk interleaved complex) or trlan (op).  Implemented in trlan.c. */

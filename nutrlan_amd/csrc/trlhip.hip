@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "trlan_internal.h"
@@ -133,14 +134,14 @@ __global__ __launch_bounds__(kBlock) void k_sum(const double *__restrict__ part,
 // o1.  With a host mirror (pinned, host-coherent memory), the results and
 // the scalars `dscal` already holds are also written there, so the host
 // reads them after a stream synchronize without a copy.
-__global__ __launch_bounds__(kBlock) void k_sum2(const double *__restrict__ p0, int n0,
+__device__ __forceinline__ void sum2_body(const double *__restrict__ p0, int n0,
                                                  double *__restrict__ o0,
                                                  const double *__restrict__ p1, int n1,
                                                  double *__restrict__ o1,
                                                  const double *__restrict__ dscal,
-                                                 double *__restrict__ hmirror)
+                                                 double *__restrict__ hmirror, int bid, int nblk)
 {
-    const bool second = blockIdx.x == 1;
+    const bool second = bid == 1;
     const double t = ordered_sum(second ? p1 : p0, second ? n1 : n0);
     if (threadIdx.x == 0) {
         *(second ? o1 : o0) = t;
@@ -156,13 +157,29 @@ __global__ __launch_bounds__(kBlock) void k_sum2(const double *__restrict__ p0, 
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_sum2(const double *__restrict__ p0, int n0,
+                                                 double *__restrict__ o0,
+                                                 const double *__restrict__ p1, int n1,
+                                                 double *__restrict__ o1,
+                                                 const double *__restrict__ dscal,
+                                                 double *__restrict__ hmirror)
+{
+    sum2_body(p0, n0, o0, p1, n1, o1, dscal, hmirror, blockIdx.x, gridDim.x);
+}
+
 // Finalize a vector reduction: hout[p] = sum_b part[p * nb + b], p < np
 // (each coefficient's nb partials are contiguous).  One block per p.
+__device__ __forceinline__ void colsum_body(const double *__restrict__ part, int nb,
+                                                   double *__restrict__ hout, int bid, int nblk)
+{
+    const double t = ordered_sum(part + (size_t)bid * nb, nb);
+    if (threadIdx.x == 0) hout[bid] = t;
+}
+
 __global__ __launch_bounds__(kBlock) void k_colsum(const double *__restrict__ part, int nb,
                                                    double *__restrict__ hout)
 {
-    const double t = ordered_sum(part + (size_t)blockIdx.x * nb, nb);
-    if (threadIdx.x == 0) hout[blockIdx.x] = t;
+    colsum_body(part, nb, hout, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------- SpMV
@@ -257,20 +274,21 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
 // with a butterfly over the lane group; no LDS and no barriers.  FUSE: x
 // is fl(xs * xscale) formed on the fly from the unscaled source xs, and the
 // kernel also stores that scaled vector to xq (the basis column it
-// normalises), so the separate scale-copy pass disappears.  Epilogue: the
-// block's partial of Re(x^H y) in part[blockIdx.x].
+// normalises), so the separate scale-copy pass disappears; xq32 (real
+// only, may be null) receives its FP32 shadow.  Epilogue: the block's
+// partial of Re(x^H y) in part[blockIdx.x].
 template <typename S, typename SV, int E, int LPR, int U, bool FUSE>
-__global__ __launch_bounds__(kBlock) void k_spmv_vec(
+__device__ __forceinline__ void spmv_vec_body(
     int n, const int *__restrict__ rp, const int *__restrict__ ci, const SV *__restrict__ val,
     const S *__restrict__ x, S *__restrict__ y, double *__restrict__ part, double xscale,
-    S *__restrict__ xq)
+    S *__restrict__ xq, float *__restrict__ xq32, int bid, int nblk)
 {
     static_assert(LPR >= 1 && LPR <= 64 && (LPR & (LPR - 1)) == 0, "LPR power of 2");
     constexpr int RPW = 64 / LPR;
     const int lane = threadIdx.x & 63;
     const int sub = lane % LPR, grp = lane / LPR;
-    const long wave = (long)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    const long nw = (long)gridDim.x * (kBlock / 64);
+    const long wave = (long)bid * (kBlock / 64) + (threadIdx.x >> 6);
+    const long nw = (long)nblk * (kBlock / 64);
     auto xval = [&](int c, int e) -> double {
         if constexpr (FUSE)
             return (double)(S)((double)x[(size_t)E * c + e] * xscale);
@@ -334,7 +352,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv_vec(
                 if constexpr (E == 1) {
                     const double xr = xval((int)row, 0);
                     y[row] = (S)sr[u];
-                    if constexpr (FUSE) xq[row] = (S)xr;
+                    if constexpr (FUSE) {
+                        if (xq) xq[row] = (S)xr;
+                        if (xq32) xq32[row] = (float)xr;
+                    }
                     acc += xr * (double)(S)sr[u];
                 } else {
                     const double xr = xval((int)row, 0), xi = xval((int)row, 1);
@@ -350,7 +371,16 @@ __global__ __launch_bounds__(kBlock) void k_spmv_vec(
         }
     }
     const double bs = block_sum(acc);
-    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+    if (threadIdx.x == 0) part[bid] = bs;
+}
+
+template <typename S, typename SV, int E, int LPR, int U, bool FUSE>
+__global__ __launch_bounds__(kBlock) void k_spmv_vec(
+    int n, const int *__restrict__ rp, const int *__restrict__ ci, const SV *__restrict__ val,
+    const S *__restrict__ x, S *__restrict__ y, double *__restrict__ part, double xscale,
+    S *__restrict__ xq, float *__restrict__ xq32)
+{
+    spmv_vec_body<S, SV, E, LPR, U, FUSE>(n, rp, ci, val, x, y, part, xscale, xq, xq32, blockIdx.x, gridDim.x);
 }
 
 // Re(x^H y) over ne doubles: per-block partials in part[blockIdx.x].
@@ -375,11 +405,12 @@ __global__ __launch_bounds__(kBlock) void k_dot(long ne, const S *__restrict__ x
 // CB*E running sums, which are transposed-and-reduced over the wave and
 // combined over waves in LDS; the block writes its partial of coefficient p
 // to part[p*gx + rb] (k_colsum sums each coefficient's partials in row
-// block order).
-template <typename S, int E, int CB, int RU>
-__global__ __launch_bounds__(kBlock) void k_proj(
-    int n, const S *__restrict__ V, long ld, int c0, int m, const S *__restrict__ r,
-    double *__restrict__ part, int gx, int gy)
+// block order).  VS = basis storage (float for the FP32 shadow of a
+// mixed-precision basis), RS = storage of r.
+template <typename VS, typename RS, int E, int CB, int RU>
+__device__ __forceinline__ void proj_body(
+    int n, const VS *__restrict__ V, long ld, int c0, int m, const RS *__restrict__ r,
+    double *__restrict__ part, int gx, int gy, int bid, int nblk)
 {
     constexpr int N = CB * E;
     static_assert(N <= 64, "chunk too wide");
@@ -391,17 +422,17 @@ __global__ __launch_bounds__(kBlock) void k_proj(
     // HBM once and then served by that XCD's L2.
     int rb, chunk;
     if ((gx & 7) == 0) {
-        const int local = blockIdx.x >> 3;
-        rb = (local / gy) * 8 + (blockIdx.x & 7);
+        const int local = bid >> 3;
+        rb = (local / gy) * 8 + (bid & 7);
         chunk = local % gy;
     } else {
-        rb = blockIdx.x / gy;
-        chunk = blockIdx.x % gy;
+        rb = bid / gy;
+        chunk = bid % gy;
     }
     const int cb = chunk * CB;
     const int last = min(CB, m - cb) - 1;
     // wave-uniform column bases; rows index them
-    const S *col[CB];
+    const VS *col[CB];
 #pragma unroll
     for (int c = 0; c < CB; ++c) col[c] = V + ((size_t)(c0 + cb + min(c, last)) * ld) * E;
     double acc[N];
@@ -477,14 +508,22 @@ __global__ __launch_bounds__(kBlock) void k_proj(
     }
 }
 
+template <typename VS, typename RS, int E, int CB, int RU>
+__global__ __launch_bounds__(kBlock) void k_proj(
+    int n, const VS *__restrict__ V, long ld, int c0, int m, const RS *__restrict__ r,
+    double *__restrict__ part, int gx, int gy)
+{
+    proj_body<VS, RS, E, CB, RU>(n, V, ld, c0, m, r, part, gx, gy, blockIdx.x, gridDim.x);
+}
+
 // Rows row + j*kBlock (j < R) of rout = rin - Vc hs; returns the sum of
 // their |rout|^2.  Per row the products are summed into four interleaved
 // partial sums (column c into sum c mod 4 within each group of 8 columns,
 // E == 1; pairs of columns, E == 2), the same order for any R; R rows at a
 // time put R*8 (R*4) column loads in flight per thread.
-template <typename S, int E, int R>
-__device__ __forceinline__ double update_rows(int row, const S *__restrict__ Vc, long ld, int m,
-                                              const double *hs, const S *rin, S *rout)
+template <typename VS, typename RS, int E, int R>
+__device__ __forceinline__ double update_rows(int row, const VS *__restrict__ Vc, long ld, int m,
+                                              const double *hs, const RS *rin, RS *rout)
 {
     double nrm = 0.0;
     if constexpr (E == 1) {
@@ -509,7 +548,7 @@ __device__ __forceinline__ double update_rows(int row, const S *__restrict__ Vc,
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             const int rr = row + j * kBlock;
-            const S rn = (S)((double)rin[rr] - ((s[j][0] + s[j][1]) + (s[j][2] + s[j][3])));
+            const RS rn = (RS)((double)rin[rr] - ((s[j][0] + s[j][1]) + (s[j][2] + s[j][3])));
             rout[rr] = rn;
             nrm += (double)rn * (double)rn;
         }
@@ -565,12 +604,13 @@ __device__ __forceinline__ double update_rows(int row, const S *__restrict__ Vc,
 // SpMV/dot left as per-block partials); every block forms that sum in the
 // same order, and block 0 also stores it to out_coef.  Without h the
 // coefficients are (h0, 0, ..., 0) apart from cslot.  A block takes RU
-// rows of its grid stride per iteration.  rin may equal rout.
-template <typename S, int E, int RU>
-__global__ __launch_bounds__(kBlock) void k_update(
-    int n, const S *__restrict__ V, long ld, int c0, int m, const double *__restrict__ h, double h0,
-    const double *__restrict__ cpart, int cnp, int cslot, const S *rin, S *rout,
-    double *__restrict__ part, double *__restrict__ out_hh, double *__restrict__ out_coef)
+// rows of its grid stride per iteration.  rin may equal rout.  VS = basis
+// storage, RS = storage of r.
+template <typename VS, typename RS, int E, int RU>
+__device__ __forceinline__ void update_body(
+    int n, const VS *__restrict__ V, long ld, int c0, int m, const double *__restrict__ h, double h0,
+    const double *__restrict__ cpart, int cnp, int cslot, const RS *rin, RS *rout,
+    double *__restrict__ part, double *__restrict__ out_hh, double *__restrict__ out_coef, int bid, int nblk)
 {
     extern __shared__ double hs[];
     for (int i = threadIdx.x; i < E * m; i += kBlock) hs[i] = h ? h[i] : (i == 0 ? h0 : 0.0);
@@ -579,34 +619,49 @@ __global__ __launch_bounds__(kBlock) void k_update(
         if (threadIdx.x == 0) {
             hs[E * cslot] = a;
             if (E == 2) hs[E * cslot + 1] = 0.0;
-            if (blockIdx.x == 0 && out_coef) out_coef[0] = a;
+            if (bid == 0 && out_coef) out_coef[0] = a;
         }
     }
     __syncthreads();
-    const S *Vc = V + (size_t)c0 * ld * E;
+    const VS *Vc = V + (size_t)c0 * ld * E;
     double acc = 0.0;
-    const int stride = gridDim.x * kBlock * RU;
-    int row = blockIdx.x * kBlock * RU + threadIdx.x;
+    const int stride = nblk * kBlock * RU;
+    int row = bid * kBlock * RU + threadIdx.x;
     for (; row + (RU - 1) * kBlock < n; row += stride)
-        acc += update_rows<S, E, RU>(row, Vc, ld, m, hs, rin, rout);
+        acc += update_rows<VS, RS, E, RU>(row, Vc, ld, m, hs, rin, rout);
 #pragma unroll
     for (int j = 0; j < RU - 1; ++j)
-        if (row + j * kBlock < n) acc += update_rows<S, E, 1>(row + j * kBlock, Vc, ld, m, hs, rin, rout);
+        if (row + j * kBlock < n)
+            acc += update_rows<VS, RS, E, 1>(row + j * kBlock, Vc, ld, m, hs, rin, rout);
     const double bs = block_sum(acc);
-    if (threadIdx.x == 0) part[blockIdx.x] = bs;
-    if (out_hh && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (threadIdx.x == 0) part[bid] = bs;
+    if (out_hh && bid == 0 && threadIdx.x == 0) {
         double t = 0.0;
         for (int i = 0; i < E * m; ++i) t += hs[i] * hs[i];
         out_hh[0] = t;
     }
 }
 
-// dst = s * src over ne storage elements (dst may equal src).
-template <typename S>
-__global__ __launch_bounds__(kBlock) void k_scale_copy(long ne, const S *src, S *dst, double s)
+template <typename VS, typename RS, int E, int RU>
+__global__ __launch_bounds__(kBlock) void k_update(
+    int n, const VS *__restrict__ V, long ld, int c0, int m, const double *__restrict__ h, double h0,
+    const double *__restrict__ cpart, int cnp, int cslot, const RS *rin, RS *rout,
+    double *__restrict__ part, double *__restrict__ out_hh, double *__restrict__ out_coef)
 {
-    for (long i = blockIdx.x * kBlock + threadIdx.x; i < ne; i += (long)gridDim.x * kBlock)
-        dst[i] = (S)((double)src[i] * s);
+    update_body<VS, RS, E, RU>(n, V, ld, c0, m, h, h0, cpart, cnp, cslot, rin, rout, part, out_hh, out_coef, blockIdx.x, gridDim.x);
+}
+
+// dst = s * src over ne storage elements (dst may equal src); with dst32
+// also its FP32 rounding (the shadow column of a mixed-precision basis).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_scale_copy(long ne, const S *src, S *dst, double s,
+                                                       float *__restrict__ dst32)
+{
+    for (long i = blockIdx.x * kBlock + threadIdx.x; i < ne; i += (long)gridDim.x * kBlock) {
+        const S v = (S)((double)src[i] * s);
+        dst[i] = v;
+        if (dst32) dst32[i] = (float)v;
+    }
 }
 
 // ------------------------------------------------------------- restart
@@ -616,16 +671,17 @@ __global__ __launch_bounds__(kBlock) void k_scale_copy(long ne, const S *src, S 
 // disjoint across waves), reads its rows over all jl input columns into
 // the accumulators before writing any output column, so the update is safe
 // in place.  Y (jl x k) is staged in LDS as Ys[kk * kp + col], padded with
-// zeros to jl4 = ceil4(jl) rows and kp = 16*KT columns.  FP32 bases are
-// widened to FP64 on load and rounded on store.
+// zeros to jl4 = ceil4(jl) rows and kp = 16*KT columns.  With V32 (the
+// FP32 shadow slab of a mixed-precision basis, same ld) each output is
+// also stored rounded to FP32.
 // MFMA f64 16x16x4 operand layout (lane l): A[l%16][l/16], B[l/16][l%16],
 // D[l/16 + 4*r][l%16] for accumulator element r (measured on gfx950).
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-template <typename S, int KT>
+template <int KT>
 __global__ __launch_bounds__(kBlock) void k_ritz_mfma(
-    long nrows, S *__restrict__ V, long ld, int lck, int jl, int k, const double *__restrict__ Y,
-    int ldy, int tiles_per_wave)
+    long nrows, double *__restrict__ V, long ld, int lck, int jl, int k, const double *__restrict__ Y,
+    int ldy, int tiles_per_wave, float *__restrict__ V32)
 {
     extern __shared__ double Ys[];
     constexpr int kp = 16 * KT;
@@ -637,7 +693,7 @@ __global__ __launch_bounds__(kBlock) void k_ritz_mfma(
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int li = lane & 15, lq = lane >> 4;
-    S *Vin = V + (size_t)lck * ld;
+    double *Vin = V + (size_t)lck * ld;
     for (int t = 0; t < tiles_per_wave; ++t) {
         const long row0 = ((long)(blockIdx.x * (kBlock / 64) + w) * tiles_per_wave + t) * 16;
         if (row0 >= nrows) break;
@@ -647,7 +703,7 @@ __global__ __launch_bounds__(kBlock) void k_ritz_mfma(
         const long arow = row0 + li;
         for (int kk = 0; kk < jl4; kk += 4) {
             const int col = kk + lq;
-            const double a = (arow < nrows && col < jl) ? (double)Vin[(size_t)col * ld + arow] : 0.0;
+            const double a = (arow < nrows && col < jl) ? Vin[(size_t)col * ld + arow] : 0.0;
 #pragma unroll
             for (int q = 0; q < KT; ++q) {
                 const double b = Ys[(kk + lq) * kp + q * 16 + li];
@@ -660,7 +716,10 @@ __global__ __launch_bounds__(kBlock) void k_ritz_mfma(
 #pragma unroll
             for (int rr = 0; rr < 4; ++rr) {
                 const long row = row0 + lq + 4 * rr;
-                if (row < nrows && col < k) Vin[(size_t)col * ld + row] = (S)acc[q][rr];
+                if (row < nrows && col < k) {
+                    Vin[(size_t)col * ld + row] = acc[q][rr];
+                    if (V32) V32[(size_t)(lck + col) * ld + row] = (float)acc[q][rr];
+                }
             }
         }
     }
@@ -781,7 +840,6 @@ struct trlhip_csr {
     int *rp;
     int *ci;
     double *val;    // FP64 values (E per nonzero)
-    float *val32;   // FP32 copy for a mixed-precision basis (R32 only)
     // staging for host-called matvec
     double *dx, *dy, *part, *dscal;
     hipStream_t s;
@@ -795,6 +853,7 @@ struct trlhip_ctx {
     int E;          // doubles per element (complex 2)
     size_t esz;     // bytes per element
     char *V;
+    float *V32;     // FP32 shadow of the basis (mixed precision), same ld
     double *part;
     size_t partcap; // doubles
     int np_alpha;   // partial count of the last SpMV / dot
@@ -812,6 +871,32 @@ struct trlhip_ctx {
     size_t hcap;
     double *hY;     // pinned staging for Y
     hipStream_t s;
+    trlhip_batch *batch;  // batch the context steps in (its stream is the batch's), or null
+};
+
+// One recorded Lanczos step of a batched context (see the batch section).
+struct BatchStep {
+    trlhip_ctx *c;
+    const trlhip_csr *A;
+    int first, qa, qb, rcol, jnd;
+    double beta_prev;
+    int fuse;         // q_a is the deferred scale-copy of the scratch column
+    double fuse_s;
+};
+
+namespace {
+struct BDesc;  // defined with the batched kernels
+}
+
+struct trlhip_batch {
+    hipStream_t s = nullptr;
+    void (*yield)(void *) = nullptr;
+    void *user = nullptr;
+    int err = 0;                  // sticky device error of a flush
+    std::vector<BatchStep> steps;
+    BDesc *hdesc = nullptr, *ddesc = nullptr;   // pinned staging / device copy
+    int *hpre = nullptr, *dpre = nullptr;       // BP_COUNT prefix tables
+    size_t cap = 0;               // requests the buffers hold
 };
 
 namespace {
@@ -826,7 +911,6 @@ int grid_for(long work, int per_block_min, int cap)
     return (int)g;
 }
 
-bool is_f32(const trlhip_ctx *c) { return c->vtype == TRLHIP_R32; }
 
 template <typename S>
 S *colp(trlhip_ctx *c, int col)
@@ -868,18 +952,18 @@ int spmv_grid(const trlhip_csr *A)
     return A->nb < kMaxGrid ? A->nb : kMaxGrid;
 }
 
-template <typename S, typename SV, int E, bool FUSE>
-void launch_vec(const trlhip_csr *A, const SV *val, const void *x, void *y, double *part,
-                hipStream_t s, double xscale, void *xq)
+template <int E, bool FUSE>
+void launch_vec(const trlhip_csr *A, const void *x, void *y, double *part, hipStream_t s,
+                double xscale, void *xq, float *xq32)
 {
     constexpr int U = E == 1 ? 4 : 2;
     const dim3 g(A->vgrid), b(kBlock);
-    const S *xs = (const S *)x;
-    S *ys = (S *)y, *q = (S *)xq;
-#define VEC_CASE(L)                                                                          \
-    case L:                                                                                  \
-        KLAUNCH((k_spmv_vec<S, SV, E, L, U, FUSE>), g, b, 0, s, A->n, A->rp, A->ci, val, xs, \
-                ys, part, xscale, q);                                                        \
+    const double *xs = (const double *)x;
+    double *ys = (double *)y, *q = (double *)xq;
+#define VEC_CASE(L)                                                                            \
+    case L:                                                                                    \
+        KLAUNCH((k_spmv_vec<double, double, E, L, U, FUSE>), g, b, 0, s, A->n, A->rp, A->ci,   \
+                A->val, xs, ys, part, xscale, q, xq32);                                        \
         break;
     switch (A->lpr) {
         VEC_CASE(2) VEC_CASE(4) VEC_CASE(8) VEC_CASE(16) VEC_CASE(32)
@@ -887,36 +971,32 @@ void launch_vec(const trlhip_csr *A, const SV *val, const void *x, void *y, doub
 #undef VEC_CASE
 }
 
-// y = A x on stream s with the kernel chosen for the matrix; partials in
-// part.  With xq (vector-CSR kernel only) x is an unscaled source: the
-// product uses fl(x * xscale), which is also stored to xq.
+// y = A x on stream s with the kernel chosen for the matrix (FP64 values
+// and vectors; complex when vtype is TRLHIP_C128); partials in part.  With
+// xq (vector-CSR kernel only) x is an unscaled source: the product uses
+// fl(x * xscale), which is also stored to xq (and rounded to xq32).
 int launch_spmv(const trlhip_csr *A, int vtype, const void *x, void *y, double *part, hipStream_t s,
-                double xscale = 1.0, void *xq = nullptr)
+                double xscale = 1.0, void *xq = nullptr, float *xq32 = nullptr)
 {
+    const bool cplx = vtype == TRLHIP_C128;
     if (A->lpr) {
-        if (vtype == TRLHIP_R64) {
-            if (xq) launch_vec<double, double, 1, true>(A, A->val, x, y, part, s, xscale, xq);
-            else launch_vec<double, double, 1, false>(A, A->val, x, y, part, s, xscale, xq);
-        } else if (vtype == TRLHIP_C128) {
-            if (xq) launch_vec<double, double, 2, true>(A, A->val, x, y, part, s, xscale, xq);
-            else launch_vec<double, double, 2, false>(A, A->val, x, y, part, s, xscale, xq);
+        if (cplx) {
+            if (xq) launch_vec<2, true>(A, x, y, part, s, xscale, xq, nullptr);
+            else launch_vec<2, false>(A, x, y, part, s, xscale, xq, nullptr);
         } else {
-            if (xq) launch_vec<float, float, 1, true>(A, A->val32, x, y, part, s, xscale, xq);
-            else launch_vec<float, float, 1, false>(A, A->val32, x, y, part, s, xscale, xq);
+            if (xq) launch_vec<1, true>(A, x, y, part, s, xscale, xq, xq32);
+            else launch_vec<1, false>(A, x, y, part, s, xscale, xq, nullptr);
         }
         return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
     }
     if (xq) return TRLHIP_EINVAL;
     const dim3 g(spmv_grid(A)), b(kBlock);
-    if (vtype == TRLHIP_R64)
-        KLAUNCH((k_spmv<double, double, 1>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
-                           A->val, (const double *)x, (double *)y, part);
-    else if (vtype == TRLHIP_C128)
-        KLAUNCH((k_spmv<double, double, 2>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
-                           A->val, (const double *)x, (double *)y, part);
+    if (cplx)
+        KLAUNCH((k_spmv<double, double, 2>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci, A->val,
+                (const double *)x, (double *)y, part);
     else
-        KLAUNCH((k_spmv<float, float, 1>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
-                           A->val32, (const float *)x, (float *)y, part);
+        KLAUNCH((k_spmv<double, double, 1>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci, A->val,
+                (const double *)x, (double *)y, part);
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
@@ -930,32 +1010,37 @@ bool csr_fits(const trlhip_ctx *c, const trlhip_csr *A)
 {
     if (!A || A->magic != kCsrMagic || A->n != c->n) return false;
     if (c->vtype == TRLHIP_C128) return A->vtype == TRLHIP_C128;
-    if (c->vtype == TRLHIP_R32) return A->val32 != nullptr;
     return A->vtype == TRLHIP_R64 || A->vtype == TRLHIP_R32;
+}
+
+// FP32 shadow of basis column col (mixed-precision contexts), else null.
+float *shadow32(trlhip_ctx *c, int col)
+{
+    return c->V32 ? c->V32 + (size_t)col * c->ld : nullptr;
 }
 
 // y col = A x col; Re(x^H y) left as partials in part_alpha (c->np_alpha).
 int run_spmv(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol)
 {
-    const size_t vsz = is_f32(c) ? 4 : 8;
     c->np_alpha = spmv_grid(A);
-    Timed t(TRLHIP_K_SPMV, c->s, spmv_bytes(A, c->E, vsz, vsz));
+    Timed t(TRLHIP_K_SPMV, c->s, spmv_bytes(A, c->E, 8, 8));
     return launch_spmv(A, c->vtype, c->V + (size_t)xcol * c->ld * c->esz,
                        c->V + (size_t)ycol * c->ld * c->esz, part_alpha(c), c->s);
 }
 
 // The deferred scale-copy into column xcol fused into the SpMV that uses
-// it: y = A (s * scratch), and s * scratch stored to column xcol.
+// it: y = A (s * scratch), and s * scratch stored to column xcol (and to
+// its FP32 shadow in a mixed-precision context).
 int run_spmv_fused(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol)
 {
-    const size_t vsz = is_f32(c) ? 4 : 8;
     c->np_alpha = spmv_grid(A);
     const double s = c->pend_s;
     c->pend_dst = -1;
-    Timed t(TRLHIP_K_SPMV, c->s, spmv_bytes(A, c->E, vsz, vsz) + (double)c->esz * c->n);
+    Timed t(TRLHIP_K_SPMV, c->s,
+            spmv_bytes(A, c->E, 8, 8) + (double)c->esz * c->n + (c->V32 ? 4.0 * c->n : 0.0));
     return launch_spmv(A, c->vtype, c->V + (size_t)c->ncols * c->ld * c->esz,
                        c->V + (size_t)ycol * c->ld * c->esz, part_alpha(c), c->s, s,
-                       c->V + (size_t)xcol * c->ld * c->esz);
+                       c->V + (size_t)xcol * c->ld * c->esz, shadow32(c, xcol));
 }
 
 // Re(x^H y) of two columns as partials in part_alpha.
@@ -965,24 +1050,20 @@ int run_dot(trlhip_ctx *c, int xcol, int ycol)
     const int g = grid_for(ne, kBlock * 8, kMaxGrid);
     c->np_alpha = g;
     Timed t(TRLHIP_K_MISC, c->s, 2.0 * (double)c->esz * c->n);
-    if (is_f32(c))
-        KLAUNCH((k_dot<float>), dim3(g), dim3(kBlock), 0, c->s, ne, colp<float>(c, xcol),
-                           colp<float>(c, ycol), part_alpha(c));
-    else
-        KLAUNCH((k_dot<double>), dim3(g), dim3(kBlock), 0, c->s, ne,
-                           colp<double>(c, xcol), colp<double>(c, ycol), part_alpha(c));
+    KLAUNCH((k_dot<double>), dim3(g), dim3(kBlock), 0, c->s, ne, colp<double>(c, xcol),
+            colp<double>(c, ycol), part_alpha(c));
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
-template <typename S, int E, int RU>
-void launch_update(trlhip_ctx *c, int g, int c0, int m, const double *dh, double h0, int aslot,
-                   int rin, int rout, double *o_hh, double *pdst)
+template <typename VS, int E, int RU>
+void launch_update(trlhip_ctx *c, int g, const VS *V, int c0, int m, const double *dh, double h0,
+                   int aslot, int rin, int rout, double *o_hh, double *pdst)
 {
     const size_t shm = sizeof(double) * (size_t)c->E * (m > 0 ? m : 1);
     const double *cp = aslot >= 0 ? part_alpha(c) : nullptr;
-    KLAUNCH((k_update<S, E, RU>), dim3(g), dim3(kBlock), shm, c->s, c->n, colp<S>(c, 0),
-                       (long)c->ld, c0, m, dh, h0, cp, c->np_alpha, aslot, colp<S>(c, rin),
-                       colp<S>(c, rout), pdst, o_hh, c->dscal + S_ALPHA);
+    KLAUNCH((k_update<VS, double, E, RU>), dim3(g), dim3(kBlock), shm, c->s, c->n, V, (long)c->ld,
+            c0, m, dh, h0, cp, c->np_alpha, aslot, colp<double>(c, rin), colp<double>(c, rout),
+            pdst, o_hh, c->dscal + S_ALPHA);
 }
 
 // rout = rin - V(:, c0:c0+m) dh; coefficient aslot (if >= 0) is the alpha
@@ -990,32 +1071,37 @@ void launch_update(trlhip_ctx *c, int g, int c0, int m, const double *dh, double
 // and ||dh||^2 -> o_hh when requested.  dh == nullptr: coefficients
 // (h0, 0, ...) apart from aslot.  The per-block partials of ||rout||^2 go
 // to pdst (default part_update) and their count to *g_out when given.
+// shadow: a Gram-Schmidt update, which reads the FP32 shadow of V in a
+// mixed-precision context (r itself stays FP64).
 int run_update(trlhip_ctx *c, int c0, int m, const double *dh, int aslot, int rin, int rout,
                double *o_rr2, double *o_hh, int kind, double h0 = 0.0, double *pdst = nullptr,
-               int *g_out = nullptr)
+               int *g_out = nullptr, bool shadow = false)
 {
     const int g = grid_for(c->n, kBlock * 4, kMaxGrid);
     if (!pdst) pdst = part_update(c);
     if (g_out) *g_out = g;
+    const bool s32 = shadow && c->V32;
     {
-        Timed t(kind, c->s, (double)c->esz * c->n * (m + 2));
-        if (is_f32(c))
-            launch_update<float, 1, 4>(c, g, c0, m, dh, h0, aslot, rin, rout, o_hh, pdst);
+        Timed t(kind, c->s, s32 ? (4.0 * m + 16.0) * c->n : (double)c->esz * c->n * (m + 2));
+        if (s32)
+            launch_update<float, 1, 4>(c, g, c->V32, c0, m, dh, h0, aslot, rin, rout, o_hh, pdst);
         else if (c->E == 1)
-            launch_update<double, 1, 4>(c, g, c0, m, dh, h0, aslot, rin, rout, o_hh, pdst);
+            launch_update<double, 1, 4>(c, g, colp<double>(c, 0), c0, m, dh, h0, aslot, rin, rout,
+                                        o_hh, pdst);
         else
-            launch_update<double, 2, 4>(c, g, c0, m, dh, h0, aslot, rin, rout, o_hh, pdst);
+            launch_update<double, 2, 4>(c, g, colp<double>(c, 0), c0, m, dh, h0, aslot, rin, rout,
+                                        o_hh, pdst);
         if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
     }
     return o_rr2 ? launch_sum(pdst, g, o_rr2, c->s) : 0;
 }
 
-template <typename S, int E, int CB, int RU>
-void launch_proj(trlhip_ctx *c, int gx, int c0, int m, int rcol)
+template <typename VS, int E, int CB, int RU>
+void launch_proj(trlhip_ctx *c, int gx, const VS *V, int c0, int m, int rcol)
 {
     const int gy = (m + CB - 1) / CB;
-    KLAUNCH((k_proj<S, E, CB, RU>), dim3(gx * gy), dim3(kBlock), 0, c->s, c->n, colp<S>(c, 0),
-            (long)c->ld, c0, m, colp<S>(c, rcol), part_proj(c), gx, gy);
+    KLAUNCH((k_proj<VS, double, E, CB, RU>), dim3(gx * gy), dim3(kBlock), 0, c->s, c->n, V,
+            (long)c->ld, c0, m, colp<double>(c, rcol), part_proj(c), gx, gy);
 }
 
 // Row blocks of the projection: about kProjBlocks blocks over all column
@@ -1029,24 +1115,26 @@ int proj_gx(const trlhip_ctx *c, int m, int cb)
     return g >= 8 ? g & ~7 : g;
 }
 
-// hout = V(:, c0:c0+m)^H r (device, E*m doubles)
-int run_proj(trlhip_ctx *c, int c0, int m, int rcol, double *hout)
+// hout = V(:, c0:c0+m)^H r (device, E*m doubles).  shadow: a Gram-Schmidt
+// projection, which reads the FP32 shadow of V in a mixed-precision context.
+int run_proj(trlhip_ctx *c, int c0, int m, int rcol, double *hout, bool shadow = false)
 {
     if (m <= 0) return 0;
+    const bool s32 = shadow && c->V32;
     int g;
     {
-        Timed t(TRLHIP_K_PROJ, c->s, (double)c->esz * c->n * (m + 1));
+        Timed t(TRLHIP_K_PROJ, c->s, s32 ? (4.0 * m + 8.0) * c->n : (double)c->esz * c->n * (m + 1));
         // chunk widths and row unrolling from a sweep at n = 2^21, m = 16..64
         // (about 5.7 TB/s for both element types)
-        if (is_f32(c)) {
+        if (s32) {
             g = proj_gx(c, m, 8);
-            launch_proj<float, 1, 8, 4>(c, g, c0, m, rcol);
+            launch_proj<float, 1, 8, 4>(c, g, c->V32, c0, m, rcol);
         } else if (c->E == 1) {
             g = proj_gx(c, m, 8);
-            launch_proj<double, 1, 8, 4>(c, g, c0, m, rcol);
+            launch_proj<double, 1, 8, 4>(c, g, colp<double>(c, 0), c0, m, rcol);
         } else {
             g = proj_gx(c, m, 4);
-            launch_proj<double, 2, 4, 4>(c, g, c0, m, rcol);
+            launch_proj<double, 2, 4, 4>(c, g, colp<double>(c, 0), c0, m, rcol);
         }
         if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
     }
@@ -1055,17 +1143,15 @@ int run_proj(trlhip_ctx *c, int c0, int m, int rcol, double *hout)
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
+// dst = s * src (columns; src may be the scratch column); the FP32 shadow
+// of dst follows in a mixed-precision context.
 int run_scale_copy(trlhip_ctx *c, int dst, int src, double s)
 {
     const long ne = (long)c->n * c->E;
     const int g = grid_for(ne, kBlock * 8, kMaxGrid);
     Timed t(TRLHIP_K_MISC, c->s, 2.0 * (double)c->esz * c->n);
-    if (is_f32(c))
-        KLAUNCH((k_scale_copy<float>), dim3(g), dim3(kBlock), 0, c->s, ne,
-                           colp<float>(c, src), colp<float>(c, dst), s);
-    else
-        KLAUNCH((k_scale_copy<double>), dim3(g), dim3(kBlock), 0, c->s, ne,
-                           colp<double>(c, src), colp<double>(c, dst), s);
+    KLAUNCH((k_scale_copy<double>), dim3(g), dim3(kBlock), 0, c->s, ne, colp<double>(c, src),
+            colp<double>(c, dst), s, shadow32(c, dst));
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
@@ -1079,13 +1165,25 @@ int flush_pending(trlhip_ctx *c)
     return run_scale_copy(c, dst, c->ncols, c->pend_s);
 }
 
+// Wait until the context's stream is idle: synchronize it, or, in a batch,
+// yield to the batch scheduler, which synchronizes the shared stream once
+// for all its coroutines (trlhip_batch_flush) before resuming this one.
+int ctx_wait(trlhip_ctx *c)
+{
+    if (c->batch) {
+        c->batch->yield(c->batch->user);
+        return c->batch->err;
+    }
+    TRY(hipStreamSynchronize(c->s));
+    stats_resolve();
+    return 0;
+}
+
 // copy scalars back and wait
 int fetch(trlhip_ctx *c, const double *dsrc, double *hdst, size_t cnt)
 {
     TRY(hipMemcpyAsync(hdst, dsrc, cnt * sizeof(double), hipMemcpyDeviceToHost, c->s));
-    TRY(hipStreamSynchronize(c->s));
-    stats_resolve();
-    return 0;
+    return ctx_wait(c);
 }
 
 bool valid_col(const trlhip_ctx *c, int col) { return col >= 0 && col < c->ncols; }
@@ -1110,6 +1208,235 @@ std::vector<int> row_blocks(int n, const int *rp)
         r = e;
     }
     return rb;
+}
+
+// ------------------------------------------------------------------ batch
+//
+// A batch runs many independent solves (one context each, all on the
+// batch's stream).  Their host drivers are coroutines: where a context
+// must wait for the device it yields instead of synchronizing, and a
+// Lanczos step (trlhip_step_cgs) is only recorded.  The scheduler then
+// calls trlhip_batch_flush, which runs every recorded step with one
+// launch per phase for all problems (block -> problem through a prefix
+// table), synchronizes once, and resumes the coroutines.
+
+// Parameters of one recorded step (device copy, one per request).
+struct BDesc {
+    int n, ld, jnd, np_alpha, g_upd, gx, gy, rm, rc0;
+    const int *rp, *ci;
+    const double *val;
+    const double *x;      // SpMV input: the basis column q_a, or the unscaled scratch column
+    double xscale;        // 1, or the deferred normalisation of the scratch column
+    double *xq;           // fused store of q_a (null when not fused)
+    float *xq32;          // its FP32 shadow (mixed precision)
+    double *r;            // residual column
+    double *scratch;      // speculative Gram-Schmidt output
+    const double *V;      // FP64 basis
+    const void *VG;       // basis the Gram-Schmidt pass reads (FP32 shadow or FP64)
+    const double *rh;     // recurrence coefficients (first step after a restart) or null
+    double rh0;           // beta_prev
+    double *part_alpha, *part_recur, *part_update, *part_proj;
+    double *dh, *dscal, *dres;
+};
+
+enum { BP_SPMV, BP_RECUR, BP_PROJ, BP_COLSUM, BP_UPDATE, BP_SUM2, BP_COUNT };
+
+// Request p owning global block blk: the last p with pre[p] <= blk.
+__device__ __forceinline__ int batch_find(const int *__restrict__ pre, int nreq, int blk)
+{
+    int lo = 0, hi = nreq - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+#define BATCH_LOCATE(phase)                                                   \
+    const int *pre_ = pre + (size_t)(phase) * (nreq + 1);                     \
+    const int p_ = batch_find(pre_, nreq, blockIdx.x);                        \
+    const int bid_ = blockIdx.x - pre_[p_], nblk_ = pre_[p_ + 1] - pre_[p_];  \
+    const BDesc &q = d[p_];
+
+template <int LPR>
+__global__ __launch_bounds__(kBlock) void kb_spmv(const BDesc *__restrict__ d, const int *pre, int nreq)
+{
+    BATCH_LOCATE(BP_SPMV)
+    spmv_vec_body<double, double, 1, LPR, 4, true>(q.n, q.rp, q.ci, q.val, q.x, q.r, q.part_alpha,
+                                                   q.xscale, q.xq, q.xq32, bid_, nblk_);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_recur(const BDesc *__restrict__ d, const int *pre, int nreq)
+{
+    BATCH_LOCATE(BP_RECUR)
+    update_body<double, double, 1, 4>(q.n, q.V, q.ld, q.rc0, q.rm, q.rh, q.rh0, q.part_alpha,
+                                      q.np_alpha, q.rm - 1, q.r, q.r, q.part_recur, nullptr,
+                                      q.dscal + S_ALPHA, bid_, nblk_);
+}
+
+template <typename VS>
+__global__ __launch_bounds__(kBlock) void kb_proj(const BDesc *__restrict__ d, const int *pre, int nreq)
+{
+    BATCH_LOCATE(BP_PROJ)
+    proj_body<VS, double, 1, 8, 4>(q.n, (const VS *)q.VG, q.ld, 0, q.jnd, q.r, q.part_proj, q.gx,
+                                   q.gy, bid_, nblk_);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_colsum(const BDesc *__restrict__ d, const int *pre, int nreq)
+{
+    BATCH_LOCATE(BP_COLSUM)
+    colsum_body(q.part_proj, q.gx, q.dh, bid_, nblk_);
+}
+
+template <typename VS>
+__global__ __launch_bounds__(kBlock) void kb_update(const BDesc *__restrict__ d, const int *pre, int nreq)
+{
+    BATCH_LOCATE(BP_UPDATE)
+    update_body<VS, double, 1, 4>(q.n, (const VS *)q.VG, q.ld, 0, q.jnd, q.dh, 0.0, nullptr, 0, -1,
+                                  q.r, q.scratch, q.part_update, q.dscal + S_HH, nullptr, bid_,
+                                  nblk_);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_sum2(const BDesc *__restrict__ d, const int *pre, int nreq)
+{
+    BATCH_LOCATE(BP_SUM2)
+    sum2_body(q.part_recur, q.g_upd, q.dscal + S_RR2, q.part_update, q.g_upd, q.dscal + S_RR2B,
+              q.dscal, q.dres, bid_, nblk_);
+}
+#undef BATCH_LOCATE
+
+// Runs every recorded step of the batch: one launch per phase over all
+// requests, in the order form_residual + run_proj + run_update + k_sum2 run
+// them for a single context.  Requires vector-CSR operators with one lanes-
+// per-row value and one basis precision across the batch.
+int batch_launch(trlhip_batch *b)
+{
+    const int R = (int)b->steps.size();
+    const BatchStep &s0 = b->steps[0];
+    const int lpr = s0.A->lpr;
+    const bool mixed = s0.c->V32 != nullptr;
+    for (const BatchStep &st : b->steps)
+        if (st.A->lpr != lpr || (st.c->V32 != nullptr) != mixed) return TRLHIP_EINVAL;
+    if ((size_t)R > b->cap) {
+        if (b->hdesc) (void)hipHostFree(b->hdesc);
+        if (b->ddesc) (void)hipFree(b->ddesc);
+        if (b->hpre) (void)hipHostFree(b->hpre);
+        if (b->dpre) (void)hipFree(b->dpre);
+        b->hdesc = b->ddesc = nullptr;
+        b->hpre = b->dpre = nullptr;
+        b->cap = 0;
+        const size_t cap = (size_t)R + R / 2 + 16;
+        const size_t npre = (size_t)BP_COUNT * (cap + 1);
+        if (hipHostMalloc((void **)&b->hdesc, sizeof(BDesc) * cap, 0) != hipSuccess ||
+            hipMalloc((void **)&b->ddesc, sizeof(BDesc) * cap) != hipSuccess ||
+            hipHostMalloc((void **)&b->hpre, sizeof(int) * npre, 0) != hipSuccess ||
+            hipMalloc((void **)&b->dpre, sizeof(int) * npre) != hipSuccess)
+            return TRLHIP_ENOMEM;
+        b->cap = cap;
+    }
+    double bytes[BP_COUNT] = {};
+    int maxrm = 1, maxj = 1;
+    int *pre = b->hpre;
+    for (int ph = 0; ph < BP_COUNT; ++ph) pre[(size_t)ph * (R + 1)] = 0;
+    for (int i = 0; i < R; ++i) {
+        const BatchStep &st = b->steps[i];
+        trlhip_ctx *c = st.c;
+        const trlhip_csr *A = st.A;
+        BDesc &d = b->hdesc[i];
+        const int n = c->n, jnd = st.jnd;
+        d.n = n;
+        d.ld = c->ld;
+        d.jnd = jnd;
+        d.np_alpha = A->vgrid;
+        c->np_alpha = A->vgrid;
+        d.g_upd = grid_for(n, kBlock * 4, kMaxGrid);
+        d.gx = proj_gx(c, jnd, 8);
+        d.gy = (jnd + 7) / 8;
+        if (st.first) {
+            d.rm = jnd;
+            d.rc0 = 0;
+            d.rh = c->dcoef;
+            d.rh0 = 0.0;
+        } else if (st.qb >= 0) {
+            d.rm = 2;
+            d.rc0 = st.qb;
+            d.rh = nullptr;
+            d.rh0 = st.beta_prev;
+        } else {
+            d.rm = 1;
+            d.rc0 = st.qa;
+            d.rh = nullptr;
+            d.rh0 = 0.0;
+        }
+        d.rp = A->rp;
+        d.ci = A->ci;
+        d.val = A->val;
+        d.x = colp<double>(c, st.fuse ? c->ncols : st.qa);
+        d.xscale = st.fuse ? st.fuse_s : 1.0;
+        d.xq = st.fuse ? colp<double>(c, st.qa) : nullptr;
+        d.xq32 = st.fuse ? shadow32(c, st.qa) : nullptr;
+        d.r = colp<double>(c, st.rcol);
+        d.scratch = colp<double>(c, c->ncols);
+        d.V = colp<double>(c, 0);
+        d.VG = mixed ? (const void *)c->V32 : (const void *)colp<double>(c, 0);
+        d.part_alpha = part_alpha(c);
+        d.part_recur = part_recur(c);
+        d.part_update = part_update(c);
+        d.part_proj = part_proj(c);
+        d.dh = c->dh;
+        d.dscal = c->dscal;
+        d.dres = c->dres;
+        const int nb[BP_COUNT] = {A->vgrid, d.g_upd, d.gx * d.gy, jnd, d.g_upd, 2};
+        for (int ph = 0; ph < BP_COUNT; ++ph) {
+            int *pp = pre + (size_t)ph * (R + 1);
+            pp[i + 1] = pp[i] + nb[ph];
+        }
+        const double vs = mixed ? 4.0 : 8.0;
+        bytes[BP_SPMV] += spmv_bytes(A, 1, 8, 8) + (st.fuse ? (mixed ? 12.0 : 8.0) * n : 0.0);
+        bytes[BP_RECUR] += 8.0 * n * (d.rm + 2);
+        bytes[BP_PROJ] += (vs * jnd + 8.0) * n;
+        bytes[BP_UPDATE] += (vs * jnd + 16.0) * n;
+        maxrm = std::max(maxrm, d.rm);
+        maxj = std::max(maxj, jnd);
+    }
+    TRY(hipMemcpyAsync(b->ddesc, b->hdesc, sizeof(BDesc) * (size_t)R, hipMemcpyHostToDevice, b->s));
+    TRY(hipMemcpyAsync(b->dpre, b->hpre, sizeof(int) * (size_t)BP_COUNT * (R + 1),
+                       hipMemcpyHostToDevice, b->s));
+    auto blocks = [&](int ph) { return dim3(pre[(size_t)ph * (R + 1) + R]); };
+    const BDesc *dd = b->ddesc;
+    const int *dp = b->dpre;
+    {
+        Timed t(TRLHIP_K_SPMV, b->s, bytes[BP_SPMV]);
+        switch (lpr) {
+        case 2: KLAUNCH((kb_spmv<2>), blocks(BP_SPMV), dim3(kBlock), 0, b->s, dd, dp, R); break;
+        case 4: KLAUNCH((kb_spmv<4>), blocks(BP_SPMV), dim3(kBlock), 0, b->s, dd, dp, R); break;
+        case 8: KLAUNCH((kb_spmv<8>), blocks(BP_SPMV), dim3(kBlock), 0, b->s, dd, dp, R); break;
+        case 16: KLAUNCH((kb_spmv<16>), blocks(BP_SPMV), dim3(kBlock), 0, b->s, dd, dp, R); break;
+        default: KLAUNCH((kb_spmv<32>), blocks(BP_SPMV), dim3(kBlock), 0, b->s, dd, dp, R); break;
+        }
+    }
+    {
+        Timed t(TRLHIP_K_RECUR, b->s, bytes[BP_RECUR]);
+        KLAUNCH(kb_recur, blocks(BP_RECUR), dim3(kBlock), sizeof(double) * maxrm, b->s, dd, dp, R);
+    }
+    {
+        Timed t(TRLHIP_K_PROJ, b->s, bytes[BP_PROJ]);
+        if (mixed) KLAUNCH((kb_proj<float>), blocks(BP_PROJ), dim3(kBlock), 0, b->s, dd, dp, R);
+        else KLAUNCH((kb_proj<double>), blocks(BP_PROJ), dim3(kBlock), 0, b->s, dd, dp, R);
+    }
+    KLAUNCH(kb_colsum, blocks(BP_COLSUM), dim3(kBlock), 0, b->s, dd, dp, R);
+    {
+        Timed t(TRLHIP_K_UPDATE, b->s, bytes[BP_UPDATE]);
+        if (mixed)
+            KLAUNCH((kb_update<float>), blocks(BP_UPDATE), dim3(kBlock), sizeof(double) * maxj, b->s,
+                    dd, dp, R);
+        else
+            KLAUNCH((kb_update<double>), blocks(BP_UPDATE), dim3(kBlock), sizeof(double) * maxj,
+                    b->s, dd, dp, R);
+    }
+    KLAUNCH(kb_sum2, blocks(BP_SUM2), dim3(kBlock), 0, b->s, dd, dp, R);
+    return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
 }  // namespace
@@ -1193,13 +1520,6 @@ int trlhip_csr_create(trlhip_csr **out, int n, long nnz, const int *row_ptr, con
     if (!rc && hipMemcpy(A->rp, row_ptr, sizeof(int) * (size_t)(n + 1), hipMemcpyHostToDevice) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && nnz > 0 && hipMemcpy(A->ci, col_idx, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && nnz > 0 && hipMemcpy(A->val, values, sizeof(double) * A->E * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess) rc = TRLHIP_EHIP;
-    if (!rc && vtype == TRLHIP_R32) {
-        std::vector<float> v32((size_t)nnz);
-        const double *vd = (const double *)values;
-        for (long k = 0; k < nnz; ++k) v32[(size_t)k] = (float)vd[k];
-        if (hipMalloc((void **)&A->val32, sizeof(float) * nz1) != hipSuccess) rc = TRLHIP_ENOMEM;
-        if (!rc && nnz > 0 && hipMemcpy(A->val32, v32.data(), sizeof(float) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess) rc = TRLHIP_EHIP;
-    }
     if (rc) {
         trlhip_csr_destroy(A);
         return rc;
@@ -1216,7 +1536,6 @@ int trlhip_csr_destroy(trlhip_csr *A)
     if (A->rp) (void)hipFree(A->rp);
     if (A->ci) (void)hipFree(A->ci);
     if (A->val) (void)hipFree(A->val);
-    if (A->val32) (void)hipFree(A->val32);
     if (A->dx) (void)hipFree(A->dx);
     if (A->dy) (void)hipFree(A->dy);
     if (A->part) (void)hipFree(A->part);
@@ -1279,6 +1598,7 @@ int trlhip_ctx_destroy(trlhip_ctx *c)
     if (!c) return 0;
     if (c->s) (void)hipStreamSynchronize(c->s);
     if (c->V) (void)hipFree(c->V);
+    if (c->V32) (void)hipFree(c->V32);
     if (c->part) (void)hipFree(c->part);
     if (c->dscal) (void)hipFree(c->dscal);
     if (c->dcoef) (void)hipFree(c->dcoef);
@@ -1287,12 +1607,17 @@ int trlhip_ctx_destroy(trlhip_ctx *c)
     if (c->hbuf) (void)hipHostFree(c->hbuf);
     if (c->hres) (void)hipHostFree(c->hres);
     if (c->hY) (void)hipHostFree(c->hY);
-    if (c->s) (void)hipStreamDestroy(c->s);
+    if (c->s && !c->batch) (void)hipStreamDestroy(c->s);
     free(c);
     return 0;
 }
 
 int trlhip_ctx_create(trlhip_ctx **out, int n, int ncols, int vtype)
+{
+    return trlhip_ctx_create_batched(out, n, ncols, vtype, nullptr);
+}
+
+int trlhip_ctx_create_batched(trlhip_ctx **out, int n, int ncols, int vtype, trlhip_batch *b)
 {
     if (!out || n <= 0 || ncols <= 0) return TRLHIP_EINVAL;
     if (vtype != TRLHIP_R64 && vtype != TRLHIP_C128 && vtype != TRLHIP_R32) return TRLHIP_EINVAL;
@@ -1304,12 +1629,21 @@ int trlhip_ctx_create(trlhip_ctx **out, int n, int ncols, int vtype)
     c->ncols = ncols;
     c->vtype = vtype;
     c->E = vtype == TRLHIP_C128 ? 2 : 1;
-    c->esz = vtype == TRLHIP_R32 ? 4 : 8 * (size_t)c->E;
+    c->esz = 8 * (size_t)c->E;
     const size_t ncoef = 2 * (size_t)(ncols + 2);
     const size_t vbytes = c->esz * (size_t)c->ld * (ncols + 1);  // + scratch column
     int rc = 0;
-    if (hipStreamCreate(&c->s) != hipSuccess) rc = TRLHIP_EHIP;
+    if (b) {
+        c->batch = b;
+        c->s = b->s;
+    } else if (hipStreamCreate(&c->s) != hipSuccess) {
+        rc = TRLHIP_EHIP;
+    }
     if (!rc && hipMalloc((void **)&c->V, vbytes) != hipSuccess) rc = TRLHIP_ENOMEM;
+    const size_t sbytes = sizeof(float) * (size_t)c->ld * (ncols + 1);
+    if (!rc && vtype == TRLHIP_R32 && hipMalloc((void **)&c->V32, sbytes) != hipSuccess)
+        rc = TRLHIP_ENOMEM;
+    if (!rc && c->V32 && hipMemsetAsync(c->V32, 0, sbytes, c->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc) rc = ensure_part(c, part_need(c->E, ncols));
     if (!rc && hipMalloc((void **)&c->dscal, sizeof(double) * 16) != hipSuccess) rc = TRLHIP_ENOMEM;
     if (!rc && hipMalloc((void **)&c->dcoef, sizeof(double) * ncoef) != hipSuccess) rc = TRLHIP_ENOMEM;
@@ -1323,6 +1657,7 @@ int trlhip_ctx_create(trlhip_ctx **out, int n, int ncols, int vtype)
     if (!rc && hipHostGetDevicePointer((void **)&c->dres, c->hres, 0) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipMemsetAsync(c->V, 0, vbytes, c->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipMemsetAsync(c->dscal, 0, sizeof(double) * 16, c->s) != hipSuccess) rc = TRLHIP_EHIP;
+    // a batched context is created outside the coroutines: plain synchronize
     if (!rc && hipStreamSynchronize(c->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (rc) {
         trlhip_ctx_destroy(c);
@@ -1340,8 +1675,10 @@ int trlhip_col_set(trlhip_ctx *c, int col, const void *x)
     if (int rc = flush_pending(c)) return rc;
     TRY(hipMemcpyAsync(c->V + (size_t)col * c->ld * c->esz, x, c->esz * (size_t)c->n,
                        hipMemcpyHostToDevice, c->s));
-    TRY(hipStreamSynchronize(c->s));
-    return 0;
+    if (c->V32) {  // refresh the FP32 shadow
+        if (int rc = run_scale_copy(c, col, col, 1.0)) return rc;
+    }
+    return ctx_wait(c);  // x may be released by the caller on return
 }
 
 int trlhip_col_get(trlhip_ctx *c, int col, void *x)
@@ -1350,9 +1687,7 @@ int trlhip_col_get(trlhip_ctx *c, int col, void *x)
     if (int rc = flush_pending(c)) return rc;
     TRY(hipMemcpyAsync(x, c->V + (size_t)col * c->ld * c->esz, c->esz * (size_t)c->n,
                        hipMemcpyDeviceToHost, c->s));
-    TRY(hipStreamSynchronize(c->s));
-    stats_resolve();
-    return 0;
+    return ctx_wait(c);
 }
 
 int trlhip_col_copy(trlhip_ctx *c, int dst, int src)
@@ -1483,22 +1818,47 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
     if (!first && qb >= 0 && qb != qa - 1) return TRLHIP_EINVAL;
     if (jnd < 1 || jnd > c->ncols || (rcol >= 0 && rcol < jnd)) return TRLHIP_EINVAL;
     if (A && !csr_fits(c, A)) return TRLHIP_EINVAL;
+    if (c->batch && A && A->lpr && c->E == 1) {
+        // batched: record the step; trlhip_batch_flush runs it with the
+        // other contexts' steps
+        if (first && jnd > 1) {
+            double *h = c->hbuf + 16;
+            for (int i = 0; i < jnd - 1; ++i) h[i] = beta[i];
+            TRY(hipMemcpyAsync(c->dcoef, h, sizeof(double) * (size_t)(jnd - 1),
+                               hipMemcpyHostToDevice, c->s));
+        }
+        BatchStep st{c, A, first, qa, qb, rcol, jnd, beta_prev, 0, 1.0};
+        if (c->pend_dst == qa) {
+            st.fuse = 1;
+            st.fuse_s = c->pend_s;
+            c->pend_dst = -1;
+        } else if (int rc = flush_pending(c)) {
+            return rc;
+        }
+        c->batch->steps.push_back(st);
+        if (int rc = ctx_wait(c)) return rc;
+        const volatile double *hr = c->hres;
+        out[0] = hr[S_ALPHA];
+        out[1] = hr[S_RR2];
+        out[2] = hr[S_HH];
+        out[3] = hr[S_RR2B];
+        return 0;
+    }
     int rc = form_residual(c, A, qa, first, qb, beta_prev, beta, jnd, rcol, true);
     if (rc) return rc;
     // speculative first Gram-Schmidt pass into the scratch column
-    rc = run_proj(c, 0, jnd, rcol, c->dh);
+    rc = run_proj(c, 0, jnd, rcol, c->dh, true);
     if (rc) return rc;
     int g2 = 0;
     rc = run_update(c, 0, jnd, c->dh, -1, rcol, c->ncols, nullptr, c->dscal + S_HH,
-                    TRLHIP_K_UPDATE, 0.0, nullptr, &g2);
+                    TRLHIP_K_UPDATE, 0.0, nullptr, &g2, true);
     if (rc) return rc;
     // both norms (before and after the pass) in one finalize launch, which
     // also writes the four results to host-coherent memory
     KLAUNCH(k_sum2, dim3(2), dim3(kBlock), 0, c->s, part_recur(c), c->np_recur,
             c->dscal + S_RR2, part_update(c), g2, c->dscal + S_RR2B, c->dscal, c->dres);
     if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
-    TRY(hipStreamSynchronize(c->s));
-    stats_resolve();
+    if (int rc2 = ctx_wait(c)) return rc2;
     const volatile double *hr = c->hres;
     out[0] = hr[S_ALPHA];
     out[1] = hr[S_RR2];
@@ -1512,10 +1872,10 @@ int trlhip_cgs_pass(trlhip_ctx *c, int c0, int m, int rcol, double *hh, double *
     if (!c || m < 1 || !valid_cols(c, c0, m) || !valid_col(c, rcol) || (rcol >= c0 && rcol < c0 + m))
         return TRLHIP_EINVAL;
     int rc = flush_pending(c);
-    if (!rc) rc = run_proj(c, c0, m, rcol, c->dh);
+    if (!rc) rc = run_proj(c, c0, m, rcol, c->dh, true);
     if (rc) return rc;
     rc = run_update(c, c0, m, c->dh, -1, rcol, rcol, c->dscal + S_RR2, c->dscal + S_HH,
-                    TRLHIP_K_UPDATE);
+                    TRLHIP_K_UPDATE, 0.0, nullptr, nullptr, true);
     if (rc) return rc;
     const int E = c->E;
     if (h_out)
@@ -1588,27 +1948,18 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y, int ldy)
     const size_t shm = sizeof(double) * (size_t)jl4 * 16 * KT;
     if (shm > 160 * 1024) return TRLHIP_EINVAL;  // gfx950: 160 KB of LDS per CU
     Timed t(TRLHIP_K_RITZ, c->s, (double)c->esz * c->n * (jl + k));
-#define RITZ_CASE(S, K)                                                                      \
-    case K:                                                                                  \
-        KLAUNCH((k_ritz_mfma<S, K>), dim3(g), dim3(kBlock), shm, c->s, nrows,     \
-                           reinterpret_cast<S *>(c->V), ldr, lck, jl, k, c->dY, ldy, tpw); \
+#define RITZ_CASE(K)                                                                          \
+    case K:                                                                                   \
+        KLAUNCH((k_ritz_mfma<K>), dim3(g), dim3(kBlock), shm, c->s, nrows,                     \
+                reinterpret_cast<double *>(c->V), ldr, lck, jl, k, c->dY, ldy, tpw, c->V32);    \
         break;
-    if (is_f32(c)) {
-        switch (KT) {
-            RITZ_CASE(float, 1) RITZ_CASE(float, 2) RITZ_CASE(float, 3) RITZ_CASE(float, 4)
-            RITZ_CASE(float, 5) RITZ_CASE(float, 6) RITZ_CASE(float, 7) RITZ_CASE(float, 8)
-        }
-    } else {
-        switch (KT) {
-            RITZ_CASE(double, 1) RITZ_CASE(double, 2) RITZ_CASE(double, 3) RITZ_CASE(double, 4)
-            RITZ_CASE(double, 5) RITZ_CASE(double, 6) RITZ_CASE(double, 7) RITZ_CASE(double, 8)
-        }
+    switch (KT) {
+        RITZ_CASE(1) RITZ_CASE(2) RITZ_CASE(3) RITZ_CASE(4)
+        RITZ_CASE(5) RITZ_CASE(6) RITZ_CASE(7) RITZ_CASE(8)
     }
 #undef RITZ_CASE
     if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
-    TRY(hipStreamSynchronize(c->s));
-    stats_resolve();
-    return 0;
+    return ctx_wait(c);  // the pinned staging of Y is reused by the next call
 }
 
 void trlhip_stats_enable(int on)
@@ -1632,6 +1983,67 @@ int trlhip_stats_get(double *out, int nout)
         out[3 * k + 2] = g_stats.bytes[k];
     }
     return TRLHIP_K_COUNT;
+}
+
+int trlhip_batch_create(trlhip_batch **out)
+{
+    if (!out) return TRLHIP_EINVAL;
+    if (trlhip_device_count() <= 0) return TRLHIP_ENODEV;
+    trlhip_batch *b = new (std::nothrow) trlhip_batch();
+    if (!b) return TRLHIP_ENOMEM;
+    if (hipStreamCreate(&b->s) != hipSuccess) {
+        delete b;
+        return TRLHIP_EHIP;
+    }
+    *out = b;
+    return 0;
+}
+
+void trlhip_batch_destroy(trlhip_batch *b)
+{
+    if (!b) return;
+    (void)hipStreamSynchronize(b->s);
+    if (b->hdesc) (void)hipHostFree(b->hdesc);
+    if (b->ddesc) (void)hipFree(b->ddesc);
+    if (b->hpre) (void)hipHostFree(b->hpre);
+    if (b->dpre) (void)hipFree(b->dpre);
+    (void)hipStreamDestroy(b->s);
+    delete b;
+}
+
+void trlhip_batch_set_yield(trlhip_batch *b, void (*yield)(void *), void *user)
+{
+    if (!b) return;
+    b->yield = yield;
+    b->user = user;
+}
+
+int trlhip_batch_flush(trlhip_batch *b)
+{
+    if (!b) return TRLHIP_EINVAL;
+    int rc = b->err;
+    if (!rc && !b->steps.empty()) {
+        // one launch set per (lanes per row, basis precision) group
+        std::vector<BatchStep> all;
+        all.swap(b->steps);
+        while (!all.empty() && !rc) {
+            const int lpr = all[0].A->lpr;
+            const bool mixed = all[0].c->V32 != nullptr;
+            std::vector<BatchStep> rest;
+            for (const BatchStep &st : all) {
+                if (st.A->lpr == lpr && (st.c->V32 != nullptr) == mixed) b->steps.push_back(st);
+                else rest.push_back(st);
+            }
+            rc = batch_launch(b);
+            b->steps.clear();
+            all.swap(rest);
+        }
+    }
+    if (hipStreamSynchronize(b->s) != hipSuccess && !rc) rc = TRLHIP_EHIP;
+    stats_resolve();
+    b->steps.clear();
+    if (rc) b->err = rc;
+    return rc;
 }
 
 }  // extern "C"

@@ -80,6 +80,8 @@ def lib():
         L.trl_init_info.argtypes = [ctypes.POINTER(TrlInfo), I, I, I, I, D, I, I, P]
         L.trl_init_info.restype = None
         L.trl_set_iguess.argtypes = [ctypes.POINTER(TrlInfo), I, I, I, ctypes.c_char_p]
+        L.trl_set_checkpoint.argtypes = [ctypes.POINTER(TrlInfo), I, ctypes.c_char_p]
+        L.trl_set_checkpoint.restype = None
         L.trl_set_iguess.restype = None
         L.trl_set_restart.argtypes = [ctypes.POINTER(TrlInfo), D]
         L.trl_set_restart.restype = None
@@ -123,10 +125,13 @@ class Csr:
     """Device CSR operator (trlhip_csr); usable as mvparam with the op
     returned by .op (trlhip_csr_matvec / trlhip_csr_zmatvec)."""
 
-    def __init__(self, rp, ci, val, complex_=False):
+    def __init__(self, rp, ci, val, complex_=False, mixed=False):
+        """mixed: mark the operator for the mixed-precision solver
+        (TRLHIP_R32: FP32 shadow basis for Gram-Schmidt, FP64 elsewhere)."""
         n = len(rp) - 1
         self.n = n
         self.complex = complex_
+        self.mixed = bool(mixed) and not complex_
         rp = np.ascontiguousarray(rp, dtype=np.int32)
         ci = np.ascontiguousarray(ci, dtype=np.int32)
         if complex_:
@@ -134,7 +139,7 @@ class Csr:
             vt = TRLHIP_C128
         else:
             v = np.ascontiguousarray(val, dtype=np.float64)
-            vt = TRLHIP_R64
+            vt = TRLHIP_R32 if self.mixed else TRLHIP_R64
         h = ctypes.c_void_p()
         rc = lib().trlhip_csr_create(ctypes.byref(h), n, len(ci), _ptr(rp), _ptr(ci), _ptr(v), vt)
         if rc != 0:
@@ -168,14 +173,26 @@ def _result(info, ev, evec, wrk, wall):
     return out
 
 
-def solve_csr(A, lohi, ned, maxlan, tol, restart, maxmv, mev, start, rfact=0.0):
+def solve_csr(A, lohi, ned, maxlan, tol, restart, maxmv, mev, start, rfact=0.0,
+              checkpoint=None, resume=None):
     """trlan()/ztrlan() on the device CSR operator A with start vector
-    `start` (guess = 1).  Returns the same dict keys as oracle/ref.py."""
+    `start` (guess = 1).  checkpoint=(cpflag, prefix) writes checkpoint
+    files (trl_set_checkpoint); resume=prefix restarts from the file written
+    under that prefix (guess = 2, oldcpf).  Returns the same dict keys as
+    oracle/ref.py."""
     import time
 
     n = A.n
     info = init_info(n, maxlan, lohi, ned, tol, restart, maxmv)
-    lib().trl_set_iguess(ctypes.byref(info), 0, 1, 0, None)
+    if checkpoint is not None:
+        info.verbose = -1
+        lib().trl_set_checkpoint(ctypes.byref(info), int(checkpoint[0]),
+                                 str(checkpoint[1]).encode())
+    if resume is not None:
+        info.oldcpf = str(resume).encode()
+        lib().trl_set_iguess(ctypes.byref(info), 0, 2, 0, None)
+    else:
+        lib().trl_set_iguess(ctypes.byref(info), 0, 1, 0, None)
     if rfact:
         lib().trl_set_restart(ctypes.byref(info), rfact)
     info.mvparam = A.handle.value
@@ -200,6 +217,49 @@ def solve_csr(A, lohi, ned, maxlan, tol, restart, maxmv, mev, start, rfact=0.0):
     if info.stat == TRLHIP_STAT_DEVICE:
         raise RuntimeError("device failure inside trlan()")
     return _result(info, ev, evec.T, wrk, wall)
+
+
+class TrlBatchJob(ctypes.Structure):
+    """Mirror of trl_batch_job (include/trlan_batch.h)."""
+    _fields_ = [("A", ctypes.c_void_p), ("info", ctypes.POINTER(TrlInfo)),
+                ("nrow", ctypes.c_int), ("mev", ctypes.c_int),
+                ("eval", ctypes.c_void_p), ("evec", ctypes.c_void_p),
+                ("lde", ctypes.c_int), ("lwrk", ctypes.c_int), ("wrk", ctypes.c_void_p)]
+
+
+def solve_batch(problems):
+    """trlan_batch over problems = [(A, lohi, ned, maxlan, tol, restart,
+    maxmv, mev, start), ...] with real device operators A (Csr, mixed or
+    not).  Returns (rc, [result dict per problem], wall seconds)."""
+    import time
+
+    L = lib()
+    L.trlan_batch.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.trlan_batch.restype = ctypes.c_int
+    jobs = (TrlBatchJob * len(problems))()
+    keep = []
+    for j, (A, lohi, ned, maxlan, tol, restart, maxmv, mev, start) in enumerate(problems):
+        n = A.n
+        info = init_info(n, maxlan, lohi, ned, tol, restart, maxmv)
+        L.trl_set_iguess(ctypes.byref(info), 0, 1, 0, None)
+        ev = np.zeros(mev)
+        evec = np.zeros((mev, n))
+        evec[0] = start
+        wrk = np.zeros(mev)
+        keep.append((info, ev, evec, wrk))
+        jobs[j].A = A.handle.value
+        jobs[j].info = ctypes.pointer(info)
+        jobs[j].nrow = n
+        jobs[j].mev = mev
+        jobs[j].eval = ev.ctypes.data
+        jobs[j].evec = evec.ctypes.data
+        jobs[j].lde = n
+        jobs[j].lwrk = mev
+        jobs[j].wrk = wrk.ctypes.data
+    t0 = time.perf_counter()
+    rc = L.trlan_batch(jobs, len(problems))
+    wall = time.perf_counter() - t0
+    return rc, [_result(info, ev, evec.T, wrk, wall) for info, ev, evec, wrk in keep], wall
 
 
 def solve_hostop(matvec, n, lohi, ned, maxlan, tol, restart, maxmv, mev, start):
@@ -250,7 +310,7 @@ def device_count():
 class Ctx:
     """A device basis context (trlhip_ctx) for kernel-level checks."""
 
-    def __init__(self, n, ncols, complex_=False):
+    def __init__(self, n, ncols, complex_=False, mixed=False):
         L = lib()
         P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
         for name, args in [
@@ -276,7 +336,8 @@ class Ctx:
         self.n, self.ncols, self.complex = n, ncols, complex_
         self.E = 2 if complex_ else 1
         h = ctypes.c_void_p()
-        rc = L.trlhip_ctx_create(ctypes.byref(h), n, ncols, TRLHIP_C128 if complex_ else TRLHIP_R64)
+        vt = TRLHIP_C128 if complex_ else (TRLHIP_R32 if mixed else TRLHIP_R64)
+        rc = L.trlhip_ctx_create(ctypes.byref(h), n, ncols, vt)
         if rc != 0:
             raise RuntimeError(f"trlhip_ctx_create failed ({rc})")
         self.h = h

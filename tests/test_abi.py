@@ -25,7 +25,7 @@ GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "ref_cases.json")))
 
 def declared_functions():
     names = set()
-    for h in ("trlan.h", "ztrlan.h", "trlhip.h"):
+    for h in ("trlan.h", "ztrlan.h", "trlhip.h", "trlan_batch.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(",

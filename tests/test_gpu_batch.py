@@ -1,0 +1,82 @@
+"""GPU checks of trlan_batch: many independent solves advancing together
+(one batched launch per step phase, one synchronisation per round).  Each
+job must make exactly the decisions of a lone trlan() call, so its
+eigenvalues and counters equal the single solve's bit for bit, and its
+eigenvalues match the oracle's to 1e-10 ||A||."""
+import numpy as np
+import pytest
+
+from nutrlan_amd import workloads as W
+from oracle import trlan_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs(T, count, mixed_every=0):
+    rng = np.random.default_rng(11)
+    out = []
+    for i in range(count):
+        L = int(rng.integers(24, 72))
+        Wd = float(rng.uniform(1.0, 12.0))
+        rp, ci, v = W.anderson2d_csr(L, Wd, seed=100 + i)[0]
+        n = len(rp) - 1
+        mixed = bool(mixed_every) and i % mixed_every == 0
+        A = T.Csr(rp, ci, v, mixed=mixed)
+        x0 = W.start_vector(n, seed=200 + i)
+        out.append(((rp, ci, v), A, (A, -1, 8, 32, 1e-8, 1, 40000, 8, x0), x0))
+    return out
+
+
+def test_batch_matches_single_solves(gpu_lib):
+    T = gpu_lib
+    jobs = _jobs(T, 24, mixed_every=3)
+    rc, res, wall = T.solve_batch([j[2] for j in jobs])
+    assert rc == 0
+    for (csr, A, args, x0), g in zip(jobs, res):
+        s = T.solve_csr(*args)
+        assert g["stat"] == 0 and s["stat"] == 0
+        keys = ["nec", "matvec", "nloop", "north", "nrand", "locked"]
+        assert [g[k] for k in keys] == [s[k] for k in keys]
+        assert np.array_equal(g["eval"], s["eval"])
+        assert np.array_equal(g["evec"], s["evec"])
+    for csr, A, args, x0 in jobs:
+        A.close()
+
+
+def test_batch_vs_oracle(gpu_lib):
+    T = gpu_lib
+    jobs = _jobs(T, 6)
+    rc, res, wall = T.solve_batch([j[2] for j in jobs])
+    assert rc == 0
+    for (csr, A, args, x0), g in zip(jobs, res):
+        rp, ci, v = csr
+        n = len(rp) - 1
+        info = O.init_info(n, 32, -1, 8, 1e-8, 1, 40000)
+        O.set_iguess(info, 0, 1)
+        o = O.trlan(W.csr_matvec(rp, ci, v), info, n, 8, x0)
+        assert g["nec"] == info.nec == 8
+        assert (g["matvec"], g["nloop"], g["north"]) == (info.matvec, info.nloop, info.north)
+        assert np.abs(g["eval"][:8] - o["eval"][:8]).max() <= 1e-10 * info.anrm
+        A.close()
+
+
+def test_batch_mixed_operators_and_early_exit(gpu_lib):
+    """Jobs with different lanes-per-row groups (2D and 3D stencils), a job
+    with nothing to do (ned <= nec) and a job with an argument error run in
+    one batch; the good ones still equal their single solves."""
+    T = gpu_lib
+    a = W.laplace3d_csr(14, seed=1)
+    b = W.anderson2d_csr(40, 5.0, seed=2)[0]
+    Aa, Ab = T.Csr(*a), T.Csr(*b)
+    xa = W.start_vector(len(a[0]) - 1, seed=3)
+    xb = W.start_vector(len(b[0]) - 1, seed=4)
+    probs = [(Aa, 1, 6, 30, 1e-8, 1, 20000, 6, xa), (Ab, -1, 5, 24, 1e-8, 1, 20000, 5, xb),
+             (Ab, -1, 0, 24, 1e-8, 1, 20000, 5, xb)]
+    rc, res, wall = T.solve_batch(probs)
+    assert rc == 0
+    for p, g in zip(probs[:2], res[:2]):
+        s = T.solve_csr(*p)
+        assert g["matvec"] == s["matvec"] and np.array_equal(g["eval"], s["eval"])
+    assert res[2]["matvec"] == 0
+    Aa.close()
+    Ab.close()

@@ -1,0 +1,86 @@
+"""GPU checks of the mixed-precision path (TRLHIP_R32 operator): FP64
+recurrence, basis and restart GEMM; Gram-Schmidt against an FP32 shadow of
+the basis (one pass, a second when the FP32-level test asks for it); an FP64
+Rayleigh-quotient refinement of every returned pair.
+
+Bars: every returned pair has a true residual ||A v - lambda v|| <= tol ||A||
+(recomputed here in numpy), the reported wrk[i] is that residual, and each
+eigenvalue's error against the FP64 oracle (the reference restated) is
+printed and bounded by 1e-10 ||A||.
+"""
+import numpy as np
+import pytest
+
+from nutrlan_amd import workloads as W
+from oracle import trlan_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # name, builder, lohi, ned, maxlan, tol, start seed
+    ("lap24_lg16", lambda: W.laplace3d_csr(24, seed=0), 1, 16, 64, 1e-8, 1),
+    ("rand30k_lg16", lambda: W.random_sym_csr(30000, seed=1), 1, 16, 64, 1e-8, 1),
+    ("and2d96_sm8", lambda: W.anderson2d_csr(96, 6.0, seed=2)[0], -1, 8, 32, 1e-8, 3),
+    ("diag2000_sm10", lambda: W.diag_csr(2000), -1, 10, 100, 1.4901e-8, 4),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_mixed_vs_fp64_oracle(gpu_lib, case):
+    T = gpu_lib
+    name, build, lohi, ned, maxlan, tol, seed = case
+    rp, ci, v = build()
+    n = len(rp) - 1
+    x0 = W.start_vector(n, seed=seed)
+    print(f"case {name} n={n}", flush=True)
+    A = T.Csr(rp, ci, v, mixed=True)
+    g = T.solve_csr(A, lohi, ned, maxlan, tol, 1, 40000, ned, x0)
+    A.close()
+    P = T.Csr(rp, ci, v)
+    p = T.solve_csr(P, lohi, ned, maxlan, tol, 1, 40000, ned, x0)
+    P.close()
+    assert g["stat"] == 0 and g["nec"] == ned, (g["stat"], g["nec"])
+
+    info = O.init_info(n, maxlan, lohi, ned, tol, 1, 40000)
+    O.set_iguess(info, 0, 1)
+    o = O.trlan(W.csr_matvec(rp, ci, v), info, n, ned, x0)
+    anrm = max(abs(info.anrm), 1e-300)
+    f = W.csr_matvec(rp, ci, v)
+    X = g["evec"][:, :ned]
+    err = []
+    for i in range(ned):
+        x = X[:, i]
+        r = np.linalg.norm(f(x) - g["eval"][i] * x) / np.linalg.norm(x)
+        assert r <= tol * anrm, (i, r / anrm)
+        assert abs(g["resid"][i] - r) <= 1e-3 * r + 1e-14 * anrm  # wrk = true residual
+        j = np.abs(o["eval"][:ned] - g["eval"][i]).argmin()
+        err.append(abs(o["eval"][j] - g["eval"][i]) / anrm)
+    print(f"{name}: matvec mixed {g['matvec']} plain {p['matvec']} oracle {info.matvec}; "
+          f"|dlambda|/||A|| per pair: " + " ".join(f"{e:.1e}" for e in err), flush=True)
+    assert max(err) <= 1e-10
+    # orthonormal to the FP32 shadow's level
+    assert np.abs(X.T @ X - np.eye(ned)).max() <= 1e-6
+
+
+def test_mixed_cgs_pass_reads_fp32_shadow(gpu_lib):
+    """A Gram-Schmidt pass in a mixed context projects against the FP32
+    rounding of the basis, while r stays FP64."""
+    T = gpu_lib
+    n, m = 50000, 12
+    rng = np.random.default_rng(4)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, m)))
+    r = rng.standard_normal(n)
+    c = T.Ctx(n, m + 2, mixed=True)
+    for j in range(m):
+        c.set(j, Q[:, j])
+    c.set(m, r)
+    hh, rr2, h = c.cgs_pass(0, m, m)
+    S = Q.astype(np.float32).astype(np.float64)
+    h_ref = S.T @ r
+    r_ref = r - S @ h_ref
+    assert np.abs(h - h_ref).max() <= 1e-12 * np.abs(h_ref).max()
+    assert np.abs(c.get(m) - r_ref).max() <= 1e-12 * np.abs(r).max()
+    assert abs(rr2 - r_ref @ r_ref) <= 1e-10 * (r_ref @ r_ref)
+    # the FP64 basis itself is untouched
+    assert np.array_equal(c.get(0), Q[:, 0])
+    c.close()

@@ -228,3 +228,122 @@ def test_smooth_rr_quirks(kit):
     z = list(x)
     O.smooth_rr(z := np.array(z))
     assert np.array_equal(y, z)
+
+
+# ------------------------------------------------------------ checkpoints
+
+COLCB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double))
+
+
+def _ck_bind(kit):
+    kit.trlkit_write_checkpoint.argtypes = [ctypes.c_char_p, I, I, P, P, I, COLCB, P, P]
+    kit.trlkit_write_checkpoint.restype = I
+    kit.trlkit_read_checkpoint.argtypes = [ctypes.c_char_p, I, I, I, ctypes.POINTER(I), P, P,
+                                          COLCB, P, P]
+    kit.trlkit_read_checkpoint.restype = I
+
+
+def _ck_write(kit, path, V, alpha, beta):
+    nrow, ncol = V.shape
+    buf = np.zeros(nrow)
+
+    def get(_u, i, b):
+        np.ctypeslib.as_array(b, shape=(nrow,))[:] = V[:, i]
+        return 0
+
+    cb = COLCB(get)
+    return kit.trlkit_write_checkpoint(path.encode(), nrow, len(alpha), p(alpha), p(beta), ncol,
+                                       cb, None, p(buf))
+
+
+def _ck_read(kit, path, nrow, mev, maxjnd):
+    cols = {}
+    buf = np.zeros(nrow)
+
+    def put(_u, i, b):
+        cols[i] = np.ctypeslib.as_array(b, shape=(nrow,)).copy()
+        return 0
+
+    cb = COLCB(put)
+    jnd = I(0)
+    alpha = np.zeros(maxjnd + 1)
+    beta = np.zeros(maxjnd + 1)
+    rc = kit.trlkit_read_checkpoint(path.encode(), nrow, mev, maxjnd, ctypes.byref(jnd), p(alpha),
+                                    p(beta), cb, None, p(buf))
+    return rc, jnd.value, alpha[:jnd.value], beta[:jnd.value], cols
+
+
+def test_checkpoint_round_trip_and_layout(kit, tmp_path):
+    """Byte layout of the reference's trl_write_checkpoint (SRC/trlaux.c:
+    780-880): int nrow, int jnd, alpha, beta, then the columns."""
+    _ck_bind(kit)
+    rng = np.random.default_rng(3)
+    nrow, jnd = 37, 5
+    V = rng.standard_normal((nrow, jnd + 1))
+    alpha, beta = rng.standard_normal(jnd), rng.standard_normal(jnd)
+    path = str(tmp_path / "cp0")
+    assert _ck_write(kit, path, V, alpha, beta) == 0
+    raw = open(path, "rb").read()
+    assert np.frombuffer(raw[:8], dtype=np.int32).tolist() == [nrow, jnd]
+    body = np.frombuffer(raw[8:], dtype=np.float64)
+    assert np.array_equal(body[:jnd], alpha) and np.array_equal(body[jnd:2 * jnd], beta)
+    assert np.array_equal(body[2 * jnd:].reshape(jnd + 1, nrow).T, V)
+    # read back: mev > jnd -> jnd + 1 columns
+    rc, j, a2, b2, cols = _ck_read(kit, path, nrow, 8, 10)
+    assert rc == 0 and j == jnd and np.array_equal(a2, alpha) and np.array_equal(b2, beta)
+    assert sorted(cols) == list(range(jnd + 1))
+    assert all(np.array_equal(cols[i], V[:, i]) for i in cols)
+    # mev <= jnd -> jnd columns (the reference reads j1 + j2 = jnd)
+    rc, j, _, _, cols = _ck_read(kit, path, nrow, 3, 10)
+    assert rc == 0 and sorted(cols) == list(range(jnd))
+    # error codes of trl_read_checkpoint
+    assert _ck_read(kit, path, nrow + 1, 8, 10)[0] == -213
+    assert _ck_read(kit, path, nrow, 8, jnd - 1)[0] == -214
+    assert _ck_read(kit, str(tmp_path / "missing0"), nrow, 8, 10)[0] == -212
+    open(str(tmp_path / "short0"), "wb").write(raw[: len(raw) // 2])
+    assert _ck_read(kit, str(tmp_path / "short0"), nrow, 8, 10)[0] in (-215, -216)
+
+
+@pytest.mark.skipif(not __import__("oracle.ref", fromlist=["available"]).available(),
+                    reason="compiled reference not built here")
+def test_checkpoint_files_interchange_with_reference(kit, tmp_path):
+    """Files written here are read by the reference's trl_read_checkpoint
+    and files the reference writes are read here, with identical data."""
+    from oracle import ref
+
+    _ck_bind(kit)
+    R = ref.lib()
+    R.trl_write_checkpoint.argtypes = [ctypes.c_char_p, I, P, P, P, I, I, P, I, I]
+    R.trl_write_checkpoint.restype = I
+    R.trl_read_checkpoint.argtypes = [ctypes.c_char_p, I, P, I, I, ctypes.POINTER(I), P, I, I,
+                                      ctypes.POINTER(I), I, P, I, P]
+    R.trl_read_checkpoint.restype = I
+    rng = np.random.default_rng(5)
+    nrow, me, nb = 23, 3, 2          # evec part and base part
+    jnd = me + nb - 1
+    E = np.asfortranarray(rng.standard_normal((nrow, me)))
+    B = np.asfortranarray(rng.standard_normal((nrow, nb)))
+    alpha, beta = rng.standard_normal(jnd), rng.standard_normal(jnd)
+    # reference -> here
+    path = str(tmp_path / "ref0")
+    assert R.trl_write_checkpoint(path.encode(), nrow, p(alpha), p(beta), p(E), nrow, me, p(B),
+                                  nrow, nb) == 0
+    rc, j, a2, b2, cols = _ck_read(kit, path, nrow, 10, 10)
+    assert rc == 0 and j == jnd and np.array_equal(a2, alpha) and np.array_equal(b2, beta)
+    V = np.hstack([E, B])
+    assert all(np.array_equal(cols[i], V[:, i]) for i in range(jnd + 1))
+    # here -> reference
+    path2 = str(tmp_path / "ours0")
+    assert _ck_write(kit, path2, V, alpha, beta) == 0
+    mev, nbas = 10, 4
+    ev = np.zeros(nrow * mev)
+    bs = np.zeros(nrow * nbas)
+    j1, j2 = I(0), I(0)
+    al = np.zeros(mev + nbas)
+    be = np.zeros(mev + nbas)
+    rc = R.trl_read_checkpoint(path2.encode(), nrow, p(ev), nrow, mev, ctypes.byref(j1), p(bs),
+                               nrow, nbas, ctypes.byref(j2), mev + nbas - 1, p(al),
+                               mev + nbas - 1, p(be))
+    assert rc == 0 and j1.value == jnd and j2.value == 0
+    assert np.array_equal(al[:jnd], alpha) and np.array_equal(be[:jnd], beta)
+    assert np.array_equal(ev[: nrow * (jnd + 1)].reshape(jnd + 1, nrow).T, V)
