@@ -1091,7 +1091,7 @@ void trl_solve_device(trl_matvec op, ztrl_matvec zop, trl_uo user_ortho, trl_inf
  * with its results.  Each solve makes exactly the decisions it would make
  * alone, so its eigenpairs and counters match a single trlan() call. */
 
-enum { LZ_STACK = 256 * 1024 };
+enum { LZ_STACK = 1024 * 1024 };  /* per coroutine; HIP runtime calls run on it */
 
 typedef struct {
     ucontext_t uc;

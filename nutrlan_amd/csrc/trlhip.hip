@@ -895,8 +895,9 @@ struct trlhip_batch {
     int err = 0;                  // sticky device error of a flush
     std::vector<BatchStep> steps;
     BDesc *hdesc = nullptr, *ddesc = nullptr;   // pinned staging / device copy
-    int *hpre = nullptr, *dpre = nullptr;       // BP_COUNT prefix tables
-    size_t cap = 0;               // requests the buffers hold
+    int *hpre = nullptr, *dpre = nullptr;       // prefix tables of all groups
+    size_t cap = 0;               // requests the descriptor buffers hold
+    size_t pcap = 0;              // ints the prefix buffers hold
 };
 
 namespace {
@@ -1306,44 +1307,28 @@ __global__ __launch_bounds__(kBlock) void kb_sum2(const BDesc *__restrict__ d, c
 }
 #undef BATCH_LOCATE
 
-// Runs every recorded step of the batch: one launch per phase over all
-// requests, in the order form_residual + run_proj + run_update + k_sum2 run
-// them for a single context.  Requires vector-CSR operators with one lanes-
-// per-row value and one basis precision across the batch.
-int batch_launch(trlhip_batch *b)
+// Host staging of one group of recorded steps (same lanes per row, same
+// basis precision): descriptors d[0:R) and BP_COUNT prefix tables of R + 1
+// block offsets each; returns the phases' algorithmic bytes and the largest
+// recurrence / projection widths.
+struct GroupInfo {
+    int R, lpr;
+    bool mixed;
+    size_t doff, poff;  // offsets into the staging arrays
+    double bytes[BP_COUNT];
+    int maxrm, maxj;
+};
+
+void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo &gi)
 {
-    const int R = (int)b->steps.size();
-    const BatchStep &s0 = b->steps[0];
-    const int lpr = s0.A->lpr;
-    const bool mixed = s0.c->V32 != nullptr;
-    for (const BatchStep &st : b->steps)
-        if (st.A->lpr != lpr || (st.c->V32 != nullptr) != mixed) return TRLHIP_EINVAL;
-    if ((size_t)R > b->cap) {
-        if (b->hdesc) (void)hipHostFree(b->hdesc);
-        if (b->ddesc) (void)hipFree(b->ddesc);
-        if (b->hpre) (void)hipHostFree(b->hpre);
-        if (b->dpre) (void)hipFree(b->dpre);
-        b->hdesc = b->ddesc = nullptr;
-        b->hpre = b->dpre = nullptr;
-        b->cap = 0;
-        const size_t cap = (size_t)R + R / 2 + 16;
-        const size_t npre = (size_t)BP_COUNT * (cap + 1);
-        if (hipHostMalloc((void **)&b->hdesc, sizeof(BDesc) * cap, 0) != hipSuccess ||
-            hipMalloc((void **)&b->ddesc, sizeof(BDesc) * cap) != hipSuccess ||
-            hipHostMalloc((void **)&b->hpre, sizeof(int) * npre, 0) != hipSuccess ||
-            hipMalloc((void **)&b->dpre, sizeof(int) * npre) != hipSuccess)
-            return TRLHIP_ENOMEM;
-        b->cap = cap;
-    }
-    double bytes[BP_COUNT] = {};
-    int maxrm = 1, maxj = 1;
-    int *pre = b->hpre;
     for (int ph = 0; ph < BP_COUNT; ++ph) pre[(size_t)ph * (R + 1)] = 0;
+    for (double &x : gi.bytes) x = 0.0;
+    gi.maxrm = gi.maxj = 1;
     for (int i = 0; i < R; ++i) {
-        const BatchStep &st = b->steps[i];
+        const BatchStep &st = steps[i];
         trlhip_ctx *c = st.c;
         const trlhip_csr *A = st.A;
-        BDesc &d = b->hdesc[i];
+        BDesc &d = desc[i];
         const int n = c->n, jnd = st.jnd;
         d.n = n;
         d.ld = c->ld;
@@ -1379,7 +1364,7 @@ int batch_launch(trlhip_batch *b)
         d.r = colp<double>(c, st.rcol);
         d.scratch = colp<double>(c, c->ncols);
         d.V = colp<double>(c, 0);
-        d.VG = mixed ? (const void *)c->V32 : (const void *)colp<double>(c, 0);
+        d.VG = gi.mixed ? (const void *)c->V32 : (const void *)colp<double>(c, 0);
         d.part_alpha = part_alpha(c);
         d.part_recur = part_recur(c);
         d.part_update = part_update(c);
@@ -1392,51 +1377,114 @@ int batch_launch(trlhip_batch *b)
             int *pp = pre + (size_t)ph * (R + 1);
             pp[i + 1] = pp[i] + nb[ph];
         }
-        const double vs = mixed ? 4.0 : 8.0;
-        bytes[BP_SPMV] += spmv_bytes(A, 1, 8, 8) + (st.fuse ? (mixed ? 12.0 : 8.0) * n : 0.0);
-        bytes[BP_RECUR] += 8.0 * n * (d.rm + 2);
-        bytes[BP_PROJ] += (vs * jnd + 8.0) * n;
-        bytes[BP_UPDATE] += (vs * jnd + 16.0) * n;
-        maxrm = std::max(maxrm, d.rm);
-        maxj = std::max(maxj, jnd);
+        const double vs = gi.mixed ? 4.0 : 8.0;
+        gi.bytes[BP_SPMV] += spmv_bytes(A, 1, 8, 8) + (st.fuse ? (gi.mixed ? 12.0 : 8.0) * n : 0.0);
+        gi.bytes[BP_RECUR] += 8.0 * n * (d.rm + 2);
+        gi.bytes[BP_PROJ] += (vs * jnd + 8.0) * n;
+        gi.bytes[BP_UPDATE] += (vs * jnd + 16.0) * n;
+        gi.maxrm = std::max(gi.maxrm, d.rm);
+        gi.maxj = std::max(gi.maxj, jnd);
     }
-    TRY(hipMemcpyAsync(b->ddesc, b->hdesc, sizeof(BDesc) * (size_t)R, hipMemcpyHostToDevice, b->s));
-    TRY(hipMemcpyAsync(b->dpre, b->hpre, sizeof(int) * (size_t)BP_COUNT * (R + 1),
-                       hipMemcpyHostToDevice, b->s));
-    auto blocks = [&](int ph) { return dim3(pre[(size_t)ph * (R + 1) + R]); };
-    const BDesc *dd = b->ddesc;
-    const int *dp = b->dpre;
+}
+
+// The launches of one staged group: one kernel per phase over its R
+// requests (descriptors dd, prefix tables dp on the device).
+int batch_run(hipStream_t s, const GroupInfo &gi, const BDesc *dd, const int *dp, const int *hpre)
+{
+    const int R = gi.R;
+    auto blocks = [&](int ph) { return dim3(hpre[(size_t)ph * (R + 1) + R]); };
     {
-        Timed t(TRLHIP_K_SPMV, b->s, bytes[BP_SPMV]);
-        switch (lpr) {
-        case 2: KLAUNCH((kb_spmv<2>), blocks(BP_SPMV), dim3(kBlock), 0, b->s, dd, dp, R); break;
-        case 4: KLAUNCH((kb_spmv<4>), blocks(BP_SPMV), dim3(kBlock), 0, b->s, dd, dp, R); break;
-        case 8: KLAUNCH((kb_spmv<8>), blocks(BP_SPMV), dim3(kBlock), 0, b->s, dd, dp, R); break;
-        case 16: KLAUNCH((kb_spmv<16>), blocks(BP_SPMV), dim3(kBlock), 0, b->s, dd, dp, R); break;
-        default: KLAUNCH((kb_spmv<32>), blocks(BP_SPMV), dim3(kBlock), 0, b->s, dd, dp, R); break;
+        Timed t(TRLHIP_K_SPMV, s, gi.bytes[BP_SPMV]);
+        switch (gi.lpr) {
+        case 2: KLAUNCH((kb_spmv<2>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
+        case 4: KLAUNCH((kb_spmv<4>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
+        case 8: KLAUNCH((kb_spmv<8>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
+        case 16: KLAUNCH((kb_spmv<16>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
+        default: KLAUNCH((kb_spmv<32>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
         }
     }
     {
-        Timed t(TRLHIP_K_RECUR, b->s, bytes[BP_RECUR]);
-        KLAUNCH(kb_recur, blocks(BP_RECUR), dim3(kBlock), sizeof(double) * maxrm, b->s, dd, dp, R);
+        Timed t(TRLHIP_K_RECUR, s, gi.bytes[BP_RECUR]);
+        KLAUNCH(kb_recur, blocks(BP_RECUR), dim3(kBlock), sizeof(double) * gi.maxrm, s, dd, dp, R);
     }
     {
-        Timed t(TRLHIP_K_PROJ, b->s, bytes[BP_PROJ]);
-        if (mixed) KLAUNCH((kb_proj<float>), blocks(BP_PROJ), dim3(kBlock), 0, b->s, dd, dp, R);
-        else KLAUNCH((kb_proj<double>), blocks(BP_PROJ), dim3(kBlock), 0, b->s, dd, dp, R);
+        Timed t(TRLHIP_K_PROJ, s, gi.bytes[BP_PROJ]);
+        if (gi.mixed) KLAUNCH((kb_proj<float>), blocks(BP_PROJ), dim3(kBlock), 0, s, dd, dp, R);
+        else KLAUNCH((kb_proj<double>), blocks(BP_PROJ), dim3(kBlock), 0, s, dd, dp, R);
     }
-    KLAUNCH(kb_colsum, blocks(BP_COLSUM), dim3(kBlock), 0, b->s, dd, dp, R);
+    KLAUNCH(kb_colsum, blocks(BP_COLSUM), dim3(kBlock), 0, s, dd, dp, R);
     {
-        Timed t(TRLHIP_K_UPDATE, b->s, bytes[BP_UPDATE]);
-        if (mixed)
-            KLAUNCH((kb_update<float>), blocks(BP_UPDATE), dim3(kBlock), sizeof(double) * maxj, b->s,
+        Timed t(TRLHIP_K_UPDATE, s, gi.bytes[BP_UPDATE]);
+        if (gi.mixed)
+            KLAUNCH((kb_update<float>), blocks(BP_UPDATE), dim3(kBlock), sizeof(double) * gi.maxj, s,
                     dd, dp, R);
         else
-            KLAUNCH((kb_update<double>), blocks(BP_UPDATE), dim3(kBlock), sizeof(double) * maxj,
-                    b->s, dd, dp, R);
+            KLAUNCH((kb_update<double>), blocks(BP_UPDATE), dim3(kBlock), sizeof(double) * gi.maxj, s,
+                    dd, dp, R);
     }
-    KLAUNCH(kb_sum2, blocks(BP_SUM2), dim3(kBlock), 0, b->s, dd, dp, R);
+    KLAUNCH(kb_sum2, blocks(BP_SUM2), dim3(kBlock), 0, s, dd, dp, R);
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
+}
+
+// Runs every recorded step of the batch, in the order form_residual +
+// run_proj + run_update + k_sum2 run them for a single context.  Steps are
+// grouped by (lanes per row, basis precision); all groups are staged in
+// one host buffer and copied to the device once, then each group's kernel
+// set is launched.  Steps whose operator has no vector-CSR kernel never get
+// here (trlhip_step_cgs runs them directly).
+int batch_launch(trlhip_batch *b)
+{
+    // group the steps (stable)
+    std::vector<BatchStep> order;
+    std::vector<GroupInfo> groups;
+    order.reserve(b->steps.size());
+    std::vector<char> taken(b->steps.size(), 0);
+    size_t npre = 0;
+    for (size_t i = 0; i < b->steps.size(); ++i) {
+        if (taken[i]) continue;
+        GroupInfo gi{};
+        gi.lpr = b->steps[i].A->lpr;
+        gi.mixed = b->steps[i].c->V32 != nullptr;
+        gi.doff = order.size();
+        gi.poff = npre;
+        for (size_t k = i; k < b->steps.size(); ++k) {
+            const BatchStep &st = b->steps[k];
+            if (!taken[k] && st.A->lpr == gi.lpr && (st.c->V32 != nullptr) == gi.mixed) {
+                order.push_back(st);
+                taken[k] = 1;
+            }
+        }
+        gi.R = (int)(order.size() - gi.doff);
+        npre += (size_t)BP_COUNT * (gi.R + 1);
+        groups.push_back(gi);
+    }
+    const size_t R = order.size();
+    if (R > b->cap || npre > b->pcap) {
+        if (b->hdesc) (void)hipHostFree(b->hdesc);
+        if (b->ddesc) (void)hipFree(b->ddesc);
+        if (b->hpre) (void)hipHostFree(b->hpre);
+        if (b->dpre) (void)hipFree(b->dpre);
+        b->hdesc = b->ddesc = nullptr;
+        b->hpre = b->dpre = nullptr;
+        b->cap = b->pcap = 0;
+        const size_t cap = R + R / 2 + 16;
+        const size_t pcap = npre + npre / 2 + 16 * BP_COUNT;
+        if (hipHostMalloc((void **)&b->hdesc, sizeof(BDesc) * cap, 0) != hipSuccess ||
+            hipMalloc((void **)&b->ddesc, sizeof(BDesc) * cap) != hipSuccess ||
+            hipHostMalloc((void **)&b->hpre, sizeof(int) * pcap, 0) != hipSuccess ||
+            hipMalloc((void **)&b->dpre, sizeof(int) * pcap) != hipSuccess)
+            return TRLHIP_ENOMEM;
+        b->cap = cap;
+        b->pcap = pcap;
+    }
+    for (GroupInfo &gi : groups)
+        batch_fill(order.data() + gi.doff, gi.R, b->hdesc + gi.doff, b->hpre + gi.poff, gi);
+    TRY(hipMemcpyAsync(b->ddesc, b->hdesc, sizeof(BDesc) * R, hipMemcpyHostToDevice, b->s));
+    TRY(hipMemcpyAsync(b->dpre, b->hpre, sizeof(int) * npre, hipMemcpyHostToDevice, b->s));
+    for (const GroupInfo &gi : groups)
+        if (int rc = batch_run(b->s, gi, b->ddesc + gi.doff, b->dpre + gi.poff, b->hpre + gi.poff))
+            return rc;
+    return 0;
 }
 
 }  // namespace
@@ -2022,23 +2070,7 @@ int trlhip_batch_flush(trlhip_batch *b)
 {
     if (!b) return TRLHIP_EINVAL;
     int rc = b->err;
-    if (!rc && !b->steps.empty()) {
-        // one launch set per (lanes per row, basis precision) group
-        std::vector<BatchStep> all;
-        all.swap(b->steps);
-        while (!all.empty() && !rc) {
-            const int lpr = all[0].A->lpr;
-            const bool mixed = all[0].c->V32 != nullptr;
-            std::vector<BatchStep> rest;
-            for (const BatchStep &st : all) {
-                if (st.A->lpr == lpr && (st.c->V32 != nullptr) == mixed) b->steps.push_back(st);
-                else rest.push_back(st);
-            }
-            rc = batch_launch(b);
-            b->steps.clear();
-            all.swap(rest);
-        }
-    }
+    if (!rc && !b->steps.empty()) rc = batch_launch(b);
     if (hipStreamSynchronize(b->s) != hipSuccess && !rc) rc = TRLHIP_EHIP;
     stats_resolve();
     b->steps.clear();
