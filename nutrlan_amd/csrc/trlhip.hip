@@ -28,6 +28,7 @@ namespace {
 
 constexpr int kBlock = 256;            // 4 waves of 64
 constexpr int kMaxGrid = 1024;         // partial slots per streaming reduction
+constexpr int kSpmvGrid = 2048;        // SpMV blocks (and alpha partial slots)
 constexpr int kProjGrid = 2048;        // row blocks (partial rows) of the projection
 constexpr int kProjBlocks = 4096;      // projection blocks wanted over all column chunks
 constexpr int kNnzBlk = 2048;          // nonzeros staged per SpMV row block
@@ -191,15 +192,34 @@ __global__ __launch_bounds__(kBlock) void k_colsum(const double *__restrict__ pa
 // by the whole block.  Epilogue: the block's partial of Re(x^H y) in
 // part[blockIdx.x] (summed by the consumer).
 // S = vector storage, SV = matrix value storage, E = doubles per element.
-template <typename S, typename SV, int E>
-__global__ __launch_bounds__(kBlock) void k_spmv(
+// FUSE: x is fl(x_src * xscale) formed on the fly from the unscaled
+// source (see spmv_vec_body); the row owner also stores it to xq (and its
+// FP32 rounding to xq32).
+template <typename S, typename SV, int E, bool FUSE>
+__device__ __forceinline__ void spmv_stream_body(
     int nb, const int *__restrict__ rb, const int *__restrict__ rp, const int *__restrict__ ci,
     const SV *__restrict__ val, const S *__restrict__ x, S *__restrict__ y,
-    double *__restrict__ part)
+    double *__restrict__ part, double xscale, S *__restrict__ xq, float *__restrict__ xq32, int bid,
+    int nblk)
 {
     __shared__ double prod[kNnzBlk * E];
+    auto xv = [&](int c, int e) -> double {
+        if constexpr (FUSE)
+            return (double)(S)((double)x[(size_t)E * c + e] * xscale);
+        else
+            return (double)x[(size_t)E * c + e];
+    };
+    auto own = [&](int row) {  // the row's x value, stored when fused
+        if constexpr (FUSE) {
+            for (int e = 0; e < E; ++e) {
+                const double v = xv(row, e);
+                if (xq) xq[(size_t)E * row + e] = (S)v;
+                if (E == 1 && xq32) xq32[row] = (float)v;
+            }
+        }
+    };
     double acc = 0.0;
-    for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+    for (int blk = bid; blk < nb; blk += nblk) {
         const int r0 = rb[blk], r1 = rb[blk + 1];
         const int k0 = rp[r0], nz = rp[r1] - k0;
         if (r1 - r0 > 1 || nz <= kNnzBlk) {
@@ -207,10 +227,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
             for (int k = threadIdx.x; k < nz; k += kBlock) {
                 const int c = ci[k0 + k];
                 if constexpr (E == 1) {
-                    prod[k] = (double)val[k0 + k] * (double)x[c];
+                    prod[k] = (double)val[k0 + k] * xv(c, 0);
                 } else {
                     const double ar = val[2 * (k0 + k)], ai = val[2 * (k0 + k) + 1];
-                    const double xr = x[2 * c], xi = x[2 * c + 1];
+                    const double xr = xv(c, 0), xi = xv(c, 1);
                     prod[2 * k] = ar * xr - ai * xi;
                     prod[2 * k + 1] = ar * xi + ai * xr;
                 }
@@ -220,11 +240,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
             if (t < r1 - r0) {
                 const int row = r0 + t;
                 const int a = rp[row] - k0, b = rp[row + 1] - k0;
+                own(row);
                 if constexpr (E == 1) {
                     double s = 0.0;
                     for (int k = a; k < b; ++k) s += prod[k];
                     y[row] = (S)s;
-                    acc += (double)x[row] * (double)(S)s;
+                    acc += xv(row, 0) * (double)(S)s;
                 } else {
                     double sr = 0.0, si = 0.0;
                     for (int k = a; k < b; ++k) {
@@ -233,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
                     }
                     y[2 * row] = sr;
                     y[2 * row + 1] = si;
-                    acc += x[2 * row] * sr + x[2 * row + 1] * si;
+                    acc += xv(row, 0) * sr + xv(row, 1) * si;
                 }
             }
             __syncthreads();
@@ -242,29 +263,40 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
             for (int k = threadIdx.x; k < nz; k += kBlock) {
                 const int c = ci[k0 + k];
                 if constexpr (E == 1) {
-                    sr += (double)val[k0 + k] * (double)x[c];
+                    sr += (double)val[k0 + k] * xv(c, 0);
                 } else {
                     const double ar = val[2 * (k0 + k)], ai = val[2 * (k0 + k) + 1];
-                    sr += ar * x[2 * c] - ai * x[2 * c + 1];
-                    si += ar * x[2 * c + 1] + ai * x[2 * c];
+                    sr += ar * xv(c, 0) - ai * xv(c, 1);
+                    si += ar * xv(c, 1) + ai * xv(c, 0);
                 }
             }
             sr = block_sum(sr);
             if (E == 2) si = block_sum(si);
             if (threadIdx.x == 0) {
+                own(r0);
                 if constexpr (E == 1) {
                     y[r0] = (S)sr;
-                    acc += (double)x[r0] * (double)(S)sr;
+                    acc += xv(r0, 0) * (double)(S)sr;
                 } else {
                     y[2 * r0] = sr;
                     y[2 * r0 + 1] = si;
-                    acc += x[2 * r0] * sr + x[2 * r0 + 1] * si;
+                    acc += xv(r0, 0) * sr + xv(r0, 1) * si;
                 }
             }
         }
     }
     const double bs = block_sum(acc);
-    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+    if (threadIdx.x == 0) part[bid] = bs;
+}
+
+template <typename S, typename SV, int E, bool FUSE>
+__global__ __launch_bounds__(kBlock) void k_spmv(
+    int nb, const int *__restrict__ rb, const int *__restrict__ rp, const int *__restrict__ ci,
+    const SV *__restrict__ val, const S *__restrict__ x, S *__restrict__ y,
+    double *__restrict__ part, double xscale, S *__restrict__ xq, float *__restrict__ xq32)
+{
+    spmv_stream_body<S, SV, E, FUSE>(nb, rb, rp, ci, val, x, y, part, xscale, xq, xq32, blockIdx.x,
+                                     gridDim.x);
 }
 
 // y = A x, vector CSR: LPR lanes per row (a power of two), 64/LPR rows per
@@ -834,7 +866,8 @@ struct trlhip_csr {
     int vtype;
     int E;          // doubles per value (complex 2)
     int nb;         // SpMV row blocks (CSR-stream kernel)
-    int lpr;        // lanes per row of the vector-CSR kernel; 0: CSR-stream
+    int lpr;        // lanes per row of the vector-CSR kernel in use; 0: CSR-stream
+    int lpr_vec;    // the vector kernel's lanes per row for this matrix (0: unsuitable)
     int vgrid;      // blocks of the vector-CSR kernel
     int *rb;        // nb + 1 row-block starts
     int *rp;
@@ -936,10 +969,13 @@ int ensure_part(trlhip_ctx *c, size_t need)
 // Partial-sum regions of c->part: alpha partials (SpMV / dot), update
 // partials (||r||^2), projection partial rows.
 double *part_alpha(trlhip_ctx *c) { return c->part; }
-double *part_update(trlhip_ctx *c) { return c->part + kMaxGrid; }
-double *part_recur(trlhip_ctx *c) { return c->part + 2 * kMaxGrid; }
-double *part_proj(trlhip_ctx *c) { return c->part + 3 * kMaxGrid; }
-size_t part_need(int E, int ncols) { return 3 * (size_t)kMaxGrid + (size_t)kProjGrid * E * (ncols + 1) + 64; }
+double *part_update(trlhip_ctx *c) { return c->part + kSpmvGrid; }
+double *part_recur(trlhip_ctx *c) { return c->part + kSpmvGrid + kMaxGrid; }
+double *part_proj(trlhip_ctx *c) { return c->part + kSpmvGrid + 2 * kMaxGrid; }
+size_t part_need(int E, int ncols)
+{
+    return (size_t)kSpmvGrid + 2 * (size_t)kMaxGrid + (size_t)kProjGrid * E * (ncols + 1) + 64;
+}
 
 double spmv_bytes(const trlhip_csr *A, int E, size_t vsz, size_t xsz)
 {
@@ -950,7 +986,7 @@ double spmv_bytes(const trlhip_csr *A, int E, size_t vsz, size_t xsz)
 int spmv_grid(const trlhip_csr *A)
 {
     if (A->lpr) return A->vgrid;
-    return A->nb < kMaxGrid ? A->nb : kMaxGrid;
+    return A->nb < kSpmvGrid ? A->nb : kSpmvGrid;
 }
 
 template <int E, bool FUSE>
@@ -990,14 +1026,24 @@ int launch_spmv(const trlhip_csr *A, int vtype, const void *x, void *y, double *
         }
         return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
     }
-    if (xq) return TRLHIP_EINVAL;
     const dim3 g(spmv_grid(A)), b(kBlock);
-    if (cplx)
-        KLAUNCH((k_spmv<double, double, 2>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci, A->val,
-                (const double *)x, (double *)y, part);
-    else
-        KLAUNCH((k_spmv<double, double, 1>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci, A->val,
-                (const double *)x, (double *)y, part);
+    const double *xs = (const double *)x;
+    double *ys = (double *)y, *q = (double *)xq;
+    if (cplx) {
+        if (xq)
+            KLAUNCH((k_spmv<double, double, 2, true>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
+                    A->val, xs, ys, part, xscale, q, (float *)nullptr);
+        else
+            KLAUNCH((k_spmv<double, double, 2, false>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
+                    A->val, xs, ys, part, xscale, q, (float *)nullptr);
+    } else {
+        if (xq)
+            KLAUNCH((k_spmv<double, double, 1, true>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
+                    A->val, xs, ys, part, xscale, q, xq32);
+        else
+            KLAUNCH((k_spmv<double, double, 1, false>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
+                    A->val, xs, ys, part, xscale, q, (float *)nullptr);
+    }
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
@@ -1223,8 +1269,8 @@ std::vector<int> row_blocks(int n, const int *rp)
 
 // Parameters of one recorded step (device copy, one per request).
 struct BDesc {
-    int n, ld, jnd, np_alpha, g_upd, gx, gy, rm, rc0;
-    const int *rp, *ci;
+    int n, ld, jnd, np_alpha, g_upd, gx, gy, rm, rc0, nb;
+    const int *rb, *rp, *ci;
     const double *val;
     const double *x;      // SpMV input: the basis column q_a, or the unscaled scratch column
     double xscale;        // 1, or the deferred normalisation of the scratch column
@@ -1259,6 +1305,14 @@ __device__ __forceinline__ int batch_find(const int *__restrict__ pre, int nreq,
     const int p_ = batch_find(pre_, nreq, blockIdx.x);                        \
     const int bid_ = blockIdx.x - pre_[p_], nblk_ = pre_[p_ + 1] - pre_[p_];  \
     const BDesc &q = d[p_];
+
+__global__ __launch_bounds__(kBlock) void kb_spmv_stream(const BDesc *__restrict__ d, const int *pre,
+                                                        int nreq)
+{
+    BATCH_LOCATE(BP_SPMV)
+    spmv_stream_body<double, double, 1, true>(q.nb, q.rb, q.rp, q.ci, q.val, q.x, q.r, q.part_alpha,
+                                              q.xscale, q.xq, q.xq32, bid_, nblk_);
+}
 
 template <int LPR>
 __global__ __launch_bounds__(kBlock) void kb_spmv(const BDesc *__restrict__ d, const int *pre, int nreq)
@@ -1333,8 +1387,8 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
         d.n = n;
         d.ld = c->ld;
         d.jnd = jnd;
-        d.np_alpha = A->vgrid;
-        c->np_alpha = A->vgrid;
+        d.np_alpha = spmv_grid(A);
+        c->np_alpha = d.np_alpha;
         d.g_upd = grid_for(n, kBlock * 4, kMaxGrid);
         d.gx = proj_gx(c, jnd, 8);
         d.gy = (jnd + 7) / 8;
@@ -1354,6 +1408,8 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
             d.rh = nullptr;
             d.rh0 = 0.0;
         }
+        d.nb = A->nb;
+        d.rb = A->rb;
         d.rp = A->rp;
         d.ci = A->ci;
         d.val = A->val;
@@ -1372,7 +1428,7 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
         d.dh = c->dh;
         d.dscal = c->dscal;
         d.dres = c->dres;
-        const int nb[BP_COUNT] = {A->vgrid, d.g_upd, d.gx * d.gy, jnd, d.g_upd, 2};
+        const int nb[BP_COUNT] = {d.np_alpha, d.g_upd, d.gx * d.gy, jnd, d.g_upd, 2};
         for (int ph = 0; ph < BP_COUNT; ++ph) {
             int *pp = pre + (size_t)ph * (R + 1);
             pp[i + 1] = pp[i] + nb[ph];
@@ -1396,6 +1452,7 @@ int batch_run(hipStream_t s, const GroupInfo &gi, const BDesc *dd, const int *dp
     {
         Timed t(TRLHIP_K_SPMV, s, gi.bytes[BP_SPMV]);
         switch (gi.lpr) {
+        case 0: KLAUNCH(kb_spmv_stream, blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
         case 2: KLAUNCH((kb_spmv<2>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
         case 4: KLAUNCH((kb_spmv<4>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
         case 8: KLAUNCH((kb_spmv<8>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
@@ -1428,10 +1485,10 @@ int batch_run(hipStream_t s, const GroupInfo &gi, const BDesc *dd, const int *dp
 
 // Runs every recorded step of the batch, in the order form_residual +
 // run_proj + run_update + k_sum2 run them for a single context.  Steps are
-// grouped by (lanes per row, basis precision); all groups are staged in
-// one host buffer and copied to the device once, then each group's kernel
-// set is launched.  Steps whose operator has no vector-CSR kernel never get
-// here (trlhip_step_cgs runs them directly).
+// grouped by (SpMV kernel: lanes per row, 0 for CSR-stream; basis
+// precision); all groups are staged in one host buffer and copied to the
+// device once, then each group's kernel set is launched.  Complex steps
+// never get here (trlhip_step_cgs runs them directly).
 int batch_launch(trlhip_batch *b)
 {
     // group the steps (stable)
@@ -1528,7 +1585,11 @@ static void choose_spmv(trlhip_csr *A, const int *row_ptr)
     const double avg = (double)A->nnz / n;
     int lpr = 2;
     while (lpr < 32 && lpr < avg) lpr <<= 1;
-    A->lpr = maxlen <= 8 * lpr ? lpr : 0;
+    // the row-block (CSR-stream) kernel is the default: measured faster on
+    // regular stencils too (6.0 vs 4.0 TB/s on the 128^3 Laplacian); the
+    // vector kernel's lanes per row are kept for trlhip_csr_autotune
+    A->lpr_vec = maxlen <= 8 * lpr ? lpr : 0;
+    A->lpr = 0;
     const int U = A->E == 1 ? 4 : 2;
     const long rows_per_block = (long)(64 / lpr) * U * (kBlock / 64);
     const long g = (n + rows_per_block - 1) / rows_per_block;
@@ -1604,7 +1665,7 @@ static int csr_matvec_host(trlhip_csr *A, int nrow, int ncol, const double *x, i
     if (!A->dx) {
         if (hipMalloc((void **)&A->dx, sizeof(double) * E * (size_t)A->n) != hipSuccess) return TRLHIP_ENOMEM;
         if (hipMalloc((void **)&A->dy, sizeof(double) * E * (size_t)A->n) != hipSuccess) return TRLHIP_ENOMEM;
-        if (hipMalloc((void **)&A->part, sizeof(double) * (size_t)kMaxGrid) != hipSuccess) return TRLHIP_ENOMEM;
+        if (hipMalloc((void **)&A->part, sizeof(double) * (size_t)kSpmvGrid) != hipSuccess) return TRLHIP_ENOMEM;
         if (hipMalloc((void **)&A->dscal, sizeof(double) * 4) != hipSuccess) return TRLHIP_ENOMEM;
     }
     for (int j = 0; j < ncol; ++j) {
@@ -1812,7 +1873,7 @@ static int form_residual(trlhip_ctx *c, const trlhip_csr *A, int qa, int first, 
         m = 1;
     }
     int rc;
-    if (A && A->lpr && c->pend_dst == qa) {
+    if (A && c->pend_dst == qa) {
         rc = run_spmv_fused(c, A, qa, rcol);
     } else {
         rc = flush_pending(c);
@@ -1866,7 +1927,7 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
     if (!first && qb >= 0 && qb != qa - 1) return TRLHIP_EINVAL;
     if (jnd < 1 || jnd > c->ncols || (rcol >= 0 && rcol < jnd)) return TRLHIP_EINVAL;
     if (A && !csr_fits(c, A)) return TRLHIP_EINVAL;
-    if (c->batch && A && A->lpr && c->E == 1) {
+    if (c->batch && A && c->E == 1) {
         // batched: record the step; trlhip_batch_flush runs it with the
         // other contexts' steps
         if (first && jnd > 1) {
