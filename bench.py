@@ -132,6 +132,99 @@ def leg(T, A, x0, steps, rp, ci, v, stats_every=0):
     return t1 - t0, g, good, stats
 
 
+def batched_anderson2d(args):
+    """The batched workload: 1024 independent periodic 2D Anderson lattices,
+    L ~ U{64..256}, hopping -1, on-site U[-W/2, W/2] with W ~ U[1, 12]
+    (seeded), 8 lowest eigenpairs each, maxlan 32, tol 1e-8.  value = the
+    mixed-precision batched solver started from the learned net (inference
+    inside the timed region); beside it the plain FP64 solver one lattice at
+    a time (timed on a sample of the same lattices), the plain FP64 batched
+    solver and the mixed batched solver from random starts."""
+    import numpy as np
+    import scipy.sparse as sp
+
+    from nutrlan_amd import learned as LN
+    from nutrlan_amd import trlan as T
+    from nutrlan_amd import workloads as W
+
+    rng = np.random.default_rng(5)
+    count = args.models
+    Ls = rng.integers(64, 257, count)
+    Ws = rng.uniform(1.0, 12.0, count)
+    models = [LN.lattice(int(L), float(w), 7000 + i) for i, (L, w) in enumerate(zip(Ls, Ws))]
+    nev, maxlan, tol = LN.NEV, 32, 1e-8
+    starts = [W.start_vector(len(c[0]) - 1, seed=9000 + i) for i, (c, d) in enumerate(models)]
+    plain = [T.Csr(*c) for c, d in models]
+    mixed = [T.Csr(*c, mixed=True) for c, d in models]
+    net = LN.load_net()
+    assert net is not None, "learned-start weights missing (run nutrlan_amd/train_start_net.py)"
+
+    def good_pairs(res, mods=models):
+        g = 0
+        for (c, d), r in zip(mods, res):
+            rp, ci, v = c
+            n = len(rp) - 1
+            A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+            X = r["evec"][:, :r["nec"]]
+            R = A @ X - X * r["eval"][:r["nec"]]
+            g += int(((R * R).sum(axis=0) ** 0.5 <= 1e-8 * r["anrm"]).sum())
+        return g
+
+    def batch(ops, x0s):
+        probs = [(A, -1, nev, maxlan, tol, 1, 100000, nev, x) for A, x in zip(ops, x0s)]
+        t0 = time.perf_counter()
+        rc, res, _ = T.solve_batch(probs)
+        el = time.perf_counter() - t0
+        assert rc == 0, rc
+        return el, res
+
+    batch(mixed[:64], starts[:64])  # warm-up
+    # headline: learned start + mixed precision, batched
+    t0 = time.perf_counter()
+    x_net = LN.start_vectors(net, [d for c, d in models])
+    t_inf = time.perf_counter() - t0
+    el, res = batch(mixed, x_net)
+    el += t_inf
+    good = good_pairs(res)
+    mv_net = sum(r["matvec"] for r in res)
+    el_r, res_r = batch(mixed, starts)
+    good_r = good_pairs(res_r)
+    mv_rand = sum(r["matvec"] for r in res_r)
+    el_p, res_p = batch(plain, starts)
+    good_p = good_pairs(res_p)
+    # plain FP64, one lattice at a time, on a sample
+    ns = min(args.plain_sample, count)
+    t0 = time.perf_counter()
+    res_s = [T.solve_csr(plain[i], -1, nev, maxlan, tol, 1, 100000, nev, starts[i]) for i in range(ns)]
+    el_s = time.perf_counter() - t0
+    g_s = good_pairs(res_s, models[:ns])
+    plain_single = g_s / el_s
+    value = good / el
+    for A in plain + mixed:
+        A.close()
+    net.close()
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "eigenpairs/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": round(1e3 * el, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "anderson2d_batch_%d" % count, "L": "U{64..256}",
+                   "W": "U[1,12]", "sites_total": int(sum(len(c[0]) - 1 for c, d in models)),
+                   "nev": nev, "lohi": -1, "maxlan": maxlan, "tol": tol,
+                   "path": "batched, mixed precision, learned start",
+                   "converged_at_1e-8": good, "matvec_total": mv_net,
+                   "learned_start_inference_s": round(t_inf, 3)},
+        "mixed_batched_random_start": {"value": round(good_r / el_r, 1), "s": round(el_r, 3),
+                                       "matvec_total": mv_rand, "converged_at_1e-8": good_r},
+        "plain_batched_random_start": {"value": round(good_p / el_p, 1), "s": round(el_p, 3),
+                                       "converged_at_1e-8": good_p},
+        "plain_fp64_one_at_a_time": {"value": round(plain_single, 2),
+                                     "sample": f"first {ns} lattices, {el_s:.2f} s"},
+        "vs_plain_fp64": round(value / plain_single, 2),
+        "learned_vs_random_matvec": round(mv_net / mv_rand, 4),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +234,11 @@ def main():
                     help="solver path measured for `value` (the other is reported beside it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=96)
+    ap.add_argument("--workload", choices=["laplace3d", "anderson2d-batch"], default="laplace3d",
+                    help="laplace3d: the 128^3 Laplacian (default line); anderson2d-batch: "
+                         "1024 uneven 2D Anderson lattices through the batched solver")
+    ap.add_argument("--models", type=int, default=1024)
+    ap.add_argument("--plain-sample", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): the multi-GPU "
@@ -152,6 +250,9 @@ def main():
     from nutrlan_amd import trlan as T
 
     assert T.device_count() > 0, "no GPU"
+    if args.workload == "anderson2d-batch":
+        batched_anderson2d(args)
+        return
     rp, ci, v, x0 = build_problem()
     n = len(rp) - 1
     ops = {"plain": T.Csr(rp, ci, v), "mixed": T.Csr(rp, ci, v, mixed=True)}

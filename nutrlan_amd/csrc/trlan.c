@@ -15,12 +15,15 @@
  * them between evec and base, SRC/trlcore.c:883-910).
  */
 #include <float.h>
+#include <stdio.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <pthread.h>
 #include <ucontext.h>
+#include <unistd.h>
 
 #include "trl_kit.h"
 #include "trlan.h"
@@ -45,9 +48,20 @@ double trl_rand_uniform(uint64_t *state)
     return (double)((x * 0x2545F4914F6CDD1Dull) >> 11) * (1.0 / 9007199254740992.0);
 }
 
+/* Clock ticks (CLOCKS_PER_SEC per second) of wall time for the trl_info
+ * timers.  The reference reads clock() (process CPU time); here the work
+ * runs on the device, and clock() is a system call on the per-step path,
+ * so the timers count monotonic wall time (a vDSO read) in the same unit. */
+static clock_t lz_clock(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (clock_t)((double)ts.tv_sec * CLOCKS_PER_SEC + (double)ts.tv_nsec * (CLOCKS_PER_SEC / 1e9));
+}
+
 static void add_ticks(trl_info *info, clock_t *acc, double *spill, clock_t t0)
 {
-    const clock_t t1 = clock();
+    const clock_t t1 = lz_clock();
     const clock_t d = t1 >= t0 ? t1 - t0 : 0;
     if (*acc + d >= *acc) {
         *acc += d;
@@ -545,10 +559,10 @@ static void lz_write_checkpoint(lz_t *z, int jnd, int rcol)
     char file[160];
     trlkit_pe_filename(file, sizeof file, info->cpfile);
     lz_colio_t io = {z, 0, jnd, rcol};
-    const clock_t c1 = clock();
+    const clock_t c1 = lz_clock();
     const int ii = trlkit_write_checkpoint(file, z->n, jnd, z->alpha, z->beta, jnd + 1, lz_get_col,
                                            &io, z->hx);
-    const clock_t c2 = clock();
+    const clock_t c2 = lz_clock();
     info->clk_out += c2 >= c1 ? c2 - c1 : 0;
     info->wrds_out += jnd * (z->n + z->n + 2) + z->n + 2;
     info->stat = ii;
@@ -571,10 +585,10 @@ static int lz_initial_guess(lz_t *z, const double *evec, int lde, int *j1)
         trlkit_pe_filename(file, sizeof file, strlen(info->oldcpf) > 0 ? info->oldcpf : info->cpfile);
         lz_colio_t io = {z, j, 0, 0};
         int jnd = 0;
-        const clock_t c1 = clock();
+        const clock_t c1 = lz_clock();
         const int rc = trlkit_read_checkpoint(file, z->n, z->mev - j, z->ncols - 1 - j, &jnd,
                                               z->alpha + j, z->beta + j, lz_put_col, &io, x);
-        const clock_t c2 = clock();
+        const clock_t c2 = lz_clock();
         info->clk_in = c2 >= c1 ? c2 - c1 : 0;
         info->wrds_in = jnd * (z->n + z->n + 2) + z->n + 2;
         info->stat = rc;
@@ -693,7 +707,7 @@ static void lz_free(lz_t *z)
 static int lz_matvec_step(lz_t *z, int first, int qa, int qb, double beta_prev, int jnd,
                           int rcol, double *alpha, double *rr2, double *pass1)
 {
-    clock_t t0 = clock();
+    clock_t t0 = lz_clock();
     if (!z->A) {
         if (lz_apply_host_op(z, qa, rcol)) return TRLHIP_STAT_DEVICE;
     }
@@ -779,7 +793,7 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
         if (lz_matvec_step(z, 1, qa, -1, 0.0, jnd, rr, &a, &rr2, pass1)) return;
         alpha[jnd - 1] = a;
         beta[jnd - 1] = a;
-        clock_t t0 = clock();
+        clock_t t0 = lz_clock();
         lz_orth(z, jnd, kept, rr, rr2, pass1);
         add_ticks(info, &info->clk_orth, &info->tick_h, t0);
         if (info->stat != 0) return;
@@ -801,7 +815,7 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
             if (lz_matvec_step(z, 0, qa, qb, beta[jnd - 2], jnd, rr, &a, &rr2, pass1)) return;
             alpha[jnd - 1] = a;
             info->flop += 6 * z->n;
-            t0 = clock();
+            t0 = lz_clock();
             lz_orth(z, jnd, kept, rr, rr2, pass1);
             add_ticks(info, &info->clk_orth, &info->tick_h, t0);
             alfrot[jnd - 1] = alpha[jnd - 1];
@@ -821,7 +835,7 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
             }
         }
         /* ---- restart or finish */
-        clock_t tr = clock();
+        clock_t tr = lz_clock();
         const int prek = kept;
         const int jml = jnd - locked;
         i2 = kept - locked + 1;
@@ -1018,7 +1032,7 @@ static int lz_setup(lz_t *z, trl_matvec op, ztrl_matvec zop, trl_uo user_ortho, 
                     int nrow, int mev, int lde, int cplx, trlhip_batch *b)
 {
     memset(z, 0, sizeof *z);
-    info->clk_tot = clock();
+    info->clk_tot = lz_clock();
     if (info->ned > mev) {
         printf("info->ned (%d) is larger than mev (%d) reducing info->ned to %d\n", info->ned,
                mev, mev);
@@ -1062,7 +1076,7 @@ static void lz_teardown(lz_t *z)
     trl_info *info = z->info;
     trlhip_ctx_destroy(z->dev);
     lz_free(z);
-    const clock_t t_end = clock();
+    const clock_t t_end = lz_clock();
     info->tick_t += (double)(t_end - info->clk_tot) / (double)info->clk_rate;
     info->clk_tot = 0;
 }
@@ -1098,93 +1112,211 @@ typedef struct {
     void *stack;
     lz_t z;
     trl_batch_job *job;
-    int done;
+    int done;     /* lz_iterate returned */
+    int ended;    /* torn down */
 } lz_co_t;
 
-typedef struct {
+/* One host thread of the scheduler: it owns the coroutines i with
+ * i % nthr == tid and resumes them every round. */
+typedef struct lz_worker {
     ucontext_t sched;
     lz_co_t *cur;
-} lz_sched_t;
+    struct lz_pool *pool;
+    int tid;
+    double busy;  /* seconds spent resuming its coroutines */
+} lz_worker_t;
 
-static lz_sched_t *g_sched;  /* the running scheduler (one batch at a time per process) */
+typedef struct lz_pool {
+    lz_co_t *cos;
+    int njobs, nthr;
+    lz_worker_t *w;
+    pthread_barrier_t start, stop;
+    volatile int quit;
+} lz_pool_t;
+
+static __thread lz_worker_t *tl_worker;  /* the scheduler thread running a coroutine */
+
+static double lz_now(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
 
 static void lz_co_yield(void *user)
 {
-    lz_sched_t *sc = (lz_sched_t *)user;
-    lz_co_t *co = sc->cur;
-    swapcontext(&co->uc, &sc->sched);
+    (void)user;
+    lz_worker_t *w = tl_worker;
+    swapcontext(&w->cur->uc, &w->sched);
 }
 
 static void lz_co_main(void)
 {
-    lz_co_t *co = g_sched->cur;
+    lz_co_t *co = tl_worker->cur;
     trl_batch_job *j = co->job;
     lz_iterate(&co->z, j->eval, j->evec, j->lde, j->wrk, j->lwrk);
     co->done = 1;
-    /* returning resumes uc_link, the scheduler */
+    /* back to this thread's scheduler for good (a finished coroutine is
+     * never resumed; returning would follow uc_link instead) */
+    for (;;) swapcontext(&co->uc, &tl_worker->sched);
+}
+
+/* Resume every unfinished coroutine of this worker once (each runs until
+ * it waits on the device or finishes). */
+static void lz_round(lz_worker_t *w)
+{
+    tl_worker = w;
+    const double t0 = lz_now();
+    lz_pool_t *p = w->pool;
+    for (int i = w->tid; i < p->njobs; i += p->nthr) {
+        lz_co_t *co = &p->cos[i];
+        if (co->done) continue;
+        w->cur = co;
+        swapcontext(&w->sched, &co->uc);
+    }
+    w->busy += lz_now() - t0;
+}
+
+static void *lz_worker_main(void *arg)
+{
+    lz_worker_t *w = (lz_worker_t *)arg;
+    for (;;) {
+        pthread_barrier_wait(&w->pool->start);
+        if (w->pool->quit) break;
+        lz_round(w);
+        pthread_barrier_wait(&w->pool->stop);
+    }
+    return NULL;
+}
+
+/* Host threads for the coroutines: TRLHIP_BATCH_THREADS, else up to 16
+ * (and no more than the online CPUs or the jobs). */
+static int lz_batch_threads(int njobs)
+{
+    const char *e = getenv("TRLHIP_BATCH_THREADS");
+    long n = e ? strtol(e, NULL, 10) : 0;
+    if (n <= 0) {
+        n = sysconf(_SC_NPROCESSORS_ONLN);
+        if (n > 16) n = 16;
+    }
+    if (n < 1) n = 1;
+    if (n > njobs) n = njobs;
+    return (int)n;
 }
 
 int trlan_batch(trl_batch_job *jobs, int njobs)
 {
     if (njobs <= 0) return 0;
-    if (!jobs || g_sched) return TRLHIP_EINVAL;
+    if (!jobs) return TRLHIP_EINVAL;
     trlhip_batch *b = NULL;
     int rc = trlhip_batch_create(&b);
     if (rc) {
         for (int i = 0; i < njobs; ++i) jobs[i].info->stat = TRLHIP_STAT_DEVICE;
         return rc;
     }
-    lz_sched_t sc;
-    memset(&sc, 0, sizeof sc);
-    trlhip_batch_set_yield(b, lz_co_yield, &sc);
+    trlhip_batch_set_yield(b, lz_co_yield, NULL);
     lz_co_t *cos = (lz_co_t *)calloc((size_t)njobs, sizeof(lz_co_t));
     if (!cos) {
         trlhip_batch_destroy(b);
         return TRLHIP_ENOMEM;
     }
-    g_sched = &sc;
     int live = 0;
     for (int i = 0; i < njobs; ++i) {
         trl_batch_job *j = &jobs[i];
         lz_co_t *co = &cos[i];
         co->job = j;
         co->done = 1;
+        co->ended = 1;
         j->info->mvparam = j->A;
         if (!lz_setup(&co->z, trlhip_csr_matvec, NULL, NULL, j->info, j->nrow, j->mev, j->lde, 0,
                       b))
             continue;
+        co->ended = 0;
         co->stack = malloc(LZ_STACK);
         if (!co->stack || getcontext(&co->uc) != 0) {
             j->info->stat = -4;
-            lz_teardown(&co->z);
             continue;
         }
         co->uc.uc_stack.ss_sp = co->stack;
         co->uc.uc_stack.ss_size = LZ_STACK;
-        co->uc.uc_link = &sc.sched;
         makecontext(&co->uc, lz_co_main, 0);
         co->done = 0;
         ++live;
     }
-    while (live > 0) {
-        for (int i = 0; i < njobs; ++i) {
-            lz_co_t *co = &cos[i];
-            if (co->done) continue;
-            sc.cur = co;
-            swapcontext(&sc.sched, &co->uc);
-            if (co->done) {
-                lz_teardown(&co->z);
-                free(co->stack);
-                co->stack = NULL;
-                --live;
-            }
+    lz_pool_t pool;
+    memset(&pool, 0, sizeof pool);
+    pool.cos = cos;
+    pool.njobs = njobs;
+    pool.nthr = lz_batch_threads(njobs);
+    pool.w = (lz_worker_t *)calloc((size_t)pool.nthr, sizeof(lz_worker_t));
+    pthread_t *th = (pthread_t *)calloc((size_t)pool.nthr, sizeof(pthread_t));
+    int nstarted = 0;
+    if (pool.w && th) {
+        pthread_barrier_init(&pool.start, NULL, (unsigned)pool.nthr);
+        pthread_barrier_init(&pool.stop, NULL, (unsigned)pool.nthr);
+        for (int t = 0; t < pool.nthr; ++t) {
+            pool.w[t].pool = &pool;
+            pool.w[t].tid = t;
         }
+        for (int t = 1; t < pool.nthr; ++t)
+            if (pthread_create(&th[t], NULL, lz_worker_main, &pool.w[t]) == 0) ++nstarted;
+    }
+    if (!pool.w || !th || nstarted != pool.nthr - 1) {
+        /* could not start the threads: stop any started, run on this one */
+        if (nstarted > 0) {
+            pool.quit = 1;
+            pthread_barrier_wait(&pool.start);
+            for (int t = 1; t <= nstarted; ++t) pthread_join(th[t], NULL);
+        }
+        if (pool.w && th) {
+            pthread_barrier_destroy(&pool.start);
+            pthread_barrier_destroy(&pool.stop);
+        }
+        free(th);
+        th = NULL;
+        if (!pool.w) pool.w = (lz_worker_t *)calloc(1, sizeof(lz_worker_t));
+        pool.nthr = 1;
+        pool.w[0].pool = &pool;
+        pool.w[0].tid = 0;
+    }
+    /* TRLHIP_BATCH_PROFILE=1: host time in the solves vs in the flushes */
+    const char *prof = getenv("TRLHIP_BATCH_PROFILE");
+    double t_co = 0.0, t_fl = 0.0;
+    long rounds = 0;
+    while (live > 0) {
+        const double t0 = lz_now();
+        if (pool.nthr > 1) pthread_barrier_wait(&pool.start);
+        lz_round(&pool.w[0]);
+        if (pool.nthr > 1) pthread_barrier_wait(&pool.stop);
+        live = 0;
+        for (int i = 0; i < njobs; ++i) live += !cos[i].done;
+        const double t1 = lz_now();
         if (live > 0) {
             const int frc = trlhip_batch_flush(b);
             if (frc && !rc) rc = frc;
         }
+        t_co += t1 - t0;
+        t_fl += lz_now() - t1;
+        ++rounds;
     }
-    g_sched = NULL;
+    if (pool.nthr > 1) {
+        pool.quit = 1;
+        pthread_barrier_wait(&pool.start);
+        for (int t = 1; t < pool.nthr; ++t) pthread_join(th[t], NULL);
+        pthread_barrier_destroy(&pool.start);
+        pthread_barrier_destroy(&pool.stop);
+    }
+    if (prof && prof[0] == '1')
+        fprintf(stderr, "trlan_batch: %d host threads, %ld rounds, %.3f s in solves, %.3f s in flushes\n",
+                pool.nthr, rounds, t_co, t_fl);
+    /* teardown after the last round: freeing device memory synchronizes */
+    for (int i = 0; i < njobs; ++i) {
+        lz_co_t *co = &cos[i];
+        if (!co->ended) lz_teardown(&co->z);
+        free(co->stack);
+    }
+    free(th);
+    free(pool.w);
     free(cos);
     trlhip_batch_destroy(b);
     return rc;

@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -773,9 +774,11 @@ struct Stats {
     std::vector<hipEvent_t> pool;
 };
 Stats g_stats;
+std::mutex g_stats_mu;  // batched solves launch from several host threads
 
 hipEvent_t ev_get()
 {
+    std::lock_guard<std::mutex> lk(g_stats_mu);
     if (!g_stats.pool.empty()) {
         hipEvent_t e = g_stats.pool.back();
         g_stats.pool.pop_back();
@@ -789,6 +792,7 @@ hipEvent_t ev_get()
 // Resolve timings of completed launches (called after a stream sync).
 void stats_resolve()
 {
+    std::lock_guard<std::mutex> lk(g_stats_mu);
     for (auto &p : g_stats.pending) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) g_stats.ms[p.k] += ms;
@@ -804,7 +808,7 @@ void stats_resolve()
 struct LaunchEvents {
     hipEvent_t a = nullptr, b = nullptr;
 };
-LaunchEvents g_launch_ev;
+thread_local LaunchEvents g_launch_ev;
 
 #define KLAUNCH(kernel, grid, block, shm, stream, ...)                                      \
     do {                                                                                    \
@@ -821,6 +825,7 @@ LaunchEvents g_launch_ev;
 bool timed_pick(int k)
 {
     if (!g_stats.on) return false;
+    std::lock_guard<std::mutex> lk(g_stats_mu);
     const uint64_t i = (uint64_t)g_stats.seen[k]++;
     return g_stats.on == 1 || ((i * 0x9E3779B97F4A7C15ull) >> 40) % (uint64_t)g_stats.on == 0;
 }
@@ -838,6 +843,8 @@ struct Timed {
     }
     ~Timed()
     {
+        if (!a && !b) return;
+        std::lock_guard<std::mutex> lk(g_stats_mu);
         if (!a || !b) {
             if (a) g_stats.pool.push_back(a);
             if (b) g_stats.pool.push_back(b);
@@ -905,6 +912,7 @@ struct trlhip_ctx {
     double *hY;     // pinned staging for Y
     hipStream_t s;
     trlhip_batch *batch;  // batch the context steps in (its stream is the batch's), or null
+    int arena;            // memory comes from the batch's arenas (freed with the batch)
 };
 
 // One recorded Lanczos step of a batched context (see the batch section).
@@ -926,11 +934,20 @@ struct trlhip_batch {
     void (*yield)(void *) = nullptr;
     void *user = nullptr;
     int err = 0;                  // sticky device error of a flush
+    std::mutex mu;                // guards steps (coroutines run on several host threads)
     std::vector<BatchStep> steps;
     BDesc *hdesc = nullptr, *ddesc = nullptr;   // pinned staging / device copy
     int *hpre = nullptr, *dpre = nullptr;       // prefix tables of all groups
     size_t cap = 0;               // requests the descriptor buffers hold
     size_t pcap = 0;              // ints the prefix buffers hold
+    // arenas the batch's contexts allocate from (freed with the batch):
+    // device memory zeroed when a chunk is taken, and pinned host-coherent
+    // memory with its device address
+    struct Chunk {
+        char *p, *dp;
+        size_t size, used;
+    };
+    std::vector<Chunk> dev, host;
 };
 
 namespace {
@@ -950,6 +967,42 @@ template <typename S>
 S *colp(trlhip_ctx *c, int col)
 {
     return reinterpret_cast<S *>(c->V + (size_t)col * c->ld * c->esz);
+}
+
+// Bump allocation from a batch arena (256-byte aligned).  Device chunks of
+// at least 1 GiB are zeroed on the batch stream when taken; host chunks of
+// at least 64 MiB are pinned, host-coherent and mapped (dptr: the device
+// address of the returned block).
+void *arena_get(trlhip_batch *b, bool host, size_t bytes, void **dptr = nullptr)
+{
+    bytes = (bytes + 255) & ~(size_t)255;
+    auto &v = host ? b->host : b->dev;
+    if (v.empty() || v.back().size - v.back().used < bytes) {
+        const size_t sz = std::max(bytes, host ? (size_t)64 << 20 : (size_t)1 << 30);
+        trlhip_batch::Chunk ch{nullptr, nullptr, sz, 0};
+        if (host) {
+            if (hipHostMalloc((void **)&ch.p, sz, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
+                return nullptr;
+            if (hipHostGetDevicePointer((void **)&ch.dp, ch.p, 0) != hipSuccess) {
+                (void)hipHostFree(ch.p);
+                return nullptr;
+            }
+            memset(ch.p, 0, sz);
+        } else {
+            if (hipMalloc((void **)&ch.p, sz) != hipSuccess) return nullptr;
+            if (hipMemsetAsync(ch.p, 0, sz, b->s) != hipSuccess) {
+                (void)hipFree(ch.p);
+                return nullptr;
+            }
+            ch.dp = ch.p;
+        }
+        v.push_back(ch);
+    }
+    auto &ch = v.back();
+    void *r = ch.p + ch.used;
+    if (dptr) *dptr = ch.dp + ch.used;
+    ch.used += bytes;
+    return r;
 }
 
 int ensure_part(trlhip_ctx *c, size_t need)
@@ -1620,7 +1673,6 @@ int trlhip_csr_create(trlhip_csr **out, int n, long nnz, const int *row_ptr, con
     choose_spmv(A, row_ptr);
     const size_t nz1 = (size_t)(nnz > 0 ? nnz : 1);
     int rc = 0;
-    if (hipStreamCreate(&A->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipMalloc((void **)&A->rb, sizeof(int) * rb.size()) != hipSuccess) rc = TRLHIP_ENOMEM;
     if (!rc && hipMalloc((void **)&A->rp, sizeof(int) * (size_t)(n + 1)) != hipSuccess) rc = TRLHIP_ENOMEM;
     if (!rc && hipMalloc((void **)&A->ci, sizeof(int) * nz1) != hipSuccess) rc = TRLHIP_ENOMEM;
@@ -1663,6 +1715,8 @@ static int csr_matvec_host(trlhip_csr *A, int nrow, int ncol, const double *x, i
     const int E = A->E;
     const int vt = A->vtype == TRLHIP_C128 ? TRLHIP_C128 : TRLHIP_R64;
     if (!A->dx) {
+        // staging for host callers, created on first use
+        if (!A->s && hipStreamCreate(&A->s) != hipSuccess) return TRLHIP_EHIP;
         if (hipMalloc((void **)&A->dx, sizeof(double) * E * (size_t)A->n) != hipSuccess) return TRLHIP_ENOMEM;
         if (hipMalloc((void **)&A->dy, sizeof(double) * E * (size_t)A->n) != hipSuccess) return TRLHIP_ENOMEM;
         if (hipMalloc((void **)&A->part, sizeof(double) * (size_t)kSpmvGrid) != hipSuccess) return TRLHIP_ENOMEM;
@@ -1705,6 +1759,10 @@ void trlhip_csr_zmatvec(const int nrow, const int ncol, const void *x, const int
 int trlhip_ctx_destroy(trlhip_ctx *c)
 {
     if (!c) return 0;
+    if (c->arena) {  // the batch owns the memory and the stream
+        free(c);
+        return 0;
+    }
     if (c->s) (void)hipStreamSynchronize(c->s);
     if (c->V) (void)hipFree(c->V);
     if (c->V32) (void)hipFree(c->V32);
@@ -1742,12 +1800,37 @@ int trlhip_ctx_create_batched(trlhip_ctx **out, int n, int ncols, int vtype, trl
     const size_t ncoef = 2 * (size_t)(ncols + 2);
     const size_t vbytes = c->esz * (size_t)c->ld * (ncols + 1);  // + scratch column
     int rc = 0;
+    c->pend_dst = -1;
+    c->hcap = 16 + 2 * ncoef;
     if (b) {
+        // batched: everything from the batch's arenas, nothing to synchronize
         c->batch = b;
         c->s = b->s;
-    } else if (hipStreamCreate(&c->s) != hipSuccess) {
-        rc = TRLHIP_EHIP;
+        c->arena = 1;
+        const size_t np = part_need(c->E, ncols);
+        const size_t ny = (size_t)(ncols + 1) * (ncols + 1);
+        c->V = (char *)arena_get(b, false, vbytes);
+        if (vtype == TRLHIP_R32)
+            c->V32 = (float *)arena_get(b, false, sizeof(float) * (size_t)c->ld * (ncols + 1));
+        c->part = (double *)arena_get(b, false, sizeof(double) * np);
+        c->partcap = np;
+        c->dscal = (double *)arena_get(b, false, sizeof(double) * 16);
+        c->dcoef = (double *)arena_get(b, false, sizeof(double) * ncoef);
+        c->dh = (double *)arena_get(b, false, sizeof(double) * ncoef);
+        c->dY = (double *)arena_get(b, false, sizeof(double) * ny);
+        c->dYcap = ny;
+        c->hbuf = (double *)arena_get(b, true, sizeof(double) * c->hcap);
+        c->hres = (double *)arena_get(b, true, sizeof(double) * 8, (void **)&c->dres);
+        c->hY = (double *)arena_get(b, true, sizeof(double) * ny);
+        if (!c->V || (vtype == TRLHIP_R32 && !c->V32) || !c->part || !c->dscal || !c->dcoef ||
+            !c->dh || !c->dY || !c->hbuf || !c->hres || !c->hY) {
+            free(c);
+            return TRLHIP_ENOMEM;
+        }
+        *out = c;
+        return 0;
     }
+    if (hipStreamCreate(&c->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipMalloc((void **)&c->V, vbytes) != hipSuccess) rc = TRLHIP_ENOMEM;
     const size_t sbytes = sizeof(float) * (size_t)c->ld * (ncols + 1);
     if (!rc && vtype == TRLHIP_R32 && hipMalloc((void **)&c->V32, sbytes) != hipSuccess)
@@ -1757,8 +1840,6 @@ int trlhip_ctx_create_batched(trlhip_ctx **out, int n, int ncols, int vtype, trl
     if (!rc && hipMalloc((void **)&c->dscal, sizeof(double) * 16) != hipSuccess) rc = TRLHIP_ENOMEM;
     if (!rc && hipMalloc((void **)&c->dcoef, sizeof(double) * ncoef) != hipSuccess) rc = TRLHIP_ENOMEM;
     if (!rc && hipMalloc((void **)&c->dh, sizeof(double) * ncoef) != hipSuccess) rc = TRLHIP_ENOMEM;
-    c->hcap = 16 + 2 * ncoef;
-    c->pend_dst = -1;
     if (!rc && hipHostMalloc((void **)&c->hbuf, sizeof(double) * c->hcap, 0) != hipSuccess) rc = TRLHIP_ENOMEM;
     if (!rc && hipHostMalloc((void **)&c->hres, sizeof(double) * 8,
                              hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
@@ -1766,7 +1847,6 @@ int trlhip_ctx_create_batched(trlhip_ctx **out, int n, int ncols, int vtype, trl
     if (!rc && hipHostGetDevicePointer((void **)&c->dres, c->hres, 0) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipMemsetAsync(c->V, 0, vbytes, c->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && hipMemsetAsync(c->dscal, 0, sizeof(double) * 16, c->s) != hipSuccess) rc = TRLHIP_EHIP;
-    // a batched context is created outside the coroutines: plain synchronize
     if (!rc && hipStreamSynchronize(c->s) != hipSuccess) rc = TRLHIP_EHIP;
     if (rc) {
         trlhip_ctx_destroy(c);
@@ -1944,7 +2024,10 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
         } else if (int rc = flush_pending(c)) {
             return rc;
         }
-        c->batch->steps.push_back(st);
+        {
+            std::lock_guard<std::mutex> lk(c->batch->mu);
+            c->batch->steps.push_back(st);
+        }
         if (int rc = ctx_wait(c)) return rc;
         const volatile double *hr = c->hres;
         out[0] = hr[S_ALPHA];
@@ -2036,6 +2119,7 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y, int ldy)
     if (KT > 8) return TRLHIP_EINVAL;  // k <= 128
     const size_t ny = (size_t)ldy * k;
     if (c->dYcap < ny) {
+        if (c->arena) return TRLHIP_EINVAL;  // sized for the largest restart at creation
         if (c->dY) (void)hipFree(c->dY);
         if (c->hY) (void)hipHostFree(c->hY);
         c->dY = nullptr;
@@ -2068,7 +2152,10 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y, int ldy)
     }
 #undef RITZ_CASE
     if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
-    return ctx_wait(c);  // the pinned staging of Y is reused by the next call
+    // the pinned staging of Y is reused by the next call: wait for the
+    // copy, except in a batch, where the next call comes after a flush has
+    // synchronized the stream
+    return c->batch ? 0 : ctx_wait(c);
 }
 
 void trlhip_stats_enable(int on)
@@ -2116,6 +2203,8 @@ void trlhip_batch_destroy(trlhip_batch *b)
     if (b->ddesc) (void)hipFree(b->ddesc);
     if (b->hpre) (void)hipHostFree(b->hpre);
     if (b->dpre) (void)hipFree(b->dpre);
+    for (auto &ch : b->dev) (void)hipFree(ch.p);
+    for (auto &ch : b->host) (void)hipHostFree(ch.p);
     (void)hipStreamDestroy(b->s);
     delete b;
 }
