@@ -42,10 +42,24 @@ METRIC = "eigenpairs/sec at resid<=1e-8||A||: mixed+learned-start vs plain FP64,
 # device kernel behind each timed kernel class, per solver path
 KERNEL_NAMES = {
     "plain": {"proj": "k_proj<double, double, 1, 8, 4>", "update": "k_update<double, double, 1, 4>",
-              "spmv": "k_spmv_vec<double, double, 1, 8, 4, true>"},
+              "spmv": "k_spmv<double, double, 1, true>"},
     "mixed": {"proj": "k_proj<float, double, 1, 8, 4>", "update": "k_update<float, double, 1, 4>",
-              "spmv": "k_spmv_vec<double, double, 1, 8, 4, true>"},
+              "spmv": "k_spmv<double, double, 1, true>"},
 }
+
+
+def calibration_copy(T, n):
+    """One plain copy of n doubles on the device (k_scale_copy<double>: 8n
+    bytes read, 8n written), the first kernel dispatch of the process, so a
+    rocprofv3 --pmc run of this script holds a kernel of known traffic to
+    scale FETCH_SIZE / WRITE_SIZE by."""
+    import numpy as np
+
+    c = T.Ctx(n, 2)
+    c.set(0, np.ones(n))
+    c.scale_copy(1, 0, 1.0)
+    assert c.get(1)[-1] == 1.0
+    c.close()
 
 
 def measured_traffic(kernel):
@@ -55,7 +69,7 @@ def measured_traffic(kernel):
     if kernel is None or not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch ") and kernel in line:
+        if line.startswith("traffic_bytes_per_launch ") and line.split(" ", 1)[1].rsplit(" ", 1)[0] == kernel:
             return int(line.split()[-1])
     return None
 
@@ -255,6 +269,7 @@ def main():
         return
     rp, ci, v, x0 = build_problem()
     n = len(rp) - 1
+    calibration_copy(T, n)
     ops = {"plain": T.Csr(rp, ci, v), "mixed": T.Csr(rp, ci, v, mixed=True)}
     other = "plain" if args.path == "mixed" else "mixed"
     for _ in range(args.warmup):

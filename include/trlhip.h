@@ -156,6 +156,18 @@ int trlhip_update(trlhip_ctx *c, int c0, int m, const double *h, int rcol,
 int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y,
                 int ldy);
 
+/* FP64 Rayleigh-quotient refinement of the Ritz vectors in columns
+ * [0, k) with operator A (real bases): rho[i] = v'Av / v'v and res[i] =
+ * ||A v - rho[i] v|| / ||v||, computed on the device (A v goes to free
+ * columns after column k-1, never to column `avoid`, which is kept);
+ * NaN for a zero column.  One device round trip for all k pairs; in a
+ * batch the pairs of every context run together.  No counterpart in the
+ * reference: it is the final residual check of a mixed-precision solve,
+ * whose pairs the reference's convergence test (trl_convergence_test,
+ * SRC/trlcore.c:2149-2305) judges from the recurrence's estimate. */
+int trlhip_refine(trlhip_ctx *c, const trlhip_csr *A, int k, int avoid,
+                  double *rho, double *res);
+
 /* ---- measurement ---------------------------------------------------- */
 
 /* Kernel classes timed with HIP events on the solver stream when enabled. */

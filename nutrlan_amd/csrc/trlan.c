@@ -729,29 +729,42 @@ static int lz_matvec_step(lz_t *z, int first, int qa, int qb, double beta_prev, 
 /* Final FP64 Rayleigh-quotient refinement of a mixed-precision solve:
  * for each of the k Ritz vectors v in device columns 0..k-1,
  * eval = v'Av / v'v with the FP64 operator, and beta = ||A v - eval v|| /
- * ||v|| (the true residual, reported in wrk instead of the estimate).
- * Column `avoid` (the last residual) is kept; another free column holds A v.
- * One operator application per pair, counted in matvec. */
+ * ||v|| (the true residual, reported in wrk instead of the estimate), all
+ * k pairs in one device call (trlhip_refine); column `avoid` (the last
+ * residual) is kept.  One operator application per pair, counted in
+ * matvec. */
 static int lz_refine(lz_t *z, int k, int avoid, double *eval, double *beta)
 {
-    int tmp = k;
-    while (tmp == avoid) ++tmp;
-    if (tmp >= z->ncols || !z->A) {
+    if (k < 1) return 0;
+    if (!z->A || 2 * k > z->maxlan * z->maxlan) {
         z->info->stat = TRLHIP_STAT_DEVICE;
         return TRLHIP_STAT_DEVICE;
     }
+    double *rho = z->zq, *res = z->zq + k;  /* zq: free scratch after the last restart */
+    DEV(trlhip_refine(z->dev, z->A, k, avoid, rho, res));
+    z->info->matvec += k;
     for (int i = 0; i < k; ++i) {
-        double vav = 0.0, vv = 0.0, rr2 = 0.0;
-        DEV(trlhip_spmv(z->dev, z->A, i, tmp, &vav));
-        DEV(trlhip_proj(z->dev, i, 1, i, &vv));
-        z->info->matvec++;
-        if (!(vv > 0.0)) continue;
-        const double rho = vav / vv;
-        DEV(trlhip_update(z->dev, i, 1, &rho, tmp, &rr2));
-        eval[i] = rho;
-        beta[i] = sqrt(rr2 / vv);
+        if (isnan(rho[i])) continue;
+        eval[i] = rho[i];
+        beta[i] = res[i];
     }
     return 0;
+}
+
+/* The reference's convergence test on the current Ritz values.  In a
+ * mixed-precision solve the Gram-Schmidt correction is taken against the
+ * FP32 shadow basis, and the part of it the three-term recurrence leaves
+ * out (V32 h, of order FLT_EPSILON * beta) moves the true residual of a
+ * Ritz pair off the recurrence's estimate by up to about a tenth of
+ * tol * ||A|| at tol = 1e-8; there pairs are accepted at half the
+ * tolerance so that the refined (true) residual still meets tol. */
+static void lz_convergence_test(lz_t *z, int jnd)
+{
+    trl_info *info = z->info;
+    const double tol = info->tol;
+    if (z->mixed) info->tol = 0.5 * tol;
+    trlkit_convergence_test(jnd, z->lam, z->res, z->tag, info);
+    info->tol = tol;
 }
 
 /* The thick-restart iteration; mirrors the decision sequence of the
@@ -824,7 +837,7 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
             if (info->matvec >= next_test) {
                 if ((info->stat = lz_get_eval(z, jnd, locked)) != 0) return;
                 memcpy(eval, z->lam, sizeof(double) * (size_t)DMIN(mev, jnd));
-                trlkit_convergence_test(jnd, z->lam, z->res, z->tag, info);
+                lz_convergence_test(z, jnd);
                 if (info->nec < info->ned && info->nec > 0) {
                     next_test = (int)((double)(info->ned * info->matvec) / (double)info->nec);
                 } else if (info->nec == 0) {
@@ -841,7 +854,7 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
         i2 = kept - locked + 1;
         if (info->nec < info->ned) {
             if ((info->stat = lz_get_eval(z, jnd, locked)) != 0) return;
-            trlkit_convergence_test(jnd, z->lam, z->res, z->tag, info);
+            lz_convergence_test(z, jnd);
         }
         if (info->nec < info->ned && info->matvec < info->maxmv) {
             /* lambda[jnd] aliases res[0] in the reference's workspace */
@@ -1103,7 +1116,11 @@ void trl_solve_device(trl_matvec op, ztrl_matvec zop, trl_uo user_ortho, trl_inf
  * batch: the recorded Lanczos steps of all solves run as one launch per
  * phase, the shared stream is synchronized once, and every solve resumes
  * with its results.  Each solve makes exactly the decisions it would make
- * alone, so its eigenpairs and counters match a single trlan() call. */
+ * alone, so its eigenpairs and counters match a single trlan() call.
+ *
+ * The solves are split into lanes (two by default), each a batch with
+ * its own stream: while the device runs one lane's launch, the host
+ * threads step the other lane's solves, so host and device work overlap. */
 
 enum { LZ_STACK = 1024 * 1024 };  /* per coroutine; HIP runtime calls run on it */
 
@@ -1114,6 +1131,7 @@ typedef struct {
     trl_batch_job *job;
     int done;     /* lz_iterate returned */
     int ended;    /* torn down */
+    int lane;     /* the batch (lane) it steps in */
 } lz_co_t;
 
 /* One host thread of the scheduler: it owns the coroutines i with
@@ -1127,11 +1145,14 @@ typedef struct lz_worker {
 } lz_worker_t;
 
 typedef struct lz_pool {
-    lz_co_t *cos;
+    lz_co_t **cos;   /* per worker: its coroutines of the lane being run */
+    int *ncos;
+    lz_co_t *all;
     int njobs, nthr;
     lz_worker_t *w;
     pthread_barrier_t start, stop;
     volatile int quit;
+    volatile int lane;  /* lane of the round in progress */
 } lz_pool_t;
 
 static __thread lz_worker_t *tl_worker;  /* the scheduler thread running a coroutine */
@@ -1161,16 +1182,20 @@ static void lz_co_main(void)
     for (;;) swapcontext(&co->uc, &tl_worker->sched);
 }
 
-/* Resume every unfinished coroutine of this worker once (each runs until
- * it waits on the device or finishes). */
+/* Resume every unfinished coroutine of this worker in the current lane
+ * once (each runs until it waits on the device or finishes).  Coroutine i
+ * of lane l is run by worker (i / nlanes) % nthr, so each lane spreads
+ * over all workers. */
 static void lz_round(lz_worker_t *w)
 {
     tl_worker = w;
     const double t0 = lz_now();
     lz_pool_t *p = w->pool;
-    for (int i = w->tid; i < p->njobs; i += p->nthr) {
-        lz_co_t *co = &p->cos[i];
-        if (co->done) continue;
+    lz_co_t **mine = p->cos + (size_t)w->tid * p->njobs;
+    const int cnt = p->ncos[w->tid];
+    for (int k = 0; k < cnt; ++k) {
+        lz_co_t *co = mine[k];
+        if (co->done || co->lane != p->lane) continue;
         w->cur = co;
         swapcontext(&w->sched, &co->uc);
     }
@@ -1187,6 +1212,17 @@ static void *lz_worker_main(void *arg)
         pthread_barrier_wait(&w->pool->stop);
     }
     return NULL;
+}
+
+/* Lanes: TRLHIP_BATCH_LANES (1..4), else 2 (1 for a single job). */
+static int lz_batch_lanes(int njobs)
+{
+    const char *e = getenv("TRLHIP_BATCH_LANES");
+    long n = e ? strtol(e, NULL, 10) : 2;
+    if (n < 1) n = 1;
+    if (n > 4) n = 4;
+    if (n > njobs) n = njobs;
+    return (int)n;
 }
 
 /* Host threads for the coroutines: TRLHIP_BATCH_THREADS, else up to 16
@@ -1208,28 +1244,30 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
 {
     if (njobs <= 0) return 0;
     if (!jobs) return TRLHIP_EINVAL;
-    trlhip_batch *b = NULL;
-    int rc = trlhip_batch_create(&b);
-    if (rc) {
-        for (int i = 0; i < njobs; ++i) jobs[i].info->stat = TRLHIP_STAT_DEVICE;
-        return rc;
+    const int nlanes = lz_batch_lanes(njobs);
+    trlhip_batch *bl[4] = {NULL, NULL, NULL, NULL};
+    int rc = 0;
+    for (int l = 0; l < nlanes && !rc; ++l) {
+        rc = trlhip_batch_create(&bl[l]);
+        if (!rc) trlhip_batch_set_yield(bl[l], lz_co_yield, NULL);
     }
-    trlhip_batch_set_yield(b, lz_co_yield, NULL);
-    lz_co_t *cos = (lz_co_t *)calloc((size_t)njobs, sizeof(lz_co_t));
+    lz_co_t *cos = rc ? NULL : (lz_co_t *)calloc((size_t)njobs, sizeof(lz_co_t));
     if (!cos) {
-        trlhip_batch_destroy(b);
-        return TRLHIP_ENOMEM;
+        for (int i = 0; i < njobs; ++i) jobs[i].info->stat = TRLHIP_STAT_DEVICE;
+        for (int l = 0; l < nlanes; ++l) trlhip_batch_destroy(bl[l]);
+        return rc ? rc : TRLHIP_ENOMEM;
     }
-    int live = 0;
+    int live[4] = {0, 0, 0, 0};
     for (int i = 0; i < njobs; ++i) {
         trl_batch_job *j = &jobs[i];
         lz_co_t *co = &cos[i];
         co->job = j;
         co->done = 1;
         co->ended = 1;
+        co->lane = i % nlanes;
         j->info->mvparam = j->A;
         if (!lz_setup(&co->z, trlhip_csr_matvec, NULL, NULL, j->info, j->nrow, j->mev, j->lde, 0,
-                      b))
+                      bl[co->lane]))
             continue;
         co->ended = 0;
         co->stack = malloc(LZ_STACK);
@@ -1241,11 +1279,11 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
         co->uc.uc_stack.ss_size = LZ_STACK;
         makecontext(&co->uc, lz_co_main, 0);
         co->done = 0;
-        ++live;
+        ++live[co->lane];
     }
     lz_pool_t pool;
     memset(&pool, 0, sizeof pool);
-    pool.cos = cos;
+    pool.all = cos;
     pool.njobs = njobs;
     pool.nthr = lz_batch_threads(njobs);
     pool.w = (lz_worker_t *)calloc((size_t)pool.nthr, sizeof(lz_worker_t));
@@ -1279,25 +1317,59 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
         pool.w[0].pool = &pool;
         pool.w[0].tid = 0;
     }
-    /* TRLHIP_BATCH_PROFILE=1: host time in the solves vs in the flushes */
-    const char *prof = getenv("TRLHIP_BATCH_PROFILE");
-    double t_co = 0.0, t_fl = 0.0;
-    long rounds = 0;
-    while (live > 0) {
-        const double t0 = lz_now();
-        if (pool.nthr > 1) pthread_barrier_wait(&pool.start);
-        lz_round(&pool.w[0]);
-        if (pool.nthr > 1) pthread_barrier_wait(&pool.stop);
-        live = 0;
-        for (int i = 0; i < njobs; ++i) live += !cos[i].done;
-        const double t1 = lz_now();
-        if (live > 0) {
-            const int frc = trlhip_batch_flush(b);
-            if (frc && !rc) rc = frc;
+    /* each worker's coroutines: i of lane i % nlanes goes to worker
+     * (i / nlanes) % nthr */
+    pool.cos = (lz_co_t **)calloc((size_t)pool.nthr * njobs, sizeof(lz_co_t *));
+    pool.ncos = (int *)calloc((size_t)pool.nthr, sizeof(int));
+    if (!pool.cos || !pool.ncos) {  /* no solve runs */
+        for (int i = 0; i < njobs; ++i)
+            if (!cos[i].done) {
+                cos[i].done = 1;
+                jobs[i].info->stat = -4;
+            }
+        for (int l = 0; l < nlanes; ++l) live[l] = 0;
+        rc = TRLHIP_ENOMEM;
+    } else {
+        for (int i = 0; i < njobs; ++i) {
+            const int t = (i / nlanes) % pool.nthr;
+            pool.cos[(size_t)t * njobs + pool.ncos[t]++] = &cos[i];
         }
-        t_co += t1 - t0;
-        t_fl += lz_now() - t1;
-        ++rounds;
+    }
+    /* TRLHIP_BATCH_PROFILE=1: host time in the solves vs waiting for the
+     * device */
+    const char *prof = getenv("TRLHIP_BATCH_PROFILE");
+    double t_co = 0.0, t_wait = 0.0;
+    long rounds = 0;
+    int pending[4] = {0, 0, 0, 0};
+    for (;;) {
+        int any = 0;
+        for (int l = 0; l < nlanes; ++l) {
+            if (pending[l]) {
+                /* the lane's last launch must finish before its solves resume */
+                const double tw = lz_now();
+                const int wrc = trlhip_batch_wait(bl[l]);
+                if (wrc && !rc) rc = wrc;
+                pending[l] = 0;
+                t_wait += lz_now() - tw;
+            }
+            if (live[l] == 0) continue;
+            any = 1;
+            const double t0 = lz_now();
+            pool.lane = l;
+            if (pool.nthr > 1) pthread_barrier_wait(&pool.start);
+            lz_round(&pool.w[0]);
+            if (pool.nthr > 1) pthread_barrier_wait(&pool.stop);
+            live[l] = 0;
+            for (int i = l; i < njobs; i += nlanes) live[l] += !cos[i].done;
+            t_co += lz_now() - t0;
+            ++rounds;
+            if (live[l] > 0) {
+                const int lrc = trlhip_batch_launch(bl[l]);
+                if (lrc && !rc) rc = lrc;
+                pending[l] = 1;
+            }
+        }
+        if (!any) break;
     }
     if (pool.nthr > 1) {
         pool.quit = 1;
@@ -1307,8 +1379,10 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
         pthread_barrier_destroy(&pool.stop);
     }
     if (prof && prof[0] == '1')
-        fprintf(stderr, "trlan_batch: %d host threads, %ld rounds, %.3f s in solves, %.3f s in flushes\n",
-                pool.nthr, rounds, t_co, t_fl);
+        fprintf(stderr,
+                "trlan_batch: %d host threads, %d lanes, %ld lane rounds, %.3f s in solves, "
+                "%.3f s waiting for the device\n",
+                pool.nthr, nlanes, rounds, t_co, t_wait);
     /* teardown after the last round: freeing device memory synchronizes */
     for (int i = 0; i < njobs; ++i) {
         lz_co_t *co = &cos[i];
@@ -1317,8 +1391,10 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     }
     free(th);
     free(pool.w);
+    free(pool.cos);
+    free(pool.ncos);
     free(cos);
-    trlhip_batch_destroy(b);
+    for (int l = 0; l < nlanes; ++l) trlhip_batch_destroy(bl[l]);
     return rc;
 }
 

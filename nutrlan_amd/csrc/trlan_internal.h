@@ -39,6 +39,10 @@ int trlhip_batch_create(trlhip_batch **b);
 void trlhip_batch_destroy(trlhip_batch *b);
 void trlhip_batch_set_yield(trlhip_batch *b, void (*yield)(void *user), void *user);
 int trlhip_batch_flush(trlhip_batch *b);
+/* trlhip_batch_flush in two halves: launch the recorded work (no wait), and
+ * wait for it; the batch's contexts may resume only after the wait. */
+int trlhip_batch_launch(trlhip_batch *b);
+int trlhip_batch_wait(trlhip_batch *b);
 int trlhip_ctx_create_batched(trlhip_ctx **c, int n, int ncols, int vtype, trlhip_batch *b);
 
 /* One thick-restart solve on a device basis; cplx selects ztrlan (zop,

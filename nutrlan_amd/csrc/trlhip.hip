@@ -710,13 +710,16 @@ __global__ __launch_bounds__(kBlock) void k_scale_copy(long ne, const S *src, S 
 // MFMA f64 16x16x4 operand layout (lane l): A[l%16][l/16], B[l/16][l%16],
 // D[l/16 + 4*r][l%16] for accumulator element r (measured on gfx950).
 typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int kRitzTiles = 4;  // 16-row tiles per wave of the restart GEMM
 
+// One block's rows of the restart GEMM (block bid; tiles_per_wave 16-row
+// tiles per wave); Ys is the block's LDS (jl4 * 16 * KT doubles).
 template <int KT>
-__global__ __launch_bounds__(kBlock) void k_ritz_mfma(
-    long nrows, double *__restrict__ V, long ld, int lck, int jl, int k, const double *__restrict__ Y,
-    int ldy, int tiles_per_wave, float *__restrict__ V32)
+__device__ __forceinline__ void ritz_body(long nrows, double *__restrict__ V, long ld, int lck, int jl,
+                                          int k, const double *__restrict__ Y, int ldy,
+                                          int tiles_per_wave, float *__restrict__ V32,
+                                          double *__restrict__ Ys, int bid)
 {
-    extern __shared__ double Ys[];
     constexpr int kp = 16 * KT;
     const int jl4 = (jl + 3) & ~3;
     for (int i = threadIdx.x; i < jl4 * kp; i += kBlock) {
@@ -728,7 +731,7 @@ __global__ __launch_bounds__(kBlock) void k_ritz_mfma(
     const int li = lane & 15, lq = lane >> 4;
     double *Vin = V + (size_t)lck * ld;
     for (int t = 0; t < tiles_per_wave; ++t) {
-        const long row0 = ((long)(blockIdx.x * (kBlock / 64) + w) * tiles_per_wave + t) * 16;
+        const long row0 = ((long)(bid * (kBlock / 64) + w) * tiles_per_wave + t) * 16;
         if (row0 >= nrows) break;
         d4 acc[KT];
 #pragma unroll
@@ -756,6 +759,15 @@ __global__ __launch_bounds__(kBlock) void k_ritz_mfma(
             }
         }
     }
+}
+
+template <int KT>
+__global__ __launch_bounds__(kBlock) void k_ritz_mfma(
+    long nrows, double *__restrict__ V, long ld, int lck, int jl, int k, const double *__restrict__ Y,
+    int ldy, int tiles_per_wave, float *__restrict__ V32)
+{
+    extern __shared__ double Ys[];
+    ritz_body<KT>(nrows, V, ld, lck, jl, k, Y, ldy, tiles_per_wave, V32, Ys, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- stats
@@ -789,17 +801,23 @@ hipEvent_t ev_get()
     return e;
 }
 
-// Resolve timings of completed launches (called after a stream sync).
+// Resolve timings of completed launches (called after a stream sync);
+// launches still running on another stream stay pending.
 void stats_resolve()
 {
     std::lock_guard<std::mutex> lk(g_stats_mu);
+    size_t keep = 0;
     for (auto &p : g_stats.pending) {
+        if (hipEventQuery(p.b) == hipErrorNotReady) {
+            g_stats.pending[keep++] = p;
+            continue;
+        }
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) g_stats.ms[p.k] += ms;
         g_stats.pool.push_back(p.a);
         g_stats.pool.push_back(p.b);
     }
-    g_stats.pending.clear();
+    g_stats.pending.resize(keep);
 }
 
 // Start/stop events armed for the next kernel launch.  KLAUNCH passes them
@@ -904,7 +922,8 @@ struct trlhip_ctx {
     double *dY;     // restart matrix
     size_t dYcap;
     double *hbuf;   // pinned host mirror: 16 scalars + coefficients
-    double *hres;   // pinned host-coherent results of a fused step (4 scalars)
+    double *hres;   // pinned host-coherent results: a fused step's 4 scalars, then
+                    // 3 per refined pair from index 8 (hres_count doubles)
     double *dres;   // its device address
     int pend_dst;   // column of a deferred scale-copy from the scratch column (-1: none)
     double pend_s;  // its scale
@@ -913,6 +932,9 @@ struct trlhip_ctx {
     hipStream_t s;
     trlhip_batch *batch;  // batch the context steps in (its stream is the batch's), or null
     int arena;            // memory comes from the batch's arenas (freed with the batch)
+    int nrec;             // recorded restart GEMMs / scale-copies not yet flushed (batch)
+    char *hops, *dops;    // staging of a refinement outside a batch (pinned / device)
+    size_t ocap;
 };
 
 // One recorded Lanczos step of a batched context (see the batch section).
@@ -923,6 +945,22 @@ struct BatchStep {
     double beta_prev;
     int fuse;         // q_a is the deferred scale-copy of the scratch column
     double fuse_s;
+    long coff;        // first step after a restart: its jnd-1 coefficients at cbuf[coff]
+};
+
+// A recorded restart GEMM or scale-copy of a batched context; `phase` is
+// its ordinal among the context's recorded operations since the last
+// flush (a flush runs phase 0 of every context, then phase 1, ...).
+enum { OP_RITZ = 0, OP_COPY = 1, OP_REFINE = 2 };
+struct BatchOp {
+    trlhip_ctx *c;
+    int type, phase;
+    int lck, jl, k, ldy;  // restart GEMM; Y at ybuf[yoff], ldy x k
+    size_t yoff;
+    int dst, src;         // scale-copy dst = s * src
+    double s;
+    const trlhip_csr *A;  // refinement of columns [0, k) (k above), `avoid` kept
+    int avoid;
 };
 
 namespace {
@@ -936,6 +974,13 @@ struct trlhip_batch {
     int err = 0;                  // sticky device error of a flush
     std::mutex mu;                // guards steps (coroutines run on several host threads)
     std::vector<BatchStep> steps;
+    std::vector<double> cbuf;     // restart-step coefficients of the recorded steps
+    double *hcbuf = nullptr, *dcbuf = nullptr;  // their pinned staging / device copy
+    size_t ccap = 0;
+    std::vector<BatchOp> ops;     // recorded restart GEMMs and scale-copies
+    std::vector<double> ybuf;     // their restart matrices
+    char *hops = nullptr, *dops = nullptr;      // pinned staging / device copy of the ops
+    size_t ocap = 0;              // bytes each holds
     BDesc *hdesc = nullptr, *ddesc = nullptr;   // pinned staging / device copy
     int *hpre = nullptr, *dpre = nullptr;       // prefix tables of all groups
     size_t cap = 0;               // requests the descriptor buffers hold
@@ -1025,6 +1070,7 @@ double *part_alpha(trlhip_ctx *c) { return c->part; }
 double *part_update(trlhip_ctx *c) { return c->part + kSpmvGrid; }
 double *part_recur(trlhip_ctx *c) { return c->part + kSpmvGrid + kMaxGrid; }
 double *part_proj(trlhip_ctx *c) { return c->part + kSpmvGrid + 2 * kMaxGrid; }
+size_t hres_count(int ncols) { return 8 + 3 * (size_t)(ncols + 1); }
 size_t part_need(int E, int ncols)
 {
     return (size_t)kSpmvGrid + 2 * (size_t)kMaxGrid + (size_t)kProjGrid * E * (ncols + 1) + 64;
@@ -1247,6 +1293,18 @@ int run_proj(trlhip_ctx *c, int c0, int m, int rcol, double *hout, bool shadow =
 // of dst follows in a mixed-precision context.
 int run_scale_copy(trlhip_ctx *c, int dst, int src, double s)
 {
+    if (c->batch && c->E == 1) {  // recorded; the batch flush runs it
+        BatchOp op{};
+        op.c = c;
+        op.type = OP_COPY;
+        op.dst = dst;
+        op.src = src;
+        op.s = s;
+        std::lock_guard<std::mutex> lk(c->batch->mu);
+        op.phase = c->nrec++;
+        c->batch->ops.push_back(op);
+        return 0;
+    }
     const long ne = (long)c->n * c->E;
     const int g = grid_for(ne, kBlock * 8, kMaxGrid);
     Timed t(TRLHIP_K_MISC, c->s, 2.0 * (double)c->esz * c->n);
@@ -1277,6 +1335,19 @@ int ctx_wait(trlhip_ctx *c)
     TRY(hipStreamSynchronize(c->s));
     stats_resolve();
     return 0;
+}
+
+// In a batch, the restart GEMMs and scale-copies a context records run at
+// the next flush.  An operation that launches or copies at once applies a
+// deferred scale-copy and, when the context has recorded operations,
+// yields to that flush first, so the context's device work keeps its
+// program order.
+int batch_drain(trlhip_ctx *c) { return (c->batch && c->nrec) ? ctx_wait(c) : 0; }
+
+int settle(trlhip_ctx *c)
+{
+    if (int rc = flush_pending(c)) return rc;
+    return batch_drain(c);
 }
 
 // copy scalars back and wait
@@ -1426,7 +1497,8 @@ struct GroupInfo {
     int maxrm, maxj;
 };
 
-void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo &gi)
+void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo &gi,
+                const double *dcbuf)
 {
     for (int ph = 0; ph < BP_COUNT; ++ph) pre[(size_t)ph * (R + 1)] = 0;
     for (double &x : gi.bytes) x = 0.0;
@@ -1448,7 +1520,7 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
         if (st.first) {
             d.rm = jnd;
             d.rc0 = 0;
-            d.rh = c->dcoef;
+            d.rh = st.coff >= 0 ? dcbuf + st.coff : c->dcoef;
             d.rh0 = 0.0;
         } else if (st.qb >= 0) {
             d.rm = 2;
@@ -1587,14 +1659,343 @@ int batch_launch(trlhip_batch *b)
         b->cap = cap;
         b->pcap = pcap;
     }
+    if (b->cbuf.size() > b->ccap) {
+        if (b->hcbuf) (void)hipHostFree(b->hcbuf);
+        if (b->dcbuf) (void)hipFree(b->dcbuf);
+        b->hcbuf = b->dcbuf = nullptr;
+        b->ccap = 0;
+        const size_t cap = b->cbuf.size() + b->cbuf.size() / 2 + 256;
+        if (hipHostMalloc((void **)&b->hcbuf, sizeof(double) * cap, 0) != hipSuccess ||
+            hipMalloc((void **)&b->dcbuf, sizeof(double) * cap) != hipSuccess)
+            return TRLHIP_ENOMEM;
+        b->ccap = cap;
+    }
     for (GroupInfo &gi : groups)
-        batch_fill(order.data() + gi.doff, gi.R, b->hdesc + gi.doff, b->hpre + gi.poff, gi);
+        batch_fill(order.data() + gi.doff, gi.R, b->hdesc + gi.doff, b->hpre + gi.poff, gi, b->dcbuf);
+    if (!b->cbuf.empty()) {
+        memcpy(b->hcbuf, b->cbuf.data(), sizeof(double) * b->cbuf.size());
+        TRY(hipMemcpyAsync(b->dcbuf, b->hcbuf, sizeof(double) * b->cbuf.size(), hipMemcpyHostToDevice,
+                           b->s));
+    }
     TRY(hipMemcpyAsync(b->ddesc, b->hdesc, sizeof(BDesc) * R, hipMemcpyHostToDevice, b->s));
     TRY(hipMemcpyAsync(b->dpre, b->hpre, sizeof(int) * npre, hipMemcpyHostToDevice, b->s));
     for (const GroupInfo &gi : groups)
         if (int rc = batch_run(b->s, gi, b->ddesc + gi.doff, b->dpre + gi.poff, b->hpre + gi.poff))
             return rc;
     return 0;
+}
+
+// Device record of one recorded operation: a restart GEMM (V, ld, lck,
+// jl, k, Y, ldy over n rows, FP32 shadow slab V32) or a scale-copy
+// (V = dst column, src, s over n elements, V32 = the dst shadow column).
+struct ODesc {
+    long n, ld;
+    double *V;
+    const double *src;
+    float *V32;
+    const double *Y;
+    double s;
+    int lck, jl, k, ldy;
+};
+
+template <int KT>
+__global__ __launch_bounds__(kBlock) void kb_ritz(const ODesc *__restrict__ d, const int *pre, int nreq)
+{
+    extern __shared__ double Ys[];
+    const int p = batch_find(pre, nreq, blockIdx.x);
+    const ODesc &q = d[p];
+    ritz_body<KT>(q.n, q.V, q.ld, q.lck, q.jl, q.k, q.Y, q.ldy, kRitzTiles, q.V32, Ys,
+                  blockIdx.x - pre[p]);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_copy(const ODesc *__restrict__ d, const int *pre, int nreq)
+{
+    const int p = batch_find(pre, nreq, blockIdx.x);
+    const ODesc &q = d[p];
+    const int bid = blockIdx.x - pre[p], nblk = pre[p + 1] - pre[p];
+    for (long i = (long)bid * kBlock + threadIdx.x; i < q.n; i += (long)nblk * kBlock) {
+        const double v = q.src[i] * q.s;
+        q.V[i] = v;
+        if (q.V32) q.V32[i] = (float)v;
+    }
+}
+
+// Rayleigh-quotient refinement of one Ritz vector v (a basis column) with
+// the FP64 operator: t = A v into a free column, rho = v'Av / v'v,
+// ||A v - rho v|| / ||v||, all on the device.  Partials at part:
+// [0, kSpmvGrid) v'Av, then kMaxGrid each for v'v and ||Av - rho v||^2;
+// coef = {rho, v'v}; out = {rho, residual, v'v} in host-coherent memory.
+struct FDesc {
+    int n, nb, npx, nd;
+    const int *rb, *rp, *ci;
+    const double *val;
+    const double *v;
+    double *t, *part, *coef, *out;
+};
+constexpr int kRefinePart = kSpmvGrid + 2 * kMaxGrid;
+
+__global__ __launch_bounds__(kBlock) void kr_spmv(const FDesc *__restrict__ d, const int *pre, int nreq)
+{
+    const int p = batch_find(pre, nreq, blockIdx.x);
+    const FDesc &q = d[p];
+    spmv_stream_body<double, double, 1, false>(q.nb, q.rb, q.rp, q.ci, q.val, q.v, q.t, q.part, 1.0,
+                                               nullptr, nullptr, blockIdx.x - pre[p],
+                                               pre[p + 1] - pre[p]);
+}
+
+__global__ __launch_bounds__(kBlock) void kr_dot(const FDesc *__restrict__ d, const int *pre, int nreq)
+{
+    const int p = batch_find(pre, nreq, blockIdx.x);
+    const FDesc &q = d[p];
+    const int bid = blockIdx.x - pre[p], nblk = pre[p + 1] - pre[p];
+    double acc = 0.0;
+    for (long i = (long)bid * kBlock + threadIdx.x; i < q.n; i += (long)nblk * kBlock) {
+        const double x = q.v[i];
+        acc += x * x;
+    }
+    const double sb = block_sum(acc);
+    if (threadIdx.x == 0) q.part[kSpmvGrid + bid] = sb;
+}
+
+__global__ __launch_bounds__(kBlock) void kr_rho(const FDesc *__restrict__ d)
+{
+    const FDesc &q = d[blockIdx.x];
+    const double vav = ordered_sum(q.part, q.npx);
+    const double vv = ordered_sum(q.part + kSpmvGrid, q.nd);
+    if (threadIdx.x == 0) {
+        q.coef[0] = vv > 0.0 ? vav / vv : 0.0;
+        q.coef[1] = vv;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void kr_res(const FDesc *__restrict__ d, const int *pre, int nreq)
+{
+    const int p = batch_find(pre, nreq, blockIdx.x);
+    const FDesc &q = d[p];
+    const int bid = blockIdx.x - pre[p], nblk = pre[p + 1] - pre[p];
+    const double rho = q.coef[0];
+    double acc = 0.0;
+    for (long i = (long)bid * kBlock + threadIdx.x; i < q.n; i += (long)nblk * kBlock) {
+        const double r = q.t[i] - rho * q.v[i];
+        acc += r * r;
+    }
+    const double sb = block_sum(acc);
+    if (threadIdx.x == 0) q.part[kSpmvGrid + kMaxGrid + bid] = sb;
+}
+
+__global__ __launch_bounds__(kBlock) void kr_fin(const FDesc *__restrict__ d)
+{
+    const FDesc &q = d[blockIdx.x];
+    const double rr = ordered_sum(q.part + kSpmvGrid + kMaxGrid, q.nd);
+    if (threadIdx.x == 0) {
+        const double vv = q.coef[1];
+        q.out[0] = q.coef[0];
+        q.out[1] = vv > 0.0 ? sqrt(rr / vv) : 0.0;
+        q.out[2] = vv;
+    }
+}
+
+// Pairs a refinement handles at once: one free column each (the columns
+// after the k Ritz vectors except `avoid`, and the scratch column) and
+// kRefinePart partials each in the projection partials.
+int refine_width(const trlhip_ctx *c, int k, int avoid)
+{
+    int nfree = c->ncols + 1 - k - ((avoid >= k && avoid <= c->ncols) ? 1 : 0);
+    const int np = (int)(((size_t)kProjGrid * c->E * (c->ncols + 1)) / kRefinePart);
+    return std::max(0, std::min(std::min(k, nfree), np));
+}
+
+int refine_free_col(const trlhip_ctx *c, int k, int avoid, int j)
+{
+    int col = k;
+    for (;; ++col) {
+        if (col == avoid) continue;
+        if (j-- == 0) return col;
+    }
+}
+
+// Runs recorded operations: for each phase (see BatchOp) one kb_ritz
+// launch per restart width KT present, one kb_copy launch, and the
+// refinements' kernels (pairs in chunks of refine_width; chunk j of every
+// context together).  Records, prefix tables and restart matrices are
+// staged in one host buffer (h, d: pinned / device, cap bytes, grown as
+// needed) and copied to the device once.  Operations of one phase belong to
+// different contexts, so their order inside the phase is free.
+int ops_run(const std::vector<BatchOp> &ops, const std::vector<double> &ybuf, char *&h, char *&d,
+            size_t &cap, hipStream_t s)
+{
+    enum { L_COPY = 0, L_REFINE = 9 };  // 1..8: restart GEMMs of that width
+    struct Launch {
+        int kind, R;
+        size_t doff, poff;  // first record (ODesc or FDesc array), first prefix entry
+        size_t shm;
+        double bytes;
+    };
+    struct Pair {
+        const BatchOp *op;
+        int i, j;  // Ritz vector, slot in its chunk
+    };
+    int nph = 0;
+    for (const BatchOp &op : ops) nph = std::max(nph, op.phase + 1);
+    std::vector<const BatchOp *> order;
+    std::vector<Pair> pairs;
+    std::vector<Launch> ls;
+    order.reserve(ops.size());
+    size_t npre = 0;
+    for (int ph = 0; ph < nph; ++ph) {
+        for (int kind = 0; kind <= 8; ++kind) {
+            Launch l{kind, 0, order.size(), npre, 0, 0.0};
+            for (const BatchOp &op : ops) {
+                if (op.phase != ph || op.type == OP_REFINE) continue;
+                const int okind = op.type == OP_COPY ? L_COPY : (op.k + 15) / 16;
+                if (okind == kind) order.push_back(&op);
+            }
+            l.R = (int)(order.size() - l.doff);
+            if (l.R == 0) continue;
+            npre += (size_t)l.R + 1;
+            ls.push_back(l);
+        }
+        for (int chunk = 0;; ++chunk) {
+            Launch l{L_REFINE, 0, pairs.size(), npre, 0, 0.0};
+            bool more = false;
+            for (const BatchOp &op : ops) {
+                if (op.phase != ph || op.type != OP_REFINE) continue;
+                const int w = refine_width(op.c, op.k, op.avoid);
+                const int i0 = chunk * w, i1 = std::min(op.k, i0 + w);
+                for (int i = i0; i < i1; ++i) pairs.push_back(Pair{&op, i, i - i0});
+                if (i1 < op.k) more = true;
+            }
+            l.R = (int)(pairs.size() - l.doff);
+            if (l.R > 0) {
+                npre += 2 * ((size_t)l.R + 1);
+                ls.push_back(l);
+            }
+            if (!more) break;
+        }
+    }
+    const size_t o_f = sizeof(ODesc) * order.size();
+    const size_t o_pre = o_f + sizeof(FDesc) * pairs.size();
+    const size_t o_y = (o_pre + sizeof(int) * npre + 255) & ~(size_t)255;
+    const size_t bytes = o_y + sizeof(double) * ybuf.size();
+    if (bytes > cap) {
+        if (h) (void)hipHostFree(h);
+        if (d) (void)hipFree(d);
+        h = d = nullptr;
+        cap = 0;
+        const size_t ncap = bytes + bytes / 2 + 4096;
+        if (hipHostMalloc((void **)&h, ncap, 0) != hipSuccess || hipMalloc((void **)&d, ncap) != hipSuccess)
+            return TRLHIP_ENOMEM;
+        cap = ncap;
+    }
+    ODesc *ho = reinterpret_cast<ODesc *>(h);
+    FDesc *hf = reinterpret_cast<FDesc *>(h + o_f);
+    int *hp = reinterpret_cast<int *>(h + o_pre);
+    const double *dy = reinterpret_cast<const double *>(d + o_y);
+    if (!ybuf.empty()) memcpy(h + o_y, ybuf.data(), sizeof(double) * ybuf.size());
+    for (Launch &l : ls) {
+        int *pp = hp + l.poff;
+        pp[0] = 0;
+        if (l.kind == L_REFINE) {
+            int *pd = pp + l.R + 1;
+            pd[0] = 0;
+            for (int r = 0; r < l.R; ++r) {
+                const Pair &pr = pairs[l.doff + r];
+                trlhip_ctx *c = pr.op->c;
+                const trlhip_csr *A = pr.op->A;
+                FDesc &f = hf[l.doff + r];
+                f = FDesc{};
+                f.n = c->n;
+                f.nb = A->nb;
+                f.npx = A->nb < kSpmvGrid ? A->nb : kSpmvGrid;
+                f.nd = grid_for(c->n, kBlock * 8, kMaxGrid);
+                f.rb = A->rb;
+                f.rp = A->rp;
+                f.ci = A->ci;
+                f.val = A->val;
+                f.v = colp<double>(c, pr.i);
+                f.t = colp<double>(c, refine_free_col(c, pr.op->k, pr.op->avoid, pr.j));
+                f.part = part_proj(c) + (size_t)pr.j * kRefinePart;
+                f.coef = c->dh + 2 * pr.j;
+                f.out = c->dres + 8 + 3 * pr.i;
+                pp[r + 1] = pp[r] + f.npx;
+                pd[r + 1] = pd[r] + f.nd;
+                l.bytes += spmv_bytes(A, 1, 8, 8);
+            }
+            continue;
+        }
+        for (int i = 0; i < l.R; ++i) {
+            const BatchOp &op = *order[l.doff + i];
+            trlhip_ctx *c = op.c;
+            ODesc &o = ho[l.doff + i];
+            o = ODesc{};
+            int nb;
+            if (op.type == OP_COPY) {
+                o.n = c->n;
+                o.V = colp<double>(c, op.dst);
+                o.src = colp<double>(c, op.src);
+                o.V32 = shadow32(c, op.dst);
+                o.s = op.s;
+                nb = grid_for(c->n, kBlock * 8, kMaxGrid);
+                l.bytes += (c->V32 ? 20.0 : 16.0) * c->n;
+            } else {
+                o.n = c->n;
+                o.ld = c->ld;
+                o.V = colp<double>(c, 0);
+                o.V32 = c->V32;
+                o.Y = dy + op.yoff;
+                o.lck = op.lck;
+                o.jl = op.jl;
+                o.k = op.k;
+                o.ldy = op.ldy;
+                const long rpb = 16L * kRitzTiles * (kBlock / 64);
+                nb = (int)((c->n + rpb - 1) / rpb);
+                const size_t shm = sizeof(double) * (size_t)((op.jl + 3) & ~3) * 16 * l.kind;
+                l.shm = std::max(l.shm, shm);
+                l.bytes += 8.0 * c->n * (op.jl + op.k) + (c->V32 ? 4.0 * c->n * op.k : 0.0);
+            }
+            pp[i + 1] = pp[i] + nb;
+        }
+    }
+    TRY(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+    const ODesc *dod = reinterpret_cast<const ODesc *>(d);
+    const FDesc *dfd = reinterpret_cast<const FDesc *>(d + o_f);
+    const int *dp = reinterpret_cast<const int *>(d + o_pre);
+    for (const Launch &l : ls) {
+        const int *pp = hp + l.poff;
+        if (l.kind == L_REFINE) {
+            const FDesc *f = dfd + l.doff;
+            const int *dps = dp + l.poff, *dpd = dps + l.R + 1;
+            const dim3 gs(pp[l.R]), gd(pp[l.R + 1 + l.R]), g1(l.R);
+            {
+                Timed t(TRLHIP_K_MISC, s, l.bytes);
+                KLAUNCH(kr_spmv, gs, dim3(kBlock), 0, s, f, dps, l.R);
+            }
+            KLAUNCH(kr_dot, gd, dim3(kBlock), 0, s, f, dpd, l.R);
+            KLAUNCH(kr_rho, g1, dim3(kBlock), 0, s, f);
+            KLAUNCH(kr_res, gd, dim3(kBlock), 0, s, f, dpd, l.R);
+            KLAUNCH(kr_fin, g1, dim3(kBlock), 0, s, f);
+            continue;
+        }
+        const dim3 g(pp[l.R]);
+        const ODesc *o = dod + l.doff;
+        const int *dpp = dp + l.poff;
+        if (l.kind == L_COPY) {
+            Timed t(TRLHIP_K_MISC, s, l.bytes);
+            KLAUNCH(kb_copy, g, dim3(kBlock), 0, s, o, dpp, l.R);
+            continue;
+        }
+        Timed t(TRLHIP_K_RITZ, s, l.bytes);
+#define KB_RITZ_CASE(K)                                                            \
+    case K:                                                                        \
+        KLAUNCH((kb_ritz<K>), g, dim3(kBlock), l.shm, s, o, dpp, l.R);             \
+        break;
+        switch (l.kind) {
+            KB_RITZ_CASE(1) KB_RITZ_CASE(2) KB_RITZ_CASE(3) KB_RITZ_CASE(4)
+            KB_RITZ_CASE(5) KB_RITZ_CASE(6) KB_RITZ_CASE(7) KB_RITZ_CASE(8)
+        }
+#undef KB_RITZ_CASE
+    }
+    return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
 }  // namespace
@@ -1773,6 +2174,8 @@ int trlhip_ctx_destroy(trlhip_ctx *c)
     if (c->dY) (void)hipFree(c->dY);
     if (c->hbuf) (void)hipHostFree(c->hbuf);
     if (c->hres) (void)hipHostFree(c->hres);
+    if (c->hops) (void)hipHostFree(c->hops);
+    if (c->dops) (void)hipFree(c->dops);
     if (c->hY) (void)hipHostFree(c->hY);
     if (c->s && !c->batch) (void)hipStreamDestroy(c->s);
     free(c);
@@ -1820,7 +2223,7 @@ int trlhip_ctx_create_batched(trlhip_ctx **out, int n, int ncols, int vtype, trl
         c->dY = (double *)arena_get(b, false, sizeof(double) * ny);
         c->dYcap = ny;
         c->hbuf = (double *)arena_get(b, true, sizeof(double) * c->hcap);
-        c->hres = (double *)arena_get(b, true, sizeof(double) * 8, (void **)&c->dres);
+        c->hres = (double *)arena_get(b, true, sizeof(double) * hres_count(ncols), (void **)&c->dres);
         c->hY = (double *)arena_get(b, true, sizeof(double) * ny);
         if (!c->V || (vtype == TRLHIP_R32 && !c->V32) || !c->part || !c->dscal || !c->dcoef ||
             !c->dh || !c->dY || !c->hbuf || !c->hres || !c->hY) {
@@ -1841,7 +2244,7 @@ int trlhip_ctx_create_batched(trlhip_ctx **out, int n, int ncols, int vtype, trl
     if (!rc && hipMalloc((void **)&c->dcoef, sizeof(double) * ncoef) != hipSuccess) rc = TRLHIP_ENOMEM;
     if (!rc && hipMalloc((void **)&c->dh, sizeof(double) * ncoef) != hipSuccess) rc = TRLHIP_ENOMEM;
     if (!rc && hipHostMalloc((void **)&c->hbuf, sizeof(double) * c->hcap, 0) != hipSuccess) rc = TRLHIP_ENOMEM;
-    if (!rc && hipHostMalloc((void **)&c->hres, sizeof(double) * 8,
+    if (!rc && hipHostMalloc((void **)&c->hres, sizeof(double) * hres_count(ncols),
                              hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
         rc = TRLHIP_ENOMEM;
     if (!rc && hipHostGetDevicePointer((void **)&c->dres, c->hres, 0) != hipSuccess) rc = TRLHIP_EHIP;
@@ -1861,7 +2264,7 @@ int trlhip_ctx_ld(const trlhip_ctx *c) { return c ? c->ld : 0; }
 int trlhip_col_set(trlhip_ctx *c, int col, const void *x)
 {
     if (!c || !x || !valid_col(c, col)) return TRLHIP_EINVAL;
-    if (int rc = flush_pending(c)) return rc;
+    if (int rc = settle(c)) return rc;
     TRY(hipMemcpyAsync(c->V + (size_t)col * c->ld * c->esz, x, c->esz * (size_t)c->n,
                        hipMemcpyHostToDevice, c->s));
     if (c->V32) {  // refresh the FP32 shadow
@@ -1873,7 +2276,7 @@ int trlhip_col_set(trlhip_ctx *c, int col, const void *x)
 int trlhip_col_get(trlhip_ctx *c, int col, void *x)
 {
     if (!c || !x || !valid_col(c, col)) return TRLHIP_EINVAL;
-    if (int rc = flush_pending(c)) return rc;
+    if (int rc = settle(c)) return rc;
     TRY(hipMemcpyAsync(x, c->V + (size_t)col * c->ld * c->esz, c->esz * (size_t)c->n,
                        hipMemcpyDeviceToHost, c->s));
     return ctx_wait(c);
@@ -1910,7 +2313,7 @@ int trlhip_spmv(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol, double *
 {
     if (!c || !csr_fits(c, A)) return TRLHIP_EINVAL;
     if (!valid_col(c, xcol) || !valid_col(c, ycol) || xcol == ycol) return TRLHIP_EINVAL;
-    int rc = flush_pending(c);
+    int rc = settle(c);
     if (!rc) rc = run_spmv(c, A, xcol, ycol);
     if (!rc) rc = launch_sum(part_alpha(c), c->np_alpha, c->dscal + S_ALPHA, c->s);
     if (!rc) rc = fetch(c, c->dscal + S_ALPHA, c->hbuf, 1);
@@ -1956,7 +2359,7 @@ static int form_residual(trlhip_ctx *c, const trlhip_csr *A, int qa, int first, 
     if (A && c->pend_dst == qa) {
         rc = run_spmv_fused(c, A, qa, rcol);
     } else {
-        rc = flush_pending(c);
+        rc = settle(c);
         if (!rc) rc = A ? run_spmv(c, A, qa, rcol) : run_dot(c, qa, rcol);
     }
     if (rc) return rc;
@@ -1973,7 +2376,7 @@ int trlhip_lanczos_step(trlhip_ctx *c, const trlhip_csr *A, int qa, int qb, doub
     if (!c || !valid_col(c, qa) || !valid_col(c, rcol) || qa == rcol) return TRLHIP_EINVAL;
     if (qb >= 0 && qb != qa - 1) return TRLHIP_EINVAL;
     if (A && !csr_fits(c, A)) return TRLHIP_EINVAL;
-    if (int rc = flush_pending(c)) return rc;
+    if (int rc = settle(c)) return rc;
     int rc = form_residual(c, A, qa, 0, qb, beta_prev, nullptr, 0, rcol);
     if (rc) return rc;
     rc = fetch(c, c->dscal, c->hbuf + 4, 2);
@@ -1989,7 +2392,7 @@ int trlhip_restart_step(trlhip_ctx *c, const trlhip_csr *A, int qa, const double
     if (!c || jnd < 1 || jnd > c->ncols || qa != jnd - 1 || !valid_col(c, rcol) || rcol == qa)
         return TRLHIP_EINVAL;
     if (A && !csr_fits(c, A)) return TRLHIP_EINVAL;
-    if (int rc = flush_pending(c)) return rc;
+    if (int rc = settle(c)) return rc;
     int rc = form_residual(c, A, qa, 1, -1, 0.0, beta, jnd, rcol);
     if (rc) return rc;
     rc = fetch(c, c->dscal, c->hbuf + 4, 2);
@@ -2010,13 +2413,7 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
     if (c->batch && A && c->E == 1) {
         // batched: record the step; trlhip_batch_flush runs it with the
         // other contexts' steps
-        if (first && jnd > 1) {
-            double *h = c->hbuf + 16;
-            for (int i = 0; i < jnd - 1; ++i) h[i] = beta[i];
-            TRY(hipMemcpyAsync(c->dcoef, h, sizeof(double) * (size_t)(jnd - 1),
-                               hipMemcpyHostToDevice, c->s));
-        }
-        BatchStep st{c, A, first, qa, qb, rcol, jnd, beta_prev, 0, 1.0};
+        BatchStep st{c, A, first, qa, qb, rcol, jnd, beta_prev, 0, 1.0, -1};
         if (c->pend_dst == qa) {
             st.fuse = 1;
             st.fuse_s = c->pend_s;
@@ -2026,6 +2423,12 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
         }
         {
             std::lock_guard<std::mutex> lk(c->batch->mu);
+            if (first && jnd > 1) {  // coefficients go with the flush's staging, with
+                                     // a slot for alpha (the recurrence reads jnd)
+                st.coff = (long)c->batch->cbuf.size();
+                c->batch->cbuf.insert(c->batch->cbuf.end(), beta, beta + (jnd - 1));
+                c->batch->cbuf.push_back(0.0);
+            }
             c->batch->steps.push_back(st);
         }
         if (int rc = ctx_wait(c)) return rc;
@@ -2036,6 +2439,7 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
         out[3] = hr[S_RR2B];
         return 0;
     }
+    if (int rc = batch_drain(c)) return rc;
     int rc = form_residual(c, A, qa, first, qb, beta_prev, beta, jnd, rcol, true);
     if (rc) return rc;
     // speculative first Gram-Schmidt pass into the scratch column
@@ -2063,7 +2467,7 @@ int trlhip_cgs_pass(trlhip_ctx *c, int c0, int m, int rcol, double *hh, double *
 {
     if (!c || m < 1 || !valid_cols(c, c0, m) || !valid_col(c, rcol) || (rcol >= c0 && rcol < c0 + m))
         return TRLHIP_EINVAL;
-    int rc = flush_pending(c);
+    int rc = settle(c);
     if (!rc) rc = run_proj(c, c0, m, rcol, c->dh, true);
     if (rc) return rc;
     rc = run_update(c, c0, m, c->dh, -1, rcol, rcol, c->dscal + S_RR2, c->dscal + S_HH,
@@ -2083,7 +2487,7 @@ int trlhip_cgs_pass(trlhip_ctx *c, int c0, int m, int rcol, double *hh, double *
 int trlhip_proj(trlhip_ctx *c, int c0, int m, int rcol, double *h)
 {
     if (!c || !h || m < 1 || !valid_cols(c, c0, m) || !valid_col(c, rcol)) return TRLHIP_EINVAL;
-    int rc = flush_pending(c);
+    int rc = settle(c);
     if (!rc) rc = run_proj(c, c0, m, rcol, c->dh);
     if (rc) return rc;
     rc = fetch(c, c->dh, c->hbuf + 16, (size_t)c->E * m);
@@ -2097,7 +2501,7 @@ int trlhip_update(trlhip_ctx *c, int c0, int m, const double *h, int rcol, doubl
     if (!c || !h || m < 0 || !valid_cols(c, c0, m) || !valid_col(c, rcol) ||
         (m > 0 && rcol >= c0 && rcol < c0 + m))
         return TRLHIP_EINVAL;
-    if (int rc = flush_pending(c)) return rc;
+    if (int rc = settle(c)) return rc;
     const int E = c->E;
     memcpy(c->hbuf + 16, h, sizeof(double) * E * (size_t)m);
     if (m > 0)
@@ -2118,6 +2522,27 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y, int ldy)
     const int KT = (k + 15) / 16;
     if (KT > 8) return TRLHIP_EINVAL;  // k <= 128
     const size_t ny = (size_t)ldy * k;
+    const int jl4 = (jl + 3) & ~3;
+    const size_t shm = sizeof(double) * (size_t)jl4 * 16 * KT;
+    if (shm > 160 * 1024) return TRLHIP_EINVAL;  // gfx950: 160 KB of LDS per CU
+    if (c->batch && c->E == 1) {
+        // batched: recorded with Y; the flush runs it with the other
+        // contexts' restart GEMMs of the same phase
+        BatchOp op{};
+        op.c = c;
+        op.type = OP_RITZ;
+        op.lck = lck;
+        op.jl = jl;
+        op.k = k;
+        op.ldy = ldy;
+        std::lock_guard<std::mutex> lk(c->batch->mu);
+        op.yoff = c->batch->ybuf.size();
+        c->batch->ybuf.insert(c->batch->ybuf.end(), Y, Y + ny);
+        op.phase = c->nrec++;
+        c->batch->ops.push_back(op);
+        return 0;
+    }
+    if (int rc = batch_drain(c)) return rc;
     if (c->dYcap < ny) {
         if (c->arena) return TRLHIP_EINVAL;  // sized for the largest restart at creation
         if (c->dY) (void)hipFree(c->dY);
@@ -2134,12 +2559,9 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y, int ldy)
     // complex columns multiply as 2n-row real matrices (Y is real)
     const long nrows = (long)c->n * c->E;
     const long ldr = (long)c->ld * c->E;
-    const int tpw = 4;  // 16-row tiles per wave
+    const int tpw = kRitzTiles;
     const long rows_per_block = 16L * tpw * (kBlock / 64);
     const int g = (int)((nrows + rows_per_block - 1) / rows_per_block);
-    const int jl4 = (jl + 3) & ~3;
-    const size_t shm = sizeof(double) * (size_t)jl4 * 16 * KT;
-    if (shm > 160 * 1024) return TRLHIP_EINVAL;  // gfx950: 160 KB of LDS per CU
     Timed t(TRLHIP_K_RITZ, c->s, (double)c->esz * c->n * (jl + k));
 #define RITZ_CASE(K)                                                                          \
     case K:                                                                                   \
@@ -2152,10 +2574,39 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y, int ldy)
     }
 #undef RITZ_CASE
     if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
-    // the pinned staging of Y is reused by the next call: wait for the
-    // copy, except in a batch, where the next call comes after a flush has
-    // synchronized the stream
-    return c->batch ? 0 : ctx_wait(c);
+    // the pinned staging of Y is reused by the next call: wait for the copy
+    return ctx_wait(c);
+}
+
+int trlhip_refine(trlhip_ctx *c, const trlhip_csr *A, int k, int avoid, double *rho, double *res)
+{
+    if (!c || !rho || !res || k < 1 || k > c->ncols || c->E != 1 || !csr_fits(c, A))
+        return TRLHIP_EINVAL;
+    if (refine_width(c, k, avoid) < 1) return TRLHIP_EINVAL;
+    if (int rc = flush_pending(c)) return rc;
+    BatchOp op{};
+    op.c = c;
+    op.type = OP_REFINE;
+    op.k = k;
+    op.A = A;
+    op.avoid = avoid;
+    if (c->batch) {  // recorded; the batch flush runs it with the other contexts'
+        std::lock_guard<std::mutex> lk(c->batch->mu);
+        op.phase = c->nrec++;
+        c->batch->ops.push_back(op);
+    } else {
+        const std::vector<BatchOp> one{op};
+        const std::vector<double> none;
+        if (int rc = ops_run(one, none, c->hops, c->dops, c->ocap, c->s)) return rc;
+    }
+    if (int rc = ctx_wait(c)) return rc;
+    const volatile double *hr = c->hres + 8;
+    for (int i = 0; i < k; ++i) {
+        const bool ok = hr[3 * i + 2] > 0.0;  // v'v
+        rho[i] = ok ? hr[3 * i] : __builtin_nan("");
+        res[i] = ok ? hr[3 * i + 1] : __builtin_nan("");
+    }
+    return 0;
 }
 
 void trlhip_stats_enable(int on)
@@ -2203,6 +2654,10 @@ void trlhip_batch_destroy(trlhip_batch *b)
     if (b->ddesc) (void)hipFree(b->ddesc);
     if (b->hpre) (void)hipHostFree(b->hpre);
     if (b->dpre) (void)hipFree(b->dpre);
+    if (b->hops) (void)hipHostFree(b->hops);
+    if (b->dops) (void)hipFree(b->dops);
+    if (b->hcbuf) (void)hipHostFree(b->hcbuf);
+    if (b->dcbuf) (void)hipFree(b->dcbuf);
     for (auto &ch : b->dev) (void)hipFree(ch.p);
     for (auto &ch : b->host) (void)hipHostFree(ch.p);
     (void)hipStreamDestroy(b->s);
@@ -2216,16 +2671,37 @@ void trlhip_batch_set_yield(trlhip_batch *b, void (*yield)(void *), void *user)
     b->user = user;
 }
 
-int trlhip_batch_flush(trlhip_batch *b)
+int trlhip_batch_launch(trlhip_batch *b)
 {
     if (!b) return TRLHIP_EINVAL;
     int rc = b->err;
+    if (!rc && !b->ops.empty()) rc = ops_run(b->ops, b->ybuf, b->hops, b->dops, b->ocap, b->s);
     if (!rc && !b->steps.empty()) rc = batch_launch(b);
-    if (hipStreamSynchronize(b->s) != hipSuccess && !rc) rc = TRLHIP_EHIP;
-    stats_resolve();
+    // the records are staged; contexts stay suspended until trlhip_batch_wait
+    for (const BatchOp &op : b->ops) op.c->nrec = 0;
+    b->ops.clear();
+    b->ybuf.clear();
     b->steps.clear();
+    b->cbuf.clear();
     if (rc) b->err = rc;
     return rc;
+}
+
+int trlhip_batch_wait(trlhip_batch *b)
+{
+    if (!b) return TRLHIP_EINVAL;
+    int rc = b->err;
+    if (hipStreamSynchronize(b->s) != hipSuccess && !rc) rc = TRLHIP_EHIP;
+    stats_resolve();
+    if (rc) b->err = rc;
+    return rc;
+}
+
+int trlhip_batch_flush(trlhip_batch *b)
+{
+    const int rc = trlhip_batch_launch(b);
+    const int rc2 = trlhip_batch_wait(b);
+    return rc ? rc : rc2;
 }
 
 }  // extern "C"

@@ -329,6 +329,7 @@ class Ctx:
             ("trlhip_ritz", [P, I, I, I, P, I]),
             ("trlhip_col_scale_copy", [P, I, I, D]),
             ("trlhip_step_cgs", [P, P, I, I, I, D, P, I, I, P]),
+            ("trlhip_refine", [P, P, I, I, P, P]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -396,6 +397,14 @@ class Ctx:
     def ritz(self, lck, jl, k, Y):
         Y = np.asfortranarray(Y, dtype=np.float64)
         self._chk(lib().trlhip_ritz(self.h, lck, jl, k, _ptr(Y), Y.shape[0]), "ritz")
+
+    def refine(self, A, k, avoid):
+        """FP64 Rayleigh quotients and residual norms of columns 0..k-1
+        (trlhip_refine); returns (rho, res)."""
+        rho = np.zeros(k)
+        res = np.zeros(k)
+        self._chk(lib().trlhip_refine(self.h, A.handle, k, avoid, _ptr(rho), _ptr(res)), "refine")
+        return rho, res
 
     def close(self):
         if self.h:

@@ -155,3 +155,43 @@ def test_fused_step_sequence(gpu_lib, builder):
     assert np.abs(c.get(3) - Q[3]).max() <= 1e-10
     c.close()
     A.close()
+
+
+@pytest.mark.parametrize("n,k,ncols,avoid", [(3000, 5, 12, 5), (20000, 8, 10, 9), (4096, 7, 8, 7)])
+def test_refine_rayleigh_quotients(gpu_lib, n, k, ncols, avoid):
+    """trlhip_refine: rho_i = v'Av / v'v and ||Av - rho v|| / ||v|| for
+    columns 0..k-1, against numpy; the Ritz columns and the kept column
+    `avoid` are left as they were.  (20000, 8, 10, 9) has room for two
+    pairs at a time, so it runs in chunks; (4096, 7, 8, 7) for one."""
+    T = gpu_lib
+    rng = np.random.default_rng(n + k)
+    if n == 4096:
+        rp, ci, v = W.anderson2d_csr(64, 5.0, seed=3)[0]
+    else:
+        rp, ci, v = W.random_sym_csr(n, seed=k)
+    import scipy.sparse as sp
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    X = rng.standard_normal((n, k))
+    X[:, 1] = 0.0  # a zero column gives NaN
+    keep = rng.standard_normal(n)
+    print(f"refine n={n} k={k} ncols={ncols} avoid={avoid}", flush=True)
+    c = T.Ctx(n, ncols)
+    for j in range(k):
+        c.set(j, X[:, j])
+    c.set(avoid, keep)
+    Ad = T.Csr(rp, ci, v)
+    rho, res = c.refine(Ad, k, avoid)
+    for j in range(k):
+        x = X[:, j]
+        if j == 1:
+            assert np.isnan(rho[j]) and np.isnan(res[j])
+            continue
+        y = A @ x
+        r0 = x @ y / (x @ x)
+        r1 = np.linalg.norm(y - r0 * x) / np.linalg.norm(x)
+        assert abs(rho[j] - r0) <= 1e-13 * abs(y).max() * np.sqrt(n), (j, rho[j], r0)
+        assert abs(res[j] - r1) <= 1e-12 * r1, (j, res[j], r1)
+        assert np.array_equal(c.get(j), x)
+    assert np.array_equal(c.get(avoid), keep)
+    Ad.close()
+    c.close()
