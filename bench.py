@@ -1,24 +1,28 @@
 """Throughput of the device-resident thick-restart Lanczos solver on one GPU.
 
-Workload (one step = one complete trlan() solve, CSR resident in HBM):
-  periodic 3D 7-point Laplacian on 128^3 (n = 2,097,152, nnz = 14,680,064),
-  off-diagonal -1, diagonal 6 + U[0,1) (seed 0), FP64 CSR on the device;
-  nev = 16 largest, maxlan = 64, tol = 1e-8, restart scheme 1,
-  start vector U[0,1) (seed 1).
-value = eigenpairs with ||A v - lambda v|| <= 1e-8 ||A|| (checked on the
-host after the timed solves) per second, for the path given by --path
-(default: mixed precision); the other path is timed the same way and
-reported beside it, with the ratio and the largest eigenvalue difference.
-Each solve is synchronous (trlan() returns after its last device
-synchronisation), so the wall clock around the K solves brackets all GPU
-work.
+Default workload (--workload anderson2d-batch): 1024 independent periodic
+2D Anderson lattices, L ~ U{64..256} (n = 4096..65536), hopping -1, on-site
+U[-W/2, W/2] with W ~ U[1, 12], 8 lowest eigenpairs each, maxlan 32,
+tol 1e-8, CSR operators resident in HBM.  One step = learned-start
+inference for every lattice + one batched mixed-precision solve of all of
+them (trlan_batch).  value = eigenpairs with ||A v - lambda v|| <=
+1e-8 ||A|| (checked on the host after the timed steps) per second.  Beside
+it: the same batch from random starts, the plain FP64 batched solver, and
+the plain FP64 solver one lattice at a time (vs_plain_fp64 = value / that).
+
+--workload laplace3d: periodic 3D 7-point Laplacian on 128^3 (n =
+2,097,152, nnz = 14,680,064), diagonal 6 + U[0,1), nev = 16 largest,
+maxlan = 64, tol = 1e-8, restart 1; one step = one complete solve, the
+mixed-precision path (--path) with plain FP64 timed beside it.
 
 Prints one JSON line, including
-  roofline: the dominant kernel's algorithmic bytes / HIP-event time vs the
-            8 TB/s HBM peak, plus the SpMV + recurrence + Gram-Schmidt
-            aggregate, and the rocprofv3-measured HBM traffic per launch;
-  cpu_baseline: the compiled reference trlan (oracle/_ref, OpenBLAS +
-            OpenMP CSR operator) on a bounded sample of the same solve.
+  roofline: the dominant kernel's algorithmic bytes / HIP-event time (a
+            hashed sample of launches, events stamped from each launch's
+            own dispatch) vs the 8 TB/s HBM peak, the SpMV + recurrence +
+            Gram-Schmidt aggregate, and the rocprofv3-measured HBM traffic
+            per launch from the committed PMC summary;
+  cpu_baseline: the compiled reference trlan (oracle/_ref) on a bounded
+            sample of the same workload on the host cores.
 """
 import argparse
 import json
@@ -62,10 +66,14 @@ def calibration_copy(T, n):
     c.close()
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+LAPLACE_TRAFFIC = "hbm_traffic_laplace3d_128_fp64.txt"
+BATCH_TRAFFIC = "hbm_traffic_anderson2d_batch_1024.txt"
+
+
+def measured_traffic(kernel, summary=LAPLACE_TRAFFIC):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_laplace3d_128_fp64.txt")
+    path = os.path.join(ROOT, "profiles", summary)
     if kernel is None or not os.path.exists(path):
         return None
     for line in open(path):
@@ -146,26 +154,93 @@ def leg(T, A, x0, steps, rp, ci, v, stats_every=0):
     return t1 - t0, g, good, stats
 
 
+def anderson_params(count):
+    """(L, W, potential seed) of the batched workload's lattices: L ~
+    U{64..256}, W ~ U[1, 12], seeded."""
+    import numpy as np
+
+    m = max(count, 1024)  # lattice i is the same for every count
+    rng = np.random.default_rng(5)
+    Ls = rng.integers(64, 257, m)
+    Ws = rng.uniform(1.0, 12.0, m)
+    return [(int(L), float(w), 7000 + i) for i, (L, w) in enumerate(zip(Ls[:count], Ws[:count]))]
+
+
+def good_pairs(csr, r, tol=1e-8):
+    """Eigenpairs of result r with ||A v - lambda v|| <= tol ||A||."""
+    import scipy.sparse as sp
+
+    rp, ci, v = csr
+    n = len(rp) - 1
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    X = r["evec"][:, :r["nec"]]
+    R = A @ X - X * r["eval"][:r["nec"]]
+    return int(((R * R).sum(axis=0) ** 0.5 <= tol * r["anrm"]).sum())
+
+
+def cpu_solve_one(i):
+    """Reference trlan (oracle/_ref, 1 thread) on lattice i of the batched
+    workload; returns (eigenpairs at 1e-8, operator applications)."""
+    from nutrlan_amd import workloads as W
+    from oracle import ref
+
+    L, w, seed = anderson_params(i + 1)[i]  # same lattice as in the batch
+    csr, _ = W.anderson2d_csr(L, w, seed=seed)
+    x0 = W.start_vector(L * L, seed=9000 + i)
+    r = ref.solve_csr(*csr, -1, 8, 32, 1e-8, 1, 100000, 8, x0, nthreads=1)
+    return good_pairs(csr, r), r["matvec"]
+
+
+def cpu_baseline_batch(ns, nproc):
+    """Reference trlan on the first ns lattices, nproc single-threaded
+    solves at a time (a process pool in a child interpreter that never
+    touches the GPU).  Returns (eigenpairs/s, description) or None."""
+    from oracle import ref
+
+    if not ref.available():
+        return None
+    code = (
+        "import sys, json, time; sys.path.insert(0, %r)\n"
+        "from multiprocessing import get_context\n"
+        "import bench\n"
+        "t0 = time.perf_counter()\n"
+        "with get_context('fork').Pool(%d) as pool:\n"
+        "    out = pool.map(bench.cpu_solve_one, range(%d), chunksize=1)\n"
+        "wall = time.perf_counter() - t0\n"
+        "print(json.dumps(dict(wall=wall, good=sum(o[0] for o in out), "
+        "matvec=sum(o[1] for o in out))))\n" % (ROOT, nproc, ns))
+    env = dict(os.environ, OPENBLAS_NUM_THREADS="1", OMP_NUM_THREADS="1")
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                         timeout=900)
+    if out.returncode != 0:
+        print(out.stderr[-2000:], file=sys.stderr)
+        return None
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return r["good"] / r["wall"], (
+        f"reference trlan (1 thread each) on the first {ns} of the same lattices, {nproc} solves "
+        f"at a time: {r['good']} eigenpairs at 1e-8, {r['matvec']} operator applications, "
+        f"{r['wall']:.1f} s")
+
+
 def batched_anderson2d(args):
     """The batched workload: 1024 independent periodic 2D Anderson lattices,
-    L ~ U{64..256}, hopping -1, on-site U[-W/2, W/2] with W ~ U[1, 12]
-    (seeded), 8 lowest eigenpairs each, maxlan 32, tol 1e-8.  value = the
-    mixed-precision batched solver started from the learned net (inference
-    inside the timed region); beside it the plain FP64 solver one lattice at
-    a time (timed on a sample of the same lattices), the plain FP64 batched
-    solver and the mixed batched solver from random starts."""
+    L ~ U{64..256} (n 4096..65536, 28.7M sites in all), hopping -1, on-site
+    U[-W/2, W/2] with W ~ U[1, 12] (seeded), 8 lowest eigenpairs each,
+    maxlan 32, tol 1e-8, CSR resident in HBM.  One step = learned-start
+    inference for every lattice + the mixed-precision batched solve
+    (trlan_batch).  Beside it: the same batch from random starts, the plain
+    FP64 batched solver, and the plain FP64 solver one lattice at a time
+    (the drop-in trlan() path, timed on a sample of the same lattices)."""
     import numpy as np
-    import scipy.sparse as sp
 
     from nutrlan_amd import learned as LN
     from nutrlan_amd import trlan as T
     from nutrlan_amd import workloads as W
 
-    rng = np.random.default_rng(5)
-    count = args.models
-    Ls = rng.integers(64, 257, count)
-    Ws = rng.uniform(1.0, 12.0, count)
-    models = [LN.lattice(int(L), float(w), 7000 + i) for i, (L, w) in enumerate(zip(Ls, Ws))]
+    calibration_copy(T, 1 << 21)
+    params = anderson_params(args.models)
+    models = [LN.lattice(L, w, seed) for L, w, seed in params]
+    count = len(models)
     nev, maxlan, tol = LN.NEV, 32, 1e-8
     starts = [W.start_vector(len(c[0]) - 1, seed=9000 + i) for i, (c, d) in enumerate(models)]
     plain = [T.Csr(*c) for c, d in models]
@@ -173,68 +248,105 @@ def batched_anderson2d(args):
     net = LN.load_net()
     assert net is not None, "learned-start weights missing (run nutrlan_amd/train_start_net.py)"
 
-    def good_pairs(res, mods=models):
-        g = 0
-        for (c, d), r in zip(mods, res):
-            rp, ci, v = c
-            n = len(rp) - 1
-            A = sp.csr_matrix((v, ci, rp), shape=(n, n))
-            X = r["evec"][:, :r["nec"]]
-            R = A @ X - X * r["eval"][:r["nec"]]
-            g += int(((R * R).sum(axis=0) ** 0.5 <= 1e-8 * r["anrm"]).sum())
-        return g
+    def total_good(res, n=None):
+        return sum(good_pairs(c, r) for (c, d), r in zip(models[:n or count], res))
 
     def batch(ops, x0s):
         probs = [(A, -1, nev, maxlan, tol, 1, 100000, nev, x) for A, x in zip(ops, x0s)]
-        t0 = time.perf_counter()
         rc, res, _ = T.solve_batch(probs)
-        el = time.perf_counter() - t0
         assert rc == 0, rc
-        return el, res
+        return res
 
-    batch(mixed[:64], starts[:64])  # warm-up
-    # headline: learned start + mixed precision, batched
+    def step():
+        x_net = LN.start_vectors(net, [d for c, d in models])
+        return batch(mixed, x_net)
+
+    for _ in range(args.warmup):
+        step()
+    # headline: learned start + mixed precision, batched; one launch in
+    # STATS_EVERY of each kernel kind carries HIP start/stop events
+    T.stats_enable(True, every=STATS_EVERY)
     t0 = time.perf_counter()
-    x_net = LN.start_vectors(net, [d for c, d in models])
-    t_inf = time.perf_counter() - t0
-    el, res = batch(mixed, x_net)
-    el += t_inf
-    good = good_pairs(res)
+    for _ in range(args.steps):
+        res = step()
+    elapsed = time.perf_counter() - t0
+    stats = T.stats_get()
+    T.stats_enable(False)
+    good = total_good(res)
     mv_net = sum(r["matvec"] for r in res)
-    el_r, res_r = batch(mixed, starts)
-    good_r = good_pairs(res_r)
-    mv_rand = sum(r["matvec"] for r in res_r)
-    el_p, res_p = batch(plain, starts)
-    good_p = good_pairs(res_p)
-    # plain FP64, one lattice at a time, on a sample
+    value = good * args.steps / elapsed
+    # beside it, untimed by events
+    t0 = time.perf_counter()
+    res_r = batch(mixed, starts)
+    el_r = time.perf_counter() - t0
+    good_r, mv_rand = total_good(res_r), sum(r["matvec"] for r in res_r)
+    t0 = time.perf_counter()
+    res_p = batch(plain, starts)
+    el_p = time.perf_counter() - t0
+    good_p, mv_p = total_good(res_p), sum(r["matvec"] for r in res_p)
     ns = min(args.plain_sample, count)
     t0 = time.perf_counter()
     res_s = [T.solve_csr(plain[i], -1, nev, maxlan, tol, 1, 100000, nev, starts[i]) for i in range(ns)]
     el_s = time.perf_counter() - t0
-    g_s = good_pairs(res_s, models[:ns])
-    plain_single = g_s / el_s
-    value = good / el
+    plain_single = total_good(res_s, ns) / el_s
+    # eigenvalues of the headline path against plain FP64 (same lattices)
+    dl = max(float(np.abs(np.sort(a["eval"][:nev]) - np.sort(b["eval"][:nev])).max() / a["anrm"])
+             for a, b in zip(res, res_p))
     for A in plain + mixed:
         A.close()
     net.close()
+
+    dom = max(("spmv", "recur", "proj", "update"), key=lambda k: stats[k][1])
+    launches, ms, byts = stats[dom]
+    achieved = byts / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    path = ["spmv", "recur", "proj", "update"]
+    pb = sum(stats[k][2] for k in path)
+    pt = sum(stats[k][1] for k in path)
+    path_gbs = pb / (pt * 1e-3) / 1e9 if pt > 0 else 0.0
+    kname = {"spmv": "kb_spmv_stream", "recur": "kb_recur", "proj": "kb_proj<float>",
+             "update": "kb_update<float>"}[dom]
+    roofline = {
+        "bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": measured_traffic(kname, BATCH_TRAFFIC),
+        "avg_launch_us": round(ms * 1e3 / max(launches, 1), 2),
+        "bytes_per_launch": round(byts / max(launches, 1)),
+        "path_spmv_recur_cgs": {"achieved": round(path_gbs, 1),
+                                "frac": round(path_gbs / HBM_PEAK_GBS, 4)},
+        "kernels": {k: {"launches": s[0], "ms": round(s[1], 3),
+                        "GBps": round(s[2] / (s[1] * 1e-3) / 1e9, 1) if s[1] > 0 else 0.0}
+                    for k, s in stats.items()},
+    }
+    cpu = None
+    if not args.no_cpu_baseline:
+        nproc = min(os.cpu_count() or 1, 16)
+        cb = cpu_baseline_batch(min(args.cpu_sample, count), nproc)
+        if cb is not None:
+            cpu = {"value": round(cb[0], 2), "unit": "eigenpairs/s", "cores": nproc,
+                   "kind": "reference", "sample": cb[1]}
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "eigenpairs/s", "n_gpus": 1,
-        "steps": 1, "warmup": 1, "ms_per_step": round(1e3 * el, 1), "higher_is_better": True,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 1), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "anderson2d_batch_%d" % count, "L": "U{64..256}",
                    "W": "U[1,12]", "sites_total": int(sum(len(c[0]) - 1 for c, d in models)),
                    "nev": nev, "lohi": -1, "maxlan": maxlan, "tol": tol,
-                   "path": "batched, mixed precision, learned start",
-                   "converged_at_1e-8": good, "matvec_total": mv_net,
-                   "learned_start_inference_s": round(t_inf, 3)},
-        "mixed_batched_random_start": {"value": round(good_r / el_r, 1), "s": round(el_r, 3),
-                                       "matvec_total": mv_rand, "converged_at_1e-8": good_r},
-        "plain_batched_random_start": {"value": round(good_p / el_p, 1), "s": round(el_p, 3),
-                                       "converged_at_1e-8": good_p},
+                   "path": "batched (trlan_batch), mixed precision (FP32 shadow basis for "
+                           "Gram-Schmidt, FP64 recurrence/restart/refinement), learned start "
+                           "(BF16-MFMA conv net, inference inside each step)",
+                   "converged_at_1e-8": good, "of": nev * count, "matvec_total": mv_net},
+        "vs_plain_fp64": round(value / plain_single, 2),
         "plain_fp64_one_at_a_time": {"value": round(plain_single, 2),
                                      "sample": f"first {ns} lattices, {el_s:.2f} s"},
-        "vs_plain_fp64": round(value / plain_single, 2),
+        "plain_fp64_batched": {"value": round(good_p / el_p, 1), "s": round(el_p, 3),
+                               "matvec_total": mv_p, "converged_at_1e-8": good_p},
+        "mixed_batched_random_start": {"value": round(good_r / el_r, 1), "s": round(el_r, 3),
+                                       "matvec_total": mv_rand, "converged_at_1e-8": good_r},
         "learned_vs_random_matvec": round(mv_net / mv_rand, 4),
+        "max_eigenvalue_diff_vs_plain_fp64": dl,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
@@ -247,10 +359,14 @@ def main():
     ap.add_argument("--path", choices=["mixed", "plain"], default="mixed",
                     help="solver path measured for `value` (the other is reported beside it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=96)
-    ap.add_argument("--workload", choices=["laplace3d", "anderson2d-batch"], default="laplace3d",
-                    help="laplace3d: the 128^3 Laplacian (default line); anderson2d-batch: "
-                         "1024 uneven 2D Anderson lattices through the batched solver")
+    ap.add_argument("--cpu-sample", type=int, default=None,
+                    help="CPU baseline sample: operator applications (laplace3d, default 96) "
+                         "or lattices (anderson2d-batch, default 96)")
+    ap.add_argument("--workload", choices=["laplace3d", "anderson2d-batch"],
+                    default="anderson2d-batch",
+                    help="anderson2d-batch (default): 1024 uneven 2D Anderson lattices through "
+                         "the batched mixed-precision learned-start solver; laplace3d: the 128^3 "
+                         "Laplacian, one solve per step, mixed vs plain FP64")
     ap.add_argument("--models", type=int, default=1024)
     ap.add_argument("--plain-sample", type=int, default=64)
     args = ap.parse_args()
@@ -265,8 +381,12 @@ def main():
 
     assert T.device_count() > 0, "no GPU"
     if args.workload == "anderson2d-batch":
+        if args.cpu_sample is None:
+            args.cpu_sample = 96
         batched_anderson2d(args)
         return
+    if args.cpu_sample is None:
+        args.cpu_sample = 96
     rp, ci, v, x0 = build_problem()
     n = len(rp) - 1
     calibration_copy(T, n)

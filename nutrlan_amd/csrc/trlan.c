@@ -726,13 +726,180 @@ static int lz_matvec_step(lz_t *z, int first, int qa, int qb, double beta_prev, 
     return 0;
 }
 
+/* Eigen-decomposition of a symmetric-definite pencil (H, G) of order m
+ * (column-major, overwritten): Cholesky G = L L', C = L^-1 H L^-T, cyclic
+ * Jacobi on C, Z = L^-T W.  theta (m) ascending is not enforced; Z (m x m)
+ * holds the G-orthonormal eigenvectors.  Returns 0, or -1 when G is not
+ * positive definite or Jacobi does not converge. */
+static int lz_sygv(int m, double *H, double *G, double *theta, double *Z)
+{
+    for (int j = 0; j < m; ++j) { /* Cholesky, lower, in G */
+        double d = G[j + (size_t)j * m];
+        for (int q = 0; q < j; ++q) d -= G[j + (size_t)q * m] * G[j + (size_t)q * m];
+        if (!(d > 0.0)) return -1;
+        d = sqrt(d);
+        G[j + (size_t)j * m] = d;
+        for (int i = j + 1; i < m; ++i) {
+            double t = G[i + (size_t)j * m];
+            for (int q = 0; q < j; ++q) t -= G[i + (size_t)q * m] * G[j + (size_t)q * m];
+            G[i + (size_t)j * m] = t / d;
+        }
+    }
+    /* C = L^-1 H L^-T: forward substitution on the columns, then the rows */
+    for (int c = 0; c < m; ++c)
+        for (int i = 0; i < m; ++i) {
+            double t = H[i + (size_t)c * m];
+            for (int q = 0; q < i; ++q) t -= G[i + (size_t)q * m] * H[q + (size_t)c * m];
+            H[i + (size_t)c * m] = t / G[i + (size_t)i * m];
+        }
+    for (int r = 0; r < m; ++r)
+        for (int i = 0; i < m; ++i) {
+            double t = H[r + (size_t)i * m];
+            for (int q = 0; q < i; ++q) t -= G[i + (size_t)q * m] * H[r + (size_t)q * m];
+            H[r + (size_t)i * m] = t / G[i + (size_t)i * m];
+        }
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) Z[i + (size_t)j * m] = i == j ? 1.0 : 0.0;
+    int conv = 0;
+    for (int sweep = 0; sweep < 60 && !conv; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) {
+                const double v = H[i + (size_t)j * m];
+                tot += v * v;
+                if (i != j) off += v * v;
+            }
+        if (off <= 1e-30 * tot) {
+            conv = 1;
+            break;
+        }
+        for (int p = 0; p < m - 1; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                const double apq = 0.5 * (H[p + (size_t)q * m] + H[q + (size_t)p * m]);
+                if (apq == 0.0) continue;
+                const double app = H[p + (size_t)p * m], aqq = H[q + (size_t)q * m];
+                const double tau = (aqq - app) / (2.0 * apq);
+                const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
+                for (int k = 0; k < m; ++k) { /* columns p, q */
+                    const double hp = H[k + (size_t)p * m], hq = H[k + (size_t)q * m];
+                    H[k + (size_t)p * m] = cs * hp - sn * hq;
+                    H[k + (size_t)q * m] = sn * hp + cs * hq;
+                }
+                for (int k = 0; k < m; ++k) { /* rows p, q */
+                    const double hp = H[p + (size_t)k * m], hq = H[q + (size_t)k * m];
+                    H[p + (size_t)k * m] = cs * hp - sn * hq;
+                    H[q + (size_t)k * m] = sn * hp + cs * hq;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double zp = Z[k + (size_t)p * m], zq = Z[k + (size_t)q * m];
+                    Z[k + (size_t)p * m] = cs * zp - sn * zq;
+                    Z[k + (size_t)q * m] = sn * zp + cs * zq;
+                }
+            }
+    }
+    if (!conv) return -1;
+    for (int i = 0; i < m; ++i) theta[i] = H[i + (size_t)i * m];
+    /* Z <- L^-T Z (back substitution on each column) */
+    for (int c = 0; c < m; ++c)
+        for (int i = m - 1; i >= 0; --i) {
+            double t = Z[i + (size_t)c * m];
+            for (int q = i + 1; q < m; ++q) t -= G[q + (size_t)i * m] * Z[q + (size_t)c * m];
+            Z[i + (size_t)c * m] = t / G[i + (size_t)i * m];
+        }
+    return 0;
+}
+
+/* Rayleigh-Ritz polish of a mixed-precision solve's k Ritz vectors
+ * (columns 0..k-1, Rayleigh quotients rho): over Q = [X, R] with R the
+ * normalized FP64 residuals A x_i - rho_i x_i (columns k..2k-1), solve
+ * Q'AQ z = theta Q'Q z and keep, for each x_i in turn, the unused Ritz
+ * pair nearest rho_i; the new vectors overwrite columns 0..k-1 (restart
+ * GEMM).  The FP32 shadow's projection errors leave components outside
+ * span(X) that no rotation inside it removes; one step over [X, R]
+ * removes most of them.  Column `avoid` is preserved.  Returns 1 when
+ * the columns were replaced, 0 when skipped (too few columns, a zero
+ * residual, an indefinite Gram matrix), or a device error. */
+static int lz_polish(lz_t *z, int k, int avoid, const double *rho)
+{
+    const int m = 2 * k;
+    int save = -1, tmp = m;
+    if (avoid >= k && avoid <= m) { /* keep the last residual out of the way */
+        save = m + 1;
+        tmp = m;
+    } else if (tmp == avoid) {
+        tmp = m + 1;
+    }
+    if ((save >= 0 ? save : tmp) >= z->ncols) return 0;
+    /* G, H, Z (m x m each), theta (m), Y (m x k) */
+    double *G = (double *)calloc((size_t)m * m * 3 + (size_t)m + (size_t)m * k, sizeof(double));
+    if (!G) return 0;
+    double *H = G + (size_t)m * m, *Z = H + (size_t)m * m, *theta = Z + (size_t)m * m;
+    double *Y = theta + m;
+    int rc = 0, used = 0;
+    if (save >= 0 && trlhip_col_copy(z->dev, save, avoid)) rc = TRLHIP_STAT_DEVICE;
+    for (int i = 0; i < k && !rc; ++i) {
+        double vav = 0.0, rr2 = 0.0;
+        if (trlhip_spmv(z->dev, z->A, i, k + i, &vav) ||
+            trlhip_update(z->dev, i, 1, &rho[i], k + i, &rr2))
+            rc = TRLHIP_STAT_DEVICE;
+        else if (!(rr2 > DBL_MIN))
+            break;
+        else if (trlhip_col_scale(z->dev, k + i, 1.0 / sqrt(rr2)))
+            rc = TRLHIP_STAT_DEVICE;
+        else
+            ++used;
+    }
+    for (int j = 0; j < m && !rc && used == k; ++j) {
+        if (trlhip_proj(z->dev, 0, m, j, G + (size_t)j * m) || trlhip_spmv(z->dev, z->A, j, tmp, NULL) ||
+            trlhip_proj(z->dev, 0, m, tmp, H + (size_t)j * m))
+            rc = TRLHIP_STAT_DEVICE;
+    }
+    z->info->matvec += used + (used == k ? m : 0);
+    int done = 0;
+    if (!rc && used == k) {
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < i; ++j) {
+                const double g = 0.5 * (G[i + (size_t)j * m] + G[j + (size_t)i * m]);
+                const double h = 0.5 * (H[i + (size_t)j * m] + H[j + (size_t)i * m]);
+                G[i + (size_t)j * m] = G[j + (size_t)i * m] = g;
+                H[i + (size_t)j * m] = H[j + (size_t)i * m] = h;
+            }
+        if (lz_sygv(m, H, G, theta, Z) == 0) {
+            int *taken = (int *)calloc((size_t)m, sizeof(int));
+            if (taken) {
+                for (int i = 0; i < k; ++i) {
+                    int best = -1;
+                    for (int j = 0; j < m; ++j)
+                        if (!taken[j] && (best < 0 || fabs(theta[j] - rho[i]) < fabs(theta[best] - rho[i])))
+                            best = j;
+                    taken[best] = 1;
+                    memcpy(Y + (size_t)i * m, Z + (size_t)best * m, sizeof(double) * (size_t)m);
+                }
+                free(taken);
+                if (trlhip_ritz(z->dev, 0, m, k, Y, m)) rc = TRLHIP_STAT_DEVICE;
+                else done = 1;
+            }
+        }
+    }
+    if (!rc && save >= 0 && trlhip_col_copy(z->dev, avoid, save)) rc = TRLHIP_STAT_DEVICE;
+    free(G);
+    if (rc) {
+        z->info->stat = TRLHIP_STAT_DEVICE;
+        return rc;
+    }
+    return done;
+}
+
 /* Final FP64 Rayleigh-quotient refinement of a mixed-precision solve:
  * for each of the k Ritz vectors v in device columns 0..k-1,
  * eval = v'Av / v'v with the FP64 operator, and beta = ||A v - eval v|| /
  * ||v|| (the true residual, reported in wrk instead of the estimate), all
  * k pairs in one device call (trlhip_refine); column `avoid` (the last
- * residual) is kept.  One operator application per pair, counted in
- * matvec. */
+ * residual) is kept.  When a pair's true residual exceeds tol * ||A||,
+ * the vectors get one Rayleigh-Ritz polish over [X, residuals]
+ * (lz_polish) and are refined again.  Every operator application counts
+ * in matvec. */
 static int lz_refine(lz_t *z, int k, int avoid, double *eval, double *beta)
 {
     if (k < 1) return 0;
@@ -743,6 +910,20 @@ static int lz_refine(lz_t *z, int k, int avoid, double *eval, double *beta)
     double *rho = z->zq, *res = z->zq + k;  /* zq: free scratch after the last restart */
     DEV(trlhip_refine(z->dev, z->A, k, avoid, rho, res));
     z->info->matvec += k;
+    const double bnd = z->info->tol * z->info->anrm;
+    int over = 0, nan = 0;
+    for (int i = 0; i < k; ++i) {
+        over += res[i] > bnd;
+        nan += isnan(rho[i]) != 0;
+    }
+    if (over > 0 && nan == 0) {
+        const int p = lz_polish(z, k, avoid, rho);
+        if (p < 0 || z->info->stat != 0) return z->info->stat ? z->info->stat : p;
+        if (p == 1) {
+            DEV(trlhip_refine(z->dev, z->A, k, avoid, rho, res));
+            z->info->matvec += k;
+        }
+    }
     for (int i = 0; i < k; ++i) {
         if (isnan(rho[i])) continue;
         eval[i] = rho[i];
@@ -879,7 +1060,11 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
             }
             for (int i = 0; i < kept; ++i)
                 beta[locked + i] = z->yy[(size_t)(i + 1) * jml - 1] * betrot[jnd - 1];
-            if (jml > info->ned + (info->ned / 5 + 6))
+            /* a mixed-precision basis locks nothing: its residual estimate
+             * reaches the locking level (DBL_EPSILON |lambda|) while the
+             * true residual still sits near FLT_EPSILON-level noise, and a
+             * locked column is never improved by later restarts */
+            if (!z->mixed && jml > info->ned + (info->ned / 5 + 6))
                 i2 = trlkit_set_locking(jml, kept, alpha + locked, beta + locked, z->yy, info->anrm);
             else
                 i2 = 0;
@@ -1118,9 +1303,10 @@ void trl_solve_device(trl_matvec op, ztrl_matvec zop, trl_uo user_ortho, trl_inf
  * with its results.  Each solve makes exactly the decisions it would make
  * alone, so its eigenpairs and counters match a single trlan() call.
  *
- * The solves are split into lanes (two by default), each a batch with
- * its own stream: while the device runs one lane's launch, the host
- * threads step the other lane's solves, so host and device work overlap. */
+ * The solves are split into lanes (two by default), batches that queue on
+ * one stream and wait on their own events: while the device runs one
+ * lane's launch, the host threads step the other lane's solves, so host
+ * and device work overlap. */
 
 enum { LZ_STACK = 1024 * 1024 };  /* per coroutine; HIP runtime calls run on it */
 
@@ -1247,14 +1433,22 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     const int nlanes = lz_batch_lanes(njobs);
     trlhip_batch *bl[4] = {NULL, NULL, NULL, NULL};
     int rc = 0;
+    /* TRLHIP_BATCH_STREAMS: unset or 2, a stream per lane with each launch
+     * ordered after the other lane's (the device runs one lane at a time,
+     * so every kernel's timing is its own); 1, streams that may also run
+     * two lanes' kernels at once; 0, one shared stream */
+    const char *es = getenv("TRLHIP_BATCH_STREAMS");
+    const int mode = es && es[0] >= '0' && es[0] <= '2' ? es[0] - '0' : 2;
+    const int own = mode != 0;
+    volatile int serial = mode == 2;
     for (int l = 0; l < nlanes && !rc; ++l) {
-        rc = trlhip_batch_create(&bl[l]);
+        rc = trlhip_batch_create_on(&bl[l], (l > 0 && !own) ? bl[0] : NULL);
         if (!rc) trlhip_batch_set_yield(bl[l], lz_co_yield, NULL);
     }
     lz_co_t *cos = rc ? NULL : (lz_co_t *)calloc((size_t)njobs, sizeof(lz_co_t));
     if (!cos) {
         for (int i = 0; i < njobs; ++i) jobs[i].info->stat = TRLHIP_STAT_DEVICE;
-        for (int l = 0; l < nlanes; ++l) trlhip_batch_destroy(bl[l]);
+        for (int l = nlanes - 1; l >= 0; --l) trlhip_batch_destroy(bl[l]); /* lane 0 owns the stream */
         return rc ? rc : TRLHIP_ENOMEM;
     }
     int live[4] = {0, 0, 0, 0};
@@ -1364,6 +1558,10 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
             t_co += lz_now() - t0;
             ++rounds;
             if (live[l] > 0) {
+                if (serial && nlanes > 1) {
+                    const int orc = trlhip_batch_after(bl[l], bl[(l + nlanes - 1) % nlanes]);
+                    if (orc && !rc) rc = orc;
+                }
                 const int lrc = trlhip_batch_launch(bl[l]);
                 if (lrc && !rc) rc = lrc;
                 pending[l] = 1;
@@ -1394,7 +1592,7 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     free(pool.cos);
     free(pool.ncos);
     free(cos);
-    for (int l = 0; l < nlanes; ++l) trlhip_batch_destroy(bl[l]);
+    for (int l = nlanes - 1; l >= 0; --l) trlhip_batch_destroy(bl[l]); /* lane 0 owns the stream */
     return rc;
 }
 

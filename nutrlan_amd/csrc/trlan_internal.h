@@ -36,6 +36,9 @@ double trl_rand_uniform(uint64_t *state);
  * calls trlhip_batch_flush once every waiting context has yielded. */
 typedef struct trlhip_batch trlhip_batch;
 int trlhip_batch_create(trlhip_batch **b);
+/* A batch on the stream of `share` (a lane that interleaves with it on
+ * one stream), or on a stream of its own when share is NULL. */
+int trlhip_batch_create_on(trlhip_batch **b, trlhip_batch *share);
 void trlhip_batch_destroy(trlhip_batch *b);
 void trlhip_batch_set_yield(trlhip_batch *b, void (*yield)(void *user), void *user);
 int trlhip_batch_flush(trlhip_batch *b);
@@ -43,6 +46,9 @@ int trlhip_batch_flush(trlhip_batch *b);
  * wait for it; the batch's contexts may resume only after the wait. */
 int trlhip_batch_launch(trlhip_batch *b);
 int trlhip_batch_wait(trlhip_batch *b);
+/* Work b queues from now on starts on the device after prev's last launch
+ * (lanes on their own streams that still run one at a time). */
+int trlhip_batch_after(trlhip_batch *b, trlhip_batch *prev);
 int trlhip_ctx_create_batched(trlhip_ctx **c, int n, int ncols, int vtype, trlhip_batch *b);
 
 /* One thick-restart solve on a device basis; cplx selects ztrlan (zop,

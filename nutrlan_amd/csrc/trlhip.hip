@@ -969,6 +969,8 @@ struct BDesc;  // defined with the batched kernels
 
 struct trlhip_batch {
     hipStream_t s = nullptr;
+    bool own_stream = true;       // false: the stream belongs to another batch
+    hipEvent_t done = nullptr;    // recorded after each launch (trlhip_batch_wait)
     void (*yield)(void *) = nullptr;
     void *user = nullptr;
     int err = 0;                  // sticky device error of a flush
@@ -2632,13 +2634,23 @@ int trlhip_stats_get(double *out, int nout)
     return TRLHIP_K_COUNT;
 }
 
-int trlhip_batch_create(trlhip_batch **out)
+int trlhip_batch_create(trlhip_batch **out) { return trlhip_batch_create_on(out, nullptr); }
+
+int trlhip_batch_create_on(trlhip_batch **out, trlhip_batch *share)
 {
     if (!out) return TRLHIP_EINVAL;
     if (trlhip_device_count() <= 0) return TRLHIP_ENODEV;
     trlhip_batch *b = new (std::nothrow) trlhip_batch();
     if (!b) return TRLHIP_ENOMEM;
-    if (hipStreamCreate(&b->s) != hipSuccess) {
+    if (share) {
+        b->s = share->s;
+        b->own_stream = false;
+    } else if (hipStreamCreate(&b->s) != hipSuccess) {
+        delete b;
+        return TRLHIP_EHIP;
+    }
+    if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+        if (b->own_stream) (void)hipStreamDestroy(b->s);
         delete b;
         return TRLHIP_EHIP;
     }
@@ -2660,7 +2672,8 @@ void trlhip_batch_destroy(trlhip_batch *b)
     if (b->dcbuf) (void)hipFree(b->dcbuf);
     for (auto &ch : b->dev) (void)hipFree(ch.p);
     for (auto &ch : b->host) (void)hipHostFree(ch.p);
-    (void)hipStreamDestroy(b->s);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->own_stream) (void)hipStreamDestroy(b->s);
     delete b;
 }
 
@@ -2677,6 +2690,9 @@ int trlhip_batch_launch(trlhip_batch *b)
     int rc = b->err;
     if (!rc && !b->ops.empty()) rc = ops_run(b->ops, b->ybuf, b->hops, b->dops, b->ocap, b->s);
     if (!rc && !b->steps.empty()) rc = batch_launch(b);
+    // everything this batch queued so far (this launch and the contexts'
+    // own copies) is behind this event
+    if (!rc && hipEventRecord(b->done, b->s) != hipSuccess) rc = TRLHIP_EHIP;
     // the records are staged; contexts stay suspended until trlhip_batch_wait
     for (const BatchOp &op : b->ops) op.c->nrec = 0;
     b->ops.clear();
@@ -2687,11 +2703,18 @@ int trlhip_batch_launch(trlhip_batch *b)
     return rc;
 }
 
+int trlhip_batch_after(trlhip_batch *b, trlhip_batch *prev)
+{
+    if (!b || !prev) return TRLHIP_EINVAL;
+    return hipStreamWaitEvent(b->s, prev->done, 0) == hipSuccess ? 0 : TRLHIP_EHIP;
+}
+
 int trlhip_batch_wait(trlhip_batch *b)
 {
     if (!b) return TRLHIP_EINVAL;
     int rc = b->err;
-    if (hipStreamSynchronize(b->s) != hipSuccess && !rc) rc = TRLHIP_EHIP;
+    if (!rc && hipEventSynchronize(b->done) != hipSuccess) rc = TRLHIP_EHIP;
+    if (rc) (void)hipStreamSynchronize(b->s);  // after an error, drain what was queued
     stats_resolve();
     if (rc) b->err = rc;
     return rc;

@@ -6,7 +6,10 @@ Rayleigh-quotient refinement of every returned pair.
 Bars: every returned pair has a true residual ||A v - lambda v|| <= tol ||A||
 (recomputed here in numpy), the reported wrk[i] is that residual, and each
 eigenvalue's error against the FP64 oracle (the reference restated) is
-printed and bounded by 1e-10 ||A||.
+printed and bounded by 1e-10 ||A||.  The localized Anderson case needs the
+Rayleigh-Ritz polish to meet the residual bar (without it the ground state
+ends at 1.8e-8 ||A||, reproduced by a CPU emulation of the FP32-shadow
+projections on the oracle).
 """
 import numpy as np
 import pytest
@@ -21,6 +24,11 @@ CASES = [
     ("lap24_lg16", lambda: W.laplace3d_csr(24, seed=0), 1, 16, 64, 1e-8, 1),
     ("rand30k_lg16", lambda: W.random_sym_csr(30000, seed=1), 1, 16, 64, 1e-8, 1),
     ("and2d96_sm8", lambda: W.anderson2d_csr(96, 6.0, seed=2)[0], -1, 8, 32, 1e-8, 3),
+    # strongly localized ground state: the FP32 shadow's projection errors
+    # leave its true residual near 1.8e-8 ||A|| until the final
+    # Rayleigh-Ritz polish over [X, residuals]
+    ("and2d80_W11_polish", lambda: W.anderson2d_csr(80, 11.35921550186609, seed=7718)[0], -1, 8,
+     32, 1e-8, 9718),
     ("diag2000_sm10", lambda: W.diag_csr(2000), -1, 10, 100, 1.4901e-8, 4),
 ]
 
