@@ -265,7 +265,7 @@ def batched_anderson2d(args):
         step()
     # headline: learned start + mixed precision, batched; one launch in
     # STATS_EVERY of each kernel kind carries HIP start/stop events
-    T.stats_enable(True, every=STATS_EVERY)
+    T.stats_enable(True, every=args.stats_every)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         res = step()
@@ -275,23 +275,29 @@ def batched_anderson2d(args):
     good = total_good(res)
     mv_net = sum(r["matvec"] for r in res)
     value = good * args.steps / elapsed
-    # beside it, untimed by events
-    t0 = time.perf_counter()
-    res_r = batch(mixed, starts)
-    el_r = time.perf_counter() - t0
-    good_r, mv_rand = total_good(res_r), sum(r["matvec"] for r in res_r)
-    t0 = time.perf_counter()
-    res_p = batch(plain, starts)
-    el_p = time.perf_counter() - t0
-    good_p, mv_p = total_good(res_p), sum(r["matvec"] for r in res_p)
-    ns = min(args.plain_sample, count)
-    t0 = time.perf_counter()
-    res_s = [T.solve_csr(plain[i], -1, nev, maxlan, tol, 1, 100000, nev, starts[i]) for i in range(ns)]
-    el_s = time.perf_counter() - t0
-    plain_single = total_good(res_s, ns) / el_s
-    # eigenvalues of the headline path against plain FP64 (same lattices)
-    dl = max(float(np.abs(np.sort(a["eval"][:nev]) - np.sort(b["eval"][:nev])).max() / a["anrm"])
-             for a, b in zip(res, res_p))
+    if args.headline_only:  # a profiling run: no legs beside the headline
+        el_r = el_p = el_s = float("nan")
+        good_r = mv_rand = good_p = mv_p = 0
+        plain_single = dl = float("nan")
+        ns = 0
+    else:  # beside it, untimed by events
+        t0 = time.perf_counter()
+        res_r = batch(mixed, starts)
+        el_r = time.perf_counter() - t0
+        good_r, mv_rand = total_good(res_r), sum(r["matvec"] for r in res_r)
+        t0 = time.perf_counter()
+        res_p = batch(plain, starts)
+        el_p = time.perf_counter() - t0
+        good_p, mv_p = total_good(res_p), sum(r["matvec"] for r in res_p)
+        ns = min(args.plain_sample, count)
+        t0 = time.perf_counter()
+        res_s = [T.solve_csr(plain[i], -1, nev, maxlan, tol, 1, 100000, nev, starts[i])
+                 for i in range(ns)]
+        el_s = time.perf_counter() - t0
+        plain_single = total_good(res_s, ns) / el_s
+        # eigenvalues of the headline path against plain FP64 (same lattices)
+        dl = max(float(np.abs(np.sort(a["eval"][:nev]) - np.sort(b["eval"][:nev])).max()
+                       / a["anrm"]) for a, b in zip(res, res_p))
     for A in plain + mixed:
         A.close()
     net.close()
@@ -313,12 +319,12 @@ def batched_anderson2d(args):
         "bytes_per_launch": round(byts / max(launches, 1)),
         "path_spmv_recur_cgs": {"achieved": round(path_gbs, 1),
                                 "frac": round(path_gbs / HBM_PEAK_GBS, 4)},
-        "kernels": {k: {"launches": s[0], "ms": round(s[1], 3),
+        "kernels": {k: {"launches": s[0], "ms": round(s[1], 3), "bytes": round(s[2]),
                         "GBps": round(s[2] / (s[1] * 1e-3) / 1e9, 1) if s[1] > 0 else 0.0}
                     for k, s in stats.items()},
     }
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.headline_only:
         nproc = min(os.cpu_count() or 1, 16)
         cb = cpu_baseline_batch(min(args.cpu_sample, count), nproc)
         if cb is not None:
@@ -336,18 +342,22 @@ def batched_anderson2d(args):
                            "Gram-Schmidt, FP64 recurrence/restart/refinement), learned start "
                            "(BF16-MFMA conv net, inference inside each step)",
                    "converged_at_1e-8": good, "of": nev * count, "matvec_total": mv_net},
-        "vs_plain_fp64": round(value / plain_single, 2),
+        "vs_plain_fp64": round(value / plain_single, 2) if plain_single == plain_single else None,
         "plain_fp64_one_at_a_time": {"value": round(plain_single, 2),
                                      "sample": f"first {ns} lattices, {el_s:.2f} s"},
         "plain_fp64_batched": {"value": round(good_p / el_p, 1), "s": round(el_p, 3),
                                "matvec_total": mv_p, "converged_at_1e-8": good_p},
         "mixed_batched_random_start": {"value": round(good_r / el_r, 1), "s": round(el_r, 3),
                                        "matvec_total": mv_rand, "converged_at_1e-8": good_r},
-        "learned_vs_random_matvec": round(mv_net / mv_rand, 4),
+        "learned_vs_random_matvec": round(mv_net / mv_rand, 4) if mv_rand else None,
         "max_eigenvalue_diff_vs_plain_fp64": dl,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if args.headline_only:
+        for k in ("plain_fp64_one_at_a_time", "plain_fp64_batched", "mixed_batched_random_start",
+                  "max_eigenvalue_diff_vs_plain_fp64"):
+            line[k] = None
     print(json.dumps(line), flush=True)
 
 
@@ -369,6 +379,10 @@ def main():
                          "Laplacian, one solve per step, mixed vs plain FP64")
     ap.add_argument("--models", type=int, default=1024)
     ap.add_argument("--plain-sample", type=int, default=64)
+    ap.add_argument("--stats-every", type=int, default=STATS_EVERY,
+                    help="time one launch in N of each kernel kind with HIP events (1: all)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="anderson2d-batch: time the headline path only (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): the multi-GPU "
@@ -400,7 +414,7 @@ def main():
     # headline path: one launch in STATS_EVERY of each kernel kind (a hashed,
     # non-periodic pick) carries HIP start/stop events stamped from its own
     # dispatch on the solver's stream; timing every launch would cost ~10%.
-    elapsed, g, good, stats = leg(T, ops[args.path], x0, args.steps, rp, ci, v, STATS_EVERY)
+    elapsed, g, good, stats = leg(T, ops[args.path], x0, args.steps, rp, ci, v, args.stats_every)
     value = good * args.steps / elapsed
     other_value = o_good * args.steps / o_el
     nec = g["nec"]

@@ -58,6 +58,16 @@ typedef struct trlhip_ctx trlhip_ctx;   /* device basis + workspace  */
  * range) is rejected with EINVAL. */
 int trlhip_csr_create(trlhip_csr **A, int n, long nnz, const int *row_ptr,
                       const int *col_idx, const void *values, int vtype);
+
+/* SpMV autotuner: times the CSR-stream kernel (row blocks) and the vector
+ * kernel at 2, 4, 8, 16 and 32 lanes per row on A itself (reps launches
+ * each, 0 = 10, HIP events on A's stream) and keeps the fastest for every
+ * later use of A, single and batched solves alike.  *lpr_out = the choice
+ * (0: CSR-stream), *ms_out = its time per SpMV (either may be NULL).
+ * Opt-in: a different kernel sums each row in a different order, so
+ * results can differ from the default kernel's in the last bits.  No
+ * counterpart in the reference, whose operator is the caller's own op. */
+int trlhip_csr_autotune(trlhip_csr *A, int reps, int *lpr_out, float *ms_out);
 int trlhip_csr_destroy(trlhip_csr *A);
 
 /* trl_matvec-compatible operator for a trlhip_csr passed as mvparam.

@@ -2033,6 +2033,18 @@ int trlhip_device_count(void)
 // (balanced rows); otherwise CSR-stream, whose row blocks absorb uneven
 // rows.  A choice made from the structure alone, so a matrix always runs
 // the same kernel (and reduction order).
+// Select the SpMV kernel: 0 = CSR-stream (row blocks), else the vector
+// kernel with that many lanes per row (and its grid).
+static void set_lpr(trlhip_csr *A, int lpr)
+{
+    A->lpr = lpr;
+    if (!lpr) return;
+    const int U = A->E == 1 ? 4 : 2;
+    const long rows_per_block = (long)(64 / lpr) * U * (kBlock / 64);
+    const long g = (A->n + rows_per_block - 1) / rows_per_block;
+    A->vgrid = (int)std::min<long>(std::max<long>(g, 1), kMaxGrid);
+}
+
 static void choose_spmv(trlhip_csr *A, const int *row_ptr)
 {
     const int n = A->n;
@@ -2042,14 +2054,10 @@ static void choose_spmv(trlhip_csr *A, const int *row_ptr)
     int lpr = 2;
     while (lpr < 32 && lpr < avg) lpr <<= 1;
     // the row-block (CSR-stream) kernel is the default: measured faster on
-    // regular stencils too (6.0 vs 4.0 TB/s on the 128^3 Laplacian); the
-    // vector kernel's lanes per row are kept for trlhip_csr_autotune
+    // regular stencils too (6.0 vs 4.0 TB/s on the 128^3 Laplacian);
+    // trlhip_csr_autotune times the vector kernel against it per matrix
     A->lpr_vec = maxlen <= 8 * lpr ? lpr : 0;
-    A->lpr = 0;
-    const int U = A->E == 1 ? 4 : 2;
-    const long rows_per_block = (long)(64 / lpr) * U * (kBlock / 64);
-    const long g = (n + rows_per_block - 1) / rows_per_block;
-    A->vgrid = (int)std::min<long>(std::max<long>(g, 1), kMaxGrid);
+    set_lpr(A, 0);
 }
 
 int trlhip_csr_create(trlhip_csr **out, int n, long nnz, const int *row_ptr, const int *col_idx,
@@ -2090,6 +2098,50 @@ int trlhip_csr_create(trlhip_csr **out, int n, long nnz, const int *row_ptr, con
     }
     *out = A;
     return 0;
+}
+
+int trlhip_csr_autotune(trlhip_csr *A, int reps, int *lpr_out, float *ms_out)
+{
+    if (!A || A->magic != kCsrMagic || reps < 0) return TRLHIP_EINVAL;
+    if (reps == 0) reps = 10;
+    const size_t ne = (size_t)A->n * A->E;
+    double *x = nullptr, *y = nullptr, *part = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (!A->s && hipStreamCreate(&A->s) != hipSuccess) return TRLHIP_EHIP;
+    if (hipMalloc((void **)&x, sizeof(double) * ne) != hipSuccess ||
+        hipMalloc((void **)&y, sizeof(double) * ne) != hipSuccess ||
+        hipMalloc((void **)&part, sizeof(double) * kSpmvGrid) != hipSuccess)
+        rc = TRLHIP_ENOMEM;
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = TRLHIP_EHIP;
+    if (!rc && hipMemsetAsync(x, 0, sizeof(double) * ne, A->s) != hipSuccess) rc = TRLHIP_EHIP;
+    const int prev = A->lpr;
+    int best = prev;
+    float best_ms = 0.f;
+    const int cand[6] = {0, 2, 4, 8, 16, 32};
+    for (int i = 0; i < 6 && !rc; ++i) {
+        set_lpr(A, cand[i]);
+        for (int w = 0; w < 2 && !rc; ++w) rc = launch_spmv(A, A->vtype, x, y, part, A->s);
+        if (!rc && hipEventRecord(e0, A->s) != hipSuccess) rc = TRLHIP_EHIP;
+        for (int r = 0; r < reps && !rc; ++r) rc = launch_spmv(A, A->vtype, x, y, part, A->s);
+        if (!rc && hipEventRecord(e1, A->s) != hipSuccess) rc = TRLHIP_EHIP;
+        float ms = 0.f;
+        if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = TRLHIP_EHIP;
+        if (!rc && (i == 0 || ms < best_ms)) {
+            best_ms = ms;
+            best = cand[i];
+        }
+    }
+    set_lpr(A, rc ? prev : best);
+    if (!rc && lpr_out) *lpr_out = best;
+    if (!rc && ms_out) *ms_out = best_ms / (float)reps;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (x) (void)hipFree(x);
+    if (y) (void)hipFree(y);
+    if (part) (void)hipFree(part);
+    return rc;
 }
 
 int trlhip_csr_destroy(trlhip_csr *A)

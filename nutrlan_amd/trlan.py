@@ -147,6 +147,20 @@ class Csr:
         self.handle = h
         self.op = fn_addr("trlhip_csr_zmatvec" if complex_ else "trlhip_csr_matvec")
 
+    def autotune(self, reps=10):
+        """Time the SpMV kernels on this matrix and keep the fastest
+        (trlhip_csr_autotune); returns (lanes per row, 0 = CSR-stream;
+        ms per SpMV)."""
+        L = lib()
+        L.trlhip_csr_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)]
+        L.trlhip_csr_autotune.restype = ctypes.c_int
+        lpr, ms = ctypes.c_int(), ctypes.c_float()
+        rc = L.trlhip_csr_autotune(self.handle, reps, ctypes.byref(lpr), ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError(f"trlhip_csr_autotune failed ({rc})")
+        return lpr.value, ms.value
+
     def close(self):
         if self.handle:
             lib().trlhip_csr_destroy(self.handle)

@@ -122,3 +122,15 @@ def test_device_entry_points_fail_cleanly_without_gpu():
                              ci.ctypes.data_as(ctypes.c_void_p),
                              v.ctypes.data_as(ctypes.c_void_p), T.TRLHIP_R64)
     assert rc == -4  # TRLHIP_ENODEV
+
+
+def test_shipped_start_net_weights():
+    """The trained learned-start weights load without executing anything
+    (numpy, allow_pickle=False) and have the net's parameter count."""
+    import numpy as np
+
+    from nutrlan_amd import learned as LN
+
+    p = np.load(LN.WEIGHTS, allow_pickle=False)
+    assert p.dtype == np.float32 and p.shape == (19105,)
+    assert np.all(np.isfinite(p)) and np.abs(p).max() < 100
