@@ -29,8 +29,9 @@ typedef struct {
 
 /* Runs every job to completion.  Each job makes the decisions a lone
  * trlan() call would make (same eigenpairs and counters); their Lanczos
- * steps run as one batched launch per step phase across all jobs, with one
- * device synchronisation per round.  Per-job status in info->stat; returns
+ * steps, restart GEMMs, column copies and final refinements run as one
+ * batched launch per phase across the jobs of a lane (two lanes by default,
+ * pipelined so the host steps one lane while the device runs the other).  Per-job status in info->stat; returns
  * 0, or a TRLHIP_E* code when the batch itself failed (then every unfinished
  * job's info->stat is TRLHIP_STAT_DEVICE).  Not re-entrant: one batch at a
  * time per process. */
