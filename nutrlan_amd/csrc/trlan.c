@@ -22,6 +22,7 @@
 #include <string.h>
 #include <time.h>
 #include <pthread.h>
+#include <sched.h>
 #include <ucontext.h>
 #include <unistd.h>
 
@@ -1328,15 +1329,53 @@ typedef struct lz_worker {
     struct lz_pool *pool;
     int tid;
     double busy;  /* seconds spent resuming its coroutines */
+    long resumes;
+    double idle;  /* seconds between finishing its part of a round and the round's end */
+    double rmax, tlong;  /* longest resume; time in resumes over 100 us */
+    long nlong;
 } lz_worker_t;
 
+/* Barrier for the scheduler threads.  Rounds come ~0.1-1 ms apart and the
+ * threads are dedicated, so a waiting thread spins (pause) instead of
+ * sleeping in the kernel (futex wake-ups cost ~0.1 ms a round with 16
+ * threads); after ~10 ms of spinning it yields its core instead. */
+typedef struct {
+    int n;      /* participating threads */
+    int count;  /* arrivals in the current generation */
+    int gen;
+} lz_spin_t;
+
+static void lz_spin_init(lz_spin_t *b, int n)
+{
+    b->n = n;
+    b->count = 0;
+    b->gen = 0;
+}
+
+static void lz_spin_wait(lz_spin_t *b)
+{
+    const int gen = __atomic_load_n(&b->gen, __ATOMIC_ACQUIRE);
+    if (__atomic_add_fetch(&b->count, 1, __ATOMIC_ACQ_REL) == b->n) {
+        __atomic_store_n(&b->count, 0, __ATOMIC_RELAXED);
+        __atomic_store_n(&b->gen, gen + 1, __ATOMIC_RELEASE);
+        return;
+    }
+    for (long spins = 0; __atomic_load_n(&b->gen, __ATOMIC_ACQUIRE) == gen; ++spins) {
+        if (spins < 200000)
+            __builtin_ia32_pause();
+        else
+            sched_yield();
+    }
+}
+
 typedef struct lz_pool {
-    lz_co_t **cos;   /* per worker: its coroutines of the lane being run */
-    int *ncos;
+    lz_co_t **lanes[4];  /* per lane: its unfinished coroutines */
+    int nlane[4];
+    int next;            /* next index of the running lane's list (atomic) */
     lz_co_t *all;
     int njobs, nthr;
     lz_worker_t *w;
-    pthread_barrier_t start, stop;
+    lz_spin_t start, stop;
     volatile int quit;
     volatile int lane;  /* lane of the round in progress */
 } lz_pool_t;
@@ -1350,40 +1389,57 @@ static double lz_now(void)
     return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
+static __attribute__((noinline, noipa)) lz_worker_t *lz_cur_worker(void);
+
 static void lz_co_yield(void *user)
 {
     (void)user;
-    lz_worker_t *w = tl_worker;
+    lz_worker_t *w = lz_cur_worker();
     swapcontext(&w->cur->uc, &w->sched);
 }
 
+/* The scheduler thread running the calling coroutine.  A coroutine may be
+ * resumed by another thread after each yield, so this reads the
+ * thread-local afresh on every call (never a cached TLS address). */
+static __attribute__((noinline, noipa)) lz_worker_t *lz_cur_worker(void) { return tl_worker; }
+
 static void lz_co_main(void)
 {
-    lz_co_t *co = tl_worker->cur;
+    lz_co_t *co = lz_cur_worker()->cur;
     trl_batch_job *j = co->job;
     lz_iterate(&co->z, j->eval, j->evec, j->lde, j->wrk, j->lwrk);
     co->done = 1;
-    /* back to this thread's scheduler for good (a finished coroutine is
-     * never resumed; returning would follow uc_link instead) */
-    for (;;) swapcontext(&co->uc, &tl_worker->sched);
+    /* back to the scheduler for good (a finished coroutine is never
+     * resumed; returning would follow uc_link instead) */
+    for (;;) swapcontext(&co->uc, &lz_cur_worker()->sched);
 }
 
-/* Resume every unfinished coroutine of this worker in the current lane
- * once (each runs until it waits on the device or finishes).  Coroutine i
- * of lane l is run by worker (i / nlanes) % nthr, so each lane spreads
- * over all workers. */
+/* Resume every unfinished coroutine of the current lane once (each runs
+ * until it waits on the device or finishes).  The workers take coroutines
+ * from the lane's list through a shared index, so a round ends when the
+ * work, not one thread's share of it, is done. */
 static void lz_round(lz_worker_t *w)
 {
     tl_worker = w;
     const double t0 = lz_now();
     lz_pool_t *p = w->pool;
-    lz_co_t **mine = p->cos + (size_t)w->tid * p->njobs;
-    const int cnt = p->ncos[w->tid];
-    for (int k = 0; k < cnt; ++k) {
-        lz_co_t *co = mine[k];
-        if (co->done || co->lane != p->lane) continue;
+    lz_co_t **list = p->lanes[p->lane];
+    const int cnt = p->nlane[p->lane];
+    for (;;) {
+        const int k = __atomic_fetch_add(&p->next, 1, __ATOMIC_RELAXED);
+        if (k >= cnt) break;
+        lz_co_t *co = list[k];
+        if (co->done) continue;
         w->cur = co;
+        const double r0 = lz_now();
         swapcontext(&w->sched, &co->uc);
+        const double dr = lz_now() - r0;
+        ++w->resumes;
+        if (dr > w->rmax) w->rmax = dr;
+        if (dr > 1e-4) {
+            w->tlong += dr;
+            ++w->nlong;
+        }
     }
     w->busy += lz_now() - t0;
 }
@@ -1392,10 +1448,12 @@ static void *lz_worker_main(void *arg)
 {
     lz_worker_t *w = (lz_worker_t *)arg;
     for (;;) {
-        pthread_barrier_wait(&w->pool->start);
+        lz_spin_wait(&w->pool->start);
         if (w->pool->quit) break;
         lz_round(w);
-        pthread_barrier_wait(&w->pool->stop);
+        const double tf = lz_now();
+        lz_spin_wait(&w->pool->stop);
+        w->idle += lz_now() - tf;
     }
     return NULL;
 }
@@ -1430,6 +1488,7 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
 {
     if (njobs <= 0) return 0;
     if (!jobs) return TRLHIP_EINVAL;
+    const double t_begin = lz_now();
     const int nlanes = lz_batch_lanes(njobs);
     trlhip_batch *bl[4] = {NULL, NULL, NULL, NULL};
     int rc = 0;
@@ -1484,8 +1543,8 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     pthread_t *th = (pthread_t *)calloc((size_t)pool.nthr, sizeof(pthread_t));
     int nstarted = 0;
     if (pool.w && th) {
-        pthread_barrier_init(&pool.start, NULL, (unsigned)pool.nthr);
-        pthread_barrier_init(&pool.stop, NULL, (unsigned)pool.nthr);
+        lz_spin_init(&pool.start, pool.nthr);
+        lz_spin_init(&pool.stop, pool.nthr);
         for (int t = 0; t < pool.nthr; ++t) {
             pool.w[t].pool = &pool;
             pool.w[t].tid = t;
@@ -1497,12 +1556,9 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
         /* could not start the threads: stop any started, run on this one */
         if (nstarted > 0) {
             pool.quit = 1;
-            pthread_barrier_wait(&pool.start);
+            lz_spin_init(&pool.start, nstarted + 1);  /* only those threads wait */
+            lz_spin_wait(&pool.start);
             for (int t = 1; t <= nstarted; ++t) pthread_join(th[t], NULL);
-        }
-        if (pool.w && th) {
-            pthread_barrier_destroy(&pool.start);
-            pthread_barrier_destroy(&pool.stop);
         }
         free(th);
         th = NULL;
@@ -1511,11 +1567,13 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
         pool.w[0].pool = &pool;
         pool.w[0].tid = 0;
     }
-    /* each worker's coroutines: i of lane i % nlanes goes to worker
-     * (i / nlanes) % nthr */
-    pool.cos = (lz_co_t **)calloc((size_t)pool.nthr * njobs, sizeof(lz_co_t *));
-    pool.ncos = (int *)calloc((size_t)pool.nthr, sizeof(int));
-    if (!pool.cos || !pool.ncos) {  /* no solve runs */
+    /* each lane's list of unfinished coroutines */
+    int lists_ok = 1;
+    for (int l = 0; l < nlanes; ++l) {
+        pool.lanes[l] = (lz_co_t **)calloc((size_t)njobs, sizeof(lz_co_t *));
+        if (!pool.lanes[l]) lists_ok = 0;
+    }
+    if (!lists_ok) {  /* no solve runs */
         for (int i = 0; i < njobs; ++i)
             if (!cos[i].done) {
                 cos[i].done = 1;
@@ -1524,16 +1582,15 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
         for (int l = 0; l < nlanes; ++l) live[l] = 0;
         rc = TRLHIP_ENOMEM;
     } else {
-        for (int i = 0; i < njobs; ++i) {
-            const int t = (i / nlanes) % pool.nthr;
-            pool.cos[(size_t)t * njobs + pool.ncos[t]++] = &cos[i];
-        }
+        for (int i = 0; i < njobs; ++i)
+            if (!cos[i].done) pool.lanes[cos[i].lane][pool.nlane[cos[i].lane]++] = &cos[i];
     }
     /* TRLHIP_BATCH_PROFILE=1: host time in the solves vs waiting for the
      * device */
     const char *prof = getenv("TRLHIP_BATCH_PROFILE");
     double t_co = 0.0, t_wait = 0.0;
     long rounds = 0;
+    const double t_loop = lz_now();
     int pending[4] = {0, 0, 0, 0};
     for (;;) {
         int any = 0;
@@ -1550,11 +1607,18 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
             any = 1;
             const double t0 = lz_now();
             pool.lane = l;
-            if (pool.nthr > 1) pthread_barrier_wait(&pool.start);
+            pool.next = 0;
+            if (pool.nthr > 1) lz_spin_wait(&pool.start);
             lz_round(&pool.w[0]);
-            if (pool.nthr > 1) pthread_barrier_wait(&pool.stop);
-            live[l] = 0;
-            for (int i = l; i < njobs; i += nlanes) live[l] += !cos[i].done;
+            const double tf = lz_now();
+            if (pool.nthr > 1) lz_spin_wait(&pool.stop);
+            pool.w[0].idle += lz_now() - tf;
+            /* drop the finished ones from the lane's list */
+            int m = 0;
+            for (int k = 0; k < pool.nlane[l]; ++k)
+                if (!pool.lanes[l][k]->done) pool.lanes[l][m++] = pool.lanes[l][k];
+            pool.nlane[l] = m;
+            live[l] = m;
             t_co += lz_now() - t0;
             ++rounds;
             if (live[l] > 0) {
@@ -1571,16 +1635,31 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     }
     if (pool.nthr > 1) {
         pool.quit = 1;
-        pthread_barrier_wait(&pool.start);
+        lz_spin_wait(&pool.start);
         for (int t = 1; t < pool.nthr; ++t) pthread_join(th[t], NULL);
-        pthread_barrier_destroy(&pool.start);
-        pthread_barrier_destroy(&pool.stop);
     }
-    if (prof && prof[0] == '1')
+    if (prof && prof[0] == '1') {
+        double bsum = 0.0, bmax = 0.0, isum = 0.0, rmax = 0.0, tlong = 0.0;
+        long res = 0, nlong = 0;
+        for (int t = 0; t < pool.nthr; ++t) {
+            bsum += pool.w[t].busy;
+            bmax = pool.w[t].busy > bmax ? pool.w[t].busy : bmax;
+            res += pool.w[t].resumes;
+            isum += pool.w[t].idle;
+            rmax = pool.w[t].rmax > rmax ? pool.w[t].rmax : rmax;
+            tlong += pool.w[t].tlong;
+            nlong += pool.w[t].nlong;
+        }
+        fprintf(stderr, "trlan_batch: idle at round ends mean %.3f s; longest resume %.1f us; "
+                "%ld resumes over 100 us take %.3f s\n", isum / pool.nthr, 1e6 * rmax, nlong, tlong);
         fprintf(stderr,
                 "trlan_batch: %d host threads, %d lanes, %ld lane rounds, %.3f s in solves, "
-                "%.3f s waiting for the device\n",
-                pool.nthr, nlanes, rounds, t_co, t_wait);
+                "%.3f s waiting for the device; %ld resumes, thread busy mean %.3f max %.3f s "
+                "(%.1f us per resume)\n",
+                pool.nthr, nlanes, rounds, t_co, t_wait, res, bsum / pool.nthr, bmax,
+                res ? 1e6 * bsum / res : 0.0);
+    }
+    const double t_end_loop = lz_now();
     /* teardown after the last round: freeing device memory synchronizes */
     for (int i = 0; i < njobs; ++i) {
         lz_co_t *co = &cos[i];
@@ -1589,10 +1668,12 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     }
     free(th);
     free(pool.w);
-    free(pool.cos);
-    free(pool.ncos);
+    for (int l = 0; l < nlanes; ++l) free(pool.lanes[l]);
     free(cos);
     for (int l = nlanes - 1; l >= 0; --l) trlhip_batch_destroy(bl[l]); /* lane 0 owns the stream */
+    if (prof && prof[0] == '1')
+        fprintf(stderr, "trlan_batch: setup %.3f s, rounds %.3f s, teardown %.3f s\n",
+                t_loop - t_begin, t_end_loop - t_loop, lz_now() - t_end_loop);
     return rc;
 }
 

@@ -40,6 +40,8 @@ int trlhip_batch_create(trlhip_batch **b);
  * one stream), or on a stream of its own when share is NULL. */
 int trlhip_batch_create_on(trlhip_batch **b, trlhip_batch *share);
 void trlhip_batch_destroy(trlhip_batch *b);
+/* Frees the arena chunks that destroyed batches left for reuse. */
+void trlhip_batch_trim(void);
 void trlhip_batch_set_yield(trlhip_batch *b, void (*yield)(void *user), void *user);
 int trlhip_batch_flush(trlhip_batch *b);
 /* trlhip_batch_flush in two halves: launch the recorded work (no wait), and

@@ -935,6 +935,7 @@ struct trlhip_ctx {
     int nrec;             // recorded restart GEMMs / scale-copies not yet flushed (batch)
     char *hops, *dops;    // staging of a refinement outside a batch (pinned / device)
     size_t ocap;
+    char *hstage;         // batch: pinned staging of one column for col_set / col_get
 };
 
 // One recorded Lanczos step of a batched context (see the batch section).
@@ -968,9 +969,15 @@ struct BDesc;  // defined with the batched kernels
 }
 
 struct trlhip_batch {
-    hipStream_t s = nullptr;
-    bool own_stream = true;       // false: the stream belongs to another batch
+    hipStream_t s = nullptr;      // the batched launches
+    bool own_stream = true;       // false: s belongs to another batch
     hipEvent_t done = nullptr;    // recorded after each launch (trlhip_batch_wait)
+    // the contexts' own (immediate) work: copies, extra Gram-Schmidt passes.
+    // Kept off s, so a blocking copy from pageable memory waits only for
+    // this batch's finished work, never for another lane's queued launch;
+    // each launch waits for it (side_done) to keep program order.
+    hipStream_t side = nullptr;
+    hipEvent_t side_done = nullptr;
     void (*yield)(void *) = nullptr;
     void *user = nullptr;
     int err = 0;                  // sticky device error of a flush
@@ -1020,6 +1027,14 @@ S *colp(trlhip_ctx *c, int col)
 // at least 1 GiB are zeroed on the batch stream when taken; host chunks of
 // at least 64 MiB are pinned, host-coherent and mapped (dptr: the device
 // address of the returned block).
+// Arena chunks outlive their batch: a destroyed batch hands them here and
+// the next batch reuses them (a pinned or device allocation of this size
+// costs milliseconds).  Bytes past a chunk's `used` mark are zero; a reused
+// chunk has its used part zeroed again.  Held until trlhip_batch_trim() or
+// the end of the process.
+std::mutex g_chunk_mu;
+std::vector<trlhip_batch::Chunk> g_free_chunks[2];  // device, host
+
 void *arena_get(trlhip_batch *b, bool host, size_t bytes, void **dptr = nullptr)
 {
     bytes = (bytes + 255) & ~(size_t)255;
@@ -1027,7 +1042,25 @@ void *arena_get(trlhip_batch *b, bool host, size_t bytes, void **dptr = nullptr)
     if (v.empty() || v.back().size - v.back().used < bytes) {
         const size_t sz = std::max(bytes, host ? (size_t)64 << 20 : (size_t)1 << 30);
         trlhip_batch::Chunk ch{nullptr, nullptr, sz, 0};
-        if (host) {
+        bool reused = false;
+        {
+            std::lock_guard<std::mutex> lk(g_chunk_mu);
+            auto &fl = g_free_chunks[host ? 1 : 0];
+            for (size_t i = 0; i < fl.size(); ++i)
+                if (fl[i].size >= bytes) {
+                    ch = fl[i];
+                    fl.erase(fl.begin() + (long)i);
+                    reused = true;
+                    break;
+                }
+        }
+        if (reused) {
+            if (host)
+                memset(ch.p, 0, ch.used);
+            else if (ch.used && hipMemsetAsync(ch.p, 0, ch.used, b->side) != hipSuccess)
+                return nullptr;
+            ch.used = 0;
+        } else if (host) {
             if (hipHostMalloc((void **)&ch.p, sz, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
                 return nullptr;
             if (hipHostGetDevicePointer((void **)&ch.dp, ch.p, 0) != hipSuccess) {
@@ -1037,7 +1070,7 @@ void *arena_get(trlhip_batch *b, bool host, size_t bytes, void **dptr = nullptr)
             memset(ch.p, 0, sz);
         } else {
             if (hipMalloc((void **)&ch.p, sz) != hipSuccess) return nullptr;
-            if (hipMemsetAsync(ch.p, 0, sz, b->s) != hipSuccess) {
+            if (hipMemsetAsync(ch.p, 0, sz, b->side) != hipSuccess) {
                 (void)hipFree(ch.p);
                 return nullptr;
             }
@@ -2262,7 +2295,7 @@ int trlhip_ctx_create_batched(trlhip_ctx **out, int n, int ncols, int vtype, trl
     if (b) {
         // batched: everything from the batch's arenas, nothing to synchronize
         c->batch = b;
-        c->s = b->s;
+        c->s = b->side;
         c->arena = 1;
         const size_t np = part_need(c->E, ncols);
         const size_t ny = (size_t)(ncols + 1) * (ncols + 1);
@@ -2279,8 +2312,9 @@ int trlhip_ctx_create_batched(trlhip_ctx **out, int n, int ncols, int vtype, trl
         c->hbuf = (double *)arena_get(b, true, sizeof(double) * c->hcap);
         c->hres = (double *)arena_get(b, true, sizeof(double) * hres_count(ncols), (void **)&c->dres);
         c->hY = (double *)arena_get(b, true, sizeof(double) * ny);
+        c->hstage = (char *)arena_get(b, true, c->esz * (size_t)c->ld);
         if (!c->V || (vtype == TRLHIP_R32 && !c->V32) || !c->part || !c->dscal || !c->dcoef ||
-            !c->dh || !c->dY || !c->hbuf || !c->hres || !c->hY) {
+            !c->dh || !c->dY || !c->hbuf || !c->hres || !c->hY || !c->hstage) {
             free(c);
             return TRLHIP_ENOMEM;
         }
@@ -2319,7 +2353,14 @@ int trlhip_col_set(trlhip_ctx *c, int col, const void *x)
 {
     if (!c || !x || !valid_col(c, col)) return TRLHIP_EINVAL;
     if (int rc = settle(c)) return rc;
-    TRY(hipMemcpyAsync(c->V + (size_t)col * c->ld * c->esz, x, c->esz * (size_t)c->n,
+    // in a batch through the context's pinned staging (an asynchronous copy;
+    // a copy from pageable memory would block this host thread)
+    const void *src = x;
+    if (c->batch && c->hstage) {
+        memcpy(c->hstage, x, c->esz * (size_t)c->n);
+        src = c->hstage;
+    }
+    TRY(hipMemcpyAsync(c->V + (size_t)col * c->ld * c->esz, src, c->esz * (size_t)c->n,
                        hipMemcpyHostToDevice, c->s));
     if (c->V32) {  // refresh the FP32 shadow
         if (int rc = run_scale_copy(c, col, col, 1.0)) return rc;
@@ -2331,6 +2372,13 @@ int trlhip_col_get(trlhip_ctx *c, int col, void *x)
 {
     if (!c || !x || !valid_col(c, col)) return TRLHIP_EINVAL;
     if (int rc = settle(c)) return rc;
+    if (c->batch && c->hstage) {  // pinned staging, as in trlhip_col_set
+        TRY(hipMemcpyAsync(c->hstage, c->V + (size_t)col * c->ld * c->esz, c->esz * (size_t)c->n,
+                           hipMemcpyDeviceToHost, c->s));
+        if (int rc = ctx_wait(c)) return rc;
+        memcpy(x, c->hstage, c->esz * (size_t)c->n);
+        return 0;
+    }
     TRY(hipMemcpyAsync(x, c->V + (size_t)col * c->ld * c->esz, c->esz * (size_t)c->n,
                        hipMemcpyDeviceToHost, c->s));
     return ctx_wait(c);
@@ -2701,7 +2749,11 @@ int trlhip_batch_create_on(trlhip_batch **out, trlhip_batch *share)
         delete b;
         return TRLHIP_EHIP;
     }
-    if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+    if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->side_done, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreate(&b->side) != hipSuccess) {
+        if (b->done) (void)hipEventDestroy(b->done);
+        if (b->side_done) (void)hipEventDestroy(b->side_done);
         if (b->own_stream) (void)hipStreamDestroy(b->s);
         delete b;
         return TRLHIP_EHIP;
@@ -2713,6 +2765,7 @@ int trlhip_batch_create_on(trlhip_batch **out, trlhip_batch *share)
 void trlhip_batch_destroy(trlhip_batch *b)
 {
     if (!b) return;
+    (void)hipStreamSynchronize(b->side);
     (void)hipStreamSynchronize(b->s);
     if (b->hdesc) (void)hipHostFree(b->hdesc);
     if (b->ddesc) (void)hipFree(b->ddesc);
@@ -2722,11 +2775,25 @@ void trlhip_batch_destroy(trlhip_batch *b)
     if (b->dops) (void)hipFree(b->dops);
     if (b->hcbuf) (void)hipHostFree(b->hcbuf);
     if (b->dcbuf) (void)hipFree(b->dcbuf);
-    for (auto &ch : b->dev) (void)hipFree(ch.p);
-    for (auto &ch : b->host) (void)hipHostFree(ch.p);
+    {
+        std::lock_guard<std::mutex> lk(g_chunk_mu);
+        for (auto &ch : b->dev) g_free_chunks[0].push_back(ch);
+        for (auto &ch : b->host) g_free_chunks[1].push_back(ch);
+    }
     if (b->done) (void)hipEventDestroy(b->done);
+    if (b->side_done) (void)hipEventDestroy(b->side_done);
+    if (b->side) (void)hipStreamDestroy(b->side);
     if (b->own_stream) (void)hipStreamDestroy(b->s);
     delete b;
+}
+
+void trlhip_batch_trim(void)
+{
+    std::lock_guard<std::mutex> lk(g_chunk_mu);
+    for (auto &ch : g_free_chunks[0]) (void)hipFree(ch.p);
+    for (auto &ch : g_free_chunks[1]) (void)hipHostFree(ch.p);
+    g_free_chunks[0].clear();
+    g_free_chunks[1].clear();
 }
 
 void trlhip_batch_set_yield(trlhip_batch *b, void (*yield)(void *), void *user)
@@ -2740,6 +2807,10 @@ int trlhip_batch_launch(trlhip_batch *b)
 {
     if (!b) return TRLHIP_EINVAL;
     int rc = b->err;
+    // the contexts' immediate work of this round comes first
+    if (!rc && (hipEventRecord(b->side_done, b->side) != hipSuccess ||
+                hipStreamWaitEvent(b->s, b->side_done, 0) != hipSuccess))
+        rc = TRLHIP_EHIP;
     if (!rc && !b->ops.empty()) rc = ops_run(b->ops, b->ybuf, b->hops, b->dops, b->ocap, b->s);
     if (!rc && !b->steps.empty()) rc = batch_launch(b);
     // everything this batch queued so far (this launch and the contexts'
@@ -2766,7 +2837,10 @@ int trlhip_batch_wait(trlhip_batch *b)
     if (!b) return TRLHIP_EINVAL;
     int rc = b->err;
     if (!rc && hipEventSynchronize(b->done) != hipSuccess) rc = TRLHIP_EHIP;
-    if (rc) (void)hipStreamSynchronize(b->s);  // after an error, drain what was queued
+    if (rc) {  // after an error, drain what was queued
+        (void)hipStreamSynchronize(b->side);
+        (void)hipStreamSynchronize(b->s);
+    }
     stats_resolve();
     if (rc) b->err = rc;
     return rc;

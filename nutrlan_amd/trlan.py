@@ -187,13 +187,20 @@ def _result(info, ev, evec, wrk, wall):
     return out
 
 
+def output_buffers(n, mev):
+    """(eval, evec, wrk) arrays for one real solve, pages touched, for
+    reuse across calls (out= of solve_csr / solve_batch)."""
+    return np.zeros(mev), np.ones((mev, n)), np.zeros(mev)
+
+
 def solve_csr(A, lohi, ned, maxlan, tol, restart, maxmv, mev, start, rfact=0.0,
-              checkpoint=None, resume=None):
+              checkpoint=None, resume=None, out=None):
     """trlan()/ztrlan() on the device CSR operator A with start vector
     `start` (guess = 1).  checkpoint=(cpflag, prefix) writes checkpoint
     files (trl_set_checkpoint); resume=prefix restarts from the file written
-    under that prefix (guess = 2, oldcpf).  Returns the same dict keys as
-    oracle/ref.py."""
+    under that prefix (guess = 2, oldcpf).  out = output_buffers(n, mev)
+    reuses the caller's arrays (real operators; the returned dict refers to
+    them).  Returns the same dict keys as oracle/ref.py."""
     import time
 
     n = A.n
@@ -221,9 +228,12 @@ def solve_csr(A, lohi, ned, maxlan, tol, restart, maxmv, mev, start, rfact=0.0,
         wall = time.perf_counter() - t0
         wrk = misc.real[:mev].copy()
     else:
-        evec = np.zeros((mev, n))
+        if out is not None:
+            ev, evec, wrk = out
+        else:
+            evec = np.zeros((mev, n))
+            wrk = np.zeros(mev)
         evec[0] = start
-        wrk = np.zeros(mev)
         t0 = time.perf_counter()
         lib().trlan(A.op, None, ctypes.byref(info), n, mev, _ptr(ev), _ptr(evec), n, mev,
                     _ptr(wrk))
@@ -241,10 +251,12 @@ class TrlBatchJob(ctypes.Structure):
                 ("lde", ctypes.c_int), ("lwrk", ctypes.c_int), ("wrk", ctypes.c_void_p)]
 
 
-def solve_batch(problems):
+def solve_batch(problems, out=None):
     """trlan_batch over problems = [(A, lohi, ned, maxlan, tol, restart,
     maxmv, mev, start), ...] with real device operators A (Csr, mixed or
-    not).  Returns (rc, [result dict per problem], wall seconds)."""
+    not).  out = [output_buffers(n, mev) per problem] reuses the caller's
+    arrays (the returned dicts refer to them).  Returns (rc, [result dict
+    per problem], wall seconds)."""
     import time
 
     L = lib()
@@ -256,10 +268,11 @@ def solve_batch(problems):
         n = A.n
         info = init_info(n, maxlan, lohi, ned, tol, restart, maxmv)
         L.trl_set_iguess(ctypes.byref(info), 0, 1, 0, None)
-        ev = np.zeros(mev)
-        evec = np.zeros((mev, n))
+        if out is not None:
+            ev, evec, wrk = out[j]
+        else:
+            ev, evec, wrk = np.zeros(mev), np.zeros((mev, n)), np.zeros(mev)
         evec[0] = start
-        wrk = np.zeros(mev)
         keep.append((info, ev, evec, wrk))
         jobs[j].A = A.handle.value
         jobs[j].info = ctypes.pointer(info)
