@@ -34,6 +34,12 @@ constexpr int kProjGrid = 2048;        // row blocks (partial rows) of the proje
 constexpr int kProjBlocks = 4096;      // projection blocks wanted over all column chunks
 constexpr int kNnzBlk = 2048;          // nonzeros staged per SpMV row block
 constexpr int kRowsBlk = kBlock;       // rows per SpMV row block (at most)
+// Smallest share of work per block: rows per update or projection block
+// (4 rows per thread), row blocks per CSR-stream SpMV block.  Measured on
+// the batched 2D Anderson workload: 2x, 4x and 8x these (fewer, longer
+// blocks) run 3 %, 4 % and 10 % slower.
+constexpr int kMinRowsUpd = 4 * kBlock;
+constexpr int kMinSpmvRowBlocks = 1;
 constexpr uint64_t kCsrMagic = 0x7472686970637372ull;  // "trhipcsr"
 
 
@@ -440,6 +446,31 @@ __global__ __launch_bounds__(kBlock) void k_dot(long ne, const S *__restrict__ x
 // to part[p*gx + rb] (k_colsum sums each coefficient's partials in row
 // block order).  VS = basis storage (float for the FP32 shadow of a
 // mixed-precision basis), RS = storage of r.
+// Four consecutive elements p[0..3] (p 16-B aligned for float, 32-B for
+// double) with 16-byte loads/stores, as doubles.
+__device__ __forceinline__ void load4(const float *p, double v[4])
+{
+    const float4 t = *reinterpret_cast<const float4 *>(p);
+    v[0] = t.x;
+    v[1] = t.y;
+    v[2] = t.z;
+    v[3] = t.w;
+}
+__device__ __forceinline__ void load4(const double *p, double v[4])
+{
+    const double2 a = reinterpret_cast<const double2 *>(p)[0];
+    const double2 b = reinterpret_cast<const double2 *>(p)[1];
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+}
+__device__ __forceinline__ void store4(double *p, const double v[4])
+{
+    reinterpret_cast<double2 *>(p)[0] = double2{v[0], v[1]};
+    reinterpret_cast<double2 *>(p)[1] = double2{v[2], v[3]};
+}
+
 template <typename VS, typename RS, int E, int CB, int RU>
 __device__ __forceinline__ void proj_body(
     int n, const VS *__restrict__ V, long ld, int c0, int m, const RS *__restrict__ r,
@@ -471,9 +502,30 @@ __device__ __forceinline__ void proj_body(
     double acc[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) acc[i] = 0.0;
+    if constexpr (E == 1 && RU == 4) {
+        // four consecutive rows per thread: one 16-byte load per column
+        const int stride = gx * kBlock * 4;
+        int row = (rb * kBlock + threadIdx.x) * 4;
+        for (; row + 3 < n; row += stride) {
+            double rv[4], vv[CB][4];
+            load4(r + row, rv);
+#pragma unroll
+            for (int c = 0; c < CB; ++c) load4(col[c] + row, vv[c]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int c = 0; c < CB; ++c) acc[c] += vv[c][j] * rv[j];
+        }
+        for (; row < n; ++row) {
+            const double rv = (double)r[row];
+#pragma unroll
+            for (int c = 0; c < CB; ++c) acc[c] += (double)col[c][row] * rv;
+        }
+    }
     const int stride = gx * kBlock * RU;
     int row = rb * kBlock * RU + threadIdx.x;
-    if constexpr (E == 1) {
+    if constexpr (E == 1 && RU == 4) {
+    } else if constexpr (E == 1) {
         for (; row + (RU - 1) * kBlock < n; row += stride) {
             double rv[RU], vv[RU][CB];
 #pragma unroll
@@ -554,6 +606,44 @@ __global__ __launch_bounds__(kBlock) void k_proj(
 // partial sums (column c into sum c mod 4 within each group of 8 columns,
 // E == 1; pairs of columns, E == 2), the same order for any R; R rows at a
 // time put R*8 (R*4) column loads in flight per thread.
+// r -= V h on the four consecutive rows [row, row+4) of a real basis (row
+// a multiple of 4): one 16-byte load per column (two for FP64).  Each row
+// sums its columns in the order of update_rows.
+template <typename VS, typename RS>
+__device__ __forceinline__ double update_rows4(int row, const VS *__restrict__ Vc, long ld, int m,
+                                               const double *hs, const RS *rin, RS *rout)
+{
+    double s[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s[j][0] = s[j][1] = s[j][2] = s[j][3] = 0.0;
+    int c = 0;
+    for (; c + 8 <= m; c += 8) {
+        double v[8][4];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) load4(Vc + (size_t)(c + q) * ld + row, v[q]);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) s[j][q & 3] += v[q][j] * hs[c + q];
+    }
+    for (; c < m; ++c) {
+        double v[4];
+        load4(Vc + (size_t)c * ld + row, v);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s[j][0] += v[j] * hs[c];
+    }
+    double ri[4], ro[4];
+    load4(rin + row, ri);
+    double nrm = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        ro[j] = (double)(RS)(ri[j] - ((s[j][0] + s[j][1]) + (s[j][2] + s[j][3])));
+        nrm += ro[j] * ro[j];
+    }
+    store4(rout + row, ro);
+    return nrm;
+}
+
 template <typename VS, typename RS, int E, int R>
 __device__ __forceinline__ double update_rows(int row, const VS *__restrict__ Vc, long ld, int m,
                                               const double *hs, const RS *rin, RS *rout)
@@ -658,14 +748,22 @@ __device__ __forceinline__ void update_body(
     __syncthreads();
     const VS *Vc = V + (size_t)c0 * ld * E;
     double acc = 0.0;
-    const int stride = nblk * kBlock * RU;
-    int row = bid * kBlock * RU + threadIdx.x;
-    for (; row + (RU - 1) * kBlock < n; row += stride)
-        acc += update_rows<VS, RS, E, RU>(row, Vc, ld, m, hs, rin, rout);
+    if constexpr (E == 1 && RU == 4) {
+        // four consecutive rows per thread (16-byte loads)
+        const int stride = nblk * kBlock * 4;
+        int row = (bid * kBlock + threadIdx.x) * 4;
+        for (; row + 3 < n; row += stride) acc += update_rows4<VS, RS>(row, Vc, ld, m, hs, rin, rout);
+        for (; row < n; ++row) acc += update_rows<VS, RS, E, 1>(row, Vc, ld, m, hs, rin, rout);
+    } else {
+        const int stride = nblk * kBlock * RU;
+        int row = bid * kBlock * RU + threadIdx.x;
+        for (; row + (RU - 1) * kBlock < n; row += stride)
+            acc += update_rows<VS, RS, E, RU>(row, Vc, ld, m, hs, rin, rout);
 #pragma unroll
-    for (int j = 0; j < RU - 1; ++j)
-        if (row + j * kBlock < n)
-            acc += update_rows<VS, RS, E, 1>(row + j * kBlock, Vc, ld, m, hs, rin, rout);
+        for (int j = 0; j < RU - 1; ++j)
+            if (row + j * kBlock < n)
+                acc += update_rows<VS, RS, E, 1>(row + j * kBlock, Vc, ld, m, hs, rin, rout);
+    }
     const double bs = block_sum(acc);
     if (threadIdx.x == 0) part[bid] = bs;
     if (out_hh && bid == 0 && threadIdx.x == 0) {
@@ -1120,7 +1218,8 @@ double spmv_bytes(const trlhip_csr *A, int E, size_t vsz, size_t xsz)
 int spmv_grid(const trlhip_csr *A)
 {
     if (A->lpr) return A->vgrid;
-    return A->nb < kSpmvGrid ? A->nb : kSpmvGrid;
+    const int g = (A->nb + kMinSpmvRowBlocks - 1) / kMinSpmvRowBlocks;
+    return g < kSpmvGrid ? g : kSpmvGrid;
 }
 
 template <int E, bool FUSE>
@@ -1258,7 +1357,7 @@ int run_update(trlhip_ctx *c, int c0, int m, const double *dh, int aslot, int ri
                double *o_rr2, double *o_hh, int kind, double h0 = 0.0, double *pdst = nullptr,
                int *g_out = nullptr, bool shadow = false)
 {
-    const int g = grid_for(c->n, kBlock * 4, kMaxGrid);
+    const int g = grid_for(c->n, kMinRowsUpd, kMaxGrid);
     if (!pdst) pdst = part_update(c);
     if (g_out) *g_out = g;
     const bool s32 = shadow && c->V32;
@@ -1292,7 +1391,7 @@ int proj_gx(const trlhip_ctx *c, int m, int cb)
 {
     const int chunks = (m + cb - 1) / cb;
     const int want = (kProjBlocks + chunks - 1) / chunks;
-    const int g = grid_for(c->n, kBlock * 4, want < kProjGrid ? want : kProjGrid);
+    const int g = grid_for(c->n, kMinRowsUpd, want < kProjGrid ? want : kProjGrid);
     return g >= 8 ? g & ~7 : g;
 }
 
@@ -1447,14 +1546,23 @@ struct BDesc {
 
 enum { BP_SPMV, BP_RECUR, BP_PROJ, BP_COLSUM, BP_UPDATE, BP_SUM2, BP_COUNT };
 
-// Request p owning global block blk: the last p with pre[p] <= blk.
+// Request p owning global block blk: the last p with pre[p] <= blk
+// (pre[0] = 0, strictly increasing).  Each wave searches cooperatively: its
+// 64 lanes test 64 evenly spaced candidates per step and a ballot narrows
+// the range 64-fold, so 512 requests take 2 dependent loads instead of 9.
 __device__ __forceinline__ int batch_find(const int *__restrict__ pre, int nreq, int blk)
 {
-    int lo = 0, hi = nreq - 1;
+    const int lane = threadIdx.x & 63;
+    int lo = 0, hi = nreq - 1;  // pre[lo] <= blk, answer <= hi
     while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (pre[mid] <= blk) lo = mid;
-        else hi = mid - 1;
+        const int step = (hi - lo + 64) / 64;  // ceil((hi - lo + 1) / 64)
+        const int k = lo + lane * step;
+        const bool le = k <= hi && pre[k] <= blk;
+        const unsigned long long m = __ballot(le);  // lane 0 is always set
+        const int last = 63 - __clzll(m);
+        lo += last * step;
+        hi = min(hi, lo + step - 1);
+        if (step == 1) break;
     }
     return lo;
 }
@@ -1549,7 +1657,7 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
         d.jnd = jnd;
         d.np_alpha = spmv_grid(A);
         c->np_alpha = d.np_alpha;
-        d.g_upd = grid_for(n, kBlock * 4, kMaxGrid);
+        d.g_upd = grid_for(n, kMinRowsUpd, kMaxGrid);
         d.gx = proj_gx(c, jnd, 8);
         d.gy = (jnd + 7) / 8;
         if (st.first) {
