@@ -1592,6 +1592,10 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     long rounds = 0;
     const double t_loop = lz_now();
     int pending[4] = {0, 0, 0, 0};
+    /* profile: wall time of lane rounds by the number of solves they step */
+    double t_band[4] = {0, 0, 0, 0}, w_band[4] = {0, 0, 0, 0};
+    long n_band[4] = {0, 0, 0, 0};
+    int last_band[4] = {0, 0, 0, 0};
     for (;;) {
         int any = 0;
         for (int l = 0; l < nlanes; ++l) {
@@ -1602,9 +1606,11 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
                 if (wrc && !rc) rc = wrc;
                 pending[l] = 0;
                 t_wait += lz_now() - tw;
+                w_band[last_band[l]] += lz_now() - tw;
             }
             if (live[l] == 0) continue;
             any = 1;
+            const int active = live[l];
             const double t0 = lz_now();
             pool.lane = l;
             pool.next = 0;
@@ -1621,6 +1627,7 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
             live[l] = m;
             t_co += lz_now() - t0;
             ++rounds;
+            const int band = active >= 256 ? 0 : active >= 32 ? 1 : active >= 4 ? 2 : 3;
             if (live[l] > 0) {
                 if (serial && nlanes > 1) {
                     const int orc = trlhip_batch_after(bl[l], bl[(l + nlanes - 1) % nlanes]);
@@ -1630,6 +1637,9 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
                 if (lrc && !rc) rc = lrc;
                 pending[l] = 1;
             }
+            t_band[band] += lz_now() - t0;
+            ++n_band[band];
+            last_band[l] = band;
         }
         if (!any) break;
     }
@@ -1650,6 +1660,10 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
             tlong += pool.w[t].tlong;
             nlong += pool.w[t].nlong;
         }
+        fprintf(stderr, "trlan_batch: lane rounds by solves stepped (count, host+launch s, wait s): "
+                ">=256 %ld %.3f %.3f; 32-255 %ld %.3f %.3f; 4-31 %ld %.3f %.3f; 1-3 %ld %.3f %.3f\n",
+                n_band[0], t_band[0], w_band[0], n_band[1], t_band[1], w_band[1], n_band[2], t_band[2],
+                w_band[2], n_band[3], t_band[3], w_band[3]);
         fprintf(stderr, "trlan_batch: idle at round ends mean %.3f s; longest resume %.1f us; "
                 "%ld resumes over 100 us take %.3f s\n", isum / pool.nthr, 1e6 * rmax, nlong, tlong);
         fprintf(stderr,
