@@ -31,6 +31,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -82,12 +84,49 @@ def measured_traffic(kernel, summary=LAPLACE_TRAFFIC):
     return None
 
 
-def build_problem():
+# Single-solve workloads (one step = one complete trlan()/ztrlan() solve):
+# builder kwargs, start-vector seed, lohi, nev, maxlan, tol, restart, maxmv,
+# complex, and the committed PMC summary for the roofline's traffic field.
+SINGLE = {
+    "laplace3d": dict(name="laplace3d_128_periodic_fp64", kind="laplace3d", x0_seed=1, lohi=1,
+                      nev=16, maxlan=64, tol=1e-8, restart=1, maxmv=20000, cplx=False,
+                      traffic=None),
+    "diag1000": dict(name="diag_1000", kind="diag", x0_seed=1, lohi=1, nev=5, maxlan=40,
+                     tol=1.4e-8, restart=1, maxmv=5000, cplx=False, traffic=None),
+    "anderson3d-c": dict(name="anderson3d_96_peierls_c128", kind="anderson3d", x0_seed=1,
+                         lohi=-1, nev=24, maxlan=80, tol=1e-8, restart=1, maxmv=40000,
+                         cplx=True, traffic=None),
+    "rand8m": dict(name="random_sym_8388608_deg16", kind="rand", x0_seed=5, lohi=1, nev=32,
+                   maxlan=128, tol=1e-8, restart=1, maxmv=40000, cplx=False, traffic=None),
+}
+SINGLE["laplace3d"]["traffic"] = "hbm_traffic_laplace3d_128_fp64.txt"
+
+
+def single_problem(workload):
+    """(row_ptr, col_idx, values, start vector) of a single-solve workload:
+    laplace3d = periodic 7-point Laplacian on 128^3, diagonal 6 + U[0,1)
+    (seed 0); diag1000 = diag(1..1000); anderson3d-c = complex Hermitian 96^3
+    Anderson model, Peierls phase exp(2 pi i x / 16) on y-bonds, on-site
+    U[-2, 2] (seed 0); rand8m = symmetric random sparse n = 2^23, row degree
+    ~Poisson(16), values N(0,1)/4, diagonal U[-1,1] (seed 4)."""
     from nutrlan_amd import workloads as W
 
-    rp, ci, v = W.laplace3d_csr(L, seed=0)
-    x0 = W.start_vector(L ** 3, seed=1)
+    sp = SINGLE[workload]
+    if sp["kind"] == "laplace3d":
+        rp, ci, v = W.laplace3d_csr(L, seed=0)
+    elif sp["kind"] == "diag":
+        rp, ci, v = W.diag_csr(1000)
+    elif sp["kind"] == "anderson3d":
+        rp, ci, v = W.anderson3d_csr(96, seed=0)
+    else:
+        rp, ci, v = W.random_sym_csr(1 << 23, seed=4)
+    x0 = W.start_vector(len(rp) - 1, seed=sp["x0_seed"], complex_=sp["cplx"])
     return rp, ci, v, x0
+
+
+def build_problem():
+    """The 128^3 Laplacian workload (kept for callers of the older name)."""
+    return single_problem("laplace3d")
 
 
 def residuals(rp, ci, v, evec, ev):
@@ -96,11 +135,11 @@ def residuals(rp, ci, v, evec, ev):
     n = len(rp) - 1
     A = sp.csr_matrix((v, ci, rp), shape=(n, n))
     R = A @ evec - evec * ev[None, :]
-    return (R * R).sum(axis=0) ** 0.5
+    return np.sqrt((np.abs(R) ** 2).sum(axis=0))
 
 
-def cpu_baseline(sample_matvec, full_matvec, nthreads):
-    """Reference trlan on the same problem capped at `sample_matvec`
+def cpu_baseline(workload, sample_matvec, full_matvec, nthreads):
+    """Reference trlan/ztrlan on the same problem capped at `sample_matvec`
     operator applications, in a child process (OpenBLAS thread count is
     fixed when the library loads).  Returns (eigenpairs/s extrapolated to a
     full solve, seconds, sample description) or None."""
@@ -108,35 +147,40 @@ def cpu_baseline(sample_matvec, full_matvec, nthreads):
 
     if not ref.available():
         return None
+    sp = SINGLE[workload]
     code = (
         "import sys, json; sys.path.insert(0, %r)\n"
         "import bench\n"
         "from oracle import ref\n"
-        "rp, ci, v, x0 = bench.build_problem()\n"
-        "r = ref.solve_csr(rp, ci, v, 1, bench.NEV, bench.MAXLAN, bench.TOL, bench.RESTART, %d,"
-        " bench.NEV, x0, nthreads=%d)\n"
+        "sp = bench.SINGLE[%r]\n"
+        "rp, ci, v, x0 = bench.single_problem(%r)\n"
+        "r = ref.solve_csr(rp, ci, v, sp['lohi'], sp['nev'], sp['maxlan'], sp['tol'], sp['restart'],"
+        " %d, sp['nev'], x0, nthreads=%d, complex_=sp['cplx'])\n"
         "print(json.dumps(dict(wall=r['wall_s'], matvec=r['matvec'], north=r['north'],"
-        " stat=r['stat'])))\n" % (ROOT, sample_matvec, nthreads))
+        " stat=r['stat'])))\n" % (ROOT, workload, workload, sample_matvec, nthreads))
     env = dict(os.environ, OPENBLAS_NUM_THREADS=str(nthreads), OMP_NUM_THREADS=str(nthreads))
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
-                         timeout=600)
+                         timeout=900)
     if out.returncode != 0:
         print(out.stderr[-2000:], file=sys.stderr)
         return None
     r = json.loads(out.stdout.strip().splitlines()[-1])
     per_mv = r["wall"] / max(r["matvec"], 1)
     full_s = per_mv * full_matvec
-    return NEV / full_s, r["wall"], (
-        f"reference trlan, first {r['matvec']} operator applications of the same solve "
-        f"({r['wall']:.1f} s, {r['north']} Gram-Schmidt passes); eigenpairs/s extrapolated to "
-        f"the full solve's {full_matvec} applications")
+    return sp["nev"] / full_s, r["wall"], (
+        f"reference {'ztrlan' if sp['cplx'] else 'trlan'}, first {r['matvec']} operator "
+        f"applications of the same solve ({r['wall']:.1f} s, {r['north']} Gram-Schmidt passes); "
+        f"eigenpairs/s extrapolated to the full solve's {full_matvec} applications")
 
 
-def leg(T, A, x0, steps, rp, ci, v, stats_every=0):
+def leg(T, A, x0, steps, rp, ci, v, stats_every=0, sp=None):
     """Time `steps` complete solves on operator A (inputs resident); returns
     (seconds, last result, eigenpairs at resid <= 1e-8 ||A||, stats)."""
+    sp = sp or SINGLE["laplace3d"]
+
     def solve():
-        return T.solve_csr(A, 1, NEV, MAXLAN, TOL, RESTART, MAXMV, NEV, x0)
+        return T.solve_csr(A, sp["lohi"], sp["nev"], sp["maxlan"], sp["tol"], sp["restart"],
+                           sp["maxmv"], sp["nev"], x0)
 
     if stats_every:
         T.stats_enable(True, every=stats_every)
@@ -378,11 +422,13 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=None,
                     help="CPU baseline sample: operator applications (laplace3d, default 96) "
                          "or lattices (anderson2d-batch, default 96)")
-    ap.add_argument("--workload", choices=["laplace3d", "anderson2d-batch"],
+    ap.add_argument("--workload", choices=["anderson2d-batch"] + list(SINGLE),
                     default="anderson2d-batch",
                     help="anderson2d-batch (default): 1024 uneven 2D Anderson lattices through "
-                         "the batched mixed-precision learned-start solver; laplace3d: the 128^3 "
-                         "Laplacian, one solve per step, mixed vs plain FP64")
+                         "the batched mixed-precision learned-start solver; laplace3d (128^3 "
+                         "Laplacian), diag1000, anderson3d-c (complex 96^3), rand8m (n = 2^23 "
+                         "random sparse): one solve per step, mixed vs plain FP64 (complex: "
+                         "plain c128)")
     ap.add_argument("--models", type=int, default=1024)
     ap.add_argument("--plain-sample", type=int, default=64)
     ap.add_argument("--stats-every", type=int, default=STATS_EVERY,
@@ -407,57 +453,69 @@ def main():
         return
     if args.cpu_sample is None:
         args.cpu_sample = 96
-    rp, ci, v, x0 = build_problem()
+    single_solve(T, args)
+
+
+def single_solve(T, args):
+    """One of the single-solve workloads (SINGLE): the --path solver timed
+    with sampled HIP events, the other real path (plain or mixed) timed
+    beside it; complex operators run the plain c128 path only."""
+    sp = SINGLE[args.workload]
+    print(f"building {sp['name']}", file=sys.stderr, flush=True)
+    rp, ci, v, x0 = single_problem(args.workload)
     n = len(rp) - 1
-    calibration_copy(T, n)
-    ops = {"plain": T.Csr(rp, ci, v), "mixed": T.Csr(rp, ci, v, mixed=True)}
-    other = "plain" if args.path == "mixed" else "mixed"
+    calibration_copy(T, min(n, 1 << 21) if n >= 1 << 16 else 1 << 16)
+    cplx = sp["cplx"]
+    path = "plain" if cplx else args.path
+    ops = {"plain": T.Csr(rp, ci, v, complex_=cplx)}
+    if not cplx:
+        ops["mixed"] = T.Csr(rp, ci, v, mixed=True)
+    other = None if cplx else ("plain" if path == "mixed" else "mixed")
     for _ in range(args.warmup):
         for k in ops:
-            T.solve_csr(ops[k], 1, NEV, MAXLAN, TOL, RESTART, MAXMV, NEV, x0)
-    # the path beside the headline one, timed on its own (no events)
-    o_el, o_g, o_good, _ = leg(T, ops[other], x0, args.steps, rp, ci, v)
-    # headline path: one launch in STATS_EVERY of each kernel kind (a hashed,
+            leg(T, ops[k], x0, 1, rp, ci, v, 0, sp)
+    o = None
+    if other:  # the path beside the headline one, timed on its own (no events)
+        o = leg(T, ops[other], x0, args.steps, rp, ci, v, 0, sp)
+    # headline path: one launch in stats_every of each kernel kind (a hashed,
     # non-periodic pick) carries HIP start/stop events stamped from its own
     # dispatch on the solver's stream; timing every launch would cost ~10%.
-    elapsed, g, good, stats = leg(T, ops[args.path], x0, args.steps, rp, ci, v, args.stats_every)
+    elapsed, g, good, stats = leg(T, ops[path], x0, args.steps, rp, ci, v, args.stats_every, sp)
     value = good * args.steps / elapsed
-    other_value = o_good * args.steps / o_el
     nec = g["nec"]
-    dl = float(np.abs(np.sort(g["eval"][:nec]) - np.sort(o_g["eval"][:o_g["nec"]])).max()
-               / g["anrm"]) if o_g["nec"] == nec else None
 
     # roofline: dominant kernel by time, plus the path aggregate
     dom = max(stats, key=lambda k: stats[k][1])
     launches, ms, byts = stats[dom]
     achieved = byts / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    path = ["spmv", "recur", "proj", "update"]
-    pb = sum(stats[k][2] for k in path)
-    pt = sum(stats[k][1] for k in path)
+    pk = ["spmv", "recur", "proj", "update"]
+    pb = sum(stats[k][2] for k in pk)
+    pt = sum(stats[k][1] for k in pk)
     path_gbs = pb / (pt * 1e-3) / 1e9 if pt > 0 else 0.0
+    kname = None if cplx else KERNEL_NAMES[path].get(dom)
     roofline = {
-        "bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(KERNEL_NAMES[args.path].get(dom)),
+        "bound": "hbm", "kernel": kname or dom, "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": measured_traffic(kname, sp["traffic"]) if sp["traffic"] else None,
         "avg_launch_us": round(ms * 1e3 / max(launches, 1), 2),
         "bytes_per_launch": round(byts / max(launches, 1)),
         "path_spmv_recur_cgs": {"achieved": round(path_gbs, 1),
                                 "frac": round(path_gbs / HBM_PEAK_GBS, 4)},
-        "kernels": {k: {"launches": s[0], "ms": round(s[1], 3),
-                        "GBps": round(s[2] / (s[1] * 1e-3) / 1e9, 1) if s[1] > 0 else 0.0}
-                    for k, s in stats.items()},
+        "kernels": {k: {"launches": s_[0], "ms": round(s_[1], 3), "bytes": round(s_[2]),
+                        "GBps": round(s_[2] / (s_[1] * 1e-3) / 1e9, 1) if s_[1] > 0 else 0.0}
+                    for k, s_ in stats.items()},
     }
     cpu = None
     if not args.no_cpu_baseline:
         nthreads = min(os.cpu_count() or 1, 16)
-        cb = cpu_baseline(args.cpu_sample, o_g["matvec"] if args.path == "mixed" else g["matvec"],
-                          nthreads)
+        full_mv = o[1]["matvec"] if (o and path == "mixed") else g["matvec"]
+        cb = cpu_baseline(args.workload, min(args.cpu_sample, full_mv), full_mv, nthreads)
         if cb is not None:
             cpu = {"value": round(cb[0], 4), "unit": "eigenpairs/s", "cores": nthreads,
                    "kind": "reference", "sample": cb[2]}
     desc = {"mixed": "mixed: FP32 shadow basis for Gram-Schmidt, FP64 SpMV/recurrence/basis/"
                      "restart GEMM, FP64 Rayleigh-quotient refinement",
-            "plain": "plain FP64 device basis"}
+            "plain": "plain FP64 device basis" + (" (complex c128)" if cplx else "")}
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -469,24 +527,30 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "c128" if cplx else "f64",
         "data": "synthetic",
-        "config": {"workload": "laplace3d_128_periodic_fp64", "n": n, "nnz": int(len(ci)),
-                   "nev": NEV, "lohi": 1, "maxlan": MAXLAN, "tol": TOL, "restart": RESTART,
-                   "path": desc[args.path], "learned_start": "none (random start; the learned "
-                   "start applies to the batched 2D Anderson models)",
+        "config": {"workload": sp["name"], "n": n, "nnz": int(len(ci)), "nev": sp["nev"],
+                   "lohi": sp["lohi"], "maxlan": sp["maxlan"], "tol": sp["tol"],
+                   "restart": sp["restart"], "path": desc[path],
+                   "learned_start": "none (random start; the learned start applies to the "
+                                    "batched 2D Anderson models)",
                    "matvec": g["matvec"], "nloop": g["nloop"], "north": g["north"],
                    "converged": nec, "converged_at_1e-8": good},
-        other + "_path": {"value": round(other_value, 3),
-                          "ms_per_step": round(1e3 * o_el / args.steps, 2),
-                          "matvec": o_g["matvec"], "converged_at_1e-8": o_good,
-                          "path": desc[other]},
-        "mixed_vs_plain": round((value / other_value) if args.path == "mixed"
-                                else (other_value / value), 3),
-        "max_eigenvalue_diff_vs_" + other: dl,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if o:
+        o_el, o_g, o_good, _ = o
+        other_value = o_good * args.steps / o_el
+        line[other + "_path"] = {"value": round(other_value, 3),
+                                 "ms_per_step": round(1e3 * o_el / args.steps, 2),
+                                 "matvec": o_g["matvec"], "converged_at_1e-8": o_good,
+                                 "path": desc[other]}
+        line["mixed_vs_plain"] = round((value / other_value) if path == "mixed"
+                                       else (other_value / value), 3)
+        line["max_eigenvalue_diff_vs_" + other] = (
+            float(np.abs(np.sort(g["eval"][:nec]) - np.sort(o_g["eval"][:o_g["nec"]])).max()
+                  / g["anrm"]) if o_g["nec"] == nec else None)
     print(json.dumps(line), flush=True)
     for A in ops.values():
         A.close()
