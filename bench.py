@@ -299,18 +299,26 @@ def batched_anderson2d(args):
     # serving caller would: trlan() writes into the caller's eval/evec/wrk
     bufs = [T.output_buffers(len(c[0]) - 1, nev) for c, d in models]
 
-    def batch(ops, x0s):
-        probs = [(A, -1, nev, maxlan, tol, 1, 100000, nev, x) for A, x in zip(ops, x0s)]
-        rc, res, _ = T.solve_batch(probs, out=bufs)
+    def batch(ops, x0s, dstarts=None):
+        probs = [(A, -1, nev, maxlan, tol, 1, 100000, nev, None if x0s is None else x0s[i])
+                 for i, A in enumerate(ops)]
+        rc, res, _ = T.solve_batch(probs, out=bufs, dstarts=dstarts)
         assert rc == 0, rc
         return res
 
+    t_inf = [0.0]
+
     def step():
-        x_net = LN.start_vectors(net, [d for c, d in models])
-        return batch(mixed, x_net)
+        # the net's outputs stay on the device and go into the batch as the
+        # jobs' start vectors (trl_batch_job.dstart)
+        t0 = time.perf_counter()
+        x_dev = LN.start_vectors_device(net, [d for c, d in models])
+        t_inf[0] += time.perf_counter() - t0
+        return batch(mixed, None, x_dev)
 
     for _ in range(args.warmup):
         step()
+    t_inf[0] = 0.0
     # headline: learned start + mixed precision, batched; one launch in
     # STATS_EVERY of each kernel kind carries HIP start/stop events
     T.stats_enable(True, every=args.stats_every)
@@ -391,7 +399,8 @@ def batched_anderson2d(args):
                    "path": "batched (trlan_batch), mixed precision (FP32 shadow basis for "
                            "Gram-Schmidt, FP64 recurrence/restart/refinement), learned start "
                            "(BF16-MFMA conv net, inference inside each step)",
-                   "converged_at_1e-8": good, "of": nev * count, "matvec_total": mv_net},
+                   "converged_at_1e-8": good, "of": nev * count, "matvec_total": mv_net,
+                   "learned_start_inference_ms_per_step": round(1e3 * t_inf[0] / args.steps, 1)},
         "vs_plain_fp64": round(value / plain_single, 2) if plain_single == plain_single else None,
         "plain_fp64_one_at_a_time": {"value": round(plain_single, 2),
                                      "sample": f"first {ns} lattices, {el_s:.2f} s"},

@@ -25,6 +25,10 @@ typedef struct {
     double *eval, *evec;
     int lde, lwrk;
     double *wrk;
+    const float *dstart; /* NULL, or an FP32 start vector (nrow entries) in
+                            device memory, used instead of evec(:, nec) when
+                            info->guess == 1; it must stay valid until
+                            trlan_batch returns */
 } trl_batch_job;
 
 /* Runs every job to completion.  Each job makes the decisions a lone

@@ -94,6 +94,13 @@ int trlhip_ctx_ld(const trlhip_ctx *c);
  * vectors between evec/base (SRC/trlcore.c:1395-1403). */
 int trlhip_col_set(trlhip_ctx *c, int col, const void *x);
 int trlhip_col_get(trlhip_ctx *c, int col, void *x);
+
+/* Column col = the FP32 device vector dsrc (n entries) converted to FP64,
+ * without a host round trip (real bases).  dsrc must stay valid until the
+ * context's next synchronisation (in a batch: until the solve resumes).
+ * No counterpart in the reference, whose start vector is the host
+ * evec(:, nec); used for start vectors computed on the device. */
+int trlhip_col_set_device_f32(trlhip_ctx *c, int col, const float *dsrc);
 int trlhip_col_copy(trlhip_ctx *c, int dst, int src);
 int trlhip_col_scale(trlhip_ctx *c, int col, double s);
 

@@ -37,6 +37,20 @@ int trlnet_get_params(trlnet *net, float *out, int n);
 int trlnet_set_params(trlnet *net, const float *in, int n);
 /* out = net(in) for nimg images of h x w. */
 int trlnet_forward(trlnet *net, int nimg, int h, int w, const float *in, float *out);
+
+/* trlnet_forward over groups of images of different sizes in one call:
+ * group g holds nimg[g] images of h[g] x w[g]; in/out hold the groups back
+ * to back (sum of nimg*h*w floats).  One upload, one download and one
+ * device synchronisation for all groups. */
+int trlnet_forward_many(trlnet *net, int ngroups, const int *nimg, const int *h, const int *w,
+                        const float *in, float *out);
+
+/* trlnet_forward_many with the outputs left on the device: *out_device is
+ * the device address of the groups' outputs (same layout, FP32), valid
+ * until the next call on this net or its destruction.  For start vectors
+ * that go straight into a batch (trl_batch_job.dstart). */
+int trlnet_forward_many_device(trlnet *net, int ngroups, const int *nimg, const int *h,
+                               const int *w, const float *in, const float **out_device);
 /* Loss and its gradient w.r.t. the parameters (no update). */
 int trlnet_loss_grad(trlnet *net, int nimg, int h, int w, const float *in,
                      const float *target, float *loss, float *grad);

@@ -91,6 +91,7 @@ typedef struct {
     double *dsc, *esc;                      /* maxlan scratch */
     int *iwrk;                              /* 4 maxlan, index map leftovers */
     double *hx, *hy;                        /* host staging for host ops */
+    const float *dstart;                    /* device FP32 start vector (batch jobs), or NULL */
     uint64_t rng;
     int rot_n;                              /* order of rot */
 } lz_t;
@@ -597,7 +598,9 @@ static int lz_initial_guess(lz_t *z, const double *evec, int lde, int *j1)
         *j1 = jnd + j;
         return 0;
     }
-    if (info->guess == 1) {
+    if (info->guess == 1 && z->dstart && !z->cplx) {
+        x = NULL;  /* the start vector is already on the device */
+    } else if (info->guess == 1) {
         memcpy(x, evec + (size_t)j * lde * E, sizeof(double) * E * (size_t)z->n);
     } else if (z->cplx) {
         /* ztrl_initial_guess (SRC/ztrlcore.c:1421-1490): (1,0), or (1,1)
@@ -636,7 +639,10 @@ static int lz_initial_guess(lz_t *z, const double *evec, int lde, int *j1)
             info->nrand++;
         }
     }
-    DEV(trlhip_col_set(z->dev, j, x));
+    if (x)
+        DEV(trlhip_col_set(z->dev, j, x));
+    else
+        DEV(trlhip_col_set_device_f32(z->dev, j, z->dstart));
     /* ||x||^2 on the device; a zero/inf/nan start is replaced by random */
     double nr2 = 0.0;
     DEV(trlhip_update(z->dev, 0, 0, &nr2, j, &nr2));
@@ -770,15 +776,22 @@ static int lz_sygv(int m, double *H, double *G, double *theta, double *Z)
                 tot += v * v;
                 if (i != j) off += v * v;
             }
-        if (off <= 1e-30 * tot) {
+        /* converged when the off-diagonal part is at rounding level: below
+         * 1e-24 of the total, or when no pair needs a rotation */
+        if (off <= 1e-24 * tot) {
             conv = 1;
             break;
         }
+        int rotations = 0;
         for (int p = 0; p < m - 1; ++p)
             for (int q = p + 1; q < m; ++q) {
                 const double apq = 0.5 * (H[p + (size_t)q * m] + H[q + (size_t)p * m]);
-                if (apq == 0.0) continue;
                 const double app = H[p + (size_t)p * m], aqq = H[q + (size_t)q * m];
+                if (fabs(apq) <= DBL_EPSILON * sqrt(fabs(app * aqq)) || apq == 0.0) {
+                    H[p + (size_t)q * m] = H[q + (size_t)p * m] = 0.0;
+                    continue;
+                }
+                ++rotations;
                 const double tau = (aqq - app) / (2.0 * apq);
                 const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                 const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
@@ -798,6 +811,7 @@ static int lz_sygv(int m, double *H, double *G, double *theta, double *Z)
                     Z[k + (size_t)q * m] = sn * zp + cs * zq;
                 }
             }
+        if (rotations == 0) conv = 1;
     }
     if (!conv) return -1;
     for (int i = 0; i < m; ++i) theta[i] = H[i + (size_t)i * m];
@@ -832,11 +846,17 @@ static int lz_polish(lz_t *z, int k, int avoid, const double *rho)
         tmp = m + 1;
     }
     if ((save >= 0 ? save : tmp) >= z->ncols) return 0;
-    /* G, H, Z (m x m each), theta (m), Y (m x k) */
-    double *G = (double *)calloc((size_t)m * m * 3 + (size_t)m + (size_t)m * k, sizeof(double));
-    if (!G) return 0;
-    double *H = G + (size_t)m * m, *Z = H + (size_t)m * m, *theta = Z + (size_t)m * m;
-    double *Y = theta + m;
+    /* G, H, Z, Gs, Hs (m x m each), theta, rnorm (m each), Y (m x k) */
+    double *G = (double *)calloc((size_t)m * m * 5 + 2 * (size_t)m + (size_t)m * k, sizeof(double));
+    int *sel = (int *)calloc((size_t)m, sizeof(int));
+    if (!G || !sel) {
+        free(G);
+        free(sel);
+        return 0;
+    }
+    double *H = G + (size_t)m * m, *Z = H + (size_t)m * m, *Gs = Z + (size_t)m * m;
+    double *Hs = Gs + (size_t)m * m, *theta = Hs + (size_t)m * m, *rnorm = theta + m;
+    double *Y = rnorm + m;
     int rc = 0, used = 0;
     if (save >= 0 && trlhip_col_copy(z->dev, save, avoid)) rc = TRLHIP_STAT_DEVICE;
     for (int i = 0; i < k && !rc; ++i) {
@@ -849,7 +869,7 @@ static int lz_polish(lz_t *z, int k, int avoid, const double *rho)
         else if (trlhip_col_scale(z->dev, k + i, 1.0 / sqrt(rr2)))
             rc = TRLHIP_STAT_DEVICE;
         else
-            ++used;
+            rnorm[used++] = sqrt(rr2);
     }
     for (int j = 0; j < m && !rc && used == k; ++j) {
         if (trlhip_proj(z->dev, 0, m, j, G + (size_t)j * m) || trlhip_spmv(z->dev, z->A, j, tmp, NULL) ||
@@ -866,16 +886,65 @@ static int lz_polish(lz_t *z, int k, int avoid, const double *rho)
                 G[i + (size_t)j * m] = G[j + (size_t)i * m] = g;
                 H[i + (size_t)j * m] = H[j + (size_t)i * m] = h;
             }
-        if (lz_sygv(m, H, G, theta, Z) == 0) {
-            int *taken = (int *)calloc((size_t)m, sizeof(int));
+        /* columns of Q to use: X, then the residuals by decreasing norm,
+         * each kept only when it adds a direction (Cholesky pivot above
+         * 1e-10 of its diagonal): residuals of a cluster of pairs often
+         * share one dominant direction, which makes [X, R] singular */
+        int ns = 0;
+        for (int i = 0; i < k; ++i) sel[ns++] = i;
+        int order[256];
+        const int kk = k < 256 ? k : 256;
+        for (int i = 0; i < kk; ++i) order[i] = i;
+        for (int i = 0; i < kk; ++i)
+            for (int j = i + 1; j < kk; ++j)
+                if (rnorm[order[j]] > rnorm[order[i]]) {
+                    const int t = order[i];
+                    order[i] = order[j];
+                    order[j] = t;
+                }
+        double *L = Z;  /* scratch: Cholesky rows of the kept columns */
+        for (int a = 0; a < ns; ++a) {
+            double d = G[sel[a] + (size_t)sel[a] * m];
+            for (int q = 0; q < a; ++q) d -= L[a + (size_t)q * m] * L[a + (size_t)q * m];
+            d = d > 0.0 ? sqrt(d) : 0.0;
+            L[a + (size_t)a * m] = d;
+            for (int b = a + 1; b < ns; ++b) {
+                double t = G[sel[b] + (size_t)sel[a] * m];
+                for (int q = 0; q < a; ++q) t -= L[b + (size_t)q * m] * L[a + (size_t)q * m];
+                L[b + (size_t)a * m] = d > 0.0 ? t / d : 0.0;
+            }
+        }
+        for (int ii = 0; ii < kk; ++ii) {
+            const int c = k + order[ii];
+            double row[512];
+            double d = G[c + (size_t)c * m];
+            for (int q = 0; q < ns; ++q) {
+                double t = G[c + (size_t)sel[q] * m];
+                for (int r = 0; r < q; ++r) t -= row[r] * L[q + (size_t)r * m];
+                row[q] = L[q + (size_t)q * m] > 0.0 ? t / L[q + (size_t)q * m] : 0.0;
+                d -= row[q] * row[q];
+            }
+            if (!(d > 1e-10 * G[c + (size_t)c * m]) || ns >= 512) continue;
+            for (int q = 0; q < ns; ++q) L[ns + (size_t)q * m] = row[q];
+            L[ns + (size_t)ns * m] = sqrt(d);
+            sel[ns++] = c;
+        }
+        for (int a = 0; a < ns; ++a)
+            for (int b = 0; b < ns; ++b) {
+                Gs[a + (size_t)b * ns] = G[sel[a] + (size_t)sel[b] * m];
+                Hs[a + (size_t)b * ns] = H[sel[a] + (size_t)sel[b] * m];
+            }
+        if (lz_sygv(ns, Hs, Gs, theta, Z) == 0) {
+            int *taken = (int *)calloc((size_t)ns, sizeof(int));
             if (taken) {
+                memset(Y, 0, sizeof(double) * (size_t)m * k);
                 for (int i = 0; i < k; ++i) {
                     int best = -1;
-                    for (int j = 0; j < m; ++j)
+                    for (int j = 0; j < ns; ++j)
                         if (!taken[j] && (best < 0 || fabs(theta[j] - rho[i]) < fabs(theta[best] - rho[i])))
                             best = j;
                     taken[best] = 1;
-                    memcpy(Y + (size_t)i * m, Z + (size_t)best * m, sizeof(double) * (size_t)m);
+                    for (int a = 0; a < ns; ++a) Y[(size_t)i * m + sel[a]] = Z[a + (size_t)best * ns];
                 }
                 free(taken);
                 if (trlhip_ritz(z->dev, 0, m, k, Y, m)) rc = TRLHIP_STAT_DEVICE;
@@ -885,6 +954,7 @@ static int lz_polish(lz_t *z, int k, int avoid, const double *rho)
     }
     if (!rc && save >= 0 && trlhip_col_copy(z->dev, avoid, save)) rc = TRLHIP_STAT_DEVICE;
     free(G);
+    free(sel);
     if (rc) {
         z->info->stat = TRLHIP_STAT_DEVICE;
         return rc;
@@ -1522,6 +1592,7 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
         if (!lz_setup(&co->z, trlhip_csr_matvec, NULL, NULL, j->info, j->nrow, j->mev, j->lde, 0,
                       bl[co->lane]))
             continue;
+        co->z.dstart = j->dstart;
         co->ended = 0;
         co->stack = malloc(LZ_STACK);
         if (!co->stack || getcontext(&co->uc) != 0) {

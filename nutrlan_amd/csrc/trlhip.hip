@@ -799,6 +799,17 @@ __global__ __launch_bounds__(kBlock) void k_scale_copy(long ne, const S *src, S 
     }
 }
 
+// dst = FP64 of an FP32 device vector (and its FP32 shadow copy).
+__global__ __launch_bounds__(kBlock) void k_copy_f32(long n, const float *__restrict__ src,
+                                                     double *__restrict__ dst, float *__restrict__ dst32)
+{
+    for (long i = blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+        const float v = src[i];
+        dst[i] = (double)v;
+        if (dst32) dst32[i] = v;
+    }
+}
+
 // ------------------------------------------------------------- restart
 
 // In-place restart GEMM V(:, lck:lck+k) <- V(:, lck:lck+jl) * Y on FP64
@@ -1065,6 +1076,7 @@ struct BatchOp {
     size_t yoff;
     int dst, src;         // scale-copy dst = s * src
     double s;
+    const float *src32;   // copy from an FP32 device vector instead of column src
     const trlhip_csr *A;  // refinement of columns [0, k) (k above), `avoid` kept
     int avoid;
 };
@@ -1846,6 +1858,7 @@ struct ODesc {
     long n, ld;
     double *V;
     const double *src;
+    const float *src32;  // copy: FP32 source instead of src
     float *V32;
     const double *Y;
     double s;
@@ -1868,7 +1881,7 @@ __global__ __launch_bounds__(kBlock) void kb_copy(const ODesc *__restrict__ d, c
     const ODesc &q = d[p];
     const int bid = blockIdx.x - pre[p], nblk = pre[p + 1] - pre[p];
     for (long i = (long)bid * kBlock + threadIdx.x; i < q.n; i += (long)nblk * kBlock) {
-        const double v = q.src[i] * q.s;
+        const double v = (q.src32 ? (double)q.src32[i] : q.src[i]) * q.s;
         q.V[i] = v;
         if (q.V32) q.V32[i] = (float)v;
     }
@@ -2086,7 +2099,8 @@ int ops_run(const std::vector<BatchOp> &ops, const std::vector<double> &ybuf, ch
             if (op.type == OP_COPY) {
                 o.n = c->n;
                 o.V = colp<double>(c, op.dst);
-                o.src = colp<double>(c, op.src);
+                o.src = op.src32 ? nullptr : colp<double>(c, op.src);
+                o.src32 = op.src32;
                 o.V32 = shadow32(c, op.dst);
                 o.s = op.s;
                 nb = grid_for(c->n, kBlock * 8, kMaxGrid);
@@ -2501,6 +2515,29 @@ int trlhip_col_get(trlhip_ctx *c, int col, void *x)
     TRY(hipMemcpyAsync(x, c->V + (size_t)col * c->ld * c->esz, c->esz * (size_t)c->n,
                        hipMemcpyDeviceToHost, c->s));
     return ctx_wait(c);
+}
+
+int trlhip_col_set_device_f32(trlhip_ctx *c, int col, const float *dsrc)
+{
+    if (!c || !dsrc || !valid_col(c, col) || c->E != 1) return TRLHIP_EINVAL;
+    if (int rc = flush_pending(c)) return rc;
+    if (c->batch) {  // recorded: the flush copies it with the batch's other copies
+        BatchOp op{};
+        op.c = c;
+        op.type = OP_COPY;
+        op.dst = col;
+        op.src = col;
+        op.s = 1.0;
+        op.src32 = dsrc;
+        std::lock_guard<std::mutex> lk(c->batch->mu);
+        op.phase = c->nrec++;
+        c->batch->ops.push_back(op);
+        return 0;
+    }
+    const int g = grid_for(c->n, kBlock * 8, kMaxGrid);
+    KLAUNCH(k_copy_f32, dim3(g), dim3(kBlock), 0, c->s, (long)c->n, dsrc, colp<double>(c, col),
+            shadow32(c, col));
+    return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
 int trlhip_col_copy(trlhip_ctx *c, int dst, int src)

@@ -440,6 +440,8 @@ struct trlnet {
     long partcap = 0;
     double *lpart = nullptr;
     int lcap = 0;
+    float *hstage = nullptr;  // pinned staging of trlnet_forward_many (in, then out)
+    long hcap = 0;            // floats
     hipStream_t s = nullptr;
 };
 
@@ -496,13 +498,20 @@ int refresh_bf16(trlnet *t)
     return hipGetLastError() == hipSuccess ? 0 : TRLNET_EHIP;
 }
 
-int forward(trlnet *t, const Geo &g)
+// Forward pass of the images g at pixel offset off of the workspace
+// (activations are channel-last, pixel p at p * C).
+int forward(trlnet *t, const Geo &g, long off = 0)
 {
     const int gb = grid_px(g.np);
-    hipLaunchKernelGGL(k_fwd1, dim3(gb), dim3(NB), 0, t->s, g, t->x, t->wb + OW1, t->prm + OB1, t->a1);
-    hipLaunchKernelGGL(k_fwd32, dim3(gb), dim3(NB), 0, t->s, g, t->a1, t->wb + OW2, t->prm + OB2, t->a2);
-    hipLaunchKernelGGL(k_fwd32, dim3(gb), dim3(NB), 0, t->s, g, t->a2, t->wb + OW3, t->prm + OB3, t->a3);
-    hipLaunchKernelGGL(k_fwd4, dim3(gb), dim3(NB), 0, t->s, g, t->a3, t->wb + OW4, t->prm + OB4, t->o);
+    const long oc = off * C;
+    hipLaunchKernelGGL(k_fwd1, dim3(gb), dim3(NB), 0, t->s, g, t->x + off, t->wb + OW1, t->prm + OB1,
+                       t->a1 + oc);
+    hipLaunchKernelGGL(k_fwd32, dim3(gb), dim3(NB), 0, t->s, g, t->a1 + oc, t->wb + OW2, t->prm + OB2,
+                       t->a2 + oc);
+    hipLaunchKernelGGL(k_fwd32, dim3(gb), dim3(NB), 0, t->s, g, t->a2 + oc, t->wb + OW3, t->prm + OB3,
+                       t->a3 + oc);
+    hipLaunchKernelGGL(k_fwd4, dim3(gb), dim3(NB), 0, t->s, g, t->a3 + oc, t->wb + OW4, t->prm + OB4,
+                       t->o + off);
     return hipGetLastError() == hipSuccess ? 0 : TRLNET_EHIP;
 }
 
@@ -615,6 +624,7 @@ int trlnet_destroy(trlnet *t)
                     (void *)t->dO, (void *)t->a1, (void *)t->a2, (void *)t->a3, (void *)t->d1,
                     (void *)t->d2, (void *)t->part, (void *)t->lpart})
         if (p) (void)hipFree(p);
+    if (t->hstage) (void)hipHostFree(t->hstage);
     if (t->s) (void)hipStreamDestroy(t->s);
     delete t;
     return 0;
@@ -645,6 +655,67 @@ int trlnet_forward(trlnet *t, int nimg, int h, int w, const float *in, float *ou
     if (int rc = forward(t, g)) return rc;
     NTRY(hipMemcpyAsync(out, t->o, sizeof(float) * g.np, hipMemcpyDeviceToHost, t->s));
     NTRY(hipStreamSynchronize(t->s));
+    return 0;
+}
+
+int trlnet_forward_many(trlnet *t, int ngroups, const int *nimg, const int *h, const int *w,
+                        const float *in, float *out)
+{
+    if (!t || ngroups < 1 || !nimg || !h || !w || !in || !out) return TRLNET_EINVAL;
+    long total = 0;
+    for (int i = 0; i < ngroups; ++i) {
+        if (!geo_ok(nimg[i], h[i], w[i])) return TRLNET_EINVAL;
+        total += (long)nimg[i] * h[i] * w[i];
+    }
+    if (int rc = ensure(t, total, 1)) return rc;
+    if (total > t->hcap) {  // pinned staging: the copies run as DMA, asynchronously
+        if (t->hstage) (void)hipHostFree(t->hstage);
+        t->hstage = nullptr;
+        t->hcap = 0;
+        NTRY(hipHostMalloc((void **)&t->hstage, sizeof(float) * total, 0));
+        t->hcap = total;
+    }
+    memcpy(t->hstage, in, sizeof(float) * total);
+    NTRY(hipMemcpyAsync(t->x, t->hstage, sizeof(float) * total, hipMemcpyHostToDevice, t->s));
+    long off = 0;
+    for (int i = 0; i < ngroups; ++i) {
+        const Geo g{nimg[i], h[i], w[i], (long)nimg[i] * h[i] * w[i]};
+        if (int rc = forward(t, g, off)) return rc;
+        off += g.np;
+    }
+    NTRY(hipMemcpyAsync(t->hstage, t->o, sizeof(float) * total, hipMemcpyDeviceToHost, t->s));
+    NTRY(hipStreamSynchronize(t->s));
+    memcpy(out, t->hstage, sizeof(float) * total);
+    return 0;
+}
+
+int trlnet_forward_many_device(trlnet *t, int ngroups, const int *nimg, const int *h, const int *w,
+                               const float *in, const float **out_device)
+{
+    if (!t || ngroups < 1 || !nimg || !h || !w || !in || !out_device) return TRLNET_EINVAL;
+    long total = 0;
+    for (int i = 0; i < ngroups; ++i) {
+        if (!geo_ok(nimg[i], h[i], w[i])) return TRLNET_EINVAL;
+        total += (long)nimg[i] * h[i] * w[i];
+    }
+    if (int rc = ensure(t, total, 1)) return rc;
+    if (total > t->hcap) {
+        if (t->hstage) (void)hipHostFree(t->hstage);
+        t->hstage = nullptr;
+        t->hcap = 0;
+        NTRY(hipHostMalloc((void **)&t->hstage, sizeof(float) * total, 0));
+        t->hcap = total;
+    }
+    memcpy(t->hstage, in, sizeof(float) * total);
+    NTRY(hipMemcpyAsync(t->x, t->hstage, sizeof(float) * total, hipMemcpyHostToDevice, t->s));
+    long off = 0;
+    for (int i = 0; i < ngroups; ++i) {
+        const Geo g{nimg[i], h[i], w[i], (long)nimg[i] * h[i] * w[i]};
+        if (int rc = forward(t, g, off)) return rc;
+        off += g.np;
+    }
+    NTRY(hipStreamSynchronize(t->s));
+    *out_device = t->o;
     return 0;
 }
 

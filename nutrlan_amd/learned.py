@@ -38,10 +38,13 @@ def _lib():
         L.trlnet_get_params.argtypes = [_P, _P, _I]
         L.trlnet_set_params.argtypes = [_P, _P, _I]
         L.trlnet_forward.argtypes = [_P, _I, _I, _I, _P, _P]
+        L.trlnet_forward_many.argtypes = [_P, _I, _P, _P, _P, _P, _P]
+        L.trlnet_forward_many_device.argtypes = [_P, _I, _P, _P, _P, _P, ctypes.POINTER(_P)]
         L.trlnet_loss_grad.argtypes = [_P, _I, _I, _I, _P, _P, ctypes.POINTER(_F), _P]
         L.trlnet_train_step.argtypes = [_P, _I, _I, _I, _P, _P, _F, ctypes.POINTER(_F)]
         for f in ("trlnet_create", "trlnet_destroy", "trlnet_get_params", "trlnet_set_params",
-                  "trlnet_forward", "trlnet_loss_grad", "trlnet_train_step"):
+                  "trlnet_forward", "trlnet_forward_many", "trlnet_forward_many_device",
+                  "trlnet_loss_grad", "trlnet_train_step"):
             getattr(L, f).restype = _I
         L._trlnet_bound = True
     return L
@@ -85,6 +88,43 @@ class Net:
         out = np.zeros_like(x)
         self._chk(_lib().trlnet_forward(self.h, n, h, w, x.ctypes.data, out.ctypes.data), "forward")
         return out
+
+    def forward_flat(self, dims, flat):
+        """dims: (ngroups, 3) int array of (nimg, h, w); flat: the groups'
+        images back to back (float32) -> outputs in the same layout, one
+        device call for all (trlnet_forward_many)."""
+        dims = np.ascontiguousarray(dims, dtype=np.int32)
+        flat = _f32(flat)
+        out = np.empty_like(flat)
+        ni, hh, ww = (np.ascontiguousarray(dims[:, k]) for k in range(3))
+        self._chk(_lib().trlnet_forward_many(self.h, len(dims), ni.ctypes.data, hh.ctypes.data,
+                                             ww.ctypes.data, flat.ctypes.data, out.ctypes.data),
+                  "forward_many")
+        return out
+
+    def forward_flat_device(self, dims, flat):
+        """forward_flat with the outputs left on the device: returns the
+        device address of the outputs (FP32, same layout), valid until the
+        next call on this net (trlnet_forward_many_device)."""
+        dims = np.ascontiguousarray(dims, dtype=np.int32)
+        flat = _f32(flat)
+        ni, hh, ww = (np.ascontiguousarray(dims[:, k]) for k in range(3))
+        dptr = _P()
+        self._chk(_lib().trlnet_forward_many_device(self.h, len(dims), ni.ctypes.data,
+                                                    hh.ctypes.data, ww.ctypes.data,
+                                                    flat.ctypes.data, ctypes.byref(dptr)),
+                  "forward_many_device")
+        return dptr.value
+
+    def forward_many(self, groups):
+        """groups: list of (nimg, h, w) arrays -> list of outputs."""
+        xs = [_f32(g) for g in groups]
+        out = self.forward_flat([g.shape for g in xs], np.concatenate([g.ravel() for g in xs]))
+        res, o = [], 0
+        for g in xs:
+            res.append(out[o:o + g.size].reshape(g.shape))
+            o += g.size
+        return res
 
     def loss_grad(self, x, t):
         x, t = _f32(x), _f32(t)
@@ -169,15 +209,50 @@ def load_net(path=WEIGHTS):
     return Net(params=np.load(path, allow_pickle=False))
 
 
-def start_vectors(net, potentials):
-    """Start vectors (flattened) for lattices with these potentials; images
-    of equal size go through the net together."""
-    out = [None] * len(potentials)
+def _flat_inputs(potentials):
+    """(groups of lattice indices by shape, dims, flat scaled float32 input)."""
     by_shape = {}
     for i, d in enumerate(potentials):
         by_shape.setdefault(d.shape, []).append(i)
-    for shp, idx in by_shape.items():
-        y = net.forward(np.stack([potentials[i] for i in idx]) * SCALE)
-        for k, i in enumerate(idx):
-            out[i] = y[k].astype(np.float64).ravel()
+    groups = list(by_shape.values())
+    total = sum(potentials[i].size for idx in groups for i in idx)
+    flat = np.empty(total, dtype=np.float32)
+    dims, o = [], 0
+    for idx in groups:
+        h, w = potentials[idx[0]].shape
+        dims.append((len(idx), h, w))
+        for i in idx:
+            np.multiply(potentials[i].ravel(), SCALE, out=flat[o:o + h * w], casting="unsafe")
+            o += h * w
+    return groups, dims, flat
+
+
+def start_vectors_device(net, potentials):
+    """Device addresses of the start vectors (FP32, one per lattice, in the
+    order of `potentials`) for trl_batch_job.dstart: the net's outputs stay
+    on the device (valid until the net's next call)."""
+    groups, dims, flat = _flat_inputs(potentials)
+    base = net.forward_flat_device(dims, flat)
+    out = [None] * len(potentials)
+    o = 0
+    for idx in groups:
+        for i in idx:
+            out[i] = base + 4 * o
+            o += potentials[i].size
+    return out
+
+
+def start_vectors(net, potentials):
+    """Start vectors (flattened, float32) for lattices with these
+    potentials: one device call for all of them, images of equal size
+    together."""
+    out = [None] * len(potentials)
+    groups, dims, flat = _flat_inputs(potentials)
+    y = net.forward_flat(dims, flat)
+    o = 0
+    for idx in groups:
+        for i in idx:
+            n = potentials[i].size
+            out[i] = y[o:o + n]  # float32; trlan() callers copy it into evec(:, nec)
+            o += n
     return out

@@ -248,14 +248,17 @@ class TrlBatchJob(ctypes.Structure):
     _fields_ = [("A", ctypes.c_void_p), ("info", ctypes.POINTER(TrlInfo)),
                 ("nrow", ctypes.c_int), ("mev", ctypes.c_int),
                 ("eval", ctypes.c_void_p), ("evec", ctypes.c_void_p),
-                ("lde", ctypes.c_int), ("lwrk", ctypes.c_int), ("wrk", ctypes.c_void_p)]
+                ("lde", ctypes.c_int), ("lwrk", ctypes.c_int), ("wrk", ctypes.c_void_p),
+                ("dstart", ctypes.c_void_p)]
 
 
-def solve_batch(problems, out=None):
+def solve_batch(problems, out=None, dstarts=None):
     """trlan_batch over problems = [(A, lohi, ned, maxlan, tol, restart,
     maxmv, mev, start), ...] with real device operators A (Csr, mixed or
     not).  out = [output_buffers(n, mev) per problem] reuses the caller's
-    arrays (the returned dicts refer to them).  Returns (rc, [result dict
+    arrays (the returned dicts refer to them).  dstarts = [device address of
+    an FP32 start vector per problem] (trl_batch_job.dstart) replaces the
+    host start vectors (start may then be None).  Returns (rc, [result dict
     per problem], wall seconds)."""
     import time
 
@@ -272,7 +275,10 @@ def solve_batch(problems, out=None):
             ev, evec, wrk = out[j]
         else:
             ev, evec, wrk = np.zeros(mev), np.zeros((mev, n)), np.zeros(mev)
-        evec[0] = start
+        if dstarts is not None:
+            jobs[j].dstart = dstarts[j]
+        else:
+            evec[0] = start
         keep.append((info, ev, evec, wrk))
         jobs[j].A = A.handle.value
         jobs[j].info = ctypes.pointer(info)

@@ -80,3 +80,33 @@ def test_batch_mixed_operators_and_early_exit(gpu_lib):
     assert res[2]["matvec"] == 0
     Aa.close()
     Ab.close()
+
+
+def test_batch_device_start_vectors(gpu_lib):
+    """Start vectors handed over on the device (trl_batch_job.dstart, FP32)
+    give exactly the solves that the same FP32 values give from the host
+    (evec(:, 0), converted to FP64)."""
+    from nutrlan_amd import learned as LN
+
+    T = gpu_lib
+    net = LN.load_net()
+    assert net is not None
+    rng = np.random.default_rng(31)
+    models = [LN.lattice(int(L), float(rng.uniform(1, 12)), 800 + i)
+              for i, L in enumerate((64, 80, 64, 96, 72))]
+    pots = [d for c, d in models]
+    host = LN.start_vectors(net, pots)
+    dptrs = LN.start_vectors_device(net, pots)
+    for mixed in (False, True):
+        ops = [T.Csr(*c, mixed=mixed) for c, d in models]
+        probs = [(A, -1, 8, 32, 1e-8, 1, 100000, 8, x) for A, x in zip(ops, host)]
+        rc_h, res_h, _ = T.solve_batch(probs)
+        rc_d, res_d, _ = T.solve_batch([p[:-1] + (None,) for p in probs], dstarts=dptrs)
+        assert rc_h == 0 and rc_d == 0
+        for a, b in zip(res_h, res_d):
+            assert a["matvec"] == b["matvec"] and a["nec"] == b["nec"] == 8
+            assert np.array_equal(a["eval"], b["eval"])
+            assert np.array_equal(a["evec"], b["evec"])
+        for A in ops:
+            A.close()
+    net.close()

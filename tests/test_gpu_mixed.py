@@ -29,6 +29,10 @@ CASES = [
     # Rayleigh-Ritz polish over [X, residuals]
     ("and2d80_W11_polish", lambda: W.anderson2d_csr(80, 11.35921550186609, seed=7718)[0], -1, 8,
      32, 1e-8, 9718),
+    # a cluster of converged pairs whose residuals share one direction, so
+    # [X, residuals] is singular and the polish keeps independent columns
+    ("and2d75_W9_polish_rank", lambda: W.anderson2d_csr(75, 9.673365812110031, seed=7380)[0], -1, 8, 32,
+     1e-8, 9380),
     ("diag2000_sm10", lambda: W.diag_csr(2000), -1, 10, 100, 1.4901e-8, 4),
 ]
 

@@ -121,3 +121,15 @@ def test_adam_steps_lower_the_loss(gpu_lib):
     l1, _ = net.loss_grad(x, t)
     assert l1 < 0.5 * l0, (l0, l1)
     net.close()
+
+
+def test_forward_many_matches_forward(gpu_lib):
+    """trlnet_forward_many over groups of different sizes gives exactly what
+    one trlnet_forward per group gives."""
+    net = learned.Net(seed=11)
+    rng = np.random.default_rng(12)
+    groups = [rng.uniform(-1, 1, s).astype(np.float32) for s in ((3, 16, 16), (1, 40, 24), (2, 64, 64))]
+    many = net.forward_many(groups)
+    for g, y in zip(groups, many):
+        assert np.array_equal(y, net.forward(g))
+    net.close()
