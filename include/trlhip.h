@@ -195,7 +195,13 @@ enum {
     TRLHIP_K_UPDATE = 3,
     TRLHIP_K_RITZ = 4,
     TRLHIP_K_MISC = 5,
-    TRLHIP_K_COUNT = 6
+    /* the batched kernels of trlan_batch (one launch for many solves) */
+    TRLHIP_K_BSPMV = 6,
+    TRLHIP_K_BRECUR = 7,
+    TRLHIP_K_BPROJ = 8,
+    TRLHIP_K_BUPDATE = 9,
+    TRLHIP_K_BRITZ = 10,
+    TRLHIP_K_COUNT = 11
 };
 
 /* Per-kernel HIP-event timing: on = 0 disables, on = N > 0 times every

@@ -1737,7 +1737,7 @@ int batch_run(hipStream_t s, const GroupInfo &gi, const BDesc *dd, const int *dp
     const int R = gi.R;
     auto blocks = [&](int ph) { return dim3(hpre[(size_t)ph * (R + 1) + R]); };
     {
-        Timed t(TRLHIP_K_SPMV, s, gi.bytes[BP_SPMV]);
+        Timed t(TRLHIP_K_BSPMV, s, gi.bytes[BP_SPMV]);
         switch (gi.lpr) {
         case 0: KLAUNCH(kb_spmv_stream, blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
         case 2: KLAUNCH((kb_spmv<2>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
@@ -1748,17 +1748,17 @@ int batch_run(hipStream_t s, const GroupInfo &gi, const BDesc *dd, const int *dp
         }
     }
     {
-        Timed t(TRLHIP_K_RECUR, s, gi.bytes[BP_RECUR]);
+        Timed t(TRLHIP_K_BRECUR, s, gi.bytes[BP_RECUR]);
         KLAUNCH(kb_recur, blocks(BP_RECUR), dim3(kBlock), sizeof(double) * gi.maxrm, s, dd, dp, R);
     }
     {
-        Timed t(TRLHIP_K_PROJ, s, gi.bytes[BP_PROJ]);
+        Timed t(TRLHIP_K_BPROJ, s, gi.bytes[BP_PROJ]);
         if (gi.mixed) KLAUNCH((kb_proj<float>), blocks(BP_PROJ), dim3(kBlock), 0, s, dd, dp, R);
         else KLAUNCH((kb_proj<double>), blocks(BP_PROJ), dim3(kBlock), 0, s, dd, dp, R);
     }
     KLAUNCH(kb_colsum, blocks(BP_COLSUM), dim3(kBlock), 0, s, dd, dp, R);
     {
-        Timed t(TRLHIP_K_UPDATE, s, gi.bytes[BP_UPDATE]);
+        Timed t(TRLHIP_K_BUPDATE, s, gi.bytes[BP_UPDATE]);
         if (gi.mixed)
             KLAUNCH((kb_update<float>), blocks(BP_UPDATE), dim3(kBlock), sizeof(double) * gi.maxj, s,
                     dd, dp, R);
@@ -2152,7 +2152,7 @@ int ops_run(const std::vector<BatchOp> &ops, const std::vector<double> &ybuf, ch
             KLAUNCH(kb_copy, g, dim3(kBlock), 0, s, o, dpp, l.R);
             continue;
         }
-        Timed t(TRLHIP_K_RITZ, s, l.bytes);
+        Timed t(TRLHIP_K_BRITZ, s, l.bytes);
 #define KB_RITZ_CASE(K)                                                            \
     case K:                                                                        \
         KLAUNCH((kb_ritz<K>), g, dim3(kBlock), l.shm, s, o, dpp, l.R);             \
@@ -2817,7 +2817,9 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y, int ldy)
     // complex columns multiply as 2n-row real matrices (Y is real)
     const long nrows = (long)c->n * c->E;
     const long ldr = (long)c->ld * c->E;
-    const int tpw = kRitzTiles;
+    // big single operators have parallelism to spare: 4 tiles per wave
+    // amortise the LDS staging of Y (about 2.1 vs 1.9 TB/s at n = 2^21)
+    const int tpw = nrows >= (1L << 20) ? 4 : kRitzTiles;
     const long rows_per_block = 16L * tpw * (kBlock / 64);
     const int g = (int)((nrows + rows_per_block - 1) / rows_per_block);
     Timed t(TRLHIP_K_RITZ, c->s, (double)c->esz * c->n * (jl + k));

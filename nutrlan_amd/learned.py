@@ -161,10 +161,12 @@ def lattice(L, Wd, seed):
     return (rp, ci, v), d.reshape(L, L)
 
 
-def solver_targets(models, maxlan=32, tol=1e-8, mixed=True, seed=0):
-    """Density amplitudes sqrt(sum_k psi_k^2) of the NEV lowest states of
-    each model [(csr, potential)], found by the batched solver from random
-    starts.  Returns (targets list, solve results)."""
+def solver_targets(models, maxlan=32, tol=1e-8, mixed=True, seed=0, kind="density"):
+    """Targets made from the NEV lowest states of each model [(csr,
+    potential)], found by the batched solver from random starts:
+    kind="density": sqrt(sum_k psi_k^2); kind="signed": sum_k psi_k with
+    each psi_k's largest-magnitude entry made positive.  Returns (targets
+    list, solve results)."""
     ops, probs = [], []
     for j, ((rp, ci, v), d) in enumerate(models):
         A = T.Csr(rp, ci, v, mixed=mixed)
@@ -181,7 +183,12 @@ def solver_targets(models, maxlan=32, tol=1e-8, mixed=True, seed=0):
         if r["nec"] < NEV:
             raise RuntimeError("a training solve did not converge")
         X = r["evec"][:, :NEV]
-        tg.append(np.sqrt((X * X).sum(axis=1)).reshape(d.shape))
+        if kind == "signed":
+            piv = np.abs(X).argmax(axis=0)
+            sg = np.sign(X[piv, np.arange(NEV)])
+            tg.append((X * sg).sum(axis=1).reshape(d.shape))
+        else:
+            tg.append(np.sqrt((X * X).sum(axis=1)).reshape(d.shape))
     return tg, res
 
 
@@ -202,8 +209,9 @@ def train(net, X, Y, epochs=20, batch=64, lr=2e-3, seed=0, log=None):
     return hist
 
 
-def load_net(path=WEIGHTS):
+def load_net(path=None):
     """The trained net shipped with the package, or None."""
+    path = path or WEIGHTS
     if not os.path.exists(path):
         return None
     return Net(params=np.load(path, allow_pickle=False))

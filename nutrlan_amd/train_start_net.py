@@ -23,7 +23,7 @@ else:
     from . import learned as LN
 
 
-def make_set(count, L, seed, chunk=1024, log=print):
+def make_set(count, L, seed, chunk=1024, log=print, kind="density"):
     rng = np.random.default_rng(seed)
     X = np.zeros((count, L, L), np.float32)
     Y = np.zeros((count, L, L), np.float32)
@@ -31,7 +31,7 @@ def make_set(count, L, seed, chunk=1024, log=print):
         k = min(chunk, count - s)
         models = [LN.lattice(L, float(rng.uniform(1.0, 12.0)), seed * 1000003 + s + j) for j in range(k)]
         t0 = time.perf_counter()
-        tg, res = LN.solver_targets(models, seed=seed + s)
+        tg, res = LN.solver_targets(models, seed=seed + s, kind=kind)
         log(f"solved {s + k}/{count} lattices ({time.perf_counter() - t0:.1f} s, "
             f"{sum(r['matvec'] for r in res)} operator applications)")
         for j, ((csr, d), t) in enumerate(zip(models, tg)):
@@ -49,6 +49,9 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default=LN.WEIGHTS)
+    ap.add_argument("--target", choices=["density", "signed"], default="density",
+                    help="density: sqrt(sum_k psi_k^2); signed: sum_k psi_k, each psi_k's "
+                         "largest entry positive")
     args = ap.parse_args()
     lines = []
 
@@ -57,7 +60,7 @@ def main():
         lines.append(msg)
 
     t0 = time.perf_counter()
-    X, Y = make_set(args.count, args.L, args.seed, log=log)
+    X, Y = make_set(args.count, args.L, args.seed, log=log, kind=args.target)
     log(f"training set: {args.count} lattices {args.L}x{args.L} in {time.perf_counter() - t0:.1f} s")
     net = LN.Net(seed=args.seed)
     t1 = time.perf_counter()

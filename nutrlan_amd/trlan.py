@@ -21,7 +21,8 @@ LIBPATH = os.path.join(_HERE, "libnutrlan_amd.so")
 
 TRLHIP_R64, TRLHIP_C128, TRLHIP_R32 = 0, 1, 2
 TRLHIP_STAT_DEVICE = -1201
-KERNELS = ["spmv", "recur", "proj", "update", "ritz", "misc"]
+KERNELS = ["spmv", "recur", "proj", "update", "ritz", "misc",
+           "b_spmv", "b_recur", "b_proj", "b_update", "b_ritz"]  # b_: batched kernels
 
 _lib = None
 
