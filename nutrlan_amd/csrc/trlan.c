@@ -14,6 +14,7 @@
  * Basis columns: logical column k is device column k (the reference splits
  * them between evec and base, SRC/trlcore.c:883-910).
  */
+#define _GNU_SOURCE
 #include <float.h>
 #include <stdio.h>
 #include <math.h>
@@ -1389,6 +1390,7 @@ typedef struct {
     int done;     /* lz_iterate returned */
     int ended;    /* torn down */
     int lane;     /* the batch (lane) it steps in */
+    int home;     /* the scheduler thread that resumes it first */
 } lz_co_t;
 
 /* One host thread of the scheduler: it owns the coroutines i with
@@ -1403,6 +1405,7 @@ typedef struct lz_worker {
     double idle;  /* seconds between finishing its part of a round and the round's end */
     double rmax, tlong;  /* longest resume; time in resumes over 100 us */
     long nlong;
+    int cpu;  /* CPU the thread is pinned to, or -1 */
 } lz_worker_t;
 
 /* Barrier for the scheduler threads.  Rounds come ~0.1-1 ms apart and the
@@ -1438,10 +1441,17 @@ static void lz_spin_wait(lz_spin_t *b)
     }
 }
 
+/* A thread's share of the running lane's list: [start, start + len),
+ * taken through the atomic index next (own cache line). */
+typedef struct {
+    int next, start, len;
+    char pad[64 - 3 * sizeof(int)];
+} lz_share_t;
+
 typedef struct lz_pool {
-    lz_co_t **lanes[4];  /* per lane: its unfinished coroutines */
+    lz_co_t **lanes[4];  /* per lane: its unfinished coroutines, ordered by home */
     int nlane[4];
-    int next;            /* next index of the running lane's list (atomic) */
+    lz_share_t *share;   /* per thread */
     lz_co_t *all;
     int njobs, nthr;
     lz_worker_t *w;
@@ -1485,38 +1495,80 @@ static void lz_co_main(void)
 }
 
 /* Resume every unfinished coroutine of the current lane once (each runs
- * until it waits on the device or finishes).  The workers take coroutines
- * from the lane's list through a shared index, so a round ends when the
- * work, not one thread's share of it, is done. */
+ * until it waits on the device or finishes).  A worker first takes the
+ * coroutines whose home it is, so a coroutine's state stays in the caches
+ * of the core that stepped it last round, then takes what is left of the
+ * other workers' shares: a round ends when the work, not one thread's
+ * share of it, is done. */
 static void lz_round(lz_worker_t *w)
 {
     tl_worker = w;
     const double t0 = lz_now();
     lz_pool_t *p = w->pool;
     lz_co_t **list = p->lanes[p->lane];
-    const int cnt = p->nlane[p->lane];
-    for (;;) {
-        const int k = __atomic_fetch_add(&p->next, 1, __ATOMIC_RELAXED);
-        if (k >= cnt) break;
-        lz_co_t *co = list[k];
-        if (co->done) continue;
-        w->cur = co;
-        const double r0 = lz_now();
-        swapcontext(&w->sched, &co->uc);
-        const double dr = lz_now() - r0;
-        ++w->resumes;
-        if (dr > w->rmax) w->rmax = dr;
-        if (dr > 1e-4) {
-            w->tlong += dr;
-            ++w->nlong;
+    for (int v = 0; v < p->nthr; ++v) {
+        lz_share_t *sh = &p->share[(w->tid + v) % p->nthr];
+        for (;;) {
+            const int k = __atomic_fetch_add(&sh->next, 1, __ATOMIC_RELAXED);
+            if (k >= sh->len) break;
+            lz_co_t *co = list[sh->start + k];
+            if (co->done) continue;
+            w->cur = co;
+            const double r0 = lz_now();
+            swapcontext(&w->sched, &co->uc);
+            const double dr = lz_now() - r0;
+            ++w->resumes;
+            if (dr > w->rmax) w->rmax = dr;
+            if (dr > 1e-4) {
+                w->tlong += dr;
+                ++w->nlong;
+            }
         }
     }
     w->busy += lz_now() - t0;
 }
 
+/* Shares of lane l's list (ordered by home) for the next round. */
+static void lz_shares(lz_pool_t *p, int l)
+{
+    for (int t = 0; t < p->nthr; ++t) {
+        p->share[t].next = 0;
+        p->share[t].len = 0;
+    }
+    const int cnt = p->nlane[l];
+    for (int k = 0; k < cnt; ++k) ++p->share[p->lanes[l][k]->home].len;
+    int at = 0;
+    for (int t = 0; t < p->nthr; ++t) {
+        p->share[t].start = at;
+        at += p->share[t].len;
+    }
+}
+
+/* The CPUs this process may run on, in order (at most cap of them). */
+static int lz_cpus(int *cpu, int cap)
+{
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof set, &set) != 0) return 0;
+    int n = 0;
+    for (int c = 0; c < CPU_SETSIZE && n < cap; ++c)
+        if (CPU_ISSET(c, &set)) cpu[n++] = c;
+    return n;
+}
+
+/* Pin the calling thread to one CPU (no-op on failure). */
+static void lz_pin(int c)
+{
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    CPU_SET(c, &set);
+    pthread_setaffinity_np(pthread_self(), sizeof set, &set);
+}
+
 static void *lz_worker_main(void *arg)
 {
     lz_worker_t *w = (lz_worker_t *)arg;
+    if (w->cpu >= 0) lz_pin(w->cpu);
     for (;;) {
         lz_spin_wait(&w->pool->start);
         if (w->pool->quit) break;
@@ -1611,19 +1663,31 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     pool.njobs = njobs;
     pool.nthr = lz_batch_threads(njobs);
     pool.w = (lz_worker_t *)calloc((size_t)pool.nthr, sizeof(lz_worker_t));
+    pool.share = (lz_share_t *)aligned_alloc(64, sizeof(lz_share_t) * (size_t)pool.nthr);
     pthread_t *th = (pthread_t *)calloc((size_t)pool.nthr, sizeof(pthread_t));
+    /* TRLHIP_BATCH_PIN (default 1): pin the scheduler threads, this one
+     * included until the batch returns, to distinct CPUs of the process's
+     * set when it has enough of them */
+    const char *epin = getenv("TRLHIP_BATCH_PIN");
+    int cpus[64];
+    const int ncpu = (epin && epin[0] == '0') ? 0 : lz_cpus(cpus, 64);
+    const int pin = pool.nthr > 1 && ncpu >= pool.nthr;
+    cpu_set_t main_mask;
+    const int main_saved = pin && sched_getaffinity(0, sizeof main_mask, &main_mask) == 0;
     int nstarted = 0;
-    if (pool.w && th) {
+    if (pool.w && th && pool.share) {
         lz_spin_init(&pool.start, pool.nthr);
         lz_spin_init(&pool.stop, pool.nthr);
         for (int t = 0; t < pool.nthr; ++t) {
             pool.w[t].pool = &pool;
             pool.w[t].tid = t;
+            pool.w[t].cpu = pin ? cpus[t] : -1;
         }
+        if (main_saved) lz_pin(cpus[0]);
         for (int t = 1; t < pool.nthr; ++t)
             if (pthread_create(&th[t], NULL, lz_worker_main, &pool.w[t]) == 0) ++nstarted;
     }
-    if (!pool.w || !th || nstarted != pool.nthr - 1) {
+    if (!pool.w || !th || !pool.share || nstarted != pool.nthr - 1) {
         /* could not start the threads: stop any started, run on this one */
         if (nstarted > 0) {
             pool.quit = 1;
@@ -1634,9 +1698,11 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
         free(th);
         th = NULL;
         if (!pool.w) pool.w = (lz_worker_t *)calloc(1, sizeof(lz_worker_t));
+        if (!pool.share) pool.share = (lz_share_t *)aligned_alloc(64, sizeof(lz_share_t));
         pool.nthr = 1;
         pool.w[0].pool = &pool;
         pool.w[0].tid = 0;
+        pool.w[0].cpu = -1;
     }
     /* each lane's list of unfinished coroutines */
     int lists_ok = 1;
@@ -1655,6 +1721,11 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     } else {
         for (int i = 0; i < njobs; ++i)
             if (!cos[i].done) pool.lanes[cos[i].lane][pool.nlane[cos[i].lane]++] = &cos[i];
+        /* homes: contiguous runs of each lane's list, so it is ordered by
+         * home and stays so as finished coroutines drop out */
+        for (int l = 0; l < nlanes; ++l)
+            for (int k = 0; k < pool.nlane[l]; ++k)
+                pool.lanes[l][k]->home = (int)((long)k * pool.nthr / pool.nlane[l]);
     }
     /* TRLHIP_BATCH_PROFILE=1: host time in the solves vs waiting for the
      * device */
@@ -1664,7 +1735,7 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     const double t_loop = lz_now();
     int pending[4] = {0, 0, 0, 0};
     /* profile: wall time of lane rounds by the number of solves they step */
-    double t_band[4] = {0, 0, 0, 0}, w_band[4] = {0, 0, 0, 0};
+    double t_band[4] = {0, 0, 0, 0}, w_band[4] = {0, 0, 0, 0}, l_band[4] = {0, 0, 0, 0};
     long n_band[4] = {0, 0, 0, 0};
     int last_band[4] = {0, 0, 0, 0};
     for (;;) {
@@ -1684,7 +1755,7 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
             const int active = live[l];
             const double t0 = lz_now();
             pool.lane = l;
-            pool.next = 0;
+            lz_shares(&pool, l);
             if (pool.nthr > 1) lz_spin_wait(&pool.start);
             lz_round(&pool.w[0]);
             const double tf = lz_now();
@@ -1699,6 +1770,7 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
             t_co += lz_now() - t0;
             ++rounds;
             const int band = active >= 256 ? 0 : active >= 32 ? 1 : active >= 4 ? 2 : 3;
+            const double tl = lz_now();
             if (live[l] > 0) {
                 if (serial && nlanes > 1) {
                     const int orc = trlhip_batch_after(bl[l], bl[(l + nlanes - 1) % nlanes]);
@@ -1709,6 +1781,7 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
                 pending[l] = 1;
             }
             t_band[band] += lz_now() - t0;
+            l_band[band] += lz_now() - tl;
             ++n_band[band];
             last_band[l] = band;
         }
@@ -1731,10 +1804,14 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
             tlong += pool.w[t].tlong;
             nlong += pool.w[t].nlong;
         }
-        fprintf(stderr, "trlan_batch: lane rounds by solves stepped (count, host+launch s, wait s): "
-                ">=256 %ld %.3f %.3f; 32-255 %ld %.3f %.3f; 4-31 %ld %.3f %.3f; 1-3 %ld %.3f %.3f\n",
-                n_band[0], t_band[0], w_band[0], n_band[1], t_band[1], w_band[1], n_band[2], t_band[2],
-                w_band[2], n_band[3], t_band[3], w_band[3]);
+        fprintf(stderr, "trlan_batch: lane rounds by solves stepped (count, host+launch s, of which "
+                "launch s, wait s): >=256 %ld %.3f %.3f %.3f; 32-255 %ld %.3f %.3f %.3f; 4-31 %ld "
+                "%.3f %.3f %.3f; 1-3 %ld %.3f %.3f %.3f\n",
+                n_band[0], t_band[0], l_band[0], w_band[0], n_band[1], t_band[1], l_band[1],
+                w_band[1], n_band[2], t_band[2], l_band[2], w_band[2], n_band[3], t_band[3],
+                l_band[3], w_band[3]);
+        fprintf(stderr, "trlan_batch: threads pinned: %s (%d CPUs in the process's set)\n",
+                pin ? "yes" : "no", ncpu);
         fprintf(stderr, "trlan_batch: idle at round ends mean %.3f s; longest resume %.1f us; "
                 "%ld resumes over 100 us take %.3f s\n", isum / pool.nthr, 1e6 * rmax, nlong, tlong);
         fprintf(stderr,
@@ -1753,6 +1830,8 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     }
     free(th);
     free(pool.w);
+    free(pool.share);
+    if (main_saved) sched_setaffinity(0, sizeof main_mask, &main_mask);
     for (int l = 0; l < nlanes; ++l) free(pool.lanes[l]);
     free(cos);
     for (int l = nlanes - 1; l >= 0; --l) trlhip_batch_destroy(bl[l]); /* lane 0 owns the stream */

@@ -822,11 +822,18 @@ __global__ __launch_bounds__(kBlock) void k_copy_f32(long n, const float *__rest
 // also stored rounded to FP32.
 // MFMA f64 16x16x4 operand layout (lane l): A[l%16][l/16], B[l/16][l%16],
 // D[l/16 + 4*r][l%16] for accumulator element r (measured on gfx950).
+// The wave computes the transposed tile, out^T = Y^T V^T (A = a 16-column
+// slice of Y^T, B = V^T), so that accumulator element r of lane l is
+// output row l%16 of column l/16 + 4r: each store covers four columns of
+// 16 consecutive rows (128-byte segments) instead of sixteen 32-byte ones.
+// The products and their order over kk are those of V * Y.
 typedef double d4 __attribute__((ext_vector_type(4)));
 // 16-row tiles per wave of the restart GEMM.  Swept on the batched 2D
 // Anderson workload: 1 tile 2.65, 2 tiles 2.54, 4 tiles 2.33, 8 tiles 1.93
 // TB/s (more blocks beat amortising the LDS staging of Y).
 constexpr int kRitzTiles = 1;
+// Input columns whose loads a wave issues before their first MFMA.
+constexpr int kRitzLoads = 8;
 
 // One block's rows of the restart GEMM (block bid; tiles_per_wave 16-row
 // tiles per wave); Ys is the block's LDS (jl4 * 16 * KT doubles).
@@ -853,24 +860,34 @@ __device__ __forceinline__ void ritz_body(long nrows, double *__restrict__ V, lo
 #pragma unroll
         for (int q = 0; q < KT; ++q) acc[q] = d4{0.0, 0.0, 0.0, 0.0};
         const long arow = row0 + li;
-        for (int kk = 0; kk < jl4; kk += 4) {
-            const int col = kk + lq;
-            const double a = (arow < nrows && col < jl) ? Vin[(size_t)col * ld + arow] : 0.0;
+        for (int k0 = 0; k0 < jl4; k0 += 4 * kRitzLoads) {
+            double a[kRitzLoads];
 #pragma unroll
-            for (int q = 0; q < KT; ++q) {
-                const double b = Ys[(kk + lq) * kp + q * 16 + li];
-                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
+            for (int u = 0; u < kRitzLoads; ++u) {
+                const int col = k0 + 4 * u + lq;
+                a[u] = (arow < nrows && col < jl) ? Vin[(size_t)col * ld + arow] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kRitzLoads; ++u) {
+                const int kk = k0 + 4 * u;
+                if (kk >= jl4) break;
+#pragma unroll
+                for (int q = 0; q < KT; ++q) {
+                    const double b = Ys[(kk + lq) * kp + q * 16 + li];
+                    acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(b, a[u], acc[q], 0, 0, 0);
+                }
             }
         }
+        if (arow < nrows) {
 #pragma unroll
-        for (int q = 0; q < KT; ++q) {
-            const int col = q * 16 + li;
+            for (int q = 0; q < KT; ++q) {
 #pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-                const long row = row0 + lq + 4 * rr;
-                if (row < nrows && col < k) {
-                    Vin[(size_t)col * ld + row] = acc[q][rr];
-                    if (V32) V32[(size_t)(lck + col) * ld + row] = (float)acc[q][rr];
+                for (int rr = 0; rr < 4; ++rr) {
+                    const int col = q * 16 + lq + 4 * rr;
+                    if (col < k) {
+                        Vin[(size_t)col * ld + arow] = acc[q][rr];
+                        if (V32) V32[(size_t)(lck + col) * ld + arow] = (float)acc[q][rr];
+                    }
                 }
             }
         }
