@@ -1118,16 +1118,14 @@ struct trlhip_batch {
     std::mutex mu;                // guards steps (coroutines run on several host threads)
     std::vector<BatchStep> steps;
     std::vector<double> cbuf;     // restart-step coefficients of the recorded steps
-    double *hcbuf = nullptr, *dcbuf = nullptr;  // their pinned staging / device copy
-    size_t ccap = 0;
     std::vector<BatchOp> ops;     // recorded restart GEMMs and scale-copies
     std::vector<double> ybuf;     // their restart matrices
     char *hops = nullptr, *dops = nullptr;      // pinned staging / device copy of the ops
     size_t ocap = 0;              // bytes each holds
-    BDesc *hdesc = nullptr, *ddesc = nullptr;   // pinned staging / device copy
-    int *hpre = nullptr, *dpre = nullptr;       // prefix tables of all groups
-    size_t cap = 0;               // requests the descriptor buffers hold
-    size_t pcap = 0;              // ints the prefix buffers hold
+    // staging of the recorded steps, copied with one transfer per launch:
+    // descriptors, the prefix tables of all groups, the coefficients
+    char *hstg = nullptr, *dstg = nullptr;      // pinned / device
+    size_t scap = 0;              // bytes each holds
     // arenas the batch's contexts allocate from (freed with the batch):
     // device memory zeroed when a chunk is taken, and pinned host-coherent
     // memory with its device address
@@ -1824,46 +1822,34 @@ int batch_launch(trlhip_batch *b)
         groups.push_back(gi);
     }
     const size_t R = order.size();
-    if (R > b->cap || npre > b->pcap) {
-        if (b->hdesc) (void)hipHostFree(b->hdesc);
-        if (b->ddesc) (void)hipFree(b->ddesc);
-        if (b->hpre) (void)hipHostFree(b->hpre);
-        if (b->dpre) (void)hipFree(b->dpre);
-        b->hdesc = b->ddesc = nullptr;
-        b->hpre = b->dpre = nullptr;
-        b->cap = b->pcap = 0;
-        const size_t cap = R + R / 2 + 16;
-        const size_t pcap = npre + npre / 2 + 16 * BP_COUNT;
-        if (hipHostMalloc((void **)&b->hdesc, sizeof(BDesc) * cap, 0) != hipSuccess ||
-            hipMalloc((void **)&b->ddesc, sizeof(BDesc) * cap) != hipSuccess ||
-            hipHostMalloc((void **)&b->hpre, sizeof(int) * pcap, 0) != hipSuccess ||
-            hipMalloc((void **)&b->dpre, sizeof(int) * pcap) != hipSuccess)
+    // one staging block: descriptors | prefix tables | coefficients
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t off_pre = up(sizeof(BDesc) * R);
+    const size_t off_cb = up(off_pre + sizeof(int) * npre);
+    const size_t total = off_cb + sizeof(double) * b->cbuf.size();
+    if (total > b->scap) {
+        if (b->hstg) (void)hipHostFree(b->hstg);
+        if (b->dstg) (void)hipFree(b->dstg);
+        b->hstg = b->dstg = nullptr;
+        b->scap = 0;
+        const size_t cap = up(total + total / 2 + 65536);
+        if (hipHostMalloc((void **)&b->hstg, cap, 0) != hipSuccess ||
+            hipMalloc((void **)&b->dstg, cap) != hipSuccess)
             return TRLHIP_ENOMEM;
-        b->cap = cap;
-        b->pcap = pcap;
+        b->scap = cap;
     }
-    if (b->cbuf.size() > b->ccap) {
-        if (b->hcbuf) (void)hipHostFree(b->hcbuf);
-        if (b->dcbuf) (void)hipFree(b->dcbuf);
-        b->hcbuf = b->dcbuf = nullptr;
-        b->ccap = 0;
-        const size_t cap = b->cbuf.size() + b->cbuf.size() / 2 + 256;
-        if (hipHostMalloc((void **)&b->hcbuf, sizeof(double) * cap, 0) != hipSuccess ||
-            hipMalloc((void **)&b->dcbuf, sizeof(double) * cap) != hipSuccess)
-            return TRLHIP_ENOMEM;
-        b->ccap = cap;
-    }
+    BDesc *hdesc = reinterpret_cast<BDesc *>(b->hstg);
+    int *hpre = reinterpret_cast<int *>(b->hstg + off_pre);
+    const BDesc *ddesc = reinterpret_cast<const BDesc *>(b->dstg);
+    const int *dpre = reinterpret_cast<const int *>(b->dstg + off_pre);
+    const double *dcbuf = reinterpret_cast<const double *>(b->dstg + off_cb);
     for (GroupInfo &gi : groups)
-        batch_fill(order.data() + gi.doff, gi.R, b->hdesc + gi.doff, b->hpre + gi.poff, gi, b->dcbuf);
-    if (!b->cbuf.empty()) {
-        memcpy(b->hcbuf, b->cbuf.data(), sizeof(double) * b->cbuf.size());
-        TRY(hipMemcpyAsync(b->dcbuf, b->hcbuf, sizeof(double) * b->cbuf.size(), hipMemcpyHostToDevice,
-                           b->s));
-    }
-    TRY(hipMemcpyAsync(b->ddesc, b->hdesc, sizeof(BDesc) * R, hipMemcpyHostToDevice, b->s));
-    TRY(hipMemcpyAsync(b->dpre, b->hpre, sizeof(int) * npre, hipMemcpyHostToDevice, b->s));
+        batch_fill(order.data() + gi.doff, gi.R, hdesc + gi.doff, hpre + gi.poff, gi, dcbuf);
+    if (!b->cbuf.empty())
+        memcpy(b->hstg + off_cb, b->cbuf.data(), sizeof(double) * b->cbuf.size());
+    TRY(hipMemcpyAsync(b->dstg, b->hstg, total, hipMemcpyHostToDevice, b->s));
     for (const GroupInfo &gi : groups)
-        if (int rc = batch_run(b->s, gi, b->ddesc + gi.doff, b->dpre + gi.poff, b->hpre + gi.poff))
+        if (int rc = batch_run(b->s, gi, ddesc + gi.doff, dpre + gi.poff, hpre + gi.poff))
             return rc;
     return 0;
 }
@@ -2942,14 +2928,10 @@ void trlhip_batch_destroy(trlhip_batch *b)
     if (!b) return;
     (void)hipStreamSynchronize(b->side);
     (void)hipStreamSynchronize(b->s);
-    if (b->hdesc) (void)hipHostFree(b->hdesc);
-    if (b->ddesc) (void)hipFree(b->ddesc);
-    if (b->hpre) (void)hipHostFree(b->hpre);
-    if (b->dpre) (void)hipFree(b->dpre);
+    if (b->hstg) (void)hipHostFree(b->hstg);
+    if (b->dstg) (void)hipFree(b->dstg);
     if (b->hops) (void)hipHostFree(b->hops);
     if (b->dops) (void)hipFree(b->dops);
-    if (b->hcbuf) (void)hipHostFree(b->hcbuf);
-    if (b->dcbuf) (void)hipFree(b->dcbuf);
     {
         std::lock_guard<std::mutex> lk(g_chunk_mu);
         for (auto &ch : b->dev) g_free_chunks[0].push_back(ch);
@@ -2982,10 +2964,18 @@ int trlhip_batch_launch(trlhip_batch *b)
 {
     if (!b) return TRLHIP_EINVAL;
     int rc = b->err;
-    // the contexts' immediate work of this round comes first
-    if (!rc && (hipEventRecord(b->side_done, b->side) != hipSuccess ||
-                hipStreamWaitEvent(b->s, b->side_done, 0) != hipSuccess))
-        rc = TRLHIP_EHIP;
+    // the contexts' immediate work of this round comes first (nothing to
+    // order when the side stream has already drained, as it usually has)
+    if (!rc) {
+        const hipError_t q = hipStreamQuery(b->side);
+        if (q == hipErrorNotReady) {
+            if (hipEventRecord(b->side_done, b->side) != hipSuccess ||
+                hipStreamWaitEvent(b->s, b->side_done, 0) != hipSuccess)
+                rc = TRLHIP_EHIP;
+        } else if (q != hipSuccess) {
+            rc = TRLHIP_EHIP;
+        }
+    }
     if (!rc && !b->ops.empty()) rc = ops_run(b->ops, b->ybuf, b->hops, b->dops, b->ocap, b->s);
     if (!rc && !b->steps.empty()) rc = batch_launch(b);
     // everything this batch queued so far (this launch and the contexts'
