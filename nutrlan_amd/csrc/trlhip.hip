@@ -1120,10 +1120,8 @@ struct trlhip_batch {
     std::vector<double> cbuf;     // restart-step coefficients of the recorded steps
     std::vector<BatchOp> ops;     // recorded restart GEMMs and scale-copies
     std::vector<double> ybuf;     // their restart matrices
-    char *hops = nullptr, *dops = nullptr;      // pinned staging / device copy of the ops
-    size_t ocap = 0;              // bytes each holds
-    // staging of the recorded steps, copied with one transfer per launch:
-    // descriptors, the prefix tables of all groups, the coefficients
+    // staging of the recorded operations and steps, copied with one
+    // transfer per launch
     char *hstg = nullptr, *dstg = nullptr;      // pinned / device
     size_t scap = 0;              // bytes each holds
     // arenas the batch's contexts allocate from (freed with the batch):
@@ -1789,17 +1787,20 @@ int batch_run(hipStream_t s, const GroupInfo &gi, const BDesc *dd, const int *dp
 // in the last-level cache from phase to phase, measured 7-33 % slower on
 // the batched 2D Anderson workload: the smaller launches lose more than
 // the cache hits gain.)
-// Runs every recorded step of the batch, in the order form_residual +
-// run_proj + run_update + k_sum2 run them for a single context.  Steps are
-// grouped by (SpMV kernel: lanes per row, 0 for CSR-stream; basis
-// precision); all groups are staged in one host buffer and copied to the
-// device once, then each group's kernel set is launched.  Complex steps
-// never get here (trlhip_step_cgs runs them directly).
-int batch_launch(trlhip_batch *b)
-{
-    // group the steps (stable)
+// Staging plan of a batch's recorded steps: the steps grouped (stable) by
+// (SpMV kernel: lanes per row, 0 for CSR-stream; basis precision), and one
+// staging block of `total` bytes: descriptors | prefix tables of all
+// groups | coefficients.
+struct StepsPlan {
     std::vector<BatchStep> order;
     std::vector<GroupInfo> groups;
+    size_t npre = 0, off_pre = 0, off_cb = 0, total = 0;
+};
+
+void steps_plan(const trlhip_batch *b, StepsPlan &P)
+{
+    auto &order = P.order;
+    auto &groups = P.groups;
     order.reserve(b->steps.size());
     std::vector<char> taken(b->steps.size(), 0);
     size_t npre = 0;
@@ -1821,36 +1822,36 @@ int batch_launch(trlhip_batch *b)
         npre += (size_t)BP_COUNT * (gi.R + 1);
         groups.push_back(gi);
     }
-    const size_t R = order.size();
-    // one staging block: descriptors | prefix tables | coefficients
     auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t off_pre = up(sizeof(BDesc) * R);
-    const size_t off_cb = up(off_pre + sizeof(int) * npre);
-    const size_t total = off_cb + sizeof(double) * b->cbuf.size();
-    if (total > b->scap) {
-        if (b->hstg) (void)hipHostFree(b->hstg);
-        if (b->dstg) (void)hipFree(b->dstg);
-        b->hstg = b->dstg = nullptr;
-        b->scap = 0;
-        const size_t cap = up(total + total / 2 + 65536);
-        if (hipHostMalloc((void **)&b->hstg, cap, 0) != hipSuccess ||
-            hipMalloc((void **)&b->dstg, cap) != hipSuccess)
-            return TRLHIP_ENOMEM;
-        b->scap = cap;
-    }
-    BDesc *hdesc = reinterpret_cast<BDesc *>(b->hstg);
-    int *hpre = reinterpret_cast<int *>(b->hstg + off_pre);
-    const BDesc *ddesc = reinterpret_cast<const BDesc *>(b->dstg);
-    const int *dpre = reinterpret_cast<const int *>(b->dstg + off_pre);
-    const double *dcbuf = reinterpret_cast<const double *>(b->dstg + off_cb);
-    for (GroupInfo &gi : groups)
-        batch_fill(order.data() + gi.doff, gi.R, hdesc + gi.doff, hpre + gi.poff, gi, dcbuf);
-    if (!b->cbuf.empty())
-        memcpy(b->hstg + off_cb, b->cbuf.data(), sizeof(double) * b->cbuf.size());
-    TRY(hipMemcpyAsync(b->dstg, b->hstg, total, hipMemcpyHostToDevice, b->s));
-    for (const GroupInfo &gi : groups)
-        if (int rc = batch_run(b->s, gi, ddesc + gi.doff, dpre + gi.poff, hpre + gi.poff))
-            return rc;
+    P.npre = npre;
+    P.off_pre = up(sizeof(BDesc) * order.size());
+    P.off_cb = up(P.off_pre + sizeof(int) * npre);
+    P.total = P.off_cb + sizeof(double) * b->cbuf.size();
+}
+
+// Writes the plan's descriptors, prefix tables and coefficients into the
+// staging block (h: host, d: the device address it is copied to).
+void steps_fill(const trlhip_batch *b, StepsPlan &P, char *h, const char *d)
+{
+    BDesc *hdesc = reinterpret_cast<BDesc *>(h);
+    int *hpre = reinterpret_cast<int *>(h + P.off_pre);
+    const double *dcbuf = reinterpret_cast<const double *>(d + P.off_cb);
+    for (GroupInfo &gi : P.groups)
+        batch_fill(P.order.data() + gi.doff, gi.R, hdesc + gi.doff, hpre + gi.poff, gi, dcbuf);
+    if (!b->cbuf.empty()) memcpy(h + P.off_cb, b->cbuf.data(), sizeof(double) * b->cbuf.size());
+}
+
+// Launches every recorded step, in the order form_residual + run_proj +
+// run_update + k_sum2 run them for a single context: each group's kernel
+// set, from a staging block already copied to d.  Complex steps never get
+// here (trlhip_step_cgs runs them directly).
+int steps_launch(hipStream_t s, const StepsPlan &P, const char *h, const char *d)
+{
+    const BDesc *ddesc = reinterpret_cast<const BDesc *>(d);
+    const int *dpre = reinterpret_cast<const int *>(d + P.off_pre);
+    const int *hpre = reinterpret_cast<const int *>(h + P.off_pre);
+    for (const GroupInfo &gi : P.groups)
+        if (int rc = batch_run(s, gi, ddesc + gi.doff, dpre + gi.poff, hpre + gi.poff)) return rc;
     return 0;
 }
 
@@ -1984,17 +1985,15 @@ int refine_free_col(const trlhip_ctx *c, int k, int avoid, int j)
     }
 }
 
-// Runs recorded operations: for each phase (see BatchOp) one kb_ritz
-// launch per restart width KT present, one kb_copy launch, and the
+// Staging plan of recorded operations: for each phase (see BatchOp) one
+// kb_ritz launch per restart width KT present, one kb_copy launch, and the
 // refinements' kernels (pairs in chunks of refine_width; chunk j of every
-// context together).  Records, prefix tables and restart matrices are
-// staged in one host buffer (h, d: pinned / device, cap bytes, grown as
-// needed) and copied to the device once.  Operations of one phase belong to
-// different contexts, so their order inside the phase is free.
-int ops_run(const std::vector<BatchOp> &ops, const std::vector<double> &ybuf, char *&h, char *&d,
-            size_t &cap, hipStream_t s)
-{
-    enum { L_COPY = 0, L_REFINE = 9 };  // 1..8: restart GEMMs of that width
+// context together).  Records, prefix tables and restart matrices share one
+// staging block of `bytes` bytes, copied to the device at once.  Operations
+// of one phase belong to different contexts, so their order inside the
+// phase is free.
+enum { L_COPY = 0, L_REFINE = 9 };  // 1..8: restart GEMMs of that width
+struct OpsPlan {
     struct Launch {
         int kind, R;
         size_t doff, poff;  // first record (ODesc or FDesc array), first prefix entry
@@ -2005,11 +2004,21 @@ int ops_run(const std::vector<BatchOp> &ops, const std::vector<double> &ybuf, ch
         const BatchOp *op;
         int i, j;  // Ritz vector, slot in its chunk
     };
-    int nph = 0;
-    for (const BatchOp &op : ops) nph = std::max(nph, op.phase + 1);
     std::vector<const BatchOp *> order;
     std::vector<Pair> pairs;
     std::vector<Launch> ls;
+    size_t o_f = 0, o_pre = 0, o_y = 0, bytes = 0;
+};
+
+void ops_plan(const std::vector<BatchOp> &ops, const std::vector<double> &ybuf, OpsPlan &P)
+{
+    using Launch = OpsPlan::Launch;
+    using Pair = OpsPlan::Pair;
+    auto &order = P.order;
+    auto &pairs = P.pairs;
+    auto &ls = P.ls;
+    int nph = 0;
+    for (const BatchOp &op : ops) nph = std::max(nph, op.phase + 1);
     order.reserve(ops.size());
     size_t npre = 0;
     for (int ph = 0; ph < nph; ++ph) {
@@ -2043,26 +2052,27 @@ int ops_run(const std::vector<BatchOp> &ops, const std::vector<double> &ybuf, ch
             if (!more) break;
         }
     }
-    const size_t o_f = sizeof(ODesc) * order.size();
-    const size_t o_pre = o_f + sizeof(FDesc) * pairs.size();
-    const size_t o_y = (o_pre + sizeof(int) * npre + 255) & ~(size_t)255;
-    const size_t bytes = o_y + sizeof(double) * ybuf.size();
-    if (bytes > cap) {
-        if (h) (void)hipHostFree(h);
-        if (d) (void)hipFree(d);
-        h = d = nullptr;
-        cap = 0;
-        const size_t ncap = bytes + bytes / 2 + 4096;
-        if (hipHostMalloc((void **)&h, ncap, 0) != hipSuccess || hipMalloc((void **)&d, ncap) != hipSuccess)
-            return TRLHIP_ENOMEM;
-        cap = ncap;
-    }
+    P.o_f = sizeof(ODesc) * order.size();
+    P.o_pre = P.o_f + sizeof(FDesc) * pairs.size();
+    P.o_y = (P.o_pre + sizeof(int) * npre + 255) & ~(size_t)255;
+    P.bytes = P.o_y + sizeof(double) * ybuf.size();
+}
+
+// Writes the plan's records into the staging block (h: host, d: the
+// device address it is copied to).
+void ops_fill(OpsPlan &P, const std::vector<double> &ybuf, char *h, const char *d)
+{
+    using Launch = OpsPlan::Launch;
+    using Pair = OpsPlan::Pair;
+    const size_t o_f = P.o_f, o_pre = P.o_pre, o_y = P.o_y;
+    auto &order = P.order;
+    auto &pairs = P.pairs;
     ODesc *ho = reinterpret_cast<ODesc *>(h);
     FDesc *hf = reinterpret_cast<FDesc *>(h + o_f);
     int *hp = reinterpret_cast<int *>(h + o_pre);
     const double *dy = reinterpret_cast<const double *>(d + o_y);
     if (!ybuf.empty()) memcpy(h + o_y, ybuf.data(), sizeof(double) * ybuf.size());
-    for (Launch &l : ls) {
+    for (Launch &l : P.ls) {
         int *pp = hp + l.poff;
         pp[0] = 0;
         if (l.kind == L_REFINE) {
@@ -2127,11 +2137,18 @@ int ops_run(const std::vector<BatchOp> &ops, const std::vector<double> &ybuf, ch
             pp[i + 1] = pp[i] + nb;
         }
     }
-    TRY(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+}
+
+// Launches the plan's kernels on s from a staging block already copied to d.
+int ops_launch(const OpsPlan &P, const char *h, const char *d, hipStream_t s)
+{
+    using Launch = OpsPlan::Launch;
+    const int *hp = reinterpret_cast<const int *>(h + P.o_pre);
+    const size_t o_f = P.o_f, o_pre = P.o_pre;
     const ODesc *dod = reinterpret_cast<const ODesc *>(d);
     const FDesc *dfd = reinterpret_cast<const FDesc *>(d + o_f);
     const int *dp = reinterpret_cast<const int *>(d + o_pre);
-    for (const Launch &l : ls) {
+    for (const Launch &l : P.ls) {
         const int *pp = hp + l.poff;
         if (l.kind == L_REFINE) {
             const FDesc *f = dfd + l.doff;
@@ -2167,6 +2184,30 @@ int ops_run(const std::vector<BatchOp> &ops, const std::vector<double> &ybuf, ch
 #undef KB_RITZ_CASE
     }
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
+}
+
+
+// Runs recorded operations outside a batch: stage (h, d: pinned / device,
+// cap bytes, grown as needed), copy, launch.
+int ops_run(const std::vector<BatchOp> &ops, const std::vector<double> &ybuf, char *&h, char *&d,
+            size_t &cap, hipStream_t s)
+{
+    OpsPlan P;
+    ops_plan(ops, ybuf, P);
+    const size_t bytes = P.bytes;
+    if (bytes > cap) {
+        if (h) (void)hipHostFree(h);
+        if (d) (void)hipFree(d);
+        h = d = nullptr;
+        cap = 0;
+        const size_t ncap = bytes + bytes / 2 + 4096;
+        if (hipHostMalloc((void **)&h, ncap, 0) != hipSuccess || hipMalloc((void **)&d, ncap) != hipSuccess)
+            return TRLHIP_ENOMEM;
+        cap = ncap;
+    }
+    ops_fill(P, ybuf, h, d);
+    TRY(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+    return ops_launch(P, h, d, s);
 }
 
 }  // namespace
@@ -2930,8 +2971,6 @@ void trlhip_batch_destroy(trlhip_batch *b)
     (void)hipStreamSynchronize(b->s);
     if (b->hstg) (void)hipHostFree(b->hstg);
     if (b->dstg) (void)hipFree(b->dstg);
-    if (b->hops) (void)hipHostFree(b->hops);
-    if (b->dops) (void)hipFree(b->dops);
     {
         std::lock_guard<std::mutex> lk(g_chunk_mu);
         for (auto &ch : b->dev) g_free_chunks[0].push_back(ch);
@@ -2976,8 +3015,39 @@ int trlhip_batch_launch(trlhip_batch *b)
             rc = TRLHIP_EHIP;
         }
     }
-    if (!rc && !b->ops.empty()) rc = ops_run(b->ops, b->ybuf, b->hops, b->dops, b->ocap, b->s);
-    if (!rc && !b->steps.empty()) rc = batch_launch(b);
+    // the recorded operations and steps share one staging block and one
+    // transfer; the operations run first
+    if (!rc && (!b->ops.empty() || !b->steps.empty())) {
+        OpsPlan op;
+        StepsPlan sp;
+        size_t o_bytes = 0, total = 0;
+        if (!b->ops.empty()) {
+            ops_plan(b->ops, b->ybuf, op);
+            o_bytes = (op.bytes + 255) & ~(size_t)255;
+        }
+        if (!b->steps.empty()) steps_plan(b, sp);
+        total = o_bytes + sp.total;
+        if (total > b->scap) {
+            if (b->hstg) (void)hipHostFree(b->hstg);
+            if (b->dstg) (void)hipFree(b->dstg);
+            b->hstg = b->dstg = nullptr;
+            b->scap = 0;
+            const size_t cap = ((total + total / 2 + 65536) + 255) & ~(size_t)255;
+            if (hipHostMalloc((void **)&b->hstg, cap, 0) != hipSuccess ||
+                hipMalloc((void **)&b->dstg, cap) != hipSuccess)
+                rc = TRLHIP_ENOMEM;
+            else
+                b->scap = cap;
+        }
+        if (!rc) {
+            if (!b->ops.empty()) ops_fill(op, b->ybuf, b->hstg, b->dstg);
+            if (!b->steps.empty()) steps_fill(b, sp, b->hstg + o_bytes, b->dstg + o_bytes);
+            if (hipMemcpyAsync(b->dstg, b->hstg, total, hipMemcpyHostToDevice, b->s) != hipSuccess)
+                rc = TRLHIP_EHIP;
+        }
+        if (!rc && !b->ops.empty()) rc = ops_launch(op, b->hstg, b->dstg, b->s);
+        if (!rc && !b->steps.empty()) rc = steps_launch(b->s, sp, b->hstg + o_bytes, b->dstg + o_bytes);
+    }
     // everything this batch queued so far (this launch and the contexts'
     // own copies) is behind this event
     if (!rc && hipEventRecord(b->done, b->s) != hipSuccess) rc = TRLHIP_EHIP;
