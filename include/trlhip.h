@@ -55,9 +55,19 @@ typedef struct trlhip_ctx trlhip_ctx;   /* device basis + workspace  */
  * keeps the recurrence, basis and restart in FP64, and refines each
  * returned pair by an FP64 Rayleigh quotient whose true residual it
  * reports in wrk.  Invalid structure (decreasing row_ptr, column out of
- * range) is rejected with EINVAL. */
+ * range) is rejected with EINVAL.  A real matrix with at most 255
+ * distinct off-diagonal values (compared bit for bit), at most one
+ * diagonal entry per row and every column within 2^23 rows of its row
+ * block also gets a packed copy of its nonzeros (one 32-bit word each
+ * instead of 12 bytes, diagonal values apart), which the CSR-stream kernel
+ * then reads with bit-identical results; the environment variable
+ * TRLHIP_CSR_PACK=0 turns packing off. */
 int trlhip_csr_create(trlhip_csr **A, int n, long nnz, const int *row_ptr,
                       const int *col_idx, const void *values, int vtype);
+
+/* 1 when A's CSR-stream SpMV reads packed nonzeros, else 0.  No
+ * counterpart in the reference. */
+int trlhip_csr_is_packed(const trlhip_csr *A);
 
 /* SpMV autotuner: times the CSR-stream kernel (row blocks) and the vector
  * kernel at 2, 4, 8, 16 and 32 lanes per row on A itself (reps launches

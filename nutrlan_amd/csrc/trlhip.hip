@@ -22,6 +22,7 @@
 #include <new>
 #include <vector>
 #include <type_traits>
+#include <unordered_map>
 
 #include "trlan_internal.h"
 #include "trlhip.h"
@@ -203,13 +204,20 @@ __global__ __launch_bounds__(kBlock) void k_colsum(const double *__restrict__ pa
 // FUSE: x is fl(x_src * xscale) formed on the fly from the unscaled
 // source (see spmv_vec_body); the row owner also stores it to xq (and its
 // FP32 rounding to xq32).
-template <typename S, typename SV, int E, bool FUSE>
+// PK (real only): the nonzeros come packed instead of ci/val, one 32-bit
+// word each: bits 31..24 a value index (255: the diagonal value dg[col],
+// else tab[index]), bits 23..0 the column minus the row block's first row
+// (signed).  The decoded values are the CSR's values bit for bit, so the
+// products and sums are the unpacked kernel's.
+template <typename S, typename SV, int E, bool FUSE, bool PK = false>
 __device__ __forceinline__ void spmv_stream_body(
     int nb, const int *__restrict__ rb, const int *__restrict__ rp, const int *__restrict__ ci,
     const SV *__restrict__ val, const S *__restrict__ x, S *__restrict__ y,
     double *__restrict__ part, double xscale, S *__restrict__ xq, float *__restrict__ xq32, int bid,
-    int nblk)
+    int nblk, const uint32_t *__restrict__ pk = nullptr, const double *__restrict__ dg = nullptr,
+    const double *__restrict__ tab = nullptr)
 {
+    static_assert(!PK || E == 1, "packed nonzeros are real");
     __shared__ double prod[kNnzBlk * E];
     auto xv = [&](int c, int e) -> double {
         if constexpr (FUSE)
@@ -230,13 +238,27 @@ __device__ __forceinline__ void spmv_stream_body(
     for (int blk = bid; blk < nb; blk += nblk) {
         const int r0 = rb[blk], r1 = rb[blk + 1];
         const int k0 = rp[r0], nz = rp[r1] - k0;
+        // nonzero k of the block: its column and (real) value
+        auto nzr = [&](int k, int &c) -> double {
+            if constexpr (PK) {
+                const uint32_t w = pk[k0 + k];
+                c = r0 + ((int)(w << 8) >> 8);
+                const uint32_t vi = w >> 24;
+                return vi == 255u ? dg[c] : tab[vi];
+            } else {
+                c = ci[k0 + k];
+                return (double)val[k0 + k];
+            }
+        };
         if (r1 - r0 > 1 || nz <= kNnzBlk) {
 #pragma unroll 4
             for (int k = threadIdx.x; k < nz; k += kBlock) {
-                const int c = ci[k0 + k];
                 if constexpr (E == 1) {
-                    prod[k] = (double)val[k0 + k] * xv(c, 0);
+                    int c;
+                    const double v = nzr(k, c);
+                    prod[k] = v * xv(c, 0);
                 } else {
+                    const int c = ci[k0 + k];
                     const double ar = val[2 * (k0 + k)], ai = val[2 * (k0 + k) + 1];
                     const double xr = xv(c, 0), xi = xv(c, 1);
                     prod[2 * k] = ar * xr - ai * xi;
@@ -269,10 +291,12 @@ __device__ __forceinline__ void spmv_stream_body(
         } else {
             double sr = 0.0, si = 0.0;
             for (int k = threadIdx.x; k < nz; k += kBlock) {
-                const int c = ci[k0 + k];
                 if constexpr (E == 1) {
-                    sr += (double)val[k0 + k] * xv(c, 0);
+                    int c;
+                    const double v = nzr(k, c);
+                    sr += v * xv(c, 0);
                 } else {
+                    const int c = ci[k0 + k];
                     const double ar = val[2 * (k0 + k)], ai = val[2 * (k0 + k) + 1];
                     sr += ar * xv(c, 0) - ai * xv(c, 1);
                     si += ar * xv(c, 1) + ai * xv(c, 0);
@@ -305,6 +329,18 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
 {
     spmv_stream_body<S, SV, E, FUSE>(nb, rb, rp, ci, val, x, y, part, xscale, xq, xq32, blockIdx.x,
                                      gridDim.x);
+}
+
+// CSR-stream SpMV over packed nonzeros (real).
+template <bool FUSE>
+__global__ __launch_bounds__(kBlock) void k_spmv_pk(
+    int nb, const int *__restrict__ rb, const int *__restrict__ rp, const uint32_t *__restrict__ pk,
+    const double *__restrict__ dg, const double *__restrict__ tab, const double *__restrict__ x,
+    double *__restrict__ y, double *__restrict__ part, double xscale, double *__restrict__ xq,
+    float *__restrict__ xq32)
+{
+    spmv_stream_body<double, double, 1, FUSE, true>(nb, rb, rp, nullptr, nullptr, x, y, part, xscale,
+                                                    xq, xq32, blockIdx.x, gridDim.x, pk, dg, tab);
 }
 
 // y = A x, vector CSR: LPR lanes per row (a power of two), 64/LPR rows per
@@ -1031,6 +1067,13 @@ struct trlhip_csr {
     int *rp;
     int *ci;
     double *val;    // FP64 values (E per nonzero)
+    // packed nonzeros for the CSR-stream kernel (real matrices with at most
+    // 255 distinct off-diagonal values; null otherwise): per nonzero a value
+    // index and the column relative to its row block's first row
+    uint32_t *pk;
+    double *dg;     // diagonal values (n), value index 255
+    double *tab;    // the off-diagonal values, indices 0..254
+    int ntab;
     // staging for host-called matvec
     double *dx, *dy, *part, *dscal;
     hipStream_t s;
@@ -1243,9 +1286,17 @@ size_t part_need(int E, int ncols)
 
 double spmv_bytes(const trlhip_csr *A, int E, size_t vsz, size_t xsz)
 {
-    // values + col_idx + row_ptr + x read once + y written once
+    // values + col_idx + row_ptr + x read once + y written once; packed:
+    // one word per nonzero + row_ptr + the diagonal + x + y
+    if (A->pk && !A->lpr)
+        return 4.0 * (double)A->nnz + 4.0 * (A->n + 1) + 8.0 * A->n + 8.0 * A->ntab +
+               2.0 * (double)xsz * A->n;
     return (double)A->nnz * ((double)vsz * E + 4.0) + 4.0 * (A->n + 1) + 2.0 * (double)xsz * E * A->n;
 }
+
+// The SpMV kernel a real matrix runs: lanes per row of the vector kernel,
+// 0 for CSR-stream, -1 for CSR-stream over packed nonzeros.
+int spmv_kind(const trlhip_csr *A) { return A->lpr ? A->lpr : (A->pk ? -1 : 0); }
 
 int spmv_grid(const trlhip_csr *A)
 {
@@ -1294,6 +1345,15 @@ int launch_spmv(const trlhip_csr *A, int vtype, const void *x, void *y, double *
     const dim3 g(spmv_grid(A)), b(kBlock);
     const double *xs = (const double *)x;
     double *ys = (double *)y, *q = (double *)xq;
+    if (A->pk && !cplx) {
+        if (xq)
+            KLAUNCH((k_spmv_pk<true>), g, b, 0, s, A->nb, A->rb, A->rp, A->pk, A->dg, A->tab, xs, ys,
+                    part, xscale, q, xq32);
+        else
+            KLAUNCH((k_spmv_pk<false>), g, b, 0, s, A->nb, A->rb, A->rp, A->pk, A->dg, A->tab, xs, ys,
+                    part, xscale, q, (float *)nullptr);
+        return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
+    }
     if (cplx) {
         if (xq)
             KLAUNCH((k_spmv<double, double, 2, true>), g, b, 0, s, A->nb, A->rb, A->rp, A->ci,
@@ -1562,6 +1622,8 @@ struct BDesc {
     int n, ld, jnd, np_alpha, g_upd, gx, gy, rm, rc0, nb;
     const int *rb, *rp, *ci;
     const double *val;
+    const uint32_t *pk;   // packed nonzeros (kb_spmv_pk), with dg and tab
+    const double *dg, *tab;
     const double *x;      // SpMV input: the basis column q_a, or the unscaled scratch column
     double xscale;        // 1, or the deferred normalisation of the scratch column
     double *xq;           // fused store of q_a (null when not fused)
@@ -1611,6 +1673,15 @@ __global__ __launch_bounds__(kBlock) void kb_spmv_stream(const BDesc *__restrict
     BATCH_LOCATE(BP_SPMV)
     spmv_stream_body<double, double, 1, true>(q.nb, q.rb, q.rp, q.ci, q.val, q.x, q.r, q.part_alpha,
                                               q.xscale, q.xq, q.xq32, bid_, nblk_);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_spmv_pk(const BDesc *__restrict__ d, const int *pre,
+                                                    int nreq)
+{
+    BATCH_LOCATE(BP_SPMV)
+    spmv_stream_body<double, double, 1, true, true>(q.nb, q.rb, q.rp, nullptr, nullptr, q.x, q.r,
+                                                    q.part_alpha, q.xscale, q.xq, q.xq32, bid_, nblk_,
+                                                    q.pk, q.dg, q.tab);
 }
 
 template <int LPR>
@@ -1665,7 +1736,7 @@ __global__ __launch_bounds__(kBlock) void kb_sum2(const BDesc *__restrict__ d, c
 // block offsets each; returns the phases' algorithmic bytes and the largest
 // recurrence / projection widths.
 struct GroupInfo {
-    int R, lpr;
+    int R, lpr;  // lpr: the SpMV kernel, lanes per row (0: CSR-stream, -1: packed CSR-stream)
     bool mixed;
     size_t doff, poff;  // offsets into the staging arrays
     double bytes[BP_COUNT];
@@ -1713,6 +1784,9 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
         d.rp = A->rp;
         d.ci = A->ci;
         d.val = A->val;
+        d.pk = A->pk;
+        d.dg = A->dg;
+        d.tab = A->tab;
         d.x = colp<double>(c, st.fuse ? c->ncols : st.qa);
         d.xscale = st.fuse ? st.fuse_s : 1.0;
         d.xq = st.fuse ? colp<double>(c, st.qa) : nullptr;
@@ -1752,6 +1826,7 @@ int batch_run(hipStream_t s, const GroupInfo &gi, const BDesc *dd, const int *dp
     {
         Timed t(TRLHIP_K_BSPMV, s, gi.bytes[BP_SPMV]);
         switch (gi.lpr) {
+        case -1: KLAUNCH(kb_spmv_pk, blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
         case 0: KLAUNCH(kb_spmv_stream, blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
         case 2: KLAUNCH((kb_spmv<2>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
         case 4: KLAUNCH((kb_spmv<4>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
@@ -1807,13 +1882,13 @@ void steps_plan(const trlhip_batch *b, StepsPlan &P)
     for (size_t i = 0; i < b->steps.size(); ++i) {
         if (taken[i]) continue;
         GroupInfo gi{};
-        gi.lpr = b->steps[i].A->lpr;
+        gi.lpr = spmv_kind(b->steps[i].A);
         gi.mixed = b->steps[i].c->V32 != nullptr;
         gi.doff = order.size();
         gi.poff = npre;
         for (size_t k = i; k < b->steps.size(); ++k) {
             const BatchStep &st = b->steps[k];
-            if (!taken[k] && st.A->lpr == gi.lpr && (st.c->V32 != nullptr) == gi.mixed) {
+            if (!taken[k] && spmv_kind(st.A) == gi.lpr && (st.c->V32 != nullptr) == gi.mixed) {
                 order.push_back(st);
                 taken[k] = 1;
             }
@@ -2231,6 +2306,8 @@ const trlhip_csr *trlhip_csr_from_op(void (*op)(void), void *mvparam, int vtype)
 
 int trlhip_csr_is_mixed(const trlhip_csr *A) { return A && A->vtype == TRLHIP_R32; }
 
+int trlhip_csr_is_packed(const trlhip_csr *A) { return A && A->magic == kCsrMagic && A->pk ? 1 : 0; }
+
 int trlhip_device_count(void)
 {
     int n = 0;
@@ -2270,6 +2347,59 @@ static void choose_spmv(trlhip_csr *A, const int *row_ptr)
     set_lpr(A, 0);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Packs the nonzeros of a real CSR matrix for the CSR-stream kernel (see
+// spmv_stream_body): false when it has more than 255 distinct off-diagonal
+// values (compared bit for bit), a row with two diagonal entries, or a
+// column more than 2^23 rows from its row block's first row.
+bool pack_csr(int n, long nnz, const int *rp, const int *ci, const double *val,
+              const std::vector<int> &rb, std::vector<uint32_t> &pk, std::vector<double> &dg,
+              std::vector<double> &tab)
+{
+    pk.assign((size_t)nnz, 0u);
+    dg.assign((size_t)n, 0.0);
+    tab.clear();
+    std::unordered_map<uint64_t, uint32_t> idx;
+    for (size_t blk = 0; blk + 1 < rb.size(); ++blk) {
+        const long r0 = rb[blk];
+        for (int row = rb[blk]; row < rb[blk + 1]; ++row) {
+            bool diag = false;
+            for (long k = rp[row]; k < rp[row + 1]; ++k) {
+                const long off = (long)ci[k] - r0;
+                if (off < -(1L << 23) || off >= (1L << 23)) return false;
+                uint32_t vi;
+                if (ci[k] == row) {
+                    if (diag) return false;
+                    diag = true;
+                    dg[(size_t)row] = val[k];
+                    vi = 255u;
+                } else {
+                    uint64_t bits;
+                    memcpy(&bits, &val[k], sizeof bits);
+                    auto it = idx.find(bits);
+                    if (it == idx.end()) {
+                        if (tab.size() >= 255) return false;
+                        vi = (uint32_t)tab.size();
+                        idx.emplace(bits, vi);
+                        tab.push_back(val[k]);
+                    } else {
+                        vi = it->second;
+                    }
+                }
+                pk[(size_t)k] = (vi << 24) | ((uint32_t)off & 0xFFFFFFu);
+            }
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
 int trlhip_csr_create(trlhip_csr **out, int n, long nnz, const int *row_ptr, const int *col_idx,
                       const void *values, int vtype)
 {
@@ -2302,6 +2432,25 @@ int trlhip_csr_create(trlhip_csr **out, int n, long nnz, const int *row_ptr, con
     if (!rc && hipMemcpy(A->rp, row_ptr, sizeof(int) * (size_t)(n + 1), hipMemcpyHostToDevice) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && nnz > 0 && hipMemcpy(A->ci, col_idx, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess) rc = TRLHIP_EHIP;
     if (!rc && nnz > 0 && hipMemcpy(A->val, values, sizeof(double) * A->E * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess) rc = TRLHIP_EHIP;
+    // packed nonzeros for the CSR-stream kernel when the values allow
+    // (TRLHIP_CSR_PACK=0: never)
+    const char *ep = getenv("TRLHIP_CSR_PACK");
+    if (!rc && A->E == 1 && nnz > 0 && !(ep && ep[0] == '0')) {
+        std::vector<uint32_t> pk;
+        std::vector<double> dg, tab;
+        if (pack_csr(n, nnz, row_ptr, col_idx, (const double *)values, rb, pk, dg, tab)) {
+            A->ntab = (int)tab.size();
+            if (tab.empty()) tab.push_back(0.0);
+            if (hipMalloc((void **)&A->pk, sizeof(uint32_t) * pk.size()) != hipSuccess ||
+                hipMalloc((void **)&A->dg, sizeof(double) * dg.size()) != hipSuccess ||
+                hipMalloc((void **)&A->tab, sizeof(double) * tab.size()) != hipSuccess)
+                rc = TRLHIP_ENOMEM;
+            if (!rc && (hipMemcpy(A->pk, pk.data(), sizeof(uint32_t) * pk.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(A->dg, dg.data(), sizeof(double) * dg.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(A->tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess))
+                rc = TRLHIP_EHIP;
+        }
+    }
     if (rc) {
         trlhip_csr_destroy(A);
         return rc;
@@ -2362,6 +2511,9 @@ int trlhip_csr_destroy(trlhip_csr *A)
     if (A->rp) (void)hipFree(A->rp);
     if (A->ci) (void)hipFree(A->ci);
     if (A->val) (void)hipFree(A->val);
+    if (A->pk) (void)hipFree(A->pk);
+    if (A->dg) (void)hipFree(A->dg);
+    if (A->tab) (void)hipFree(A->tab);
     if (A->dx) (void)hipFree(A->dx);
     if (A->dy) (void)hipFree(A->dy);
     if (A->part) (void)hipFree(A->part);

@@ -148,6 +148,15 @@ class Csr:
         self.handle = h
         self.op = fn_addr("trlhip_csr_zmatvec" if complex_ else "trlhip_csr_matvec")
 
+    @property
+    def packed(self):
+        """True when the CSR-stream SpMV reads this matrix's packed nonzeros
+        (trlhip_csr_is_packed)."""
+        L = lib()
+        L.trlhip_csr_is_packed.argtypes = [ctypes.c_void_p]
+        L.trlhip_csr_is_packed.restype = ctypes.c_int
+        return bool(L.trlhip_csr_is_packed(self.handle))
+
     def autotune(self, reps=10):
         """Time the SpMV kernels on this matrix and keep the fastest
         (trlhip_csr_autotune); returns (lanes per row, 0 = CSR-stream;

@@ -247,3 +247,87 @@ def test_autotuned_solve_matches_oracle(gpu_lib):
     err = np.abs(np.sort(g["eval"][:8]) - np.sort(o["eval"][:8])).max() / anrm
     print(f"autotuned (lanes {lpr}) vs oracle: max |dlambda|/||A|| {err:.1e}", flush=True)
     assert err <= 1e-10
+
+
+def _csr_pair(T, rp, ci, v, monkeypatch, mixed=False):
+    """The same matrix as a packed operator (default) and an unpacked one
+    (TRLHIP_CSR_PACK=0 at creation)."""
+    A1 = T.Csr(rp, ci, v, mixed=mixed)
+    monkeypatch.setenv("TRLHIP_CSR_PACK", "0")
+    A0 = T.Csr(rp, ci, v, mixed=mixed)
+    monkeypatch.delenv("TRLHIP_CSR_PACK")
+    return A1, A0
+
+
+@pytest.mark.parametrize("builder", ["lap", "and2d", "diag"])
+def test_packed_spmv_bitwise(gpu_lib, builder, monkeypatch):
+    """Matrices with few distinct off-diagonal values get packed nonzeros;
+    the packed CSR-stream SpMV reproduces the unpacked one bit for bit
+    (y and the alpha partial sums), and both match numpy."""
+    T = gpu_lib
+    if builder == "lap":
+        rp, ci, v = W.laplace3d_csr(20, seed=2)
+    elif builder == "and2d":
+        (rp, ci, v), _ = W.anderson2d_csr(150, 7.5, seed=3)
+    else:
+        rp, ci, v = W.diag_csr(5000)
+    n = len(rp) - 1
+    A1, A0 = _csr_pair(T, rp, ci, v, monkeypatch)
+    print(f"packed spmv {builder}: n={n} packed={A1.packed}", flush=True)
+    assert A1.packed and not A0.packed
+    x = W.start_vector(n, seed=11)
+    c = T.Ctx(n, 3, False)
+    c.set(0, x)
+    a1 = c.spmv(A1, 0, 1)
+    a0 = c.spmv(A0, 0, 2)
+    y1, y0 = c.get(1), c.get(2)
+    assert np.array_equal(y1, y0) and a1 == a0
+    y = W.csr_matvec(rp, ci, v)(x)
+    assert np.abs(y1 - y).max() <= 1e-13 * np.abs(y).max()
+    c.close()
+    A1.close()
+    A0.close()
+
+
+def test_packing_declined(gpu_lib):
+    """More than 255 distinct off-diagonal values: no packed copy."""
+    T = gpu_lib
+    rp, ci, v = W.random_sym_csr(3000, seed=5)
+    A = T.Csr(rp, ci, v)
+    assert not A.packed
+    A.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_packed_solves_bitwise(gpu_lib, mixed, monkeypatch):
+    """Whole solves on packed and unpacked operators agree bit for bit:
+    single (trlan) and batched (trlan_batch, kb_spmv_pk against
+    kb_spmv_stream), plain FP64 and mixed precision."""
+    T = gpu_lib
+    rp, ci, v = W.laplace3d_csr(16, seed=4)
+    n = len(rp) - 1
+    x0 = W.start_vector(n, seed=12)
+    A1, A0 = _csr_pair(T, rp, ci, v, monkeypatch, mixed=mixed)
+    print(f"packed solves mixed={mixed} n={n}", flush=True)
+    g1 = T.solve_csr(A1, -1, 6, 30, 1e-9, 1, 20000, 6, x0)
+    g0 = T.solve_csr(A0, -1, 6, 30, 1e-9, 1, 20000, 6, x0)
+    assert g1["nec"] == g0["nec"] == 6
+    assert g1["matvec"] == g0["matvec"] and g1["nloop"] == g0["nloop"]
+    assert np.array_equal(g1["eval"][:6], g0["eval"][:6])
+    lat = [W.anderson2d_csr(L, 6.0, seed=20 + L)[0] for L in (48, 64, 70)]
+    ops1, ops0 = [], []
+    for (r, c_, vv) in lat:
+        a1, a0 = _csr_pair(T, r, c_, vv, monkeypatch, mixed=mixed)
+        assert a1.packed
+        ops1.append(a1)
+        ops0.append(a0)
+    probs = lambda ops: [(A, -1, 4, 24, 1e-8, 1, 20000, 4,
+                          W.start_vector(len(lat[i][0]) - 1, seed=30 + i)) for i, A in enumerate(ops)]
+    rc1, r1, _ = T.solve_batch(probs(ops1))
+    rc0, r0, _ = T.solve_batch(probs(ops0))
+    assert rc1 == 0 and rc0 == 0
+    for a, b in zip(r1, r0):
+        assert a["matvec"] == b["matvec"] and a["nec"] == b["nec"] == 4
+        assert np.array_equal(a["eval"][:4], b["eval"][:4])
+    for A in ops1 + ops0 + [A1, A0]:
+        A.close()
