@@ -356,6 +356,7 @@ def batched_anderson2d(args):
         # eigenvalues of the headline path against plain FP64 (same lattices)
         dl = max(float(np.abs(np.sort(a["eval"][:nev]) - np.sort(b["eval"][:nev])).max()
                        / a["anrm"]) for a, b in zip(res, res_p))
+    packed = all(A.packed for A in mixed)
     for A in plain + mixed:
         A.close()
     net.close()
@@ -368,8 +369,8 @@ def batched_anderson2d(args):
     pb = sum(stats[k][2] for k in path)
     pt = sum(stats[k][1] for k in path)
     path_gbs = pb / (pt * 1e-3) / 1e9 if pt > 0 else 0.0
-    kname = {"b_spmv": "kb_spmv_stream", "b_recur": "kb_recur", "b_proj": "kb_proj<float>",
-             "b_update": "kb_update<float>"}[dom]
+    kname = {"b_spmv": "kb_spmv_pk" if packed else "kb_spmv_stream", "b_recur": "kb_recur",
+             "b_proj": "kb_proj<float>", "b_update": "kb_update<float>"}[dom]
     roofline = {
         "bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -503,6 +504,8 @@ def single_solve(T, args):
     pt = sum(stats[k][1] for k in pk)
     path_gbs = pb / (pt * 1e-3) / 1e9 if pt > 0 else 0.0
     kname = None if cplx else KERNEL_NAMES[path].get(dom)
+    if dom == "spmv" and kname and ops[path].packed:
+        kname = "k_spmv_pk<true>"
     roofline = {
         "bound": "hbm", "kernel": kname or dom, "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

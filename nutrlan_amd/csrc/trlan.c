@@ -54,6 +54,19 @@ double trl_rand_uniform(uint64_t *state)
  * timers.  The reference reads clock() (process CPU time); here the work
  * runs on the device, and clock() is a system call on the per-step path,
  * so the timers count monotonic wall time (a vDSO read) in the same unit. */
+/* Host-time counters of batched solves (ns; TRLHIP_BATCH_PROFILE=1 only):
+ * restarts that continue the iteration, the convergence checks inside a
+ * cycle, and the arrowhead reduction at a cycle's start. */
+static int g_prof;
+static long g_ns_restart, g_ns_check, g_ns_tridiag, g_n_restart;
+
+static long lz_ns(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (long)ts.tv_sec * 1000000000L + ts.tv_nsec;
+}
+
 static clock_t lz_clock(void)
 {
     struct timespec ts;
@@ -1065,7 +1078,9 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
         if (info->stat != 0) return;
         memcpy(alfrot, alpha, sizeof(double) * (size_t)locked);
         memset(betrot, 0, sizeof(double) * (size_t)locked);
+        const long tt0 = g_prof ? lz_ns() : 0;
         lz_tridiag(z, jnd - locked, alpha + locked, beta + locked, alfrot + locked, betrot + locked);
+        if (g_prof) __atomic_add_fetch(&g_ns_tridiag, lz_ns() - tt0, __ATOMIC_RELAXED);
         info->flop += 8 * (jnd - locked) * (jnd - locked) * (jnd - locked) / 3;
         betrot[jnd - 1] = beta[jnd - 1];
         /* ---- three-term recurrence until the basis is full */
@@ -1088,6 +1103,7 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
             betrot[jnd - 1] = beta[jnd - 1];
             if (info->stat != 0) return;
             if (info->matvec >= next_test) {
+                const long tc0 = g_prof ? lz_ns() : 0;
                 if ((info->stat = lz_get_eval(z, jnd, locked)) != 0) return;
                 memcpy(eval, z->lam, sizeof(double) * (size_t)DMIN(mev, jnd));
                 lz_convergence_test(z, jnd);
@@ -1098,10 +1114,12 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
                     if (info->maxlan == info->ntot)
                         next_test = (int)ceil(0.5 * (info->maxlan + info->matvec));
                 }
+                if (g_prof) __atomic_add_fetch(&g_ns_check, lz_ns() - tc0, __ATOMIC_RELAXED);
             }
         }
         /* ---- restart or finish */
         clock_t tr = lz_clock();
+        const long tr0 = g_prof ? lz_ns() : 0;
         const int prek = kept;
         const int jml = jnd - locked;
         i2 = kept - locked + 1;
@@ -1164,6 +1182,10 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
                 return;
             }
             rr = jnd;
+            if (g_prof) {
+                __atomic_add_fetch(&g_ns_restart, lz_ns() - tr0, __ATOMIC_RELAXED);
+                __atomic_add_fetch(&g_n_restart, 1, __ATOMIC_RELAXED);
+            }
             if (ckpt && info->matvec >= chkpnt) {
                 lz_write_checkpoint(z, jnd, rr);
                 if (info->stat != 0) return;
@@ -1730,6 +1752,8 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     /* TRLHIP_BATCH_PROFILE=1: host time in the solves vs waiting for the
      * device */
     const char *prof = getenv("TRLHIP_BATCH_PROFILE");
+    g_prof = prof && prof[0] == '1';
+    g_ns_restart = g_ns_check = g_ns_tridiag = g_n_restart = 0;
     double t_co = 0.0, t_wait = 0.0;
     long rounds = 0;
     const double t_loop = lz_now();
@@ -1812,6 +1836,21 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
                 l_band[3], w_band[3]);
         fprintf(stderr, "trlan_batch: threads pinned: %s (%d CPUs in the process's set)\n",
                 pin ? "yes" : "no", ncpu);
+        /* the solves' own timers: restarts (host eigenproblems, recorded
+         * GEMMs, final refinement) and orthogonalization decisions */
+        double sres = 0.0, sorth = 0.0;
+        for (int i = 0; i < njobs; ++i) {
+            const trl_info *in = jobs[i].info;
+            const double r = in->clk_rate > 0 ? (double)in->clk_rate : (double)CLOCKS_PER_SEC;
+            sres += (double)in->clk_res / r + in->tick_r;
+            sorth += (double)in->clk_orth / r + in->tick_h;
+        }
+        fprintf(stderr, "trlan_batch: summed over solves: restart %.3f s, orthogonalization %.3f s\n",
+                sres, sorth);
+        fprintf(stderr, "trlan_batch: host work summed over threads: %ld restarts %.3f s "
+                "(%.1f us each), convergence checks %.3f s, arrowhead reductions %.3f s\n",
+                g_n_restart, 1e-9 * g_ns_restart, g_n_restart ? 1e-3 * g_ns_restart / g_n_restart : 0.0,
+                1e-9 * g_ns_check, 1e-9 * g_ns_tridiag);
         fprintf(stderr, "trlan_batch: idle at round ends mean %.3f s; longest resume %.1f us; "
                 "%ld resumes over 100 us take %.3f s\n", isum / pool.nthr, 1e6 * rmax, nlong, tlong);
         fprintf(stderr,

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -1145,6 +1146,21 @@ namespace {
 struct BDesc;  // defined with the batched kernels
 }
 
+// Lock for the batch's record lists: held for one push_back by the
+// coroutines' threads, so a waiter spins (pause) instead of sleeping in the
+// kernel; a std::mutex under that contention costs futex round trips.
+struct SpinLock {
+    std::atomic<int> f{0};
+    void lock()
+    {
+        for (;;) {
+            if (!f.exchange(1, std::memory_order_acquire)) return;
+            while (f.load(std::memory_order_relaxed)) __builtin_ia32_pause();
+        }
+    }
+    void unlock() { f.store(0, std::memory_order_release); }
+};
+
 struct trlhip_batch {
     hipStream_t s = nullptr;      // the batched launches
     bool own_stream = true;       // false: s belongs to another batch
@@ -1158,7 +1174,7 @@ struct trlhip_batch {
     void (*yield)(void *) = nullptr;
     void *user = nullptr;
     int err = 0;                  // sticky device error of a flush
-    std::mutex mu;                // guards steps (coroutines run on several host threads)
+    SpinLock mu;                  // guards the records (coroutines run on several host threads)
     std::vector<BatchStep> steps;
     std::vector<double> cbuf;     // restart-step coefficients of the recorded steps
     std::vector<BatchOp> ops;     // recorded restart GEMMs and scale-copies
@@ -1526,7 +1542,7 @@ int run_scale_copy(trlhip_ctx *c, int dst, int src, double s)
         op.dst = dst;
         op.src = src;
         op.s = s;
-        std::lock_guard<std::mutex> lk(c->batch->mu);
+        std::lock_guard<SpinLock> lk(c->batch->mu);
         op.phase = c->nrec++;
         c->batch->ops.push_back(op);
         return 0;
@@ -2725,7 +2741,7 @@ int trlhip_col_set_device_f32(trlhip_ctx *c, int col, const float *dsrc)
         op.src = col;
         op.s = 1.0;
         op.src32 = dsrc;
-        std::lock_guard<std::mutex> lk(c->batch->mu);
+        std::lock_guard<SpinLock> lk(c->batch->mu);
         op.phase = c->nrec++;
         c->batch->ops.push_back(op);
         return 0;
@@ -2876,7 +2892,7 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
             return rc;
         }
         {
-            std::lock_guard<std::mutex> lk(c->batch->mu);
+            std::lock_guard<SpinLock> lk(c->batch->mu);
             if (first && jnd > 1) {  // coefficients go with the flush's staging, with
                                      // a slot for alpha (the recurrence reads jnd)
                 st.coff = (long)c->batch->cbuf.size();
@@ -2989,7 +3005,7 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y, int ldy)
         op.jl = jl;
         op.k = k;
         op.ldy = ldy;
-        std::lock_guard<std::mutex> lk(c->batch->mu);
+        std::lock_guard<SpinLock> lk(c->batch->mu);
         op.yoff = c->batch->ybuf.size();
         c->batch->ybuf.insert(c->batch->ybuf.end(), Y, Y + ny);
         op.phase = c->nrec++;
@@ -3047,7 +3063,7 @@ int trlhip_refine(trlhip_ctx *c, const trlhip_csr *A, int k, int avoid, double *
     op.A = A;
     op.avoid = avoid;
     if (c->batch) {  // recorded; the batch flush runs it with the other contexts'
-        std::lock_guard<std::mutex> lk(c->batch->mu);
+        std::lock_guard<SpinLock> lk(c->batch->mu);
         op.phase = c->nrec++;
         c->batch->ops.push_back(op);
     } else {
