@@ -259,16 +259,37 @@ def _csr_pair(T, rp, ci, v, monkeypatch, mixed=False):
     return A1, A0
 
 
-@pytest.mark.parametrize("builder", ["lap", "and2d", "diag"])
+def _band_csr(n, half, seed):
+    """Symmetric band matrix: random diagonal, off-diagonals -1/d at
+    distance d <= half (rows of 2*half + 1 nonzeros)."""
+    rng = np.random.default_rng(seed)
+    rows, cols, vals = [], [], []
+    for d in range(-half, half + 1):
+        i = np.arange(max(0, -d), min(n, n - d))
+        rows.append(i)
+        cols.append(i + d)
+        vals.append(rng.standard_normal(len(i)) if d == 0 else np.full(len(i), -1.0 / abs(d)))
+    r, c, v = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    o = np.lexsort((c, r))
+    r, c, v = r[o], c[o], v[o]
+    rp = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(rp, r + 1, 1)
+    return np.cumsum(rp).astype(np.int32), c.astype(np.int32), v
+
+
+@pytest.mark.parametrize("builder", ["lap", "and2d", "diag", "band"])
 def test_packed_spmv_bitwise(gpu_lib, builder, monkeypatch):
-    """Matrices with few distinct off-diagonal values get packed nonzeros;
-    the packed CSR-stream SpMV reproduces the unpacked one bit for bit
-    (y and the alpha partial sums), and both match numpy."""
+    """Matrices with few distinct off-diagonal values get packed nonzeros
+    (short and long rows, a diagonal-only matrix); the packed CSR-stream
+    SpMV reproduces the unpacked one bit for bit (y and the alpha partial
+    sums), and both match numpy."""
     T = gpu_lib
     if builder == "lap":
         rp, ci, v = W.laplace3d_csr(20, seed=2)
     elif builder == "and2d":
         (rp, ci, v), _ = W.anderson2d_csr(150, 7.5, seed=3)
+    elif builder == "band":
+        rp, ci, v = _band_csr(20000, 12, seed=4)
     else:
         rp, ci, v = W.diag_csr(5000)
     n = len(rp) - 1
