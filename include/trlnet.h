@@ -51,6 +51,15 @@ int trlnet_forward_many(trlnet *net, int ngroups, const int *nimg, const int *h,
  * that go straight into a batch (trl_batch_job.dstart). */
 int trlnet_forward_many_device(trlnet *net, int ngroups, const int *nimg, const int *h,
                                const int *w, const float *in, const float **out_device);
+
+/* trlnet_forward_many_device with the inputs taken from FP64 images, one
+ * pointer per image (groups back to back, nimg[g] images of h[g] x w[g]
+ * each): the library forms fl32(img * scale) in its pinned staging buffer
+ * on several host threads, so the caller does no conversion or copy. */
+int trlnet_forward_many_device_f64(trlnet *net, int ngroups, const int *nimg, const int *h,
+                                   const int *w, const double *const *img, double scale,
+                                   const float **out_device);
+
 /* Loss and its gradient w.r.t. the parameters (no update). */
 int trlnet_loss_grad(trlnet *net, int nimg, int h, int w, const float *in,
                      const float *target, float *loss, float *grad);

@@ -17,11 +17,13 @@
 // D[4*(l/16) + i][l%16] in accumulator element i.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "trlnet.h"
@@ -689,24 +691,35 @@ int trlnet_forward_many(trlnet *t, int ngroups, const int *nimg, const int *h, c
     return 0;
 }
 
-int trlnet_forward_many_device(trlnet *t, int ngroups, const int *nimg, const int *h, const int *w,
-                               const float *in, const float **out_device)
+}  // extern "C"
+
+namespace {
+
+// Validates the groups, sizes the device buffers and the pinned staging
+// for `total` input floats.
+int many_prepare(trlnet *t, int ngroups, const int *nimg, const int *h, const int *w, long *total)
 {
-    if (!t || ngroups < 1 || !nimg || !h || !w || !in || !out_device) return TRLNET_EINVAL;
-    long total = 0;
+    long tot = 0;
     for (int i = 0; i < ngroups; ++i) {
         if (!geo_ok(nimg[i], h[i], w[i])) return TRLNET_EINVAL;
-        total += (long)nimg[i] * h[i] * w[i];
+        tot += (long)nimg[i] * h[i] * w[i];
     }
-    if (int rc = ensure(t, total, 1)) return rc;
-    if (total > t->hcap) {
+    if (int rc = ensure(t, tot, 1)) return rc;
+    if (tot > t->hcap) {
         if (t->hstage) (void)hipHostFree(t->hstage);
         t->hstage = nullptr;
         t->hcap = 0;
-        NTRY(hipHostMalloc((void **)&t->hstage, sizeof(float) * total, 0));
-        t->hcap = total;
+        NTRY(hipHostMalloc((void **)&t->hstage, sizeof(float) * tot, 0));
+        t->hcap = tot;
     }
-    memcpy(t->hstage, in, sizeof(float) * total);
+    *total = tot;
+    return 0;
+}
+
+// Uploads the staged inputs and runs every group; outputs stay on the device.
+int many_run(trlnet *t, int ngroups, const int *nimg, const int *h, const int *w, long total,
+             const float **out_device)
+{
     NTRY(hipMemcpyAsync(t->x, t->hstage, sizeof(float) * total, hipMemcpyHostToDevice, t->s));
     long off = 0;
     for (int i = 0; i < ngroups; ++i) {
@@ -717,6 +730,60 @@ int trlnet_forward_many_device(trlnet *t, int ngroups, const int *nimg, const in
     NTRY(hipStreamSynchronize(t->s));
     *out_device = t->o;
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int trlnet_forward_many_device(trlnet *t, int ngroups, const int *nimg, const int *h, const int *w,
+                               const float *in, const float **out_device)
+{
+    if (!t || ngroups < 1 || !nimg || !h || !w || !in || !out_device) return TRLNET_EINVAL;
+    long total = 0;
+    if (int rc = many_prepare(t, ngroups, nimg, h, w, &total)) return rc;
+    memcpy(t->hstage, in, sizeof(float) * total);
+    return many_run(t, ngroups, nimg, h, w, total, out_device);
+}
+
+int trlnet_forward_many_device_f64(trlnet *t, int ngroups, const int *nimg, const int *h,
+                                   const int *w, const double *const *img, double scale,
+                                   const float **out_device)
+{
+    if (!t || ngroups < 1 || !nimg || !h || !w || !img || !out_device) return TRLNET_EINVAL;
+    long total = 0;
+    if (int rc = many_prepare(t, ngroups, nimg, h, w, &total)) return rc;
+    // image i: its pointer, size and offset in the staging buffer
+    std::vector<const double *> src;
+    std::vector<long> len, off;
+    long o = 0;
+    for (int g = 0; g < ngroups; ++g)
+        for (int k = 0; k < nimg[g]; ++k) {
+            const long np = (long)h[g] * w[g];
+            if (!img[src.size()]) return TRLNET_EINVAL;
+            src.push_back(img[src.size()]);
+            len.push_back(np);
+            off.push_back(o);
+            o += np;
+        }
+    // fl32(x * scale) into the staging, images dealt to up to 16 threads
+    // by a shared counter
+    const int nimgs = (int)src.size();
+    unsigned hw = std::thread::hardware_concurrency();
+    const int nthr = (int)std::min<long>(std::min<unsigned>(hw ? hw : 1, 16), std::max(1, nimgs));
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int i; (i = next.fetch_add(1, std::memory_order_relaxed)) < nimgs;) {
+            const double *p = src[i];
+            float *q = t->hstage + off[i];
+            for (long j = 0; j < len[i]; ++j) q[j] = (float)(p[j] * scale);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int k = 1; k < nthr; ++k) th.emplace_back(work);
+    work();
+    for (auto &x : th) x.join();
+    return many_run(t, ngroups, nimg, h, w, total, out_device);
 }
 
 int trlnet_loss_grad(trlnet *t, int nimg, int h, int w, const float *in, const float *target,

@@ -40,10 +40,13 @@ def _lib():
         L.trlnet_forward.argtypes = [_P, _I, _I, _I, _P, _P]
         L.trlnet_forward_many.argtypes = [_P, _I, _P, _P, _P, _P, _P]
         L.trlnet_forward_many_device.argtypes = [_P, _I, _P, _P, _P, _P, ctypes.POINTER(_P)]
+        L.trlnet_forward_many_device_f64.argtypes = [_P, _I, _P, _P, _P, _P, ctypes.c_double,
+                                                     ctypes.POINTER(_P)]
         L.trlnet_loss_grad.argtypes = [_P, _I, _I, _I, _P, _P, ctypes.POINTER(_F), _P]
         L.trlnet_train_step.argtypes = [_P, _I, _I, _I, _P, _P, _F, ctypes.POINTER(_F)]
         for f in ("trlnet_create", "trlnet_destroy", "trlnet_get_params", "trlnet_set_params",
                   "trlnet_forward", "trlnet_forward_many", "trlnet_forward_many_device",
+                  "trlnet_forward_many_device_f64",
                   "trlnet_loss_grad", "trlnet_train_step"):
             getattr(L, f).restype = _I
         L._trlnet_bound = True
@@ -114,6 +117,22 @@ class Net:
                                                     hh.ctypes.data, ww.ctypes.data,
                                                     flat.ctypes.data, ctypes.byref(dptr)),
                   "forward_many_device")
+        return dptr.value
+
+    def forward_images_device(self, dims, images, scale):
+        """forward_flat_device on fl32(image * scale) of FP64 images (one
+        array per image, in group order), converted by the library into its
+        pinned staging on several threads (trlnet_forward_many_device_f64)."""
+        dims = np.ascontiguousarray(dims, dtype=np.int32)
+        imgs = [np.ascontiguousarray(a, dtype=np.float64) for a in images]
+        ptrs = (_P * len(imgs))(*[a.ctypes.data for a in imgs])
+        ni, hh, ww = (np.ascontiguousarray(dims[:, k]) for k in range(3))
+        dptr = _P()
+        self._chk(_lib().trlnet_forward_many_device_f64(self.h, len(dims), ni.ctypes.data,
+                                                        hh.ctypes.data, ww.ctypes.data,
+                                                        ctypes.cast(ptrs, _P), float(scale),
+                                                        ctypes.byref(dptr)),
+                  "forward_many_device_f64")
         return dptr.value
 
     def forward_many(self, groups):
@@ -239,8 +258,12 @@ def start_vectors_device(net, potentials):
     """Device addresses of the start vectors (FP32, one per lattice, in the
     order of `potentials`) for trl_batch_job.dstart: the net's outputs stay
     on the device (valid until the net's next call)."""
-    groups, dims, flat = _flat_inputs(potentials)
-    base = net.forward_flat_device(dims, flat)
+    by_shape = {}
+    for i, d in enumerate(potentials):
+        by_shape.setdefault(d.shape, []).append(i)
+    groups = list(by_shape.values())
+    dims = [(len(idx),) + potentials[idx[0]].shape for idx in groups]
+    base = net.forward_images_device(dims, [potentials[i] for idx in groups for i in idx], SCALE)
     out = [None] * len(potentials)
     o = 0
     for idx in groups:

@@ -83,9 +83,10 @@ def test_batch_mixed_operators_and_early_exit(gpu_lib):
 
 
 def test_batch_device_start_vectors(gpu_lib):
-    """Start vectors handed over on the device (trl_batch_job.dstart, FP32)
-    give exactly the solves that the same FP32 values give from the host
-    (evec(:, 0), converted to FP64)."""
+    """Start vectors handed over on the device (trl_batch_job.dstart, FP32;
+    the library converts the FP64 potentials into the net's input itself,
+    trlnet_forward_many_device_f64) give exactly the solves that the host
+    path's FP32 values give from evec(:, 0) (converted to FP64)."""
     from nutrlan_amd import learned as LN
 
     T = gpu_lib
