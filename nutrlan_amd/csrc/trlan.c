@@ -1762,9 +1762,33 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     double t_band[4] = {0, 0, 0, 0}, w_band[4] = {0, 0, 0, 0}, l_band[4] = {0, 0, 0, 0};
     long n_band[4] = {0, 0, 0, 0};
     int last_band[4] = {0, 0, 0, 0};
+    /* When the host is the bottleneck (the next lane's last launch has
+     * already finished), a lane's recorded work is launched while the
+     * workers already step the next lane: this thread issues the launch
+     * (about a dozen HIP calls) after the round's start barrier and then
+     * joins the round.  When the device is (that launch still runs), the
+     * launch is issued before waiting, so the device never runs dry.
+     * Launches stay in lane order, each after the one before it. */
+    int deferred = -1;
+#define LZ_LAUNCH(L)                                                                   \
+    do {                                                                               \
+        const double tl_ = lz_now();                                                   \
+        if (serial && nlanes > 1) {                                                    \
+            const int orc_ = trlhip_batch_after(bl[L], bl[((L) + nlanes - 1) % nlanes]); \
+            if (orc_ && !rc) rc = orc_;                                                \
+        }                                                                              \
+        const int lrc_ = trlhip_batch_launch(bl[L]);                                   \
+        if (lrc_ && !rc) rc = lrc_;                                                    \
+        pending[L] = 1;                                                                \
+        l_band[last_band[L]] += lz_now() - tl_;                                        \
+    } while (0)
     for (;;) {
         int any = 0;
         for (int l = 0; l < nlanes; ++l) {
+            if (deferred == l || (deferred >= 0 && pending[l] && !trlhip_batch_ready(bl[l]))) {
+                LZ_LAUNCH(deferred);
+                deferred = -1;
+            }
             if (pending[l]) {
                 /* the lane's last launch must finish before its solves resume */
                 const double tw = lz_now();
@@ -1781,6 +1805,10 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
             pool.lane = l;
             lz_shares(&pool, l);
             if (pool.nthr > 1) lz_spin_wait(&pool.start);
+            if (deferred >= 0) {
+                LZ_LAUNCH(deferred);
+                deferred = -1;
+            }
             lz_round(&pool.w[0]);
             const double tf = lz_now();
             if (pool.nthr > 1) lz_spin_wait(&pool.stop);
@@ -1794,23 +1822,14 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
             t_co += lz_now() - t0;
             ++rounds;
             const int band = active >= 256 ? 0 : active >= 32 ? 1 : active >= 4 ? 2 : 3;
-            const double tl = lz_now();
-            if (live[l] > 0) {
-                if (serial && nlanes > 1) {
-                    const int orc = trlhip_batch_after(bl[l], bl[(l + nlanes - 1) % nlanes]);
-                    if (orc && !rc) rc = orc;
-                }
-                const int lrc = trlhip_batch_launch(bl[l]);
-                if (lrc && !rc) rc = lrc;
-                pending[l] = 1;
-            }
+            if (live[l] > 0) deferred = l;
             t_band[band] += lz_now() - t0;
-            l_band[band] += lz_now() - tl;
             ++n_band[band];
             last_band[l] = band;
         }
         if (!any) break;
     }
+#undef LZ_LAUNCH
     if (pool.nthr > 1) {
         pool.quit = 1;
         lz_spin_wait(&pool.start);
@@ -1828,7 +1847,7 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
             tlong += pool.w[t].tlong;
             nlong += pool.w[t].nlong;
         }
-        fprintf(stderr, "trlan_batch: lane rounds by solves stepped (count, host+launch s, of which "
+        fprintf(stderr, "trlan_batch: lane rounds by solves stepped (count, round s, its lane's "
                 "launch s, wait s): >=256 %ld %.3f %.3f %.3f; 32-255 %ld %.3f %.3f %.3f; 4-31 %ld "
                 "%.3f %.3f %.3f; 1-3 %ld %.3f %.3f %.3f\n",
                 n_band[0], t_band[0], l_band[0], w_band[0], n_band[1], t_band[1], l_band[1],

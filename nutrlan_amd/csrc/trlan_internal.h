@@ -48,6 +48,9 @@ int trlhip_batch_flush(trlhip_batch *b);
  * wait for it; the batch's contexts may resume only after the wait. */
 int trlhip_batch_launch(trlhip_batch *b);
 int trlhip_batch_wait(trlhip_batch *b);
+/* 1 when the batch's last launch has finished on the device (or failed),
+ * 0 while it runs; never blocks. */
+int trlhip_batch_ready(trlhip_batch *b);
 /* Work b queues from now on starts on the device after prev's last launch
  * (lanes on their own streams that still run one at a time). */
 int trlhip_batch_after(trlhip_batch *b, trlhip_batch *prev);

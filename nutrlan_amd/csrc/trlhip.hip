@@ -3249,6 +3249,12 @@ int trlhip_batch_wait(trlhip_batch *b)
     return rc;
 }
 
+int trlhip_batch_ready(trlhip_batch *b)
+{
+    if (!b || b->err) return 1;
+    return hipEventQuery(b->done) == hipErrorNotReady ? 0 : 1;
+}
+
 int trlhip_batch_flush(trlhip_batch *b)
 {
     const int rc = trlhip_batch_launch(b);
