@@ -111,3 +111,33 @@ def test_batch_device_start_vectors(gpu_lib):
         for A in ops:
             A.close()
     net.close()
+
+
+def test_batch_heterogeneous_groups(gpu_lib, monkeypatch):
+    """One batch whose steps fall into four launch groups (packed and
+    unpacked CSR-stream SpMV, FP64 and mixed-precision bases) staged in one
+    transfer: every job still equals its single solve bit for bit."""
+    T = gpu_lib
+    a = W.anderson2d_csr(40, 6.0, seed=5)[0]
+    b = W.random_sym_csr(3000, seed=6)
+    c = W.laplace3d_csr(12, seed=7)
+    d = W.anderson2d_csr(48, 9.0, seed=8)[0]
+    e = W.random_sym_csr(2500, seed=9)
+    ops = [T.Csr(*a), T.Csr(*b), T.Csr(*c, mixed=True)]
+    monkeypatch.setenv("TRLHIP_CSR_PACK", "0")
+    ops += [T.Csr(*d, mixed=True), T.Csr(*e, mixed=True)]
+    monkeypatch.delenv("TRLHIP_CSR_PACK")
+    assert [A.packed for A in ops] == [True, False, True, False, False]
+    probs = [(A, -1, 5, 30, 1e-8, 1, 20000, 5, W.start_vector(A.n, seed=40 + i))
+             for i, A in enumerate(ops)]
+    print("heterogeneous batch: n =", [A.n for A in ops], flush=True)
+    rc, res, wall = T.solve_batch(probs)
+    assert rc == 0
+    for p, g in zip(probs, res):
+        s = T.solve_csr(*p)
+        assert g["stat"] == 0 and s["stat"] == 0 and g["nec"] == s["nec"] == 5
+        assert (g["matvec"], g["nloop"], g["north"]) == (s["matvec"], s["nloop"], s["north"])
+        assert np.array_equal(g["eval"], s["eval"])
+        assert np.array_equal(g["evec"], s["evec"])
+    for A in ops:
+        A.close()
