@@ -51,6 +51,8 @@ KERNEL_NAMES = {
               "spmv": "k_spmv<double, double, 1, true>"},
     "mixed": {"proj": "k_proj<float, double, 1, 8, 4>", "update": "k_update<float, double, 1, 4>",
               "spmv": "k_spmv<double, double, 1, true>"},
+    "complex": {"proj": "k_proj<double, double, 2, 4, 4>", "update": "k_update<double, double, 2, 4>",
+                "spmv": "k_spmv<double, double, 2, true>"},
 }
 
 
@@ -100,6 +102,8 @@ SINGLE = {
                    maxlan=128, tol=1e-8, restart=1, maxmv=40000, cplx=False, traffic=None),
 }
 SINGLE["laplace3d"]["traffic"] = "hbm_traffic_laplace3d_128_fp64.txt"
+SINGLE["anderson3d-c"]["traffic"] = "hbm_traffic_anderson3d_96_c128.txt"
+SINGLE["rand8m"]["traffic"] = "hbm_traffic_rand8m.txt"
 
 
 def single_problem(workload):
@@ -503,7 +507,7 @@ def single_solve(T, args):
     pb = sum(stats[k][2] for k in pk)
     pt = sum(stats[k][1] for k in pk)
     path_gbs = pb / (pt * 1e-3) / 1e9 if pt > 0 else 0.0
-    kname = None if cplx else KERNEL_NAMES[path].get(dom)
+    kname = KERNEL_NAMES["complex" if cplx else path].get(dom)
     if dom == "spmv" and kname and ops[path].packed:
         kname = "k_spmv_pk<true>"
     roofline = {
