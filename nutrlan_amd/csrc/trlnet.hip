@@ -94,12 +94,12 @@ __device__ inline bf16x8 zero8() { return __builtin_bit_cast(bf16x8, uint4{0, 0,
 
 // Layer 1: one input channel (FP32 potential, rounded to BF16), K = the 9
 // taps padded to 32.  out = relu(W1 * x + b1), BF16, 32 channels.
-__global__ __launch_bounds__(NB) void k_fwd1(Geo g, const float *__restrict__ x,
-                                             const bf16_t *__restrict__ w1, const float *__restrict__ b1,
-                                             bf16_t *__restrict__ out)
+__device__ __forceinline__ void fwd1_body(const Geo &g, long bid, const float *__restrict__ x,
+                                          const bf16_t *__restrict__ w1, const float *__restrict__ b1,
+                                          bf16_t *__restrict__ out)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const long p0 = ((long)blockIdx.x * 4 + wv) * 16;
+    const long p0 = (bid * 4 + wv) * 16;
     if (p0 >= g.np) return;
     const int m = lane & 15, kq = lane >> 4;
     const long p = p0 + m;
@@ -124,13 +124,20 @@ __global__ __launch_bounds__(NB) void k_fwd1(Geo g, const float *__restrict__ x,
     }
 }
 
+__global__ __launch_bounds__(NB) void k_fwd1(Geo g, const float *__restrict__ x,
+                                             const bf16_t *__restrict__ w1, const float *__restrict__ b1,
+                                             bf16_t *__restrict__ out)
+{
+    fwd1_body(g, blockIdx.x, x, w1, b1, out);
+}
+
 // Layers 2 and 3: 32 -> 32 channels, K = 9 taps x 32 channels.
-__global__ __launch_bounds__(NB) void k_fwd32(Geo g, const bf16_t *__restrict__ in,
-                                              const bf16_t *__restrict__ w, const float *__restrict__ bias,
-                                              bf16_t *__restrict__ out)
+__device__ __forceinline__ void fwd32_body(const Geo &g, long bid, const bf16_t *__restrict__ in,
+                                           const bf16_t *__restrict__ w, const float *__restrict__ bias,
+                                           bf16_t *__restrict__ out)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const long p0 = ((long)blockIdx.x * 4 + wv) * 16;
+    const long p0 = (bid * 4 + wv) * 16;
     if (p0 >= g.np) return;
     const int m = lane & 15, kq = lane >> 4;
     const long p = p0 + m;
@@ -152,13 +159,20 @@ __global__ __launch_bounds__(NB) void k_fwd32(Geo g, const bf16_t *__restrict__ 
     }
 }
 
+__global__ __launch_bounds__(NB) void k_fwd32(Geo g, const bf16_t *__restrict__ in,
+                                              const bf16_t *__restrict__ w, const float *__restrict__ bias,
+                                              bf16_t *__restrict__ out)
+{
+    fwd32_body(g, blockIdx.x, in, w, bias, out);
+}
+
 // Layer 4: 32 -> 1 channel (N padded to 16, column 0 real), FP32 output.
-__global__ __launch_bounds__(NB) void k_fwd4(Geo g, const bf16_t *__restrict__ in,
-                                             const bf16_t *__restrict__ w4, const float *__restrict__ b4,
-                                             float *__restrict__ out)
+__device__ __forceinline__ void fwd4_body(const Geo &g, long bid, const bf16_t *__restrict__ in,
+                                          const bf16_t *__restrict__ w4, const float *__restrict__ b4,
+                                          float *__restrict__ out)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const long p0 = ((long)blockIdx.x * 4 + wv) * 16;
+    const long p0 = (bid * 4 + wv) * 16;
     if (p0 >= g.np) return;
     const int m = lane & 15, kq = lane >> 4;
     const long p = p0 + m;
@@ -173,6 +187,67 @@ __global__ __launch_bounds__(NB) void k_fwd4(Geo g, const bf16_t *__restrict__ i
             const long q = p0 + 4 * kq + i;
             if (q < g.np) out[q] = acc[i] + b4[0];
         }
+}
+
+__global__ __launch_bounds__(NB) void k_fwd4(Geo g, const bf16_t *__restrict__ in,
+                                             const bf16_t *__restrict__ w4, const float *__restrict__ b4,
+                                             float *__restrict__ out)
+{
+    fwd4_body(g, blockIdx.x, in, w4, b4, out);
+}
+
+// Groups of images of different sizes in one launch per layer: group i
+// (geometry gg[i].g, first pixel gg[i].off) owns blocks [pre[i], pre[i+1]).
+struct GroupGeo {
+    Geo g;
+    long off;
+};
+
+// The group owning block blk (pre increasing, pre[0] = 0): each wave
+// narrows the range 64-fold per step with a ballot over 64 candidates.
+__device__ __forceinline__ int group_of(const int *__restrict__ pre, int ng, int blk)
+{
+    const int lane = threadIdx.x & 63;
+    int lo = 0, hi = ng - 1;
+    while (lo < hi) {
+        const int step = (hi - lo + 64) / 64;
+        const int k = lo + lane * step;
+        const unsigned long long m = __ballot(k <= hi && pre[k] <= blk);
+        lo += (63 - __clzll(m)) * step;
+        hi = min(hi, lo + step - 1);
+        if (step == 1) break;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(NB) void k_fwd1_many(const GroupGeo *__restrict__ gg, const int *__restrict__ pre,
+                                                  int ng, const float *__restrict__ x,
+                                                  const bf16_t *__restrict__ w1, const float *__restrict__ b1,
+                                                  bf16_t *__restrict__ out)
+{
+    const int i = group_of(pre, ng, blockIdx.x);
+    const GroupGeo q = gg[i];
+    fwd1_body(q.g, blockIdx.x - pre[i], x + q.off, w1, b1, out + q.off * C);
+}
+
+__global__ __launch_bounds__(NB) void k_fwd32_many(const GroupGeo *__restrict__ gg, const int *__restrict__ pre,
+                                                   int ng, const bf16_t *__restrict__ in,
+                                                   const bf16_t *__restrict__ w, const float *__restrict__ bias,
+                                                   bf16_t *__restrict__ out)
+{
+    const int i = group_of(pre, ng, blockIdx.x);
+    const GroupGeo q = gg[i];
+    fwd32_body(q.g, blockIdx.x - pre[i], in + q.off * C, w, bias, out + q.off * C);
+}
+
+__global__ __launch_bounds__(NB) void k_fwd4_many(const GroupGeo *__restrict__ gg, const int *__restrict__ pre,
+                                                  int ng, const bf16_t *__restrict__ in,
+                                                  const bf16_t *__restrict__ w4, const float *__restrict__ b4,
+                                                  float *__restrict__ out)
+{
+    const int i = group_of(pre, ng, blockIdx.x);
+    const GroupGeo q = gg[i];
+    fwd4_body(q.g, blockIdx.x - pre[i], in + q.off * C, w4, b4, out + q.off);
 }
 
 // ------------------------------------------------------------ loss, grads
@@ -444,6 +519,8 @@ struct trlnet {
     int lcap = 0;
     float *hstage = nullptr;  // pinned staging of trlnet_forward_many (in, then out)
     long hcap = 0;            // floats
+    char *hgeo = nullptr, *dgeo = nullptr;  // group table of the many-group launches (pinned / device)
+    long geocap = 0;                        // bytes
     hipStream_t s = nullptr;
 };
 
@@ -514,6 +591,50 @@ int forward(trlnet *t, const Geo &g, long off = 0)
                        t->a3 + oc);
     hipLaunchKernelGGL(k_fwd4, dim3(gb), dim3(NB), 0, t->s, g, t->a3 + oc, t->wb + OW4, t->prm + OB4,
                        t->o + off);
+    return hipGetLastError() == hipSuccess ? 0 : TRLNET_EHIP;
+}
+
+// The forward pass over groups back to back (inputs at t->x, outputs at
+// t->o): one launch per layer for all groups, the same per-pixel
+// arithmetic as forward() group by group.
+int forward_groups(trlnet *t, int ngroups, const int *nimg, const int *h, const int *w)
+{
+    const size_t tab = sizeof(GroupGeo) * (size_t)ngroups;
+    const size_t off_pre = (tab + 255) & ~(size_t)255;
+    const size_t bytes = off_pre + sizeof(int) * (size_t)(ngroups + 1);
+    if ((long)bytes > t->geocap) {
+        if (t->hgeo) (void)hipHostFree(t->hgeo);
+        if (t->dgeo) (void)hipFree(t->dgeo);
+        t->hgeo = t->dgeo = nullptr;
+        t->geocap = 0;
+        NTRY(hipHostMalloc((void **)&t->hgeo, bytes, 0));
+        NTRY(hipMalloc((void **)&t->dgeo, bytes));
+        t->geocap = (long)bytes;
+    }
+    GroupGeo *gg = reinterpret_cast<GroupGeo *>(t->hgeo);
+    int *pre = reinterpret_cast<int *>(t->hgeo + off_pre);
+    long off = 0;
+    pre[0] = 0;
+    for (int i = 0; i < ngroups; ++i) {
+        const Geo g{nimg[i], h[i], w[i], (long)nimg[i] * h[i] * w[i]};
+        gg[i] = GroupGeo{g, off};
+        const long nb = (long)pre[i] + grid_px(g.np);
+        if (nb > 0x7fffffffL) return TRLNET_EINVAL;
+        pre[i + 1] = (int)nb;
+        off += g.np;
+    }
+    NTRY(hipMemcpyAsync(t->dgeo, t->hgeo, bytes, hipMemcpyHostToDevice, t->s));
+    const GroupGeo *dg = reinterpret_cast<const GroupGeo *>(t->dgeo);
+    const int *dp = reinterpret_cast<const int *>(t->dgeo + off_pre);
+    const dim3 gb(pre[ngroups]);
+    hipLaunchKernelGGL(k_fwd1_many, gb, dim3(NB), 0, t->s, dg, dp, ngroups, t->x, t->wb + OW1,
+                       t->prm + OB1, t->a1);
+    hipLaunchKernelGGL(k_fwd32_many, gb, dim3(NB), 0, t->s, dg, dp, ngroups, t->a1, t->wb + OW2,
+                       t->prm + OB2, t->a2);
+    hipLaunchKernelGGL(k_fwd32_many, gb, dim3(NB), 0, t->s, dg, dp, ngroups, t->a2, t->wb + OW3,
+                       t->prm + OB3, t->a3);
+    hipLaunchKernelGGL(k_fwd4_many, gb, dim3(NB), 0, t->s, dg, dp, ngroups, t->a3, t->wb + OW4,
+                       t->prm + OB4, t->o);
     return hipGetLastError() == hipSuccess ? 0 : TRLNET_EHIP;
 }
 
@@ -627,6 +748,8 @@ int trlnet_destroy(trlnet *t)
                     (void *)t->d2, (void *)t->part, (void *)t->lpart})
         if (p) (void)hipFree(p);
     if (t->hstage) (void)hipHostFree(t->hstage);
+    if (t->hgeo) (void)hipHostFree(t->hgeo);
+    if (t->dgeo) (void)hipFree(t->dgeo);
     if (t->s) (void)hipStreamDestroy(t->s);
     delete t;
     return 0;
@@ -679,12 +802,7 @@ int trlnet_forward_many(trlnet *t, int ngroups, const int *nimg, const int *h, c
     }
     memcpy(t->hstage, in, sizeof(float) * total);
     NTRY(hipMemcpyAsync(t->x, t->hstage, sizeof(float) * total, hipMemcpyHostToDevice, t->s));
-    long off = 0;
-    for (int i = 0; i < ngroups; ++i) {
-        const Geo g{nimg[i], h[i], w[i], (long)nimg[i] * h[i] * w[i]};
-        if (int rc = forward(t, g, off)) return rc;
-        off += g.np;
-    }
+    if (int rc = forward_groups(t, ngroups, nimg, h, w)) return rc;
     NTRY(hipMemcpyAsync(t->hstage, t->o, sizeof(float) * total, hipMemcpyDeviceToHost, t->s));
     NTRY(hipStreamSynchronize(t->s));
     memcpy(out, t->hstage, sizeof(float) * total);
@@ -721,12 +839,7 @@ int many_run(trlnet *t, int ngroups, const int *nimg, const int *h, const int *w
              const float **out_device)
 {
     NTRY(hipMemcpyAsync(t->x, t->hstage, sizeof(float) * total, hipMemcpyHostToDevice, t->s));
-    long off = 0;
-    for (int i = 0; i < ngroups; ++i) {
-        const Geo g{nimg[i], h[i], w[i], (long)nimg[i] * h[i] * w[i]};
-        if (int rc = forward(t, g, off)) return rc;
-        off += g.np;
-    }
+    if (int rc = forward_groups(t, ngroups, nimg, h, w)) return rc;
     NTRY(hipStreamSynchronize(t->s));
     *out_device = t->o;
     return 0;
