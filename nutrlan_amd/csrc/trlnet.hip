@@ -82,6 +82,24 @@ __device__ inline long nbr(const Geo &g, long p, int t, int sign)
     return img * hw + (long)y * g.w + x;
 }
 
+// The 9 neighbours of pixel p (tap t = (dy+1)*3 + dx+1, periodic), with
+// one division per pixel instead of one per tap: nb[t] equals nbr(g, p, t, 1).
+__device__ inline void nbr9(const Geo &g, long p, long nb[9])
+{
+    const long hw = (long)g.h * g.w;
+    const long img = p / hw;
+    const int r = (int)(p - img * hw);
+    const int y = r / g.w, x = r - y * g.w;
+    const int ym = y == 0 ? g.h - 1 : y - 1, yp = y == g.h - 1 ? 0 : y + 1;
+    const int xm = x == 0 ? g.w - 1 : x - 1, xp = x == g.w - 1 ? 0 : x + 1;
+    const long base = img * hw;
+    const int ys[3] = {ym, y, yp}, xs[3] = {xm, x, xp};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) nb[a * 3 + b] = base + (long)ys[a] * g.w + xs[b];
+}
+
 __device__ inline bf16x8 load8(const bf16_t *p)
 {
     const uint4 v = *reinterpret_cast<const uint4 *>(p);
@@ -104,9 +122,14 @@ __device__ __forceinline__ void fwd1_body(const Geo &g, long bid, const float *_
     const int m = lane & 15, kq = lane >> 4;
     const long p = p0 + m;
     bf16x8 a = zero8();
-    for (int j = 0; j < 8; ++j) {
-        const int t = 8 * kq + j;
-        if (t < T && p < g.np) a[j] = as_bf(f2bf(x[nbr(g, p, t, 1)]));
+    if (p < g.np) {
+        long nb[9];
+        nbr9(g, p, nb);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int t = 8 * kq + j;
+            if (t < T) a[j] = as_bf(f2bf(x[nb[t]]));
+        }
     }
     for (int nt = 0; nt < 2; ++nt) {
         const int co = 16 * nt + m;
@@ -142,8 +165,11 @@ __device__ __forceinline__ void fwd32_body(const Geo &g, long bid, const bf16_t 
     const int m = lane & 15, kq = lane >> 4;
     const long p = p0 + m;
     f4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+    long nb[9];
+    if (p < g.np) nbr9(g, p, nb);
+#pragma unroll
     for (int t = 0; t < T; ++t) {
-        const bf16x8 a = p < g.np ? load8(in + nbr(g, p, t, 1) * C + 8 * kq) : zero8();
+        const bf16x8 a = p < g.np ? load8(in + nb[t] * C + 8 * kq) : zero8();
         for (int nt = 0; nt < 2; ++nt) {
             const int co = 16 * nt + m;
             const bf16x8 b = load8(w + ((long)co * T + t) * C + 8 * kq);
@@ -177,8 +203,11 @@ __device__ __forceinline__ void fwd4_body(const Geo &g, long bid, const bf16_t *
     const int m = lane & 15, kq = lane >> 4;
     const long p = p0 + m;
     f4 acc = {0.f, 0.f, 0.f, 0.f};
+    long nb[9];
+    if (p < g.np) nbr9(g, p, nb);
+#pragma unroll
     for (int t = 0; t < T; ++t) {
-        const bf16x8 a = p < g.np ? load8(in + nbr(g, p, t, 1) * C + 8 * kq) : zero8();
+        const bf16x8 a = p < g.np ? load8(in + nb[t] * C + 8 * kq) : zero8();
         const bf16x8 b = m == 0 ? load8(w4 + (long)t * C + 8 * kq) : zero8();
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
     }
