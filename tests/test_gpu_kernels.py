@@ -352,3 +352,46 @@ def test_packed_solves_bitwise(gpu_lib, mixed, monkeypatch):
         assert np.array_equal(a["eval"][:4], b["eval"][:4])
     for A in ops1 + ops0 + [A1, A0]:
         A.close()
+
+
+def test_packing_declined_far_column_and_double_diagonal(gpu_lib):
+    """Packing needs every column within 2^23 rows of its row block and at
+    most one diagonal entry per row; a matrix breaking either keeps the
+    unpacked SpMV, whose result still matches numpy."""
+    T = gpu_lib
+    # n > 2^23: diagonal plus one symmetric pair (0, n-1), 2^23 + 511 rows apart
+    n = (1 << 23) + 512
+    rows = np.concatenate([np.arange(n), [0, n - 1]])
+    cols = np.concatenate([np.arange(n), [n - 1, 0]])
+    vals = np.concatenate([np.full(n, 2.0), [-1.0, -1.0]])
+    o = np.lexsort((cols, rows))
+    rows, cols, vals = rows[o], cols[o], vals[o]
+    rp = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(rp, rows + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    A = T.Csr(rp, cols.astype(np.int32), vals)
+    print(f"far column: n={n} packed={A.packed}", flush=True)
+    assert not A.packed
+    x = W.start_vector(n, seed=13)
+    c = T.Ctx(n, 2, False)
+    c.set(0, x)
+    c.spmv(A, 0, 1)
+    y = 2.0 * x
+    y[0] -= x[n - 1]
+    y[n - 1] -= x[0]
+    assert np.abs(c.get(1) - y).max() <= 1e-14 * np.abs(y).max()
+    c.close()
+    A.close()
+    # a row with two diagonal entries (duplicates are summed)
+    rp2 = np.array([0, 2, 3, 4], dtype=np.int32)
+    ci2 = np.array([0, 0, 1, 2], dtype=np.int32)
+    v2 = np.array([1.5, 0.25, 3.0, 4.0])
+    B = T.Csr(rp2, ci2, v2)
+    assert not B.packed
+    c = T.Ctx(3, 2, False)
+    x3 = np.array([1.0, -2.0, 0.5])
+    c.set(0, x3)
+    c.spmv(B, 0, 1)
+    assert np.allclose(c.get(1), [1.75 * 1.0, 3.0 * -2.0, 4.0 * 0.5], rtol=0, atol=1e-15)
+    c.close()
+    B.close()
