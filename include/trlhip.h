@@ -78,6 +78,26 @@ int trlhip_csr_is_packed(const trlhip_csr *A);
  * results can differ from the default kernel's in the last bits.  No
  * counterpart in the reference, whose operator is the caller's own op. */
 int trlhip_csr_autotune(trlhip_csr *A, int reps, int *lpr_out, float *ms_out);
+/* Sets A's SpMV kernel as trlhip_csr_autotune would: lpr = 0 (CSR-stream)
+ * or 2, 4, 8, 16, 32 lanes per row; for a cached autotuner choice.  No
+ * counterpart in the reference. */
+int trlhip_csr_set_spmv(trlhip_csr *A, int lpr);
+
+/* Gram-Schmidt projection tiles (h = V^T r): columns per block, 4, 8 or
+ * 16, for an FP64 basis (cb_fp64) and for the FP32 shadow of a
+ * mixed-precision basis (cb_fp32); 0 restores the default 8.  Process-wide,
+ * read by every later solve, single and batched alike (so a batch still
+ * equals its single solves bit for bit); set it while no solve runs.  A
+ * width other than the default sums each coefficient in another order.
+ * No counterpart in the reference. */
+int trlhip_proj_tiles(int cb_fp64, int cb_fp32);
+int trlhip_proj_tiles_get(int *cb_fp64, int *cb_fp32);
+/* Autotuner for the projection tiles: times the projection of an n x m
+ * basis (and its FP32 shadow) at 4, 8 and 16 columns per block (reps
+ * launches each, 0 = 10) and keeps the fastest of each through
+ * trlhip_proj_tiles.  cb_out[0..1] and ms_out[0..1] (FP64, FP32) may be
+ * NULL.  No counterpart in the reference. */
+int trlhip_proj_autotune(int n, int m, int reps, int *cb_out, float *ms_out);
 int trlhip_csr_destroy(trlhip_csr *A);
 
 /* trl_matvec-compatible operator for a trlhip_csr passed as mvparam.

@@ -1492,6 +1492,13 @@ void launch_proj(trlhip_ctx *c, int gx, const VS *V, int c0, int m, int rcol)
             (long)c->ld, c0, m, colp<double>(c, rcol), part_proj(c), gx, gy);
 }
 
+// Columns per projection block (a chunk) for a real basis read in FP64
+// ([0]) or through the FP32 shadow ([1]): 4, 8 (default, from the sweep
+// below) or 16; set by trlhip_proj_autotune / trlhip_proj_tiles.  Single
+// and batched solves read the same setting, so they stay bit-identical; a
+// different width sums each coefficient's lanes in another order.
+int g_proj_cb[2] = {8, 8};
+
 // Row blocks of the projection: about kProjBlocks blocks over all column
 // chunks, at most kProjGrid (the partial rows), a multiple of 8 (the XCD
 // count) when there are at least 8.
@@ -1514,12 +1521,22 @@ int run_proj(trlhip_ctx *c, int c0, int m, int rcol, double *hout, bool shadow =
         Timed t(TRLHIP_K_PROJ, c->s, s32 ? (4.0 * m + 8.0) * c->n : (double)c->esz * c->n * (m + 1));
         // chunk widths and row unrolling from a sweep at n = 2^21, m = 16..64
         // (about 5.7 TB/s for both element types)
-        if (s32) {
-            g = proj_gx(c, m, 8);
-            launch_proj<float, 1, 8, 4>(c, g, c->V32, c0, m, rcol);
-        } else if (c->E == 1) {
-            g = proj_gx(c, m, 8);
-            launch_proj<double, 1, 8, 4>(c, g, colp<double>(c, 0), c0, m, rcol);
+        if (s32 || c->E == 1) {
+            const int cb = g_proj_cb[s32 ? 1 : 0];
+            g = proj_gx(c, m, cb);
+#define PROJ_CB(CB)                                                                      \
+    case CB:                                                                             \
+        if (s32)                                                                         \
+            launch_proj<float, 1, CB, 4>(c, g, c->V32, c0, m, rcol);                     \
+        else                                                                             \
+            launch_proj<double, 1, CB, 4>(c, g, colp<double>(c, 0), c0, m, rcol);        \
+        break;
+            switch (cb) {
+                PROJ_CB(4)
+                PROJ_CB(16)
+                default: PROJ_CB(8)
+            }
+#undef PROJ_CB
         } else {
             g = proj_gx(c, m, 4);
             launch_proj<double, 2, 4, 4>(c, g, colp<double>(c, 0), c0, m, rcol);
@@ -1716,12 +1733,12 @@ __global__ __launch_bounds__(kBlock) void kb_recur(const BDesc *__restrict__ d, 
                                       q.dscal + S_ALPHA, bid_, nblk_);
 }
 
-template <typename VS>
+template <typename VS, int CB>
 __global__ __launch_bounds__(kBlock) void kb_proj(const BDesc *__restrict__ d, const int *pre, int nreq)
 {
     BATCH_LOCATE(BP_PROJ)
-    proj_body<VS, double, 1, 8, 4>(q.n, (const VS *)q.VG, q.ld, 0, q.jnd, q.r, q.part_proj, q.gx,
-                                   q.gy, bid_, nblk_);
+    proj_body<VS, double, 1, CB, 4>(q.n, (const VS *)q.VG, q.ld, 0, q.jnd, q.r, q.part_proj, q.gx,
+                                    q.gy, bid_, nblk_);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_colsum(const BDesc *__restrict__ d, const int *pre, int nreq)
@@ -1777,8 +1794,9 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
         d.np_alpha = spmv_grid(A);
         c->np_alpha = d.np_alpha;
         d.g_upd = grid_for(n, kMinRowsUpd, kMaxGrid);
-        d.gx = proj_gx(c, jnd, 8);
-        d.gy = (jnd + 7) / 8;
+        const int cb = g_proj_cb[gi.mixed ? 1 : 0];
+        d.gx = proj_gx(c, jnd, cb);
+        d.gy = (jnd + cb - 1) / cb;
         if (st.first) {
             d.rm = jnd;
             d.rc0 = 0;
@@ -1857,8 +1875,19 @@ int batch_run(hipStream_t s, const GroupInfo &gi, const BDesc *dd, const int *dp
     }
     {
         Timed t(TRLHIP_K_BPROJ, s, gi.bytes[BP_PROJ]);
-        if (gi.mixed) KLAUNCH((kb_proj<float>), blocks(BP_PROJ), dim3(kBlock), 0, s, dd, dp, R);
-        else KLAUNCH((kb_proj<double>), blocks(BP_PROJ), dim3(kBlock), 0, s, dd, dp, R);
+#define KB_PROJ_CB(CB)                                                                        \
+    case CB:                                                                                  \
+        if (gi.mixed)                                                                         \
+            KLAUNCH((kb_proj<float, CB>), blocks(BP_PROJ), dim3(kBlock), 0, s, dd, dp, R);    \
+        else                                                                                  \
+            KLAUNCH((kb_proj<double, CB>), blocks(BP_PROJ), dim3(kBlock), 0, s, dd, dp, R);   \
+        break;
+        switch (g_proj_cb[gi.mixed ? 1 : 0]) {
+            KB_PROJ_CB(4)
+            KB_PROJ_CB(16)
+            default: KB_PROJ_CB(8)
+        }
+#undef KB_PROJ_CB
     }
     KLAUNCH(kb_colsum, blocks(BP_COLSUM), dim3(kBlock), 0, s, dd, dp, R);
     {
@@ -2516,6 +2545,72 @@ int trlhip_csr_autotune(trlhip_csr *A, int reps, int *lpr_out, float *ms_out)
     if (x) (void)hipFree(x);
     if (y) (void)hipFree(y);
     if (part) (void)hipFree(part);
+    return rc;
+}
+
+int trlhip_csr_set_spmv(trlhip_csr *A, int lpr)
+{
+    if (!A || A->magic != kCsrMagic) return TRLHIP_EINVAL;
+    if (lpr != 0 && lpr != 2 && lpr != 4 && lpr != 8 && lpr != 16 && lpr != 32) return TRLHIP_EINVAL;
+    set_lpr(A, lpr);
+    return 0;
+}
+
+int trlhip_proj_tiles(int cb_fp64, int cb_fp32)
+{
+    auto ok = [](int cb) { return cb == 0 || cb == 4 || cb == 8 || cb == 16; };
+    if (!ok(cb_fp64) || !ok(cb_fp32)) return TRLHIP_EINVAL;
+    g_proj_cb[0] = cb_fp64 ? cb_fp64 : 8;
+    g_proj_cb[1] = cb_fp32 ? cb_fp32 : 8;
+    return 0;
+}
+
+int trlhip_proj_tiles_get(int *cb_fp64, int *cb_fp32)
+{
+    if (cb_fp64) *cb_fp64 = g_proj_cb[0];
+    if (cb_fp32) *cb_fp32 = g_proj_cb[1];
+    return 0;
+}
+
+int trlhip_proj_autotune(int n, int m, int reps, int *cb_out, float *ms_out)
+{
+    if (n < 1 || m < 1 || m > 4096 || reps < 0) return TRLHIP_EINVAL;
+    if (reps == 0) reps = 10;
+    trlhip_ctx *c = nullptr;
+    if (int rc = trlhip_ctx_create(&c, n, m + 1, TRLHIP_R32)) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = TRLHIP_EHIP;
+    const int saved[2] = {g_proj_cb[0], g_proj_cb[1]};
+    const int cand[3] = {4, 8, 16};
+    for (int prec = 0; prec < 2 && !rc; ++prec) {
+        int best = 8;
+        float best_ms = 0.f;
+        for (int i = 0; i < 3 && !rc; ++i) {
+            g_proj_cb[prec] = cand[i];
+            for (int w = 0; w < 2 && !rc; ++w) rc = run_proj(c, 0, m, m, c->dh, prec == 1);
+            if (!rc && hipEventRecord(e0, c->s) != hipSuccess) rc = TRLHIP_EHIP;
+            for (int r = 0; r < reps && !rc; ++r) rc = run_proj(c, 0, m, m, c->dh, prec == 1);
+            if (!rc && hipEventRecord(e1, c->s) != hipSuccess) rc = TRLHIP_EHIP;
+            float ms = 0.f;
+            if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = TRLHIP_EHIP;
+            if (!rc && (i == 0 || ms < best_ms)) {
+                best_ms = ms;
+                best = cand[i];
+            }
+        }
+        g_proj_cb[prec] = rc ? saved[prec] : best;
+        if (!rc && cb_out) cb_out[prec] = best;
+        if (!rc && ms_out) ms_out[prec] = best_ms / (float)reps;
+    }
+    if (rc) {
+        g_proj_cb[0] = saved[0];
+        g_proj_cb[1] = saved[1];
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    trlhip_ctx_destroy(c);
     return rc;
 }
 

@@ -100,6 +100,14 @@ def lib():
         L.trlhip_stats_get.restype = I
         L.trlhip_device_count.argtypes = []
         L.trlhip_device_count.restype = I
+        L.trlhip_csr_set_spmv.argtypes = [P, I]
+        L.trlhip_csr_set_spmv.restype = I
+        L.trlhip_proj_tiles.argtypes = [I, I]
+        L.trlhip_proj_tiles.restype = I
+        L.trlhip_proj_tiles_get.argtypes = [P, P]
+        L.trlhip_proj_tiles_get.restype = I
+        L.trlhip_proj_autotune.argtypes = [I, I, I, P, P]
+        L.trlhip_proj_autotune.restype = I
         L.trl_check_ritz.argtypes = [P, ctypes.POINTER(TrlInfo), I, I, P, I, P,
                                      ctypes.POINTER(I), P, P, P, I]
         L.trl_check_ritz.restype = None
@@ -131,6 +139,7 @@ class Csr:
         (TRLHIP_R32: FP32 shadow basis for Gram-Schmidt, FP64 elsewhere)."""
         n = len(rp) - 1
         self.n = n
+        self.nnz = len(ci)
         self.complex = complex_
         self.mixed = bool(mixed) and not complex_
         rp = np.ascontiguousarray(rp, dtype=np.int32)
@@ -156,6 +165,13 @@ class Csr:
         L.trlhip_csr_is_packed.argtypes = [ctypes.c_void_p]
         L.trlhip_csr_is_packed.restype = ctypes.c_int
         return bool(L.trlhip_csr_is_packed(self.handle))
+
+    def set_spmv(self, lpr):
+        """Use the SpMV kernel with `lpr` lanes per row (0: CSR-stream), as
+        a cached autotuner choice (trlhip_csr_set_spmv)."""
+        rc = lib().trlhip_csr_set_spmv(self.handle, int(lpr))
+        if rc != 0:
+            raise RuntimeError(f"trlhip_csr_set_spmv failed ({rc})")
 
     def autotune(self, reps=10):
         """Time the SpMV kernels on this matrix and keep the fastest
@@ -195,6 +211,62 @@ def _result(info, ev, evec, wrk, wall):
     out.update(eval=ev, evec=evec, resid=wrk, anrm=float(info.anrm), wall_s=wall,
                crat=float(info.crat), tol=float(info.tol))
     return out
+
+
+def proj_tiles(cb_fp64=0, cb_fp32=0):
+    """Columns per Gram-Schmidt projection block for FP64 and FP32-shadow
+    bases (4, 8 or 16; 0 = the default 8), process-wide
+    (trlhip_proj_tiles)."""
+    rc = lib().trlhip_proj_tiles(int(cb_fp64), int(cb_fp32))
+    if rc != 0:
+        raise RuntimeError(f"trlhip_proj_tiles failed ({rc})")
+
+
+def proj_tiles_get():
+    a, b = ctypes.c_int(), ctypes.c_int()
+    lib().trlhip_proj_tiles_get(ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def proj_autotune(n, m, reps=10):
+    """Time the projection tiles on an n x m basis and keep the fastest
+    (trlhip_proj_autotune); returns ((cb_fp64, cb_fp32), (ms_fp64,
+    ms_fp32))."""
+    cb = (ctypes.c_int * 2)()
+    ms = (ctypes.c_float * 2)()
+    rc = lib().trlhip_proj_autotune(int(n), int(m), int(reps), cb, ms)
+    if rc != 0:
+        raise RuntimeError(f"trlhip_proj_autotune failed ({rc})")
+    return (cb[0], cb[1]), (ms[0], ms[1])
+
+
+def autotune_cached(path, n, m, csrs=(), reps=10):
+    """The autotuner's choices through a JSON cache at `path`: projection
+    tiles for an n x m basis and the SpMV kernel of each operator in csrs
+    (keyed by rows and nonzeros); missing entries are measured and added.
+    Returns the cache dict."""
+    import json
+    import os
+
+    cache = {"proj": {}, "spmv": {}}
+    if os.path.exists(path):
+        with open(path) as f:
+            cache.update(json.load(f))
+    key = f"{int(n)}x{int(m)}"
+    if key in cache["proj"]:
+        proj_tiles(*cache["proj"][key])
+    else:
+        cache["proj"][key] = list(proj_autotune(n, m, reps)[0])
+    for A in csrs:
+        k = f"{A.n}:{A.nnz}"
+        if k in cache["spmv"]:
+            A.set_spmv(cache["spmv"][k])
+        else:
+            cache["spmv"][k] = A.autotune(reps)[0]
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(cache, f, indent=1, sort_keys=True)
+    return cache
 
 
 def output_buffers(n, mev):

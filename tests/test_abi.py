@@ -134,3 +134,19 @@ def test_shipped_start_net_weights():
     p = np.load(LN.WEIGHTS, allow_pickle=False)
     assert p.dtype == np.float32 and p.shape == (19105,)
     assert np.all(np.isfinite(p)) and np.abs(p).max() < 100
+
+
+def test_projection_tile_setting():
+    """trlhip_proj_tiles takes 4, 8 or 16 columns per block (0 = default 8)
+    for each basis precision and rejects anything else (host-only)."""
+    assert T.proj_tiles_get() == (8, 8)
+    try:
+        T.proj_tiles(4, 16)
+        assert T.proj_tiles_get() == (4, 16)
+        for bad in ((5, 8), (8, 32), (-4, 8)):
+            with pytest.raises(RuntimeError):
+                T.proj_tiles(*bad)
+        assert T.proj_tiles_get() == (4, 16)
+    finally:
+        T.proj_tiles()
+    assert T.proj_tiles_get() == (8, 8)

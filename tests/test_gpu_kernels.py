@@ -395,3 +395,53 @@ def test_packing_declined_far_column_and_double_diagonal(gpu_lib):
     assert np.allclose(c.get(1), [1.75 * 1.0, 3.0 * -2.0, 4.0 * 0.5], rtol=0, atol=1e-15)
     c.close()
     B.close()
+
+
+@pytest.mark.parametrize("cb", [4, 16])
+def test_projection_tiles_keep_batch_equal_single(gpu_lib, cb):
+    """With other projection tiles (process-wide), batched solves still equal
+    their single solves bit for bit, plain and mixed, and the eigenvalues
+    stay within 1e-10 ||A|| of the default tiles' solve."""
+    T = gpu_lib
+    lat = [W.anderson2d_csr(L, 7.0, seed=60 + L)[0] for L in (40, 56)]
+    ops = [T.Csr(*c, mixed=(i == 1)) for i, c in enumerate(lat)]
+    probs = [(A, -1, 6, 28, 1e-8, 1, 20000, 6, W.start_vector(A.n, seed=70 + i))
+             for i, A in enumerate(ops)]
+    ref = [T.solve_csr(*p) for p in probs]
+    try:
+        T.proj_tiles(cb, cb)
+        rc, res, _ = T.solve_batch(probs)
+        assert rc == 0
+        for p, g, r0 in zip(probs, res, ref):
+            s = T.solve_csr(*p)
+            assert g["matvec"] == s["matvec"] and np.array_equal(g["eval"], s["eval"])
+            assert np.array_equal(g["evec"], s["evec"])
+            assert g["nec"] == r0["nec"] == 6
+            assert np.abs(g["eval"][:6] - r0["eval"][:6]).max() <= 1e-10 * r0["anrm"]
+    finally:
+        T.proj_tiles()
+    for A in ops:
+        A.close()
+
+
+def test_projection_autotune_and_cache(gpu_lib, tmp_path):
+    """The projection-tile autotuner picks one of 4, 8, 16 per precision and
+    sets it; the JSON cache replays that choice and the SpMV choice."""
+    T = gpu_lib
+    try:
+        (c64, c32), (m64, m32) = T.proj_autotune(1 << 18, 24, reps=3)
+        print(f"proj tiles: fp64 {c64} ({1e3 * m64:.1f} us), fp32 {c32} ({1e3 * m32:.1f} us)", flush=True)
+        assert c64 in (4, 8, 16) and c32 in (4, 8, 16) and m64 > 0 and m32 > 0
+        assert T.proj_tiles_get() == (c64, c32)
+        rp, ci, v = W.laplace3d_csr(16, seed=3)
+        A = T.Csr(rp, ci, v)
+        path = str(tmp_path / "autotune.json")
+        cache = T.autotune_cached(path, 1 << 16, 20, [A], reps=2)
+        key = f"{A.n}:{A.nnz}"
+        assert key in cache["spmv"] and "65536x20" in cache["proj"]
+        T.proj_tiles()
+        again = T.autotune_cached(path, 1 << 16, 20, [A], reps=2)
+        assert again == cache and T.proj_tiles_get() == tuple(cache["proj"]["65536x20"])
+        A.close()
+    finally:
+        T.proj_tiles()
