@@ -1102,8 +1102,9 @@ struct trlhip_ctx {
     double *hres;   // pinned host-coherent results: a fused step's 4 scalars, then
                     // 3 per refined pair from index 8 (hres_count doubles)
     double *dres;   // its device address
-    int pend_dst;   // column of a deferred scale-copy from the scratch column (-1: none)
+    int pend_dst;   // column of a deferred scale-copy (-1: none)
     double pend_s;  // its scale
+    int pend_src;   // its source: the scratch column, or (a deferred column copy) a basis column
     size_t hcap;
     double *hY;     // pinned staging for Y
     hipStream_t s;
@@ -1121,9 +1122,10 @@ struct BatchStep {
     const trlhip_csr *A;
     int first, qa, qb, rcol, jnd;
     double beta_prev;
-    int fuse;         // q_a is the deferred scale-copy of the scratch column
+    int fuse;         // q_a is a deferred scale-copy, of column fuse_src
     double fuse_s;
     long coff;        // first step after a restart: its jnd-1 coefficients at cbuf[coff]
+    int fuse_src;
 };
 
 // A recorded restart GEMM or scale-copy of a batched context; `phase` is
@@ -1417,16 +1419,18 @@ int run_spmv(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol)
 }
 
 // The deferred scale-copy into column xcol fused into the SpMV that uses
-// it: y = A (s * scratch), and s * scratch stored to column xcol (and to
-// its FP32 shadow in a mixed-precision context).
+// it: y = A (s * src), and s * src stored to column xcol (and to its FP32
+// shadow in a mixed-precision context); src is the scratch column or, for a
+// deferred column copy (s = 1, exact), another basis column.
 int run_spmv_fused(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol)
 {
     c->np_alpha = spmv_grid(A);
     const double s = c->pend_s;
+    const int src = c->pend_src;
     c->pend_dst = -1;
     Timed t(TRLHIP_K_SPMV, c->s,
             spmv_bytes(A, c->E, 8, 8) + (double)c->esz * c->n + (c->V32 ? 4.0 * c->n : 0.0));
-    return launch_spmv(A, c->vtype, c->V + (size_t)c->ncols * c->ld * c->esz,
+    return launch_spmv(A, c->vtype, c->V + (size_t)src * c->ld * c->esz,
                        c->V + (size_t)ycol * c->ld * c->esz, part_alpha(c), c->s, s,
                        c->V + (size_t)xcol * c->ld * c->esz, shadow32(c, xcol));
 }
@@ -1579,7 +1583,7 @@ int flush_pending(trlhip_ctx *c)
     if (c->pend_dst < 0) return 0;
     const int dst = c->pend_dst;
     c->pend_dst = -1;
-    return run_scale_copy(c, dst, c->ncols, c->pend_s);
+    return run_scale_copy(c, dst, c->pend_src, c->pend_s);
 }
 
 // Wait until the context's stream is idle: synchronize it, or, in a batch,
@@ -1821,7 +1825,7 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
         d.pk = A->pk;
         d.dg = A->dg;
         d.tab = A->tab;
-        d.x = colp<double>(c, st.fuse ? c->ncols : st.qa);
+        d.x = colp<double>(c, st.fuse ? st.fuse_src : st.qa);
         d.xscale = st.fuse ? st.fuse_s : 1.0;
         d.xq = st.fuse ? colp<double>(c, st.qa) : nullptr;
         d.xq32 = st.fuse ? shadow32(c, st.qa) : nullptr;
@@ -2852,6 +2856,13 @@ int trlhip_col_copy(trlhip_ctx *c, int dst, int src)
     if (!c || !valid_col(c, dst) || !valid_col(c, src)) return TRLHIP_EINVAL;
     if (int rc = flush_pending(c)) return rc;
     if (dst == src) return 0;
+    if (c->E == 1) {  // deferred like a scale-copy from the scratch column: the
+                      // next SpMV of dst reads src and stores the copy (exact)
+        c->pend_dst = dst;
+        c->pend_s = 1.0;
+        c->pend_src = src;
+        return 0;
+    }
     return run_scale_copy(c, dst, src, 1.0);
 }
 
@@ -2869,6 +2880,7 @@ int trlhip_col_scale_copy(trlhip_ctx *c, int dst, int src, double s)
     if (src == c->ncols) {  // deferred: the next SpMV of column dst applies it
         c->pend_dst = dst;
         c->pend_s = s;
+        c->pend_src = src;
         return 0;
     }
     return run_scale_copy(c, dst, src, s);
@@ -2921,7 +2933,7 @@ static int form_residual(trlhip_ctx *c, const trlhip_csr *A, int qa, int first, 
         m = 1;
     }
     int rc;
-    if (A && c->pend_dst == qa) {
+    if (A && c->pend_dst == qa && rcol != c->pend_src) {
         rc = run_spmv_fused(c, A, qa, rcol);
     } else {
         rc = settle(c);
@@ -2979,9 +2991,10 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
         // batched: record the step; trlhip_batch_flush runs it with the
         // other contexts' steps
         BatchStep st{c, A, first, qa, qb, rcol, jnd, beta_prev, 0, 1.0, -1};
-        if (c->pend_dst == qa) {
+        if (c->pend_dst == qa && rcol != c->pend_src) {
             st.fuse = 1;
             st.fuse_s = c->pend_s;
+            st.fuse_src = c->pend_src;
             c->pend_dst = -1;
         } else if (int rc = flush_pending(c)) {
             return rc;
