@@ -22,6 +22,7 @@
 #include <mutex>
 #include <new>
 #include <vector>
+#include <thread>
 #include <type_traits>
 #include <unordered_map>
 
@@ -1151,13 +1152,20 @@ struct BDesc;  // defined with the batched kernels
 // Lock for the batch's record lists: held for one push_back by the
 // coroutines' threads, so a waiter spins (pause) instead of sleeping in the
 // kernel; a std::mutex under that contention costs futex round trips.
+// After a few thousand spins (the holder was descheduled) a waiter yields
+// its CPU between checks.
 struct SpinLock {
     std::atomic<int> f{0};
     void lock()
     {
-        for (;;) {
+        for (int spins = 0;;) {
             if (!f.exchange(1, std::memory_order_acquire)) return;
-            while (f.load(std::memory_order_relaxed)) __builtin_ia32_pause();
+            while (f.load(std::memory_order_relaxed)) {
+                if (++spins < 4096)
+                    __builtin_ia32_pause();
+                else
+                    std::this_thread::yield();
+            }
         }
     }
     void unlock() { f.store(0, std::memory_order_release); }
