@@ -1655,6 +1655,8 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
         return rc ? rc : TRLHIP_ENOMEM;
     }
     int live[4] = {0, 0, 0, 0};
+    double t_su = 0.0, t_st = 0.0;  /* profile: job setup, coroutine stacks */
+    const double t_jobs = lz_now();
     for (int i = 0; i < njobs; ++i) {
         trl_batch_job *j = &jobs[i];
         lz_co_t *co = &cos[i];
@@ -1663,12 +1665,16 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
         co->ended = 1;
         co->lane = i % nlanes;
         j->info->mvparam = j->A;
-        if (!lz_setup(&co->z, trlhip_csr_matvec, NULL, NULL, j->info, j->nrow, j->mev, j->lde, 0,
-                      bl[co->lane]))
-            continue;
+        const double ts0 = lz_now();
+        const int ok = lz_setup(&co->z, trlhip_csr_matvec, NULL, NULL, j->info, j->nrow, j->mev,
+                                j->lde, 0, bl[co->lane]);
+        t_su += lz_now() - ts0;
+        if (!ok) continue;
         co->z.dstart = j->dstart;
         co->ended = 0;
+        const double ts1 = lz_now();
         co->stack = malloc(LZ_STACK);
+        t_st += lz_now() - ts1;
         if (!co->stack || getcontext(&co->uc) != 0) {
             j->info->stat = -4;
             continue;
@@ -1679,6 +1685,7 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
         co->done = 0;
         ++live[co->lane];
     }
+    const double t_pool = lz_now();
     lz_pool_t pool;
     memset(&pool, 0, sizeof pool);
     pool.all = cos;
@@ -1894,8 +1901,10 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     free(cos);
     for (int l = nlanes - 1; l >= 0; --l) trlhip_batch_destroy(bl[l]); /* lane 0 owns the stream */
     if (prof && prof[0] == '1')
-        fprintf(stderr, "trlan_batch: setup %.3f s, rounds %.3f s, teardown %.3f s\n",
-                t_loop - t_begin, t_end_loop - t_loop, lz_now() - t_end_loop);
+        fprintf(stderr, "trlan_batch: setup %.3f s (lanes %.3f, jobs %.3f of which job setup %.3f "
+                "and stacks %.3f), rounds %.3f s, teardown %.3f s\n",
+                t_loop - t_begin, t_jobs - t_begin, t_pool - t_jobs, t_su, t_st,
+                t_end_loop - t_loop, lz_now() - t_end_loop);
     return rc;
 }
 

@@ -2730,7 +2730,9 @@ int trlhip_ctx_create_batched(trlhip_ctx **out, int n, int ncols, int vtype, trl
 {
     if (!out || n <= 0 || ncols <= 0) return TRLHIP_EINVAL;
     if (vtype != TRLHIP_R64 && vtype != TRLHIP_C128 && vtype != TRLHIP_R32) return TRLHIP_EINVAL;
-    if (trlhip_device_count() <= 0) return TRLHIP_ENODEV;
+    // (a batch already holds streams on the device: no runtime query per
+    // context)
+    if (!b && trlhip_device_count() <= 0) return TRLHIP_ENODEV;
     trlhip_ctx *c = (trlhip_ctx *)calloc(1, sizeof(trlhip_ctx));
     if (!c) return TRLHIP_ENOMEM;
     c->n = n;
