@@ -36,7 +36,8 @@ enum {
     TRLHIP_EINVAL = -1,   /* bad argument                          */
     TRLHIP_ENOMEM = -2,   /* device or pinned allocation failed    */
     TRLHIP_EHIP = -3,     /* a HIP runtime call failed             */
-    TRLHIP_ENODEV = -4    /* no usable GPU                          */
+    TRLHIP_ENODEV = -4,   /* no usable GPU                          */
+    TRLHIP_ERANGE = -5    /* a size beyond a fixed maximum          */
 };
 
 /* info->stat value for a device failure inside trlan()/ztrlan(). */
@@ -187,6 +188,25 @@ int trlhip_col_scale_copy(trlhip_ctx *c, int dst, int src, double s);
  * trl_g_dot_, two dgemv, two ddot, g_sum). */
 int trlhip_cgs_pass(trlhip_ctx *c, int c0, int m, int rcol, double *hh,
                     double *rr2, double *h_out);
+
+/* The same pass with no allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.
+ * V: m device columns of n rows, leading dimension ld elements (FP64 for
+ * TRLHIP_R64, interleaved complex FP64 for TRLHIP_C128, FP32 for
+ * TRLHIP_R32: the FP32 shadow a mixed-precision solve projects against);
+ * r: n device FP64 elements (2n for complex), updated in place.  Real
+ * bases need ld a multiple of 4; V and r 16-byte aligned.  stream: a
+ * hipStream_t, NULL for the default stream; the call synchronizes it.
+ * Outputs as trlhip_cgs_pass (h_out: E*m doubles, may be NULL); results
+ * equal trlhip_cgs_pass's bit for bit on the same data.  At most
+ * TRLHIP_NOALLOC_MAX_COLS columns (else TRLHIP_ERANGE): the reduction
+ * partials live in a fixed device array of about 8.4 MiB that each GPU
+ * the library runs on holds until the process ends.  Every call on one
+ * GPU uses that one array: calls that overlap on one GPU race. */
+#define TRLHIP_NOALLOC_MAX_COLS 256
+int trlhip_cgs_pass_noalloc(int n, int m, int vtype, const void *V, long ld, void *r,
+                            double *hh, double *rr2, double *h_out, void *stream);
 
 /* h = V(:, c0:c0+m)^H r (no update).  Replaces trl_g_dot_
  * (SRC/trl_comm_none.c:238-329). */

@@ -1514,13 +1514,14 @@ int g_proj_cb[2] = {8, 8};
 // Row blocks of the projection: about kProjBlocks blocks over all column
 // chunks, at most kProjGrid (the partial rows), a multiple of 8 (the XCD
 // count) when there are at least 8.
-int proj_gx(const trlhip_ctx *c, int m, int cb)
+int proj_gx_rows(int n, int m, int cb)
 {
     const int chunks = (m + cb - 1) / cb;
     const int want = (kProjBlocks + chunks - 1) / chunks;
-    const int g = grid_for(c->n, kMinRowsUpd, want < kProjGrid ? want : kProjGrid);
+    const int g = grid_for(n, kMinRowsUpd, want < kProjGrid ? want : kProjGrid);
     return g >= 8 ? g & ~7 : g;
 }
+int proj_gx(const trlhip_ctx *c, int m, int cb) { return proj_gx_rows(c->n, m, cb); }
 
 // hout = V(:, c0:c0+m)^H r (device, E*m doubles).  shadow: a Gram-Schmidt
 // projection, which reads the FP32 shadow of V in a mixed-precision context.
@@ -3378,6 +3379,127 @@ int trlhip_batch_flush(trlhip_batch *b)
     const int rc = trlhip_batch_launch(b);
     const int rc2 = trlhip_batch_wait(b);
     return rc ? rc : rc2;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ allocation-free pass
+//
+// trlhip_cgs_pass_noalloc: one classical Gram-Schmidt pass over caller
+// device memory with no allocation at all.  Its reduction partials (the
+// projection's partial rows, h, the update's ||r||^2 partials and the
+// three result scalars) live in `scratch`, a fixed device array that every
+// GPU the library runs on holds once, for the life of the process.  Host
+// code never passes `scratch` itself to a kernel (in host code the name
+// is a host-side stand-in, not a device address): the device address is
+// taken once per device with hipGetSymbolAddress and kept in g_scratch_dev.
+// Every kernel writes each scratch element it later reads (no part of it
+// needs to start at zero); calls that overlap on one GPU share the array
+// and race.
+
+namespace {
+constexpr int kNoAllocMaxCols = 256;  // TRLHIP_NOALLOC_MAX_COLS
+constexpr int kMaxDevices = 64;
+// partial rows: kProjGrid per coefficient, 2 coefficients per complex
+// column; then h (2 per column), the update partials, 8 scalars
+constexpr size_t kScrProj = 0;
+constexpr size_t kScrH = kScrProj + (size_t)kProjGrid * 2 * kNoAllocMaxCols;
+constexpr size_t kScrUpd = kScrH + 2 * (size_t)kNoAllocMaxCols;
+constexpr size_t kScrScal = kScrUpd + kMaxGrid;
+constexpr size_t kScrCount = kScrScal + 8;
+}  // namespace
+
+static __device__ __align__(256) double scratch[kScrCount];
+
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (taken on the first
+// call on that device), or nullptr.
+double *scratch_dev(void)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        double *p = nullptr;
+        if (hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        g_scratch_dev[dev] = p;
+    }
+    return g_scratch_dev[dev];
+}
+
+template <typename VS, int E>
+int noalloc_launch(int n, int m, const VS *V, long ld, double *r, double *scr, hipStream_t s)
+{
+    double *pproj = scr + kScrProj, *dh = scr + kScrH, *pupd = scr + kScrUpd, *dscal = scr + kScrScal;
+    // the same kernels, tiles and grids as trlhip_cgs_pass on a context of
+    // n rows, so the two agree bit for bit
+    const bool s32 = std::is_same<VS, float>::value;
+    const int cb = E == 2 ? 4 : g_proj_cb[s32 ? 1 : 0];
+    const int gx = proj_gx_rows(n, m, cb);
+    const int gy = (m + cb - 1) / cb;
+#define NA_PROJ(CB)                                                                              \
+    case CB:                                                                                     \
+        KLAUNCH((k_proj<VS, double, E, CB, 4>), dim3(gx * gy), dim3(kBlock), 0, s, n, V, ld, 0, m, \
+                (const double *)r, pproj, gx, gy);                                               \
+        break;
+    if (E == 2) {
+        KLAUNCH((k_proj<VS, double, E, 4, 4>), dim3(gx * gy), dim3(kBlock), 0, s, n, V, ld, 0, m,
+                (const double *)r, pproj, gx, gy);
+    } else {
+        switch (cb) {
+            NA_PROJ(4)
+            NA_PROJ(16)
+            default: NA_PROJ(8)
+        }
+    }
+#undef NA_PROJ
+    if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
+    KLAUNCH(k_colsum, dim3(m * E), dim3(kBlock), 0, s, pproj, gx, dh);
+    const int g = grid_for(n, kMinRowsUpd, kMaxGrid);
+    const size_t shm = sizeof(double) * (size_t)E * m;
+    KLAUNCH((k_update<VS, double, E, 4>), dim3(g), dim3(kBlock), shm, s, n, V, ld, 0, m,
+            (const double *)dh, 0.0, (const double *)nullptr, 0, -1, (const double *)r, r, pupd,
+            dscal + S_HH, dscal + S_ALPHA);
+    if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
+    return launch_sum(pupd, g, dscal + S_RR2, s);
+}
+}  // namespace
+
+extern "C" {
+
+int trlhip_cgs_pass_noalloc(int n, int m, int vtype, const void *V, long ld, void *r, double *hh,
+                            double *rr2, double *h_out, void *stream)
+{
+    if (m < 1 || m > kNoAllocMaxCols) return TRLHIP_ERANGE;
+    if (n < 1 || !V || !r || ld < n) return TRLHIP_EINVAL;
+    if (vtype != TRLHIP_R64 && vtype != TRLHIP_C128 && vtype != TRLHIP_R32) return TRLHIP_EINVAL;
+    // 16-byte aligned columns: the real kernels move four rows per 16-byte
+    // load or store
+    if (((uintptr_t)V & 15) || ((uintptr_t)r & 15) || (vtype != TRLHIP_C128 && (ld & 3)))
+        return TRLHIP_EINVAL;
+    double *scr = scratch_dev();
+    if (!scr) return TRLHIP_EHIP;
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if (vtype == TRLHIP_R64)
+        rc = noalloc_launch<double, 1>(n, m, (const double *)V, ld, (double *)r, scr, s);
+    else if (vtype == TRLHIP_R32)
+        rc = noalloc_launch<float, 1>(n, m, (const float *)V, ld, (double *)r, scr, s);
+    else
+        rc = noalloc_launch<double, 2>(n, m, (const double *)V, ld, (double *)r, scr, s);
+    if (rc) return rc;
+    const int E = vtype == TRLHIP_C128 ? 2 : 1;
+    double sc[3];
+    TRY(hipMemcpyAsync(sc, scr + kScrScal, sizeof sc, hipMemcpyDeviceToHost, s));
+    if (h_out)
+        TRY(hipMemcpyAsync(h_out, scr + kScrH, sizeof(double) * E * (size_t)m, hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    if (hh) *hh = sc[S_HH];
+    if (rr2) *rr2 = sc[S_RR2];
+    return 0;
 }
 
 }  // extern "C"
