@@ -265,8 +265,17 @@ int trlkit_syev(int n, double *a, double *w)
 void trlkit_convergence_test(int nd, double *lambda, double *res, int *tag,
                              trl_info *info)
 {
-    double *wl = (double *)malloc(sizeof(double) * (size_t)nd);
-    double *wr = (double *)malloc(sizeof(double) * (size_t)nd);
+    double *w = (double *)malloc(sizeof(double) * (size_t)(2 * nd + 2));
+    trlkit_convergence_test_w(nd, lambda, res, tag, info, w);
+    free(w);
+}
+
+void trlkit_convergence_test_w(int nd, double *lambda, double *res, int *tag,
+                               trl_info *info, double *wrk)
+{
+    /* the reference's layout of its workspace: |res| then lambda, both in
+     * the sorted order (SRC/trlcore.c:2181-2201) */
+    double *wr = wrk, *wl = wrk + nd;
     if (info->lohi < -1) trlkit_sort2(nd, TRLKIT_BY_DIST, info->ref, lambda, res, tag);
     for (int i = 0; i < nd; ++i) {
         wl[i] = lambda[i];
@@ -351,8 +360,6 @@ void trlkit_convergence_test(int nd, double *lambda, double *res, int *tag,
         for (int i = ncl + 1; i < ncr; ++i)
             if (wr[i] < bnd) info->nec++;
     }
-    free(wl);
-    free(wr);
 }
 
 /* Final output order: the reference's trl_sort_eig (SRC/trlcore.c:2308). */
@@ -428,6 +435,29 @@ void trlkit_smooth_rr(int n, double *rr)
         prev = t;
     }
     rr[n - 1] = 2 * rr[n - 1] + rr[1 < n ? 1 : 0] + prev;
+}
+
+/* The reference's trl_smooth_zz (SRC/ztrlcore.c:1289-1332) on interleaved
+ * complex z[0:2n): as trl_smooth_rr per part, except that the last
+ * element's imaginary part adds twice its real part as just updated
+ * (ztrlcore.c:1328) instead of twice its own imaginary part.  Indices past
+ * the end of a vector shorter than 3 wrap (the reference reads beyond it). */
+void trlkit_smooth_zz(int n, double *z)
+{
+    if (n <= 0) return;
+    double r2 = z[0], i2 = z[1];
+    z[0] = 2 * z[0] + z[2 * (2 % n)] + z[2 * (n - 1)];
+    z[1] = 2 * z[1] + z[2 * (2 % n) + 1] + z[2 * (n - 1) + 1];
+    for (int i = 1; i < n - 1; ++i) {
+        const double r1 = z[2 * i], i1 = z[2 * i + 1];
+        z[2 * i] = 2 * z[2 * i] + z[2 * (i + 1)] + r2;
+        z[2 * i + 1] = 2 * z[2 * i + 1] + z[2 * (i + 1) + 1] + i2;
+        r2 = r1;
+        i2 = i1;
+    }
+    const int l = n - 1;
+    z[2 * l] = 2 * z[2 * l] + z[2 * (1 % n)] + r2;
+    z[2 * l + 1] = 2 * z[2 * l] + z[2 * (1 % n) + 1] + i2;
 }
 
 /* --------------------------------------------------------- checkpoints */

@@ -53,6 +53,14 @@ int trlkit_syev(int n, double *a, double *w);
 /* Restatements of the reference's host decisions, same arguments. */
 void trlkit_convergence_test(int nd, double *lambda, double *res, int *tag,
                              trl_info *info);
+/* The convergence test leaving in wrk[0:2nd) what the reference's leaves
+ * in its workspace: |res| then lambda, in the sorted order. */
+void trlkit_convergence_test_w(int nd, double *lambda, double *res, int *tag,
+                               trl_info *info, double *wrk);
+/* lambda and res are windows of one array laid out as the reference's
+ * workspace (SRC/trlcore.c:1200-1201, 1256, 1305-1306): the restart schemes
+ * read one element before and after them (lambda[-1], res[-1], lambda[nd],
+ * res[nd]), as the reference does. */
 int trlkit_shuffle_eig(int nd, int mnd, double *lambda, double *res,
                        int *tag, trl_info *info, int kept);
 int trlkit_set_locking(int jnd, int nlam, double *lambda, double *res,
@@ -62,6 +70,9 @@ void trlkit_sort_eig(int nd, int lohi, int nec, double ref, double *lambda,
 
 /* The reference's trl_smooth_rr (SRC/trlcore.c:1534-1575), end quirks kept. */
 void trlkit_smooth_rr(int n, double *rr);
+/* The reference's trl_smooth_zz (SRC/ztrlcore.c:1289-1332) on interleaved
+ * complex z[0:2n), its last-element quirk kept. */
+void trlkit_smooth_zz(int n, double *z);
 
 /* Checkpoint files, the reference's format (SRC/trlaux.c:780-1014): int
  * nrow, int jnd, alpha[jnd], beta[jnd], then columns of nrow doubles,

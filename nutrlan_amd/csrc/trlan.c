@@ -100,6 +100,9 @@ typedef struct {
     double *rot;                            /* maxlan^2 */
     double *lam, *res;                      /* maxlan + 1 */
     int *tag;                               /* maxlan + 1 */
+    double *ctw;                            /* 2 (maxlan + 2): the convergence test's workspace */
+    double *shw;                            /* 4 (maxlan + 2) + 1: lambda and res as laid out
+                                               for the restart schemes */
     double *yy;                             /* maxlan^2 */
     double *zq;                             /* maxlan^2 scratch */
     double *dsc, *esc;                      /* maxlan scratch */
@@ -140,32 +143,10 @@ static int lz_user_ortho(lz_t *z, int rcol, double *change)
     return 0;
 }
 
-/* Complex smoothing: the reference's trl_smooth_zz (SRC/ztrlcore.c:
- * 1289-1332), including its last element taking the real part of itself
- * into the imaginary sum. */
-static void smooth_zz(int n, double *z)
-{
-    if (n <= 0) return;
-    double r2 = z[0], i2 = z[1];
-    z[0] = 2 * z[0] + z[2 * (2 % n)] + z[2 * (n - 1)];
-    z[1] = 2 * z[1] + z[2 * (2 % n) + 1] + z[2 * (n - 1) + 1];
-    for (int i = 1; i < n - 1; ++i) {
-        const double r1 = z[2 * i], i1 = z[2 * i + 1];
-        z[2 * i] = 2 * z[2 * i] + z[2 * (i + 1)] + r2;
-        z[2 * i + 1] = 2 * z[2 * i + 1] + z[2 * (i + 1) + 1] + i2;
-        r2 = r1;
-        i2 = i1;
-    }
-    const int l = n - 1;
-    const double re = z[2 * l];
-    z[2 * l] = 2 * re + z[2 * (1 % n)] + r2;
-    z[2 * l + 1] = 2 * re + z[2 * (1 % n) + 1] + i2;
-}
-
 static int lz_random_column(lz_t *z, int col)
 {
     for (int i = 0; i < z->E * z->n; ++i) z->hx[i] = trl_rand_uniform(&z->rng);
-    if (z->cplx) smooth_zz(z->n, z->hx);
+    if (z->cplx) trlkit_smooth_zz(z->n, z->hx);
     else trlkit_smooth_rr(z->n, z->hx);
     DEV(trlhip_col_set(z->dev, col, z->hx));
     return 0;
@@ -199,7 +180,7 @@ static int lz_perturb_complex(lz_t *z, int rcol, int irnd, double rnrm, double o
             x[2 * k + 1] = trl_rand_uniform(&z->rng);
         }
     }
-    smooth_zz(n, x);
+    trlkit_smooth_zz(n, x);
     DEV(trlhip_col_set(z->dev, rcol, x));
     return 0;
 }
@@ -584,7 +565,32 @@ static void lz_write_checkpoint(lz_t *z, int jnd, int rcol)
     info->stat = ii;
 }
 
-static int lz_initial_guess(lz_t *z, const double *evec, int lde, int *j1)
+/* The workspace bound on the random perturbations of a guess <= 0 start:
+ * trlan() hands trlanczos nmis doubles of misc workspace (SRC/trlan.c:
+ * 172-226: maxlan (maxlan + 10), or what the caller's wrk holds beyond the
+ * basis) and trl_initial_guess gets what follows alpha, beta, alfrot and
+ * betrot in it (SRC/trlcore.c:805-806); ztrlan() passes the caller's nmis
+ * whole (SRC/ztrlcore.c:617-618). */
+static long lz_guess_cap(const lz_t *z, int lwrk)
+{
+    if (z->cplx) return lwrk;
+    const long maxlan = z->info->maxlan;
+    long ldb = ((long)(z->n + 3) / 4) * 4;
+    if (ldb % 4096 == 0) ldb += 4;
+    const long ii = DMAX(1L, maxlan - z->mev + 1) * ldb;
+    long nmis = maxlan * (maxlan + 10);
+    if (lwrk >= DMIN(ii, nmis)) {
+        if (lwrk >= ii + nmis)
+            nmis = lwrk - ii;
+        else if (lwrk >= DMAX(ii, nmis)) {
+            if (ii < nmis) nmis = lwrk;
+        } else if (ii > nmis)
+            nmis = lwrk;
+    }
+    return nmis - 4 * maxlan;
+}
+
+static int lz_initial_guess(lz_t *z, const double *evec, int lde, int lwrk, int *j1)
 {
     trl_info *info = z->info;
     const int j = info->nec;
@@ -609,8 +615,32 @@ static int lz_initial_guess(lz_t *z, const double *evec, int lde, int *j1)
         info->wrds_in = jnd * (z->n + z->n + 2) + z->n + 2;
         info->stat = rc;
         if (rc) return rc;
-        *j1 = jnd + j;
-        return 0;
+        /* The residual follows the restored basis, in column j + jnd.  The
+         * reference reads it only when the basis ends inside evec
+         * (SRC/trlaux.c:982-1007); otherwise its slot in base keeps the
+         * zeros trlan() filled it with, and trl_cgs below replaces a zero
+         * vector by a random one. */
+        const int jr = j + jnd;
+        if (DMIN(jnd, z->mev - j) >= z->mev - j) {
+            memset(x, 0, sizeof(double) * (size_t)z->n * E);
+            DEV(trlhip_col_set(z->dev, jr, x));
+        }
+        /* SRC/trlcore.c:1710-1741: the norm check looks at evec(:, nec)
+         * (the first restored vector), then trl_cgs orthogonalizes the
+         * residual against all jr restored columns and normalizes it */
+        double nr2 = 0.0;
+        DEV(trlhip_update(z->dev, 0, 0, &nr2, j, &nr2));
+        if (!(nr2 >= DBL_MIN && nr2 <= DBL_MAX)) {
+            if (lz_random_column(z, j)) return TRLHIP_STAT_DEVICE;
+            info->nrand++;
+        }
+        *j1 = jr;
+        double rnrm = 0.0;
+        int local_north = 0;
+        const int rcg = lz_cgs(z, jr, jr, &rnrm, &local_north, NULL);
+        if (rcg == TRLHIP_STAT_DEVICE) return rcg;
+        info->stat = rcg;
+        return rcg;
     }
     if (info->guess == 1 && z->dstart && !z->cplx) {
         x = NULL;  /* the start vector is already on the device */
@@ -624,7 +654,7 @@ static int lz_initial_guess(lz_t *z, const double *evec, int lde, int *j1)
             x[2 * k + 1] = info->guess == 0 ? 0.0 : 1.0;
         }
         if (info->guess < 0) {
-            int nran = DMIN(-info->guess, 4 * info->maxlan);
+            long nran = DMIN(-(long)info->guess, lz_guess_cap(z, lwrk));
             nran = 2 * (nran / 2);
             if (nran < z->n) {
                 for (int k = 0; k < nran - 1; k += 2) {
@@ -634,13 +664,13 @@ static int lz_initial_guess(lz_t *z, const double *evec, int lde, int *j1)
                 }
             } else {
                 for (int k = 0; k < 2 * z->n; ++k) x[k] = trl_rand_uniform(&z->rng);
-                smooth_zz(z->n, x);
+                trlkit_smooth_zz(z->n, x);
                 info->nrand++;
             }
         }
     } else {
         for (int k = 0; k < z->n; ++k) x[k] = 1.0;
-        int nran = DMIN(1 - info->guess, 4 * info->maxlan);
+        long nran = DMIN(1 - (long)info->guess, lz_guess_cap(z, lwrk));
         nran = 2 * (nran / 2);
         if (nran > 0 && nran < z->n) {
             for (int k = 0; k < nran - 1; k += 2) {
@@ -690,6 +720,8 @@ static int lz_alloc(lz_t *z, int maxlan)
     z->lam = (double *)calloc(m, sizeof(double));
     z->res = (double *)calloc(m, sizeof(double));
     z->tag = (int *)calloc(m, sizeof(int));
+    z->ctw = (double *)calloc(2 * m, sizeof(double));
+    z->shw = (double *)calloc(4 * m + 1, sizeof(double));
     z->yy = (double *)calloc(m * m, sizeof(double));
     z->zq = (double *)calloc(m * m, sizeof(double));
     z->dsc = (double *)calloc(m, sizeof(double));
@@ -698,7 +730,7 @@ static int lz_alloc(lz_t *z, int maxlan)
     z->hx = (double *)calloc((size_t)z->n * z->E, sizeof(double));
     z->hy = (double *)calloc((size_t)z->n * z->E, sizeof(double));
     return (z->alpha && z->beta && z->alfrot && z->betrot && z->rot && z->lam && z->res &&
-            z->tag && z->yy && z->zq && z->dsc && z->esc && z->iwrk && z->hx && z->hy)
+            z->tag && z->ctw && z->shw && z->yy && z->zq && z->dsc && z->esc && z->iwrk && z->hx && z->hy)
                ? 0
                : -1;
 }
@@ -713,6 +745,8 @@ static void lz_free(lz_t *z)
     free(z->lam);
     free(z->res);
     free(z->tag);
+    free(z->ctw);
+    free(z->shw);
     free(z->yy);
     free(z->zq);
     free(z->dsc);
@@ -1029,7 +1063,7 @@ static void lz_convergence_test(lz_t *z, int jnd)
     trl_info *info = z->info;
     const double tol = info->tol;
     if (z->mixed) info->tol = 0.5 * tol;
-    trlkit_convergence_test(jnd, z->lam, z->res, z->tag, info);
+    trlkit_convergence_test_w(jnd, z->lam, z->res, z->tag, info, z->ctw);
     info->tol = tol;
 }
 
@@ -1051,7 +1085,7 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
             }
     }
     int j1 = 0;
-    if (lz_initial_guess(z, evec, lde, &j1) != 0) return;
+    if (lz_initial_guess(z, evec, lde, lwrk, &j1) != 0) return;
     int jnd = j1, kept = jnd;
     int i1 = info->ned - jnd;
     int next_test = i1 + DMIN(i1, DMIN(6, info->ned / 2));
@@ -1128,10 +1162,21 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
             lz_convergence_test(z, jnd);
         }
         if (info->nec < info->ned && info->matvec < info->maxmv) {
-            /* lambda[jnd] aliases res[0] in the reference's workspace */
-            z->lam[jnd] = z->res[0];
-            kept = trlkit_shuffle_eig(jml, info->klan - locked, z->lam + locked, z->res + locked,
+            /* lambda and res as the reference's workspace holds them when
+             * it restarts (SRC/trlcore.c:868, 1200-1201, 1256, 1283): the
+             * last element of betrot, lambda[0:jnd), res[0:jnd), then the
+             * convergence test's |res| and lambda; the restart schemes
+             * read one element beyond either end of their windows
+             * (lambda[-1], res[-1], lambda[jml], res[jml]) */
+            double *w = z->shw;
+            w[0] = betrot[info->maxlan - 1];
+            memcpy(w + 1, z->lam, sizeof(double) * (size_t)jnd);
+            memcpy(w + 1 + jnd, z->res, sizeof(double) * (size_t)jnd);
+            memcpy(w + 1 + 2 * jnd, z->ctw, sizeof(double) * (size_t)(2 * jnd));
+            kept = trlkit_shuffle_eig(jml, info->klan - locked, w + 1 + locked, w + 1 + jnd + locked,
                                       z->tag + locked, info, kept);
+            memcpy(z->lam + locked, w + 1 + locked, sizeof(double) * (size_t)jml);
+            memcpy(z->res + locked, w + 1 + jnd + locked, sizeof(double) * (size_t)jml);
             int st = 0;
             if (kept * 3 < jml) {
                 st = lz_get_tvec(z, jml, 0, i2, alfrot + locked, betrot + locked, kept,

@@ -21,21 +21,20 @@ import numpy as np
 
 
 def _coo_to_csr(n, rows, cols, vals, dtype):
-    order = np.lexsort((cols, rows))
-    rows = rows[order]
-    cols = cols[order]
+    """CSR of a COO matrix: rows and columns sorted, duplicates summed in
+    their input order (one stable sort of row * n + col)."""
+    key = rows.astype(np.int64) * n + cols
+    order = np.argsort(key, kind="stable")
+    key = key[order]
     vals = vals[order]
-    # merge duplicates (sum), keeping sorted order
-    if len(rows):
-        key = rows.astype(np.int64) * n + cols
-        uniq, start = np.unique(key, return_index=True)
-        sums = np.add.reduceat(vals, start) if len(vals) else vals
-        rows = (uniq // n).astype(np.int64)
-        cols = (uniq % n).astype(np.int64)
-        vals = sums
+    if len(key):
+        start = np.flatnonzero(np.concatenate([[True], key[1:] != key[:-1]]))
+        vals = np.add.reduceat(vals, start)
+        key = key[start]
+    rows = key // n
+    cols = key - rows * n
     rp = np.zeros(n + 1, dtype=np.int64)
-    np.add.at(rp, rows + 1, 1)
-    rp = np.cumsum(rp)
+    rp[1:] = np.cumsum(np.bincount(rows, minlength=n))
     return rp.astype(np.int32), cols.astype(np.int32), vals.astype(dtype)
 
 

@@ -75,6 +75,28 @@ static double now_s(void)
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
+/* info->guess for the solve: 1 (the start vector in evec(:,0)) unless the
+ * environment variable TRL_REF_GUESS says otherwise (0: the reference's
+ * all-ones start; 2: resume from a checkpoint). */
+static int ref_guess(void)
+{
+    const char *g = getenv("TRL_REF_GUESS");
+    return g ? atoi(g) : 1;
+}
+
+/* Checkpoint settings from the environment: TRL_REF_CPFLAG and
+ * TRL_REF_CPFILE (write, trl_set_checkpoint), TRL_REF_OLDCPF (the prefix a
+ * guess = 2 solve resumes from), then info->guess as ref_guess() says. */
+static void ref_setup(trl_info *info)
+{
+    static char cpf[160];
+    const char *f = getenv("TRL_REF_CPFILE"), *fl = getenv("TRL_REF_CPFLAG");
+    if (f && fl) trl_set_checkpoint(info, atoi(fl), (char *)f);
+    const char *old = getenv("TRL_REF_OLDCPF");
+    if (old) strncpy(info->oldcpf, old, sizeof info->oldcpf - 1);
+    trl_set_iguess(info, 0, ref_guess(), 0, cpf);
+}
+
 static void pack_counters(const trl_info *info, int *iout, double *dout)
 {
     iout[0] = info->stat;
@@ -106,7 +128,7 @@ int ref_trlan_csr(int n, const int *rp, const int *ci, const double *val,
     trl_info info;
     memset(&info, 0, sizeof info);
     trl_init_info(&info, n, maxlan, lohi, ned, tol, restart, maxmv, NULL);
-    trl_set_iguess(&info, 0, 1, 0, NULL);
+    ref_setup(&info);
     info.mvparam = &A;
     if (getenv("TRL_REF_VERBOSE")) info.verbose = atoi(getenv("TRL_REF_VERBOSE"));
     int ldb = ((n + 3) / 4) * 4 + 4;
@@ -138,7 +160,7 @@ int ref_ztrlan_csr(int n, const int *rp, const int *ci, const double *val,
     trl_info info;
     memset(&info, 0, sizeof info);
     trl_init_info(&info, n, maxlan, lohi, ned, tol, restart, maxmv, NULL);
-    trl_set_iguess(&info, 0, 1, 0, NULL);
+    ref_setup(&info);
     info.mvparam = &A;
     int nmis = 10 * n > maxlan * (maxlan + 10) ? 10 * n : maxlan * (maxlan + 10);
     int ldwrk = maxlan * (maxlan + 10);

@@ -5,8 +5,12 @@ thick restart: [0..kl] from the left and [kr..nd) from the right, then
 compacts them to the front.  The scheme-specific choice of (kl, kr) follows
 the reference's schemes 0-8.  Integer arithmetic follows C: `nint(x)` is
 (int)(x+0.5) truncating toward zero, `/` on ints is truncating division.
-Scheme 3 reads operator vs. reorthogonalization time (restart.c:1117), so its
-trajectory is parity unpinned.
+Scheme 3 compares operator time with orthogonalization + restart time
+(restart.c:1052-1054); given the same trl_info clock fields it decides as the
+reference does (pinned on both sides of that branch by the compiled
+reference's trl_shuffle_eig, tests/golden/kit_cases.json), but in a solve
+those fields are wall-clock readings, so a scheme-3 trajectory is not
+reproducible run to run, in the reference or here.
 """
 import math
 
@@ -16,6 +20,68 @@ from .sorts import dsort2
 
 DBL_EPSILON = 2.220446049250313e-16
 DBL_MAX = 1.7976931348623157e308
+
+
+
+
+# C math library semantics for the scalar formulas of the schemes: the
+# platform libm (Python's math module calls it, as the reference's C does),
+# with IEEE results (inf, nan) where Python would raise.  numpy's own
+# vectorized exp/log/atan may differ from libm in the last bit, which can
+# flip a scheme's arg-min over many restarts.
+def _log(x):
+    x = float(x)
+    if x > 0.0:
+        return np.float64(math.log(x))
+    return np.float64(-np.inf if x == 0.0 else np.nan)
+
+
+def _sqrt(x):
+    x = float(x)
+    return np.float64(math.sqrt(x)) if x >= 0.0 else np.float64(np.nan)
+
+
+def _exp(x):
+    try:
+        return np.float64(math.exp(float(x)))
+    except OverflowError:
+        return np.float64(np.inf)
+
+
+def _atan(x):
+    return np.float64(math.atan(float(x)))
+
+
+def _pow(x, y):
+    try:
+        return np.float64(math.pow(float(x), float(y)))
+    except (OverflowError, ValueError):
+        with np.errstate(all="ignore"):
+            return np.power(np.float64(x), np.float64(y))
+
+class Strip:
+    """A window of a flat float64 array with C pointer indexing: index i is
+    element off + i of the array, for i < 0 and past the window's end too.
+    The reference's restart schemes index their lambda/res arguments one
+    element beyond either end (restart.c:633, 641, 681; 781, 788, 980),
+    reading the neighbouring workspace; windows of one array reproduce
+    that."""
+
+    def __init__(self, base, off):
+        self.base = base
+        self.off = off
+
+    def _at(self, i):
+        j = self.off + i
+        if j < 0 or j >= len(self.base):
+            raise IndexError(f"window index {i} outside the laid-out workspace")
+        return j
+
+    def __getitem__(self, i):
+        return self.base[self._at(i)]
+
+    def __setitem__(self, i, v):
+        self.base[self._at(i)] = v
 
 
 def _nint(x):
@@ -62,7 +128,7 @@ def _min_gap_ratio(info, nd, tind, res):
     gamma = info.maxmv * (info.nec + 1.0) / info.ned - info.matvec
     if gamma < info.klan:
         gamma = max((info.maxmv - info.matvec) / float(info.ned - info.nec), 2.0)
-    return min(np.log(res[tind] / (info.tol * info.anrm)) / gamma, 0.5)
+    return min(_log(res[tind] / (info.tol * info.anrm)) / gamma, 0.5)
 
 
 def _gap_ratio(i, j, tind, lam):
@@ -76,7 +142,7 @@ def _fixed(nd, mnd, tind, lam, res, info, kl, kr):
     extra = _nint((mnd - info.nec) * (0.4 + 0.1 * info.ned / float(mnd)))
     if extra > info.ned + info.ned and extra > 5:
         gmin = np.float64(mnd) / np.float64(info.ned)
-        extra = _nint((extra + (np.log(gmin) * info.ned) * gmin) / (1.0 + gmin))
+        extra = _nint((extra + (_log(gmin) * info.ned) * gmin) / (1.0 + gmin))
     minsep = max(3, max(nd // 5, nd - 4 * info.ned))
     gmin = _min_gap_ratio(info, nd, tind, res)
     if info.lohi > 0:
@@ -190,7 +256,7 @@ def _small_res(nd, mnd, tind, lam, res, info, kl, kr):
     extra = _nint((mnd - info.nec) * (0.4 + 0.1 * info.ned / float(mnd)))
     if extra > info.ned + info.ned and extra > 5:
         gmin = np.float64(mnd) / np.float64(info.ned)
-        extra = _nint((extra + (np.log(gmin) * info.ned) * gmin) / (1.0 + gmin))
+        extra = _nint((extra + (_log(gmin) * info.ned) * gmin) / (1.0 + gmin))
     kli, kri = kl, kr
     kl0, kr0 = kli, kri
     resmax = max(res[i] for i in range(nd)) if nd > 0 else 0.0
@@ -198,7 +264,7 @@ def _small_res(nd, mnd, tind, lam, res, info, kl, kr):
     if info.lohi > 0:
         if 0.0 <= acpt < 0.999:
             ii = tind - 1
-            acpt = max(np.sqrt(acpt) * res[tind], res[ii] + res[ii])
+            acpt = max(_sqrt(acpt) * res[tind], res[ii] + res[ii])
             acpt = min(acpt, resmax)
             kri = ii - 1
             while res[kri] < acpt and kri > kli + 3:
@@ -210,7 +276,7 @@ def _small_res(nd, mnd, tind, lam, res, info, kl, kr):
     elif info.lohi < 0:
         if 0.0 <= acpt < 0.999:
             ii = tind + 1
-            acpt = max(np.sqrt(acpt) * res[tind], res[ii] + res[ii])
+            acpt = max(_sqrt(acpt) * res[tind], res[ii] + res[ii])
             acpt = min(acpt, resmax)
             kli = ii + 1
             while res[kli] < acpt and kli < kri - 3:
@@ -341,11 +407,11 @@ def _dyn_gap(info, span):
         return int(span * def2)
     if z1 < 1.0:
         return int(span * def1)
-    tmp3 = np.log(z1 + np.sqrt(z1 - 1.0) * np.sqrt(z1 + 1.0)) / (2.0 * info.avgm)
-    tmp2 = np.log(z2 + np.sqrt(z2 - 1.0) * np.sqrt(z2 + 1.0)) / (info.klan - info.k + 1)
+    tmp3 = _log(z1 + _sqrt(z1 - 1.0) * _sqrt(z1 + 1.0)) / (2.0 * info.avgm)
+    tmp2 = _log(z2 + _sqrt(z2 - 1.0) * _sqrt(z2 + 1.0)) / (info.klan - info.k + 1)
     tmp = tmp2 / tmp3
-    tmp = tmp ** 2.0
-    tmp = np.arctan(tmp) * (2.0 / 3.141592654)
+    tmp = _pow(tmp, 2.0)
+    tmp = _atan(tmp) * (2.0 / 3.141592654)
     tmp = dw + (up - dw) * tmp
     return int(span * tmp)
 
@@ -353,7 +419,7 @@ def _dyn_gap(info, span):
 def _max_reduction(nd, tind, kept, lam, res, info, kl, kr):
     """trl_restart_max_reduction (restart.c:1280-1408)."""
     def merit(i, j):
-        return np.sqrt((lam[i] - info.trgt) / (lam[j] - lam[i])) * abs(j - i)
+        return _sqrt((lam[i] - info.trgt) / (lam[j] - lam[i])) * abs(j - i)
 
     lohi, klm, krm = _search_range(nd, lam, res, info, kl, kr, tind)
     kli, kri = klm, krm
@@ -392,7 +458,7 @@ def _max_gap_cost_ratio(n, tind, info, lam, res, kl, kr):
     min_l = 0
     for i in range(k1, k2 - t + 1):
         for j in range(i + t, k2 + 1):
-            gamma0 = np.sqrt((lam[i + 1] - lam[tind]) / (lam[j - 1] - lam[i + 1]))
+            gamma0 = _sqrt((lam[i + 1] - lam[tind]) / (lam[j - 1] - lam[i + 1]))
             l1 = (i + 1) + (n - j)
             l2 = l1 + info.locked
             mn2 = l2 + 1
@@ -409,7 +475,7 @@ def _max_gap_cost_ratio(n, tind, info, lam, res, kl, kr):
                     min_gamma, min_gamma0, min_l = gamma, gamma0, l2
     info.klan = min_m
     info.k1, info.k2, info.k = min_k1, min_k2, min_l
-    info.predicted_crate = np.exp(-min_gamma)
+    info.predicted_crate = _exp(-min_gamma)
     info.mgamma = min_gamma
     info.gamma0 = min_gamma0
     info.old_target = lam[tind]
@@ -425,15 +491,15 @@ def _max_gap_cost_ratio_static(n, tind, info, lam, res, kl, kr):
     lohi, k1, k2 = _search_range(n, lam, res, info, kl, kr, tind)
     info.lohi = lohi
     if info.crat > 0.0:
-        tmp2 = -np.log(info.crat)
-        tmp3 = -(tmp2 * info.maxlan) / np.log(info.tol * info.anrm)
+        tmp2 = -_log(info.crat)
+        tmp3 = -(tmp2 * info.maxlan) / _log(info.tol * info.anrm)
         if tmp3 >= 0.5 or info.klan < info.maxlan:
             t = int(abs(k2 - k1) * 0.8)
         else:
             tmp = _min_gap_ratio(info, nd, tind, res)
             tmp = tmp2 / tmp
-            tmp = tmp ** 0.25
-            tmp = np.arctan(tmp) * (4.0 / 3.141592654)
+            tmp = _pow(tmp, 0.25)
+            tmp = _atan(tmp) * (4.0 / 3.141592654)
             t = int(abs(k2 - k1) * tmp)
     else:
         t = int(abs(k2 - k1) * 0.8)
@@ -442,7 +508,7 @@ def _max_gap_cost_ratio_static(n, tind, info, lam, res, kl, kr):
     min_k1, min_k2, min_m = k1, k2, info.klan
     for i in range(k1, k2 - t + 1):
         for j in range(i + t, k2 + 1):
-            gamma0 = np.sqrt((lam[i + 1] - lam[tind]) / (lam[j - 1] - lam[i + 1]))
+            gamma0 = _sqrt((lam[i + 1] - lam[tind]) / (lam[j - 1] - lam[i + 1]))
             l1 = (i + 1) + (n - j)
             l2 = l1 + info.locked
             mn2 = mn1

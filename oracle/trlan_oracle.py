@@ -45,7 +45,7 @@ import math
 import numpy as np
 
 from . import lapack
-from .restart_oracle import shuffle_eig
+from .restart_oracle import Strip, shuffle_eig
 from .sorts import dsort2, dsort2a, dsort2s, dsort2su, dsort2sd
 
 DBL_EPSILON = np.finfo(np.float64).eps
@@ -95,7 +95,7 @@ class Info:
         self.rfact = 0.0
         self.ref = 0.0
         self.cpflag = 0
-        # timing fields used only by restart scheme 3 (SRC/restart.c:1117)
+        # timing fields used only by restart scheme 3 (SRC/restart.c:1052-1054)
         self.clk_op = 0.0
         self.tick_o = 0.0
         self.clk_orth = 0.0
@@ -230,7 +230,7 @@ class _State:
 
 def smooth_zz(z):
     """trl_smooth_zz (SRC/ztrlcore.c:1289-1332), including the last
-    element's imaginary part taking its own real part."""
+    element's imaginary part taking twice its own updated real part."""
     n = len(z)
     if n <= 0:
         return
@@ -242,8 +242,10 @@ def smooth_zz(z):
         r1, i1 = z[i].real, z[i].imag
         z[i] = complex(2 * z[i].real + z[i + 1].real + r2, 2 * z[i].imag + z[i + 1].imag + i2)
         r2, i2 = r1, i1
-    re = z[n - 1].real
-    z[n - 1] = complex(2 * re + z[1 % n].real + r2, 2 * re + z[1 % n].imag + i2)
+    re = 2 * z[n - 1].real + z[1 % n].real + r2
+    # the imaginary part adds twice the real part as just updated
+    # (ztrlcore.c:1327-1328)
+    z[n - 1] = complex(re, 2 * re + z[1 % n].imag + i2)
 
 
 def _gdot(st, c1, c2, rr):
@@ -355,6 +357,32 @@ def _perturb_complex(st, rr, irnd, rnrm, old_rnrm):
     smooth_zz(rr)
 
 
+def guess_cap(info, nrow, mev, lwrk, complex_):
+    """The workspace bound on the number of random perturbations of a
+    guess <= 0 start.  trlan() hands trlanczos `nmis` doubles of misc
+    workspace (SRC/trlan.c:172-226: maxlan*(maxlan+10), or what the
+    caller's wrk holds beyond the basis), and trl_initial_guess gets what
+    follows alpha/beta/alfrot/betrot in it (trlcore.c:805-806).  ztrlan()
+    passes the caller's nmis whole (ztrlcore.c:617-618)."""
+    if complex_:
+        return lwrk
+    maxlan = info.maxlan
+    ldb = ((nrow + 3) // 4) * 4
+    if ldb % 4096 == 0:
+        ldb += 4
+    ii = max(1, maxlan - mev + 1) * ldb
+    nmis = maxlan * (maxlan + 10)
+    if lwrk >= min(ii, nmis):
+        if lwrk >= ii + nmis:
+            nmis = lwrk - ii
+        elif lwrk >= max(ii, nmis):
+            if ii < nmis:
+                nmis = lwrk
+        elif ii > nmis:
+            nmis = lwrk
+    return nmis - 4 * maxlan
+
+
 def initial_guess(info, st, start):
     """trl_initial_guess (SRC/trlcore.c:1577-1763) / ztrl_initial_guess
     (SRC/ztrlcore.c:1335-1567) for guess <= 1.
@@ -373,7 +401,7 @@ def initial_guess(info, st, start):
     elif st.complex:
         col[:] = 1.0 if info.guess == 0 else 1.0 + 1.0j
         if info.guess < 0:
-            nran = 2 * (min(-info.guess, 4 * info.maxlan) // 2)
+            nran = 2 * (min(-info.guess, st.guess_cap) // 2)
             if nran < n:
                 w = st.rng.random(nran) + 1j * st.rng.random(nran)
                 for i in range(0, nran - 1, 2):
@@ -384,7 +412,7 @@ def initial_guess(info, st, start):
                 info.nrand += 1
     else:
         col[:] = 1.0
-        nran = min(1 - info.guess, 4 * info.maxlan)
+        nran = min(1 - info.guess, st.guess_cap)
         nran = 2 * (nran // 2)
         if 0 < nran < n:
             w = st.rng.random(nran)
@@ -610,6 +638,8 @@ def convergence_test(nd, lam, res, info):
         for i in range(ncl + 1, ncr):
             if wr[i] < bnd:
                 info.nec += 1
+    # what the reference leaves in its workspace: |res| then lambda, sorted
+    return np.concatenate([wr, wl])
 
 
 def sort_eig(nd, lohi, nec, ref, lam, res):
@@ -778,14 +808,16 @@ def ritz_vectors(st, lck, ny, yy, jl):
     st.V[:, lck:lck + ny] = out
 
 
-def trlan(matvec, info, nrow, mev, start, seed=12345, complex_=False, gdot_quirk=True):
+def trlan(matvec, info, nrow, mev, start, seed=12345, complex_=False, gdot_quirk=True, lwrk=0):
     """trlan (SRC/trlan.c:21-301) + trlanczos (SRC/trlcore.c:723-1532), or
     with complex_=True ztrlan (SRC/ztrlan.c:26-221) + ztrlanczos
     (SRC/ztrlcore.c:436-1286).
 
     matvec(x) -> A x (numpy).  start: the start vector (used when
     info.guess == 1).  gdot_quirk (complex only) reproduces the reference's
-    two-column ztrl_g_dot_ branches; False gives the exact V^H r.
+    two-column ztrl_g_dot_ branches; False gives the exact V^H r.  lwrk:
+    the caller's workspace length (trlan's lwrk; ztrlan's nmis), which
+    bounds the perturbations of a guess <= 0 start.
     Returns dict(eval, evec, wrk, info).
     """
     rng = np.random.default_rng(seed)
@@ -799,6 +831,7 @@ def trlan(matvec, info, nrow, mev, start, seed=12345, complex_=False, gdot_quirk
     if info.stat != 0:
         return dict(eval=out_eval, evec=np.zeros((nrow, mev)), wrk=np.zeros(mev), info=info)
     st = _State(nrow, mev, info.maxlan, rng, complex_, gdot_quirk)
+    st.guess_cap = guess_cap(info, nrow, mev, lwrk, complex_)
     wrk_out = _trlanczos(matvec, info, st, mev, out_eval, start)
     evec = st.V[:, :mev].copy()
     return dict(eval=out_eval, evec=evec, wrk=wrk_out, info=info)
@@ -926,17 +959,21 @@ def _trlanczos(matvec, info, st, mev, eval_, start):
             info.stat = ierr
             if ierr != 0:
                 break
-            convergence_test(jnd, lam, res, info)
+            ctw = convergence_test(jnd, lam, res, info)
             degen = -1
         if (degen != -1 or info.nec < info.ned) and info.matvec < info.maxmv:
-            # The reference's lambda sits right before res in one workspace
-            # (trlcore.c:1191-1192), so a one-past-the-end read of
-            # &lambda[locked] by a restart scheme sees res[0]; keep that.
-            lam_s = np.concatenate([lam[locked:jnd], res[:1]])
-            res_s = res[locked:jnd].copy()
+            # The reference's workspace around lambda and res when it
+            # restarts (trlcore.c:868, 1200-1201, 1256, 1283): betrot's last
+            # element, lambda[0:jnd), res[0:jnd), then the convergence
+            # test's |res| and lambda.  The restart schemes read one element
+            # beyond either end of their windows, so they get windows of
+            # this one array with C indexing (Strip).
+            w = np.concatenate([[betrot[info.maxlan - 1]], lam[:jnd], res[:jnd], ctw])
+            lam_s = Strip(w, 1 + locked)
+            res_s = Strip(w, 1 + jnd + locked)
             kept = shuffle_eig(jml, info.klan - locked, lam_s, res_s, info, kept, locked)
-            lam[locked:jnd] = lam_s[:jml]
-            res[locked:jnd] = res_s
+            lam[locked:jnd] = w[1 + locked:1 + jnd]
+            res[locked:jnd] = w[1 + jnd + locked:1 + 2 * jnd]
             yy = None
             if kept * 3 < jml:
                 yy, ierr = get_tvec(jml, alfrot[locked:], betrot[locked:], 0, i2, rot,

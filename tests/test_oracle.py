@@ -19,6 +19,7 @@ import numpy as np
 import pytest
 import scipy.linalg as sla
 
+from nutrlan_amd import trlan as T
 from nutrlan_amd import workloads as W
 from oracle import ref, restart_oracle
 from oracle import trlan_oracle as O
@@ -44,16 +45,35 @@ def build(c):
 def run_oracle(c, rp, ci, v, x0, quirk=True):
     n = len(rp) - 1
     info = O.init_info(n, c["maxlan"], c["lohi"], c["ned"], c["tol"], c["restart"], c["maxmv"])
-    O.set_iguess(info, 0, 1)
+    O.set_iguess(info, 0, c["start"][1] if c["start"][0] == "guess" else 1)
     o = O.trlan(W.csr_matvec(rp, ci, v), info, n, c["mev"], x0, complex_=c["complex"],
                 gdot_quirk=quirk)
     return o, info
 
 
-# scheme 7 (restart.c:121-299) diverges from the reference on strongly
-# degenerate spectra after hundreds of restarts; it is pinned where it
-# agrees (diag2000 cases) and otherwise only through eigenvalues.
-@pytest.mark.parametrize("c", REAL, ids=[c["name"] for c in REAL])
+# Restart schemes 7 and 8 (restart.c:121-432) pick the kept pairs and the
+# next basis size by an arg-min over real-valued costs; over hundreds of
+# restarts their trajectories follow the last bits of the Ritz values, so
+# away from diagonal spectra they are held to eigenvalues (and shown to make
+# the reference routine's own choices at every restart in
+# test_adaptive_scheme_decisions_match_reference_routine).
+ADAPTIVE = {"lap10_lg8_r7", "and2d48_sm8_r7", "and2d48_sm8_r8", "rand3000_lg10_r7"}
+EXACT = [c for c in REAL if c["name"] not in ADAPTIVE]
+ADAPT = [c for c in REAL if c["name"] in ADAPTIVE]
+
+
+@pytest.mark.parametrize("c", ADAPT, ids=[c["name"] for c in ADAPT])
+def test_adaptive_golden(c):
+    rp, ci, v, x0 = build(c)
+    o, info = run_oracle(c, rp, ci, v, x0)
+    assert info.stat == 0 and info.nec == c["nec"]
+    nec = c["nec"]
+    assert np.abs(o["eval"][:nec] - np.array(c["eval"])).max() <= 1e-10 * c["anrm"]
+    assert np.abs(o["wrk"][:nec]).max() <= c["tol"] * info.anrm
+    assert abs(info.matvec - c["matvec"]) <= 0.1 * c["matvec"]
+
+
+@pytest.mark.parametrize("c", EXACT, ids=[c["name"] for c in EXACT])
 def test_real_golden(c):
     rp, ci, v, x0 = build(c)
     o, info = run_oracle(c, rp, ci, v, x0)
@@ -153,3 +173,53 @@ def test_live_reference_matches_golden():
     r = ref.solve_csr(rp, ci, v, c["lohi"], c["ned"], c["maxlan"], c["tol"], c["restart"],
                       c["maxmv"], c["mev"], x0)
     assert [r[k] for k in KEYS] == [c[k] for k in KEYS]
+
+
+@pytest.mark.skipif(not ref.available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("case", [
+    ("anderson2d", [48, 6.0, 6], -1, 8, 32, 1e-8, 7, 20000, 8, ["uniform", 7]),
+    ("laplace3d", [10, 3], 1, 8, 40, 1e-8, 7, 20000, 8, ["uniform", 2]),
+    ("laplace3d", [10, 3], 0, 8, 40, 1e-8, 8, 3000, 8, ["uniform", 2]),
+])
+def test_adaptive_scheme_decisions_match_reference_routine(case, monkeypatch):
+    """Restart schemes 7/8 (restart.c:121-432) choose kept pairs and the
+    next basis size by an arg-min over real-valued costs, so a trajectory
+    over hundreds of restarts is sensitive to the last bits of the Ritz
+    values and residual norms, which the oracle's tridiagonal eigensolver
+    and BLAS order do not reproduce.  This shows that the restatement is
+    not the cause: at every restart of the oracle's own solve, the compiled
+    reference's trl_shuffle_eig, given the same state (the workspace array
+    around lambda/res and the trl_info fields), makes the same choice."""
+    import sys
+
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    import make_golden as MG
+
+    gen, args, lohi, ned, maxlan, tol, restart, maxmv, mev, start = case
+    rp, ci, v = MG.make_operator(gen, args)
+    n = len(rp) - 1
+    x0 = MG.make_start(start, n)
+    orig = O.shuffle_eig
+    seen = []
+
+    def checked(nd, mnd, lam, res, info, kept, locked):
+        w0 = lam.base.copy()
+        snap = {f: getattr(info, f) for f, _ in T.TrlInfo._fields_ if hasattr(info, f)
+                and isinstance(getattr(info, f), (int, float, np.integer, np.floating))}
+        k = orig(nd, mnd, lam, res, info, kept, locked)
+        ti = T.TrlInfo()
+        for f, val in snap.items():
+            setattr(ti, f, type(getattr(ti, f))(val))
+        w1 = w0.copy()
+        kr = ref.shuffle_eig(nd, mnd, w1, lam.off, res.off, ti, kept, locked)
+        assert kr == k and ti.klan == info.klan
+        assert np.array_equal(w1[lam.off:lam.off + k], lam.base[lam.off:lam.off + k])
+        assert np.array_equal(w1[res.off:res.off + k], res.base[res.off:res.off + k])
+        seen.append(k)
+        return k
+
+    monkeypatch.setattr(O, "shuffle_eig", checked)
+    info = O.init_info(n, maxlan, lohi, ned, tol, restart, maxmv)
+    O.set_iguess(info, 0, 1)
+    O.trlan(W.csr_matvec(rp, ci, v), info, n, mev, x0)
+    assert info.stat == 0 and len(seen) > 20, (info.stat, len(seen))
