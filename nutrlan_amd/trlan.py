@@ -276,9 +276,10 @@ def output_buffers(n, mev):
 
 
 def solve_csr(A, lohi, ned, maxlan, tol, restart, maxmv, mev, start, rfact=0.0,
-              checkpoint=None, resume=None, out=None):
+              checkpoint=None, resume=None, out=None, guess=1):
     """trlan()/ztrlan() on the device CSR operator A with start vector
-    `start` (guess = 1).  checkpoint=(cpflag, prefix) writes checkpoint
+    `start` (guess = 1; guess <= 0: the reference's own start, `start` is
+    not read).  checkpoint=(cpflag, prefix) writes checkpoint
     files (trl_set_checkpoint); resume=prefix restarts from the file written
     under that prefix (guess = 2, oldcpf).  out = output_buffers(n, mev)
     reuses the caller's arrays (real operators; the returned dict refers to
@@ -295,7 +296,7 @@ def solve_csr(A, lohi, ned, maxlan, tol, restart, maxmv, mev, start, rfact=0.0,
         info.oldcpf = str(resume).encode()
         lib().trl_set_iguess(ctypes.byref(info), 0, 2, 0, None)
     else:
-        lib().trl_set_iguess(ctypes.byref(info), 0, 1, 0, None)
+        lib().trl_set_iguess(ctypes.byref(info), 0, int(guess), 0, None)
     if rfact:
         lib().trl_set_restart(ctypes.byref(info), rfact)
     info.mvparam = A.handle.value
@@ -416,6 +417,18 @@ def stats_get():
     lib().trlhip_stats_get(_ptr(out), len(out))
     return {k: (int(out[3 * i]), float(out[3 * i + 1]), float(out[3 * i + 2]))
             for i, k in enumerate(KERNELS)}
+
+
+def stats_flops():
+    """{kernel: floating-point operations of the timed launches} since
+    stats_enable (the restart GEMMs' 2 n jl k per launch; 0 for kernels
+    whose bound is bandwidth)."""
+    L = lib()
+    L.trlhip_stats_flops.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.trlhip_stats_flops.restype = ctypes.c_int
+    out = np.zeros(len(KERNELS))
+    L.trlhip_stats_flops(_ptr(out), len(out))
+    return {k: float(out[i]) for i, k in enumerate(KERNELS)}
 
 
 def device_count():
