@@ -471,6 +471,14 @@ int trlkit_write_checkpoint(const char *file, int nrow, int jnd, const double *a
                             const double *beta, int ncol,
                             int (*col)(void *u, int i, double *buf), void *u, double *buf)
 {
+    return trlkit_write_checkpoint_e(file, nrow, 1, jnd, alpha, beta, ncol, col, u, buf);
+}
+
+int trlkit_write_checkpoint_e(const char *file, int nrow, int E, int jnd, const double *alpha,
+                              const double *beta, int ncol,
+                              int (*col)(void *u, int i, double *buf), void *u, double *buf)
+{
+    const size_t cw = (size_t)nrow * E;
     FILE *f = fopen(file, "w");
     if (!f) {
         printf("TRL_WRITE_CHECKPOINT: failed to open file: %s.\n", file);
@@ -481,7 +489,7 @@ int trlkit_write_checkpoint(const char *file, int nrow, int jnd, const double *a
         (jnd > 0 && fwrite(beta, sizeof(double), (size_t)jnd, f) != (size_t)jnd))
         return ck_close(f, -223, -222);
     for (int i = 0; i < ncol; ++i) {
-        if (col(u, i, buf) != 0 || fwrite(buf, sizeof(double), (size_t)nrow, f) != (size_t)nrow)
+        if (col(u, i, buf) != 0 || fwrite(buf, sizeof(double), cw, f) != cw)
             return ck_close(f, -223, -222);
     }
     return ck_close(f, 0, -223);
@@ -491,6 +499,14 @@ int trlkit_read_checkpoint(const char *file, int nrow, int mev, int maxjnd, int 
                            double *alpha, double *beta,
                            int (*col)(void *u, int i, double *buf), void *u, double *buf)
 {
+    return trlkit_read_checkpoint_e(file, nrow, 1, mev, maxjnd, jnd_out, alpha, beta, col, u, buf);
+}
+
+int trlkit_read_checkpoint_e(const char *file, int nrow, int E, int mev, int maxjnd, int *jnd_out,
+                             double *alpha, double *beta,
+                             int (*col)(void *u, int i, double *buf), void *u, double *buf)
+{
+    const size_t cw = (size_t)nrow * E;
     FILE *f = fopen(file, "r");
     if (!f) {
         printf("TRL_READ_CHECKPOINT: failed to open check-point file %s.\n", file);
@@ -515,7 +531,7 @@ int trlkit_read_checkpoint(const char *file, int nrow, int mev, int maxjnd, int 
     const int j1 = jnd < mev ? jnd : mev;
     const int ncol = j1 < mev ? j1 + 1 : jnd;
     for (int i = 0; i < ncol; ++i) {
-        if (fread(buf, sizeof(double), (size_t)nrow, f) != (size_t)nrow || col(u, i, buf) != 0)
+        if (fread(buf, sizeof(double), cw, f) != cw || col(u, i, buf) != 0)
             return ck_close(f, -215, -216);
     }
     *jnd_out = jnd;

@@ -93,6 +93,16 @@ int trlkit_write_checkpoint(const char *file, int nrow, int jnd, const double *a
 int trlkit_read_checkpoint(const char *file, int nrow, int mev, int maxjnd, int *jnd_out,
                            double *alpha, double *beta,
                            int (*col)(void *u, int i, double *buf), void *u, double *buf);
+/* The same with E doubles per element: E = 2 is the complex format of the
+ * reference's ztrl_write_checkpoint / ztrl_read_checkpoint
+ * (SRC/ztrlaux.c:69-307): the same header and real alpha/beta, then
+ * columns of nrow interleaved (re, im) pairs. */
+int trlkit_write_checkpoint_e(const char *file, int nrow, int E, int jnd, const double *alpha,
+                              const double *beta, int ncol,
+                              int (*col)(void *u, int i, double *buf), void *u, double *buf);
+int trlkit_read_checkpoint_e(const char *file, int nrow, int E, int mev, int maxjnd, int *jnd_out,
+                             double *alpha, double *beta,
+                             int (*col)(void *u, int i, double *buf), void *u, double *buf);
 /* The reference's per-process file name (SRC/trlaux.c:1067-1104) for one
  * process: base up to its first blank, then rank 0 as one digit. */
 void trlkit_pe_filename(char *out, size_t cap, const char *base);

@@ -547,9 +547,10 @@ static int lz_get_col(void *u, int i, double *buf)
     return trlhip_col_get(io->z->dev, i < io->jnd ? i : io->rcol, buf);
 }
 
-/* The reference's write_checkpoint (SRC/trlcore.c:434-460): alpha/beta of
- * the jnd basis vectors, the vectors and the residual (column rcol) to
- * cpfile + process number.  Real bases only. */
+/* The reference's write_checkpoint (SRC/trlcore.c:434-460) /
+ * zwrite_checkpoint (SRC/ztrlcore.c:360-430): alpha/beta of the jnd basis
+ * vectors, the vectors and the residual (column rcol) to cpfile + process
+ * number; complex columns as interleaved (re, im) pairs. */
 static void lz_write_checkpoint(lz_t *z, int jnd, int rcol)
 {
     trl_info *info = z->info;
@@ -557,8 +558,8 @@ static void lz_write_checkpoint(lz_t *z, int jnd, int rcol)
     trlkit_pe_filename(file, sizeof file, info->cpfile);
     lz_colio_t io = {z, 0, jnd, rcol};
     const clock_t c1 = lz_clock();
-    const int ii = trlkit_write_checkpoint(file, z->n, jnd, z->alpha, z->beta, jnd + 1, lz_get_col,
-                                           &io, z->hx);
+    const int ii = trlkit_write_checkpoint_e(file, z->n, z->E, jnd, z->alpha, z->beta, jnd + 1,
+                                             lz_get_col, &io, z->hx);
     const clock_t c2 = lz_clock();
     info->clk_out += c2 >= c1 ? c2 - c1 : 0;
     info->wrds_out += jnd * (z->n + z->n + 2) + z->n + 2;
@@ -597,19 +598,16 @@ static int lz_initial_guess(lz_t *z, const double *evec, int lde, int lwrk, int 
     double *x = z->hx;
     const int E = z->E;
     if (info->guess > 1) {
-        /* restore a checkpoint (SRC/trlcore.c:1653-1679): oldcpf if set,
-         * else cpfile; the basis continues from its saved state */
-        if (z->cplx) {
-            info->stat = TRLAN_STAT_NO_CHECKPOINT;
-            return info->stat;
-        }
+        /* restore a checkpoint (SRC/trlcore.c:1653-1679, complex
+         * ztrlcore.c:1413-1440): oldcpf if set, else cpfile; the basis
+         * continues from its saved state */
         char file[160];
         trlkit_pe_filename(file, sizeof file, strlen(info->oldcpf) > 0 ? info->oldcpf : info->cpfile);
         lz_colio_t io = {z, j, 0, 0};
         int jnd = 0;
         const clock_t c1 = lz_clock();
-        const int rc = trlkit_read_checkpoint(file, z->n, z->mev - j, z->ncols - 1 - j, &jnd,
-                                              z->alpha + j, z->beta + j, lz_put_col, &io, x);
+        const int rc = trlkit_read_checkpoint_e(file, z->n, E, z->mev - j, z->ncols - 1 - j, &jnd,
+                                                z->alpha + j, z->beta + j, lz_put_col, &io, x);
         const clock_t c2 = lz_clock();
         info->clk_in = c2 >= c1 ? c2 - c1 : 0;
         info->wrds_in = jnd * (z->n + z->n + 2) + z->n + 2;
@@ -1089,8 +1087,8 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
     int jnd = j1, kept = jnd;
     int i1 = info->ned - jnd;
     int next_test = i1 + DMIN(i1, DMIN(6, info->ned / 2));
-    /* checkpoints every maxmv/cpflag operator applications (real bases) */
-    const int ckpt = info->cpflag > 0 && !z->cplx;
+    /* checkpoints every maxmv/cpflag operator applications */
+    const int ckpt = info->cpflag > 0;
     int chkpnt = ckpt ? info->maxmv / info->cpflag : info->maxmv + info->maxlan;
     int rr = jnd, qa = -1, qb = -1, i2 = 0;
     while (info->matvec < info->maxmv && info->nec < info->ned) {

@@ -15,10 +15,6 @@
 extern "C" {
 #endif
 
-/* info->stat when guess > 1 asks for a checkpoint file that cannot be
- * read (the reference's trl_read_checkpoint codes are -212..-215). */
-#define TRLAN_STAT_NO_CHECKPOINT (-212)
-
 /* The device CSR handle behind an operator callback, or NULL when `op` is
  * not this library's device operator for `vtype` or mvparam is not a live
  * handle of that type. */

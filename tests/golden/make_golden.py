@@ -77,6 +77,9 @@ CKPT_CASES = [
     # starts its next vector from a zero column
     ("lap12_ckpt_full_ned_eq_mev", "laplace3d", [12, 4], -1, 6, 32, 1e-8, 1, 1.0, 20000, 6,
      ["uniform", 5]),
+    # complex (ztrl_write_checkpoint / ztrl_read_checkpoint)
+    ("zand8_ckpt_cut", "anderson3d", [8, 3], -1, 6, 40, 1e-8, 1, 0.6, 20000, 6,
+     ["zuniform", 4]),
 ]
 
 
@@ -145,19 +148,20 @@ def run_ckpt_case(case):
     rp, ci, v = make_operator(gen, args)
     n = len(rp) - 1
     x0 = make_start(start, n)
-    full = ref.solve_csr(rp, ci, v, lohi, ned, maxlan, tol, restart, 40000, mev, x0)
+    cx = np.iscomplexobj(x0)
+    full = ref.solve_csr(rp, ci, v, lohi, ned, maxlan, tol, restart, 40000, mev, x0, complex_=cx)
     cut = int(full["matvec"] * frac) if frac < 1.0 else 40000
     with tempfile.TemporaryDirectory() as d:
         prefix = os.path.join(d, "cp")
         part = ref.solve_csr(rp, ci, v, lohi, ned, maxlan, tol, restart, cut, mev, x0,
-                             checkpoint=(1, prefix))
+                             checkpoint=(1, prefix), complex_=cx)
         raw = open(prefix + "0", "rb").read()
         jnd = int(np.frombuffer(raw[4:8], dtype=np.int32)[0])
         res = ref.solve_csr(rp, ci, v, lohi, ned, maxlan, tol, restart, maxmv2, mev,
-                            np.zeros(n), resume=prefix)
+                            np.zeros(n, dtype=x0.dtype), resume=prefix, complex_=cx)
     keys = ["stat", "nec", "matvec", "nloop", "north", "nrand", "locked"]
     return dict(
-        name=name, generator=gen, args=args, lohi=lohi, ned=ned, maxlan=maxlan, tol=tol,
+        name=name, complex=cx, generator=gen, args=args, lohi=lohi, ned=ned, maxlan=maxlan, tol=tol,
         restart=restart, maxmv_cut=cut, maxmv_resume=maxmv2, mev=mev, start=start, cpflag=1,
         full={k: full[k] for k in keys}, part={k: part[k] for k in keys}, file_jnd=jnd,
         resume=dict({k: res[k] for k in keys}, anrm=res["anrm"],

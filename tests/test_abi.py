@@ -47,7 +47,7 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
-def test_info_layout_matches_reference():
+def test_info_layout_matches_reference(tmp_path):
     lay = GOLD["info_layout"]  # as the compiled reference reported it
     fields = ["stat", "lohi", "ned", "nec", "tol", "mpicomp", "maxlan", "klan", "maxmv",
               "restart", "locked", "guess", "matvec", "nloop", "north", "nrand", "anrm",
@@ -57,8 +57,7 @@ def test_info_layout_matches_reference():
     for f, off in zip(fields, lay[1:]):
         assert getattr(T.TrlInfo, f).offset == off, f
     # the C header itself, compiled here
-    src = os.path.join(ROOT, "_work", "layout_probe.c")
-    os.makedirs(os.path.dirname(src), exist_ok=True)
+    src = str(tmp_path / "layout_probe.c")
     with open(src, "w") as fh:
         fh.write('#include <stddef.h>\n#include <stdio.h>\n#include "trlan.h"\n'
                  'int main(void){printf("%zu %zu %zu %zu\\n", sizeof(trl_info),'

@@ -485,3 +485,72 @@ def test_sort2_vs_reference(kit):
         y1, y2 = list(a1), list(a2)
         fn[c["key"]](n, c["s"], y1, y2)
         assert y1 == c["a1_out"] and y2 == c["a2_out"]
+
+
+@pytest.mark.skipif(not __import__("oracle.ref", fromlist=["available"]).available(),
+                    reason="compiled reference not built here")
+def test_complex_checkpoint_files_interchange_with_reference(kit, tmp_path):
+    """Complex checkpoint files (interleaved (re, im) columns,
+    SRC/ztrlaux.c:69-307) written by the kit are read by the reference's
+    ztrl_read_checkpoint and the reference's ztrl_write_checkpoint files are
+    read by the kit, with identical data."""
+    from oracle import ref
+
+    R = ref.lib()
+    R.ztrl_write_checkpoint.argtypes = [ctypes.c_char_p, I, P, P, P, I, I, P, I, I]
+    R.ztrl_write_checkpoint.restype = I
+    R.ztrl_read_checkpoint.argtypes = [ctypes.c_char_p, I, P, I, I, ctypes.POINTER(I), P, I, I,
+                                       ctypes.POINTER(I), I, P, I, P]
+    R.ztrl_read_checkpoint.restype = I
+    COL = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                           ctypes.POINTER(ctypes.c_double))
+    kit.trlkit_write_checkpoint_e.argtypes = [ctypes.c_char_p, I, I, I, P, P, I, COL, P, P]
+    kit.trlkit_write_checkpoint_e.restype = I
+    kit.trlkit_read_checkpoint_e.argtypes = [ctypes.c_char_p, I, I, I, I, ctypes.POINTER(I), P, P,
+                                             COL, P, P]
+    kit.trlkit_read_checkpoint_e.restype = I
+    rng = np.random.default_rng(6)
+    nrow, me, nb = 19, 3, 2
+    jnd = me + nb - 1
+    Z = rng.standard_normal((nrow, me + nb)) + 1j * rng.standard_normal((nrow, me + nb))
+    E, B = np.asfortranarray(Z[:, :me]), np.asfortranarray(Z[:, me:])
+    alpha, beta = rng.standard_normal(jnd), rng.standard_normal(jnd)
+    # reference -> kit
+    path = str(tmp_path / "zref0")
+    assert R.ztrl_write_checkpoint(path.encode(), nrow, p(alpha), p(beta), p(E), nrow, me, p(B),
+                                   nrow, nb) == 0
+    cols = {}
+    buf = np.zeros(2 * nrow)
+
+    def put(_u, i, b):
+        cols[i] = np.ctypeslib.as_array(b, shape=(2 * nrow,)).copy().view(np.complex128)
+        return 0
+
+    cb = COL(put)
+    j = I(0)
+    a2, b2 = np.zeros(16), np.zeros(16)
+    assert kit.trlkit_read_checkpoint_e(path.encode(), nrow, 2, 10, 10, ctypes.byref(j), p(a2),
+                                        p(b2), cb, None, p(buf)) == 0
+    assert j.value == jnd and np.array_equal(a2[:jnd], alpha) and np.array_equal(b2[:jnd], beta)
+    assert all(np.array_equal(cols[i], Z[:, i]) for i in range(jnd + 1))
+    # kit -> reference
+    path2 = str(tmp_path / "zours0")
+
+    def get(_u, i, b):
+        np.ctypeslib.as_array(b, shape=(2 * nrow,))[:] = np.ascontiguousarray(Z[:, i]).view(np.float64)
+        return 0
+
+    cb2 = COL(get)
+    assert kit.trlkit_write_checkpoint_e(path2.encode(), nrow, 2, jnd, p(alpha), p(beta), jnd + 1,
+                                         cb2, None, p(buf)) == 0
+    assert open(path2, "rb").read() == open(path, "rb").read()
+    mev, nbas = 10, 4
+    ev = np.zeros(nrow * mev, dtype=np.complex128)
+    bs = np.zeros(nrow * nbas, dtype=np.complex128)
+    j1, j2 = I(0), I(0)
+    al, be = np.zeros(mev + nbas), np.zeros(mev + nbas)
+    assert R.ztrl_read_checkpoint(path2.encode(), nrow, p(ev), nrow, mev, ctypes.byref(j1), p(bs),
+                                  nrow, nbas, ctypes.byref(j2), mev + nbas - 1, p(al),
+                                  mev + nbas - 1, p(be)) == 0
+    assert j1.value == jnd and np.array_equal(al[:jnd], alpha)
+    assert np.array_equal(ev[: nrow * (jnd + 1)].reshape(jnd + 1, nrow).T, Z)
