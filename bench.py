@@ -37,6 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6  # dense FP64 matrix-core peak of one MI355X
 L = 128
 NEV, MAXLAN, TOL, RESTART, MAXMV = 16, 64, 1e-8, 1, 20000
 STATS_EVERY = 7
@@ -45,15 +46,17 @@ STATS_EVERY = 7
 METRIC = "eigenpairs/sec at resid<=1e-8||A||: mixed+learned-start vs plain FP64, 1 GPU"
 
 
-# device kernel behind each timed kernel class, per solver path
-KERNEL_NAMES = {
-    "plain": {"proj": "k_proj<double, double, 1, 8, 4>", "update": "k_update<double, double, 1, 4>",
-              "spmv": "k_spmv<double, double, 1, true>"},
-    "mixed": {"proj": "k_proj<float, double, 1, 8, 4>", "update": "k_update<float, double, 1, 4>",
-              "spmv": "k_spmv<double, double, 1, true>"},
-    "complex": {"proj": "k_proj<double, double, 2, 4, 4>", "update": "k_update<double, double, 2, 4>",
-                "spmv": "k_spmv<double, double, 2, true>"},
-}
+def kernel_names(T, path):
+    """Device kernel behind each timed kernel class of a solver path, as
+    rocprofv3 names the dispatches (the projection's columns per block come
+    from the library's current tile setting)."""
+    cb64, cb32 = T.proj_tiles_get()
+    if path == "complex":
+        return {"proj": "k_proj<double, double, 2, 4, 4>", "update": "k_update<double, double, 2, 4>",
+                "spmv": "k_spmv<double, double, 2, true>"}
+    vs, cb = ("float", cb32) if path == "mixed" else ("double", cb64)
+    return {"proj": f"k_proj<{vs}, double, 1, {cb}, 4>", "update": f"k_update<{vs}, double, 1, 4>",
+            "spmv": "k_spmv<double, double, 1, true>"}
 
 
 def calibration_copy(T, n):
@@ -76,13 +79,17 @@ BATCH_TRAFFIC = "hbm_traffic_anderson2d_batch_1024.txt"
 
 def measured_traffic(kernel, summary=LAPLACE_TRAFFIC):
     """HBM bytes per launch of `kernel` from a committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None
+    (with a warning on stderr when the summary exists but does not list the
+    kernel: a renamed kernel must not pass silently)."""
     path = os.path.join(ROOT, "profiles", summary)
     if kernel is None or not os.path.exists(path):
         return None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch ") and line.split(" ", 1)[1].rsplit(" ", 1)[0] == kernel:
             return int(line.split()[-1])
+    print(f"bench.py: profiles/{summary} has no traffic line for kernel {kernel!r}",
+          file=sys.stderr, flush=True)
     return None
 
 
@@ -193,6 +200,7 @@ def leg(T, A, x0, steps, rp, ci, v, stats_every=0, sp=None):
     t1 = time.perf_counter()
     stats = T.stats_get() if stats_every else None
     if stats_every:
+        stats["_flops"] = T.stats_flops()
         T.stats_enable(False)
     g = outs[-1]
     assert g["stat"] == 0, g["stat"]
@@ -200,6 +208,21 @@ def leg(T, A, x0, steps, rp, ci, v, stats_every=0, sp=None):
     res = residuals(rp, ci, v, g["evec"][:, :nec], g["eval"][:nec])
     good = int((res <= 1e-8 * g["anrm"]).sum())
     return t1 - t0, g, good, stats
+
+
+def restart_gemm(stats, flops, key, kernel):
+    """FP64-MFMA use of the restart GEMM (V(:, lck:lck+k) <- V Y): its
+    2 n jl k flops per timed launch over their event time, against the
+    dense FP64 matrix-core peak, with its HBM rate beside it."""
+    launches, ms, byts = stats[key]
+    if launches <= 0 or ms <= 0:
+        return None
+    tf = flops[key] / (ms * 1e-3) / 1e12
+    return {"kernel": kernel, "launches_timed": launches, "avg_launch_us": round(ms * 1e3 / launches, 2),
+            "achieved_tflops": round(tf, 3), "peak_tflops": FP64_MFMA_PEAK_TFLOPS,
+            "mfma_frac": round(tf / FP64_MFMA_PEAK_TFLOPS, 4),
+            "GBps": round(byts / (ms * 1e-3) / 1e9, 1),
+            "hbm_frac": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
 def anderson_params(count):
@@ -331,6 +354,7 @@ def batched_anderson2d(args):
         res = step()
     elapsed = time.perf_counter() - t0
     stats = T.stats_get()
+    flops = T.stats_flops()
     T.stats_enable(False)
     good = total_good(res)
     mv_net = sum(r["matvec"] for r in res)
@@ -379,7 +403,8 @@ def batched_anderson2d(args):
     pt = sum(stats[k][1] for k in path)
     path_gbs = pb / (pt * 1e-3) / 1e9 if pt > 0 else 0.0
     kname = {"b_spmv": "kb_spmv_pk" if packed else "kb_spmv_stream", "b_recur": "kb_recur",
-             "b_proj": "kb_proj<float>", "b_update": "kb_update<float>"}[dom]
+             "b_proj": f"kb_proj<float, {T.proj_tiles_get()[1]}>",
+             "b_update": "kb_update<float>"}[dom]
     roofline = {
         "bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -391,14 +416,19 @@ def batched_anderson2d(args):
         "kernels": {k: {"launches": s[0], "ms": round(s[1], 3), "bytes": round(s[2]),
                         "GBps": round(s[2] / (s[1] * 1e-3) / 1e9, 1) if s[1] > 0 else 0.0}
                     for k, s in stats.items() if s[0] > 0},
+        "restart_gemm": restart_gemm(stats, flops, "b_ritz", "kb_ritz"),
     }
-    cpu = None
+    cpu = cpu_seq = None
     if not args.no_cpu_baseline and not args.headline_only:
         nproc = min(os.cpu_count() or 1, 16)
         cb = cpu_baseline_batch(min(args.cpu_sample, count), nproc)
         if cb is not None:
             cpu = {"value": round(cb[0], 2), "unit": "eigenpairs/s", "cores": nproc,
                    "kind": "reference", "sample": cb[1]}
+        cb = cpu_baseline_batch(min(args.cpu_seq_sample, count), 1)
+        if cb is not None:
+            cpu_seq = {"value": round(cb[0], 2), "unit": "eigenpairs/s", "cores": 1,
+                       "kind": "reference", "sample": cb[1]}
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "eigenpairs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -424,6 +454,7 @@ def batched_anderson2d(args):
         "eigenvalue_diff_vs_plain_fp64_per_pair": dstat,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_sequential": cpu_seq,
     }
     if args.headline_only:
         for k in ("plain_fp64_one_at_a_time", "plain_fp64_batched", "mixed_batched_random_start",
@@ -450,6 +481,9 @@ def main():
                          "Laplacian), diag1000, anderson3d-c (complex 96^3), rand8m (n = 2^23 "
                          "random sparse): one solve per step, mixed vs plain FP64 (complex: "
                          "plain c128)")
+    ap.add_argument("--cpu-seq-sample", type=int, default=24,
+                    help="anderson2d-batch: lattices of the sequential (one process, one "
+                         "thread) reference baseline")
     ap.add_argument("--models", type=int, default=1024)
     ap.add_argument("--plain-sample", type=int, default=64)
     ap.add_argument("--stats-every", type=int, default=STATS_EVERY,
@@ -502,6 +536,7 @@ def single_solve(T, args):
     # non-periodic pick) carries HIP start/stop events stamped from its own
     # dispatch on the solver's stream; timing every launch would cost ~10%.
     elapsed, g, good, stats = leg(T, ops[path], x0, args.steps, rp, ci, v, args.stats_every, sp)
+    flops = stats.pop("_flops")
     value = good * args.steps / elapsed
     nec = g["nec"]
 
@@ -513,7 +548,7 @@ def single_solve(T, args):
     pb = sum(stats[k][2] for k in pk)
     pt = sum(stats[k][1] for k in pk)
     path_gbs = pb / (pt * 1e-3) / 1e9 if pt > 0 else 0.0
-    kname = KERNEL_NAMES["complex" if cplx else path].get(dom)
+    kname = kernel_names(T, "complex" if cplx else path).get(dom)
     if dom == "spmv" and kname and ops[path].packed:
         kname = "k_spmv_pk<true>"
     roofline = {
@@ -527,6 +562,7 @@ def single_solve(T, args):
         "kernels": {k: {"launches": s_[0], "ms": round(s_[1], 3), "bytes": round(s_[2]),
                         "GBps": round(s_[2] / (s_[1] * 1e-3) / 1e9, 1) if s_[1] > 0 else 0.0}
                     for k, s_ in stats.items() if s_[0] > 0},
+        "restart_gemm": restart_gemm(stats, flops, "ritz", "k_ritz_mfma"),
     }
     cpu = None
     if not args.no_cpu_baseline:

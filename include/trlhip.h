@@ -70,18 +70,45 @@ int trlhip_csr_create(trlhip_csr **A, int n, long nnz, const int *row_ptr,
  * counterpart in the reference. */
 int trlhip_csr_is_packed(const trlhip_csr *A);
 
-/* SpMV autotuner: times the CSR-stream kernel (row blocks) and the vector
- * kernel at 2, 4, 8, 16 and 32 lanes per row on A itself (reps launches
- * each, 0 = 10, HIP events on A's stream) and keeps the fastest for every
- * later use of A, single and batched solves alike.  *lpr_out = the choice
- * (0: CSR-stream), *ms_out = its time per SpMV (either may be NULL).
+/* 1 when A's SpMV in use is the column-panel kernel (see
+ * trlhip_csr_set_spmv), else 0.  No counterpart in the reference. */
+int trlhip_csr_is_paneled(const trlhip_csr *A);
+
+/* D = 2 or 3 when A is a periodic nearest-neighbour stencil on an L^D
+ * lattice (site x + L y (+ L^2 z), L >= 3, each row holding its 2D
+ * neighbours and itself with ascending columns, one off-diagonal value bit
+ * for bit), else 0; *L (may be NULL) gets L.  Such an operator's CSR-stream
+ * SpMV forms its columns from the lattice instead of reading them, with
+ * results equal to the CSR kernel's bit for bit; the environment variable
+ * TRLHIP_CSR_STENCIL=0 turns that off.  No counterpart in the reference. */
+int trlhip_csr_stencil(const trlhip_csr *A, int *L);
+
+/* SpMV autotuner: times the column-panel kernel (when A has panels), the
+ * CSR-stream kernel (row blocks) and the vector kernel at 2, 4, 8, 16 and
+ * 32 lanes per row on A itself (reps launches each, 0 = 10, HIP events on
+ * A's stream) and keeps the fastest for every later use of A, single and
+ * batched solves alike.  *lpr_out = the choice (1: column panels, 0:
+ * CSR-stream), *ms_out = its time per SpMV (either may be NULL).
  * Opt-in: a different kernel sums each row in a different order, so
  * results can differ from the default kernel's in the last bits.  No
  * counterpart in the reference, whose operator is the caller's own op. */
 int trlhip_csr_autotune(trlhip_csr *A, int reps, int *lpr_out, float *ms_out);
-/* Sets A's SpMV kernel as trlhip_csr_autotune would: lpr = 0 (CSR-stream)
- * or 2, 4, 8, 16, 32 lanes per row; for a cached autotuner choice.  No
- * counterpart in the reference. */
+/* Sets A's SpMV kernel as trlhip_csr_autotune would: lpr = 1 (column
+ * panels, when A has them), 0 (CSR-stream) or 2, 4, 8, 16, 32 lanes per
+ * row; for a cached autotuner choice.  No counterpart in the reference.
+ *
+ * Column panels: a real operator with at least 4 M rows whose nonzeros
+ * mostly lie farther than 128 K columns from their row (a random sparse
+ * pattern) also gets its nonzeros split into column panels of 256 K
+ * columns (about 16 bytes per nonzero more device memory); its SpMV then
+ * reads each panel's share of x through one XCD's L2 and combines per-row
+ * partials in panel order, which changes the sums' grouping (results
+ * differ from the CSR-stream kernel's in the last bits; run to run they
+ * are the same).  Batched solves of such operators use the CSR-stream
+ * kernel.  The panel kernel keeps its partials in the operator: one such
+ * operator serves one solve at a time.  TRLHIP_CSR_PANEL=0 turns panels
+ * off, =1 builds them for any real unpacked operator; TRLHIP_CSR_PANEL_COLS
+ * sets the panel width (a multiple of 64). */
 int trlhip_csr_set_spmv(trlhip_csr *A, int lpr);
 
 /* Gram-Schmidt projection tiles (h = V^T r): columns per block, 4, 8 or
@@ -261,6 +288,11 @@ void trlhip_stats_enable(int on);
  * out[k*3+2] = their algorithmic bytes, for k < TRLHIP_K_COUNT.  Returns
  * TRLHIP_K_COUNT. */
 int trlhip_stats_get(double *out, int nout);
+/* out[k] = floating-point operations of the timed launches of kernel class
+ * k (the restart GEMMs: 2 n jl k per launch, complex bases 4 n jl k as a
+ * 2n-row real product; 0 for the bandwidth-bound classes).  Returns
+ * TRLHIP_K_COUNT. */
+int trlhip_stats_flops(double *out, int nout);
 
 /* Number of visible GPUs (0 when none); never aborts. */
 int trlhip_device_count(void);

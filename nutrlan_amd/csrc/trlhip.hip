@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <cstdint>
 #include <cstdlib>
@@ -196,6 +197,15 @@ __global__ __launch_bounds__(kBlock) void k_colsum(const double *__restrict__ pa
 
 // ---------------------------------------------------------------- SpMV
 
+// a + b * c rounded twice (no FMA contraction), so that kernels that form
+// the same sums in different code agree bit for bit
+__device__ __forceinline__ double add_prod(double a, double b, double c)
+{
+#pragma clang fp contract(off)
+    const double p = b * c;
+    return a + p;
+}
+
 // y = A x, CSR-stream: a block walks row blocks [rb[b], rb[b+1]) (grid
 // stride); it stages each row block's nonzeros (at most kNnzBlk) as
 // products val*x[col] in LDS with fully coalesced loads, then each thread
@@ -277,7 +287,7 @@ __device__ __forceinline__ void spmv_stream_body(
                     double s = 0.0;
                     for (int k = a; k < b; ++k) s += prod[k];
                     y[row] = (S)s;
-                    acc += xv(row, 0) * (double)(S)s;
+                    acc = add_prod(acc, xv(row, 0), (double)(S)s);
                 } else {
                     double sr = 0.0, si = 0.0;
                     for (int k = a; k < b; ++k) {
@@ -310,7 +320,7 @@ __device__ __forceinline__ void spmv_stream_body(
                 own(r0);
                 if constexpr (E == 1) {
                     y[r0] = (S)sr;
-                    acc += xv(r0, 0) * (double)(S)sr;
+                    acc = add_prod(acc, xv(r0, 0), (double)(S)sr);
                 } else {
                     y[2 * r0] = sr;
                     y[2 * r0 + 1] = si;
@@ -343,6 +353,187 @@ __global__ __launch_bounds__(kBlock) void k_spmv_pk(
 {
     spmv_stream_body<double, double, 1, FUSE, true>(nb, rb, rp, nullptr, nullptr, x, y, part, xscale,
                                                     xq, xq32, blockIdx.x, gridDim.x, pk, dg, tab);
+}
+
+// y = A x for a periodic nearest-neighbour stencil on an L^D lattice (D = 2
+// or 3; site x + L y (+ L^2 z)) whose off-diagonal entries all hold `off`
+// and whose diagonal is dg: the 2D+1 columns of a row are computed from
+// its lattice coordinates instead of read, sorted ascending as the CSR
+// stores them, and the products (each rounded before it is added, as the
+// CSR-stream kernel's LDS-staged products are) summed in that order.  Row
+// blocks, the row-to-thread map and the alpha partials follow
+// spmv_stream_body, so y and the partials equal the CSR-stream kernel's on
+// the same matrix bit for bit, while the only matrix bytes read are the
+// diagonal's.
+template <int D, bool FUSE>
+__device__ __forceinline__ void spmv_stencil_body(
+    int nb, const int *__restrict__ rb, int L, double off, const double *__restrict__ dg,
+    const double *__restrict__ x, double *__restrict__ y, double *__restrict__ part, double xscale,
+    double *__restrict__ xq, float *__restrict__ xq32, int bid, int nblk)
+{
+    constexpr int K = 2 * D + 1;
+    auto xv = [&](int c) -> double {
+        if constexpr (FUSE)
+            return (double)(x[c] * xscale);
+        else
+            return x[c];
+    };
+    double acc = 0.0;
+    for (int blk = bid; blk < nb; blk += nblk) {
+        const int r0 = rb[blk], r1 = rb[blk + 1];
+        const int t = threadIdx.x;
+        if (t >= r1 - r0) continue;
+        const int row = r0 + t;
+        int c[K];
+        {
+            const int xs = row % L, rest = row / L;
+            const int ys = D == 2 ? rest : rest % L;
+            const int base = row - xs;  // the site with x = 0 on this row's line
+            c[0] = row;
+            c[1] = base + (xs == 0 ? L - 1 : xs - 1);
+            c[2] = base + (xs == L - 1 ? 0 : xs + 1);
+            const int ybase = row - ys * L;
+            c[3] = ybase + (ys == 0 ? L - 1 : ys - 1) * L;
+            c[4] = ybase + (ys == L - 1 ? 0 : ys + 1) * L;
+            if constexpr (D == 3) {
+                const int zs = rest / L;
+                const int L2 = L * L;
+                const int zbase = row - zs * L2;
+                c[5] = zbase + (zs == 0 ? L - 1 : zs - 1) * L2;
+                c[6] = zbase + (zs == L - 1 ? 0 : zs + 1) * L2;
+            }
+        }
+        // odd-even transposition sort: fixed indices, so c stays in registers
+#pragma unroll
+        for (int r = 0; r < K; ++r)
+#pragma unroll
+            for (int i = r & 1; i + 1 < K; i += 2) {
+                const int lo = min(c[i], c[i + 1]), hi = max(c[i], c[i + 1]);
+                c[i] = lo;
+                c[i + 1] = hi;
+            }
+        if constexpr (FUSE) {
+            const double v = xv(row);
+            if (xq) xq[row] = v;
+            if (xq32) xq32[row] = (float)v;
+        }
+        const double d = dg[row];
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) s = add_prod(s, c[k] == row ? d : off, xv(c[k]));
+        y[row] = s;
+        acc = add_prod(acc, xv(row), s);
+    }
+    const double bs = block_sum(acc);
+    if (threadIdx.x == 0) part[bid] = bs;
+}
+
+template <int D, bool FUSE>
+__global__ __launch_bounds__(kBlock) void k_spmv_st(
+    int nb, const int *__restrict__ rb, int L, double off, const double *__restrict__ dg,
+    const double *__restrict__ x, double *__restrict__ y, double *__restrict__ part, double xscale,
+    double *__restrict__ xq, float *__restrict__ xq32)
+{
+    spmv_stencil_body<D, FUSE>(nb, rb, L, off, dg, x, y, part, xscale, xq, xq32, blockIdx.x,
+                               gridDim.x);
+}
+
+// Column-panel SpMV, for large operators whose rows gather x from all
+// over (random sparse patterns): CSR-stream gathers then move a whole
+// cache line from the fabric per 8-byte x element.  Here the nonzeros are
+// kept once more, split by column into panels of kPanelCols columns (x's
+// share of a panel, 2 MiB, fits an XCD's 4 MiB L2), each panel's in row
+// order and cut into chunks of kPanelChunk.  Stage 1 (k_spmv_panel): block
+// w takes chunk perm[w]; every chunk of a panel goes to blocks of one XCD
+// (workgroups go to the 8 XCDs round-robin by index), back to back, so the
+// panel's x is read from HBM about once and then served by that L2.  It
+// forms the products (rounded, as the CSR-stream kernel's are), and one
+// partial per run of equal rows, summed in nonzero order by the thread
+// owning the run's first nonzero, into sp (runs numbered in chunk order,
+// from seg_base[chunk]).  Stage 2 (k_spmv_panel_rows): each row sums its
+// partials in panel order (rs_ptr / rs_idx), stores y and the fused
+// normalised x, and the alpha partials as the other kernels do.  The sums
+// are grouped by panel, so results differ from the CSR-stream kernel's in
+// the last bits; they are deterministic run to run.
+constexpr int kPanelCols = 1 << 18;
+constexpr int kPanelChunk = 2048;
+
+template <bool FUSE>
+__global__ __launch_bounds__(kBlock) void k_spmv_panel(
+    const int *__restrict__ perm, const long *__restrict__ cstart, const int *__restrict__ seg_base,
+    const int *__restrict__ erow, const int *__restrict__ ecol, const double *__restrict__ eval,
+    const double *__restrict__ x, double xscale, double *__restrict__ sp)
+{
+    constexpr int PER = kPanelChunk / kBlock;
+    __shared__ double prod[kPanelChunk];
+    __shared__ int rows[kPanelChunk + 1];
+    __shared__ int wsum[kBlock / 64];
+    const int chunk = perm[blockIdx.x];
+    if (chunk < 0) return;  // an idle slot of the XCD map: the whole block leaves
+    const long k0 = cstart[chunk];
+    const int nz = (int)(cstart[chunk + 1] - k0);
+    for (int k = threadIdx.x; k < nz; k += kBlock) {
+        const int c = ecol[k0 + k];
+        const double xv = FUSE ? (double)(x[c] * xscale) : x[c];
+        prod[k] = eval[k0 + k] * xv;
+        rows[k] = erow[k0 + k];
+    }
+    if (threadIdx.x == 0) rows[nz] = -1;
+    __syncthreads();
+    const int a = threadIdx.x * PER;
+    int starts = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int k = a + i;
+        if (k < nz && (k == 0 || rows[k] != rows[k - 1])) ++starts;
+    }
+    // exclusive scan of the run starts over the block: wave scan, then LDS
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int incl = starts;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int id = seg_base[chunk] + incl - starts;
+    for (int i = 0; i < w; ++i) id += wsum[i];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int k = a + i;
+        if (k < nz && (k == 0 || rows[k] != rows[k - 1])) {
+            double sum = 0.0;
+            int e = k;
+            do {
+                sum += prod[e];
+                ++e;
+            } while (rows[e] == rows[k]);
+            sp[id++] = sum;
+        }
+    }
+}
+
+template <bool FUSE>
+__global__ __launch_bounds__(kBlock) void k_spmv_panel_rows(
+    int n, const int *__restrict__ rs_ptr, const int *__restrict__ rs_idx,
+    const double *__restrict__ sp, const double *__restrict__ x, double *__restrict__ y,
+    double *__restrict__ part, double xscale, double *__restrict__ xq, float *__restrict__ xq32)
+{
+    double acc = 0.0;
+    for (int row = blockIdx.x * kBlock + threadIdx.x; row < n; row += gridDim.x * kBlock) {
+        double sum = 0.0;
+        for (int k = rs_ptr[row]; k < rs_ptr[row + 1]; ++k) sum += sp[rs_idx[k]];
+        const double xv = FUSE ? (double)(x[row] * xscale) : x[row];
+        if constexpr (FUSE) {
+            if (xq) xq[row] = xv;
+            if (xq32) xq32[row] = (float)xv;
+        }
+        y[row] = sum;
+        acc = add_prod(acc, xv, sum);
+    }
+    const double bs = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
 // y = A x, vector CSR: LPR lanes per row (a power of two), 64/LPR rows per
@@ -949,6 +1140,7 @@ struct Stats {
     double launches[TRLHIP_K_COUNT] = {};
     double ms[TRLHIP_K_COUNT] = {};
     double bytes[TRLHIP_K_COUNT] = {};
+    double flops[TRLHIP_K_COUNT] = {};
     struct Pending {
         hipEvent_t a, b;
         int k;
@@ -1022,8 +1214,9 @@ bool timed_pick(int k)
 struct Timed {
     hipEvent_t a = nullptr, b = nullptr;
     int k;
-    double bytes;
-    Timed(int kind, hipStream_t, double nbytes) : k(kind), bytes(nbytes)
+    double bytes, flops;
+    Timed(int kind, hipStream_t, double nbytes, double nflops = 0.0)
+        : k(kind), bytes(nbytes), flops(nflops)
     {
         if (!timed_pick(k)) return;
         a = ev_get();
@@ -1047,6 +1240,7 @@ struct Timed {
         }
         g_stats.launches[k] += 1.0;
         g_stats.bytes[k] += bytes;
+        g_stats.flops[k] += flops;
         g_stats.pending.push_back({a, b, k});
     }
 };
@@ -1076,6 +1270,17 @@ struct trlhip_csr {
     double *dg;     // diagonal values (n), value index 255
     double *tab;    // the off-diagonal values, indices 0..254
     int ntab;
+    // periodic nearest-neighbour stencil on an st_L^st_dim lattice with one
+    // off-diagonal value st_off (and the packed diagonal dg); st_dim = 0:
+    // not a stencil (see spmv_stencil_body)
+    int st_dim, st_L;
+    double st_off;
+    // column panels (see k_spmv_panel); pn_on: the SpMV in use
+    int pn_on, pn_nchunk, pn_nslot, pn_grid;
+    long pn_nseg;
+    int *pn_perm, *pn_segbase, *pn_erow, *pn_ecol, *pn_rsptr, *pn_rsidx;
+    long *pn_cstart;
+    double *pn_eval, *pn_sp;
     // staging for host-called matvec
     double *dx, *dy, *part, *dscal;
     hipStream_t s;
@@ -1314,6 +1519,12 @@ double spmv_bytes(const trlhip_csr *A, int E, size_t vsz, size_t xsz)
 {
     // values + col_idx + row_ptr + x read once + y written once; packed:
     // one word per nonzero + row_ptr + the diagonal + x + y
+    if (A->pn_on && !A->lpr)  // panels: nonzeros (row, col, value) + the runs' partials
+                              // written and gathered + row lists + x + y
+        return 16.0 * (double)A->nnz + 20.0 * (double)A->pn_nseg + 4.0 * (A->n + 1) +
+               12.0 * A->pn_nchunk + 4.0 * A->pn_nslot + 2.0 * (double)xsz * A->n;
+    if (A->st_dim && !A->lpr)  // stencil: row blocks + the diagonal + x + y
+        return 4.0 * (A->nb + 1) + 8.0 * A->n + 2.0 * (double)xsz * A->n;
     if (A->pk && !A->lpr)
         return 4.0 * (double)A->nnz + 4.0 * (A->n + 1) + 8.0 * A->n + 8.0 * A->ntab +
                2.0 * (double)xsz * A->n;
@@ -1321,12 +1532,27 @@ double spmv_bytes(const trlhip_csr *A, int E, size_t vsz, size_t xsz)
 }
 
 // The SpMV kernel a real matrix runs: lanes per row of the vector kernel,
-// 0 for CSR-stream, -1 for CSR-stream over packed nonzeros.
-int spmv_kind(const trlhip_csr *A) { return A->lpr ? A->lpr : (A->pk ? -1 : 0); }
+// 0 for CSR-stream, -1 for CSR-stream over packed nonzeros, -2 / -3 for
+// the 2D / 3D stencil kernel.
+int spmv_kind(const trlhip_csr *A)
+{
+    return A->lpr ? A->lpr : (A->pn_on ? -4 : (A->st_dim ? -A->st_dim : (A->pk ? -1 : 0)));
+}
+
+// The kernel a batched step runs: the column-panel kernel has no batched
+// form (its operators are far larger than a batch's), they run CSR-stream.
+int spmv_kind_batch(const trlhip_csr *A) { return A->pn_on && !A->lpr ? 0 : spmv_kind(A); }
+
+int spmv_grid_stream(const trlhip_csr *A)
+{
+    const int g = (A->nb + kMinSpmvRowBlocks - 1) / kMinSpmvRowBlocks;
+    return g < kSpmvGrid ? g : kSpmvGrid;
+}
 
 int spmv_grid(const trlhip_csr *A)
 {
     if (A->lpr) return A->vgrid;
+    if (A->pn_on) return A->pn_grid;
     const int g = (A->nb + kMinSpmvRowBlocks - 1) / kMinSpmvRowBlocks;
     return g < kSpmvGrid ? g : kSpmvGrid;
 }
@@ -1371,6 +1597,35 @@ int launch_spmv(const trlhip_csr *A, int vtype, const void *x, void *y, double *
     const dim3 g(spmv_grid(A)), b(kBlock);
     const double *xs = (const double *)x;
     double *ys = (double *)y, *q = (double *)xq;
+    if (A->pn_on && !cplx) {
+        const dim3 g1(A->pn_nslot);
+        if (xq) {
+            KLAUNCH((k_spmv_panel<true>), g1, b, 0, s, A->pn_perm, A->pn_cstart, A->pn_segbase,
+                    A->pn_erow, A->pn_ecol, A->pn_eval, xs, xscale, A->pn_sp);
+            KLAUNCH((k_spmv_panel_rows<true>), g, b, 0, s, A->n, A->pn_rsptr, A->pn_rsidx, A->pn_sp,
+                    xs, ys, part, xscale, q, xq32);
+        } else {
+            KLAUNCH((k_spmv_panel<false>), g1, b, 0, s, A->pn_perm, A->pn_cstart, A->pn_segbase,
+                    A->pn_erow, A->pn_ecol, A->pn_eval, xs, xscale, A->pn_sp);
+            KLAUNCH((k_spmv_panel_rows<false>), g, b, 0, s, A->n, A->pn_rsptr, A->pn_rsidx,
+                    A->pn_sp, xs, ys, part, xscale, (double *)nullptr, (float *)nullptr);
+        }
+        return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
+    }
+    if (A->st_dim && !cplx) {
+#define ST_CASE(D, F)                                                                             \
+    KLAUNCH((k_spmv_st<D, F>), g, b, 0, s, A->nb, A->rb, A->st_L, A->st_off, A->dg, xs, ys, part, \
+            xscale, q, F ? xq32 : (float *)nullptr)
+        if (A->st_dim == 2) {
+            if (xq) ST_CASE(2, true);
+            else ST_CASE(2, false);
+        } else {
+            if (xq) ST_CASE(3, true);
+            else ST_CASE(3, false);
+        }
+#undef ST_CASE
+        return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
+    }
     if (A->pk && !cplx) {
         if (xq)
             KLAUNCH((k_spmv_pk<true>), g, b, 0, s, A->nb, A->rb, A->rp, A->pk, A->dg, A->tab, xs, ys,
@@ -1509,7 +1764,9 @@ void launch_proj(trlhip_ctx *c, int gx, const VS *V, int c0, int m, int rcol)
 // below) or 16; set by trlhip_proj_autotune / trlhip_proj_tiles.  Single
 // and batched solves read the same setting, so they stay bit-identical; a
 // different width sums each coefficient's lanes in another order.
-int g_proj_cb[2] = {8, 8};
+// Read once per launch (a solve's group, or one projection) so that a
+// change from another thread never splits one launch between two widths.
+std::atomic<int> g_proj_cb[2] = {8, 8};
 
 // Row blocks of the projection: about kProjBlocks blocks over all column
 // chunks, at most kProjGrid (the partial rows), a multiple of 8 (the XCD
@@ -1525,7 +1782,10 @@ int proj_gx(const trlhip_ctx *c, int m, int cb) { return proj_gx_rows(c->n, m, c
 
 // hout = V(:, c0:c0+m)^H r (device, E*m doubles).  shadow: a Gram-Schmidt
 // projection, which reads the FP32 shadow of V in a mixed-precision context.
-int run_proj(trlhip_ctx *c, int c0, int m, int rcol, double *hout, bool shadow = false)
+// cb_force (4, 8, 16; 0: the process-wide setting) fixes the columns per
+// block, as the autotuner's trials do.
+int run_proj(trlhip_ctx *c, int c0, int m, int rcol, double *hout, bool shadow = false,
+             int cb_force = 0)
 {
     if (m <= 0) return 0;
     const bool s32 = shadow && c->V32;
@@ -1535,7 +1795,7 @@ int run_proj(trlhip_ctx *c, int c0, int m, int rcol, double *hout, bool shadow =
         // chunk widths and row unrolling from a sweep at n = 2^21, m = 16..64
         // (about 5.7 TB/s for both element types)
         if (s32 || c->E == 1) {
-            const int cb = g_proj_cb[s32 ? 1 : 0];
+            const int cb = cb_force ? cb_force : g_proj_cb[s32 ? 1 : 0].load(std::memory_order_relaxed);
             g = proj_gx(c, m, cb);
 #define PROJ_CB(CB)                                                                      \
     case CB:                                                                             \
@@ -1670,6 +1930,8 @@ struct BDesc {
     const double *val;
     const uint32_t *pk;   // packed nonzeros (kb_spmv_pk), with dg and tab
     const double *dg, *tab;
+    int st_L;             // stencil lattice side (kb_spmv_st), with st_off and dg
+    double st_off;
     const double *x;      // SpMV input: the basis column q_a, or the unscaled scratch column
     double xscale;        // 1, or the deferred normalisation of the scratch column
     double *xq;           // fused store of q_a (null when not fused)
@@ -1730,6 +1992,15 @@ __global__ __launch_bounds__(kBlock) void kb_spmv_pk(const BDesc *__restrict__ d
                                                     q.pk, q.dg, q.tab);
 }
 
+template <int D>
+__global__ __launch_bounds__(kBlock) void kb_spmv_st(const BDesc *__restrict__ d, const int *pre,
+                                                    int nreq)
+{
+    BATCH_LOCATE(BP_SPMV)
+    spmv_stencil_body<D, true>(q.nb, q.rb, q.st_L, q.st_off, q.dg, q.x, q.r, q.part_alpha, q.xscale,
+                               q.xq, q.xq32, bid_, nblk_);
+}
+
 template <int LPR>
 __global__ __launch_bounds__(kBlock) void kb_spmv(const BDesc *__restrict__ d, const int *pre, int nreq)
 {
@@ -1782,8 +2053,10 @@ __global__ __launch_bounds__(kBlock) void kb_sum2(const BDesc *__restrict__ d, c
 // block offsets each; returns the phases' algorithmic bytes and the largest
 // recurrence / projection widths.
 struct GroupInfo {
-    int R, lpr;  // lpr: the SpMV kernel, lanes per row (0: CSR-stream, -1: packed CSR-stream)
+    int R, lpr;  // lpr: the SpMV kernel (spmv_kind: lanes per row, 0: CSR-stream, -1: packed
+                 // CSR-stream, -2 / -3: 2D / 3D stencil)
     bool mixed;
+    int cb;             // projection columns per block, read once for the group
     size_t doff, poff;  // offsets into the staging arrays
     double bytes[BP_COUNT];
     int maxrm, maxj;
@@ -1804,10 +2077,10 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
         d.n = n;
         d.ld = c->ld;
         d.jnd = jnd;
-        d.np_alpha = spmv_grid(A);
+        d.np_alpha = spmv_kind_batch(A) == spmv_kind(A) ? spmv_grid(A) : spmv_grid_stream(A);
         c->np_alpha = d.np_alpha;
         d.g_upd = grid_for(n, kMinRowsUpd, kMaxGrid);
-        const int cb = g_proj_cb[gi.mixed ? 1 : 0];
+        const int cb = gi.cb;
         d.gx = proj_gx(c, jnd, cb);
         d.gy = (jnd + cb - 1) / cb;
         if (st.first) {
@@ -1834,6 +2107,8 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
         d.pk = A->pk;
         d.dg = A->dg;
         d.tab = A->tab;
+        d.st_L = A->st_L;
+        d.st_off = A->st_off;
         d.x = colp<double>(c, st.fuse ? st.fuse_src : st.qa);
         d.xscale = st.fuse ? st.fuse_s : 1.0;
         d.xq = st.fuse ? colp<double>(c, st.qa) : nullptr;
@@ -1855,7 +2130,10 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
             pp[i + 1] = pp[i] + nb[ph];
         }
         const double vs = gi.mixed ? 4.0 : 8.0;
-        gi.bytes[BP_SPMV] += spmv_bytes(A, 1, 8, 8) + (st.fuse ? (gi.mixed ? 12.0 : 8.0) * n : 0.0);
+        const double sb = spmv_kind_batch(A) == spmv_kind(A)
+                              ? spmv_bytes(A, 1, 8, 8)
+                              : 12.0 * (double)A->nnz + 4.0 * (n + 1) + 16.0 * n;
+        gi.bytes[BP_SPMV] += sb + (st.fuse ? (gi.mixed ? 12.0 : 8.0) * n : 0.0);
         gi.bytes[BP_RECUR] += 8.0 * n * (d.rm + 2);
         gi.bytes[BP_PROJ] += (vs * jnd + 8.0) * n;
         gi.bytes[BP_UPDATE] += (vs * jnd + 16.0) * n;
@@ -1874,6 +2152,8 @@ int batch_run(hipStream_t s, const GroupInfo &gi, const BDesc *dd, const int *dp
         Timed t(TRLHIP_K_BSPMV, s, gi.bytes[BP_SPMV]);
         switch (gi.lpr) {
         case -1: KLAUNCH(kb_spmv_pk, blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
+        case -2: KLAUNCH((kb_spmv_st<2>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
+        case -3: KLAUNCH((kb_spmv_st<3>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
         case 0: KLAUNCH(kb_spmv_stream, blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
         case 2: KLAUNCH((kb_spmv<2>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
         case 4: KLAUNCH((kb_spmv<4>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
@@ -1895,7 +2175,7 @@ int batch_run(hipStream_t s, const GroupInfo &gi, const BDesc *dd, const int *dp
         else                                                                                  \
             KLAUNCH((kb_proj<double, CB>), blocks(BP_PROJ), dim3(kBlock), 0, s, dd, dp, R);   \
         break;
-        switch (g_proj_cb[gi.mixed ? 1 : 0]) {
+        switch (gi.cb) {
             KB_PROJ_CB(4)
             KB_PROJ_CB(16)
             default: KB_PROJ_CB(8)
@@ -1940,13 +2220,14 @@ void steps_plan(const trlhip_batch *b, StepsPlan &P)
     for (size_t i = 0; i < b->steps.size(); ++i) {
         if (taken[i]) continue;
         GroupInfo gi{};
-        gi.lpr = spmv_kind(b->steps[i].A);
+        gi.lpr = spmv_kind_batch(b->steps[i].A);
         gi.mixed = b->steps[i].c->V32 != nullptr;
+        gi.cb = g_proj_cb[gi.mixed ? 1 : 0].load(std::memory_order_relaxed);
         gi.doff = order.size();
         gi.poff = npre;
         for (size_t k = i; k < b->steps.size(); ++k) {
             const BatchStep &st = b->steps[k];
-            if (!taken[k] && spmv_kind(st.A) == gi.lpr && (st.c->V32 != nullptr) == gi.mixed) {
+            if (!taken[k] && spmv_kind_batch(st.A) == gi.lpr && (st.c->V32 != nullptr) == gi.mixed) {
                 order.push_back(st);
                 taken[k] = 1;
             }
@@ -2131,7 +2412,7 @@ struct OpsPlan {
         int kind, R;
         size_t doff, poff;  // first record (ODesc or FDesc array), first prefix entry
         size_t shm;
-        double bytes;
+        double bytes, flops;
     };
     struct Pair {
         const BatchOp *op;
@@ -2266,6 +2547,7 @@ void ops_fill(OpsPlan &P, const std::vector<double> &ybuf, char *h, const char *
                 const size_t shm = sizeof(double) * (size_t)((op.jl + 3) & ~3) * 16 * l.kind;
                 l.shm = std::max(l.shm, shm);
                 l.bytes += 8.0 * c->n * (op.jl + op.k) + (c->V32 ? 4.0 * c->n * op.k : 0.0);
+                l.flops += 2.0 * c->n * op.jl * op.k;
             }
             pp[i + 1] = pp[i] + nb;
         }
@@ -2305,7 +2587,7 @@ int ops_launch(const OpsPlan &P, const char *h, const char *d, hipStream_t s)
             KLAUNCH(kb_copy, g, dim3(kBlock), 0, s, o, dpp, l.R);
             continue;
         }
-        Timed t(TRLHIP_K_BRITZ, s, l.bytes);
+        Timed t(TRLHIP_K_BRITZ, s, l.bytes, l.flops);
 #define KB_RITZ_CASE(K)                                                            \
     case K:                                                                        \
         KLAUNCH((kb_ritz<K>), g, dim3(kBlock), l.shm, s, o, dpp, l.R);             \
@@ -2365,6 +2647,18 @@ const trlhip_csr *trlhip_csr_from_op(void (*op)(void), void *mvparam, int vtype)
 int trlhip_csr_is_mixed(const trlhip_csr *A) { return A && A->vtype == TRLHIP_R32; }
 
 int trlhip_csr_is_packed(const trlhip_csr *A) { return A && A->magic == kCsrMagic && A->pk ? 1 : 0; }
+
+int trlhip_csr_is_paneled(const trlhip_csr *A)
+{
+    return A && A->magic == kCsrMagic && A->pn_on && !A->lpr ? 1 : 0;
+}
+
+int trlhip_csr_stencil(const trlhip_csr *A, int *L)
+{
+    if (!A || A->magic != kCsrMagic) return 0;
+    if (L) *L = A->st_L;
+    return A->st_dim;
+}
 
 int trlhip_device_count(void)
 {
@@ -2454,6 +2748,165 @@ bool pack_csr(int n, long nnz, const int *rp, const int *ci, const double *val,
     return true;
 }
 
+// A periodic nearest-neighbour stencil on an L^D lattice (D = 2 or 3,
+// L >= 3, site x + L y (+ L^2 z)): every row holds exactly its 2D lattice
+// neighbours and itself, columns ascending, and every off-diagonal value is
+// the same double (bit for bit), returned in *off.  Returns D, or 0.
+int detect_stencil(int n, long nnz, const int *rp, const int *ci, const double *val, int *L_out,
+                   double *off)
+{
+    for (int D = 2; D <= 3; ++D) {
+        const int K = 2 * D + 1;
+        int L = (int)std::lround(std::pow((double)n, 1.0 / D));
+        while ((long)std::pow((double)L, D) > n) --L;
+        while ((long)std::pow((double)(L + 1), D) <= n) ++L;
+        if (L < 3 || (D == 2 ? (long)L * L : (long)L * L * L) != n || nnz != (long)K * n) continue;
+        bool ok = true;
+        bool have_off = false;
+        uint64_t offbits = 0;
+        for (int row = 0; row < n && ok; ++row) {
+            if (rp[row] != (long)K * row) {
+                ok = false;
+                break;
+            }
+            const int xs = row % L, rest = row / L;
+            const int ys = D == 2 ? rest : rest % L, zs = D == 2 ? 0 : rest / L;
+            int c[7];
+            c[0] = row;
+            c[1] = row - xs + (xs + L - 1) % L;
+            c[2] = row - xs + (xs + 1) % L;
+            c[3] = row - ys * L + ((ys + L - 1) % L) * L;
+            c[4] = row - ys * L + ((ys + 1) % L) * L;
+            if (D == 3) {
+                c[5] = row - zs * L * L + ((zs + L - 1) % L) * L * L;
+                c[6] = row - zs * L * L + ((zs + 1) % L) * L * L;
+            }
+            std::sort(c, c + K);
+            for (int k = 0; k < K && ok; ++k) {
+                const long e = (long)rp[row] + k;
+                if (ci[e] != c[k]) ok = false;
+                if (c[k] == row) continue;
+                uint64_t bits;
+                memcpy(&bits, &val[e], sizeof bits);
+                if (!have_off) {
+                    have_off = true;
+                    offbits = bits;
+                    *off = val[e];
+                } else if (bits != offbits) {
+                    ok = false;
+                }
+            }
+        }
+        if (ok && have_off) {
+            *L_out = L;
+            return D;
+        }
+    }
+    return 0;
+}
+
+// Host side of the column-panel SpMV (see k_spmv_panel): W columns per
+// panel.  Fills the nonzeros per panel in row order, the chunks and the
+// XCD map of blocks to chunks, the run (partial) numbering and each row's
+// runs in panel order, and copies them to the device.  Returns 0, or an
+// error (A's panel fields stay partly allocated; trlhip_csr_destroy frees
+// them).
+int build_panels(trlhip_csr *A, const int *rp, const int *ci, const double *val, int W)
+{
+    const int n = A->n;
+    const long nnz = A->nnz;
+    const int P = (n + W - 1) / W;
+    std::vector<long> pe((size_t)P + 1, 0);
+    for (long k = 0; k < nnz; ++k) ++pe[(size_t)ci[k] / W + 1];
+    for (int p = 0; p < P; ++p) pe[p + 1] += pe[p];
+    std::vector<int> erow((size_t)nnz), ecol((size_t)nnz);
+    std::vector<double> ev((size_t)nnz);
+    {
+        std::vector<long> cur(pe.begin(), pe.end() - 1);
+        for (int i = 0; i < n; ++i)
+            for (long k = rp[i]; k < rp[i + 1]; ++k) {
+                const long q = cur[(size_t)ci[k] / W]++;
+                erow[q] = i;
+                ecol[q] = ci[k];
+                ev[q] = val[k];
+            }
+    }
+    // chunks (panel-major), their runs, each row's runs in panel order
+    std::vector<long> cstart;
+    std::vector<int> cpanel, segbase;
+    long nseg = 0;
+    std::vector<int> rcount((size_t)n + 1, 0);
+    for (int p = 0; p < P; ++p)
+        for (long k0 = pe[p]; k0 < pe[p + 1]; k0 += kPanelChunk) {
+            const long k1 = std::min(pe[p + 1], k0 + kPanelChunk);
+            cstart.push_back(k0);
+            cpanel.push_back(p);
+            segbase.push_back((int)nseg);
+            for (long k = k0; k < k1; ++k)
+                if (k == k0 || erow[k] != erow[k - 1]) {
+                    ++nseg;
+                    ++rcount[(size_t)erow[k] + 1];
+                }
+        }
+    const int nc = (int)cstart.size();
+    cstart.push_back(nnz);
+    if (nseg >= (1L << 31)) return TRLHIP_EINVAL;
+    for (int i = 0; i < n; ++i) rcount[(size_t)i + 1] += rcount[i];
+    std::vector<int> rsidx((size_t)std::max<long>(nseg, 1));
+    {
+        std::vector<int> cur(rcount.begin(), rcount.end() - 1);
+        long sid = 0;
+        for (int c = 0; c < nc; ++c)
+            for (long k = cstart[c]; k < cstart[c + 1]; ++k)
+                if (k == cstart[c] || erow[k] != erow[k - 1]) rsidx[(size_t)cur[erow[k]]++] = (int)sid++;
+    }
+    // XCD map: workgroup 8 j + x runs chunk j of XCD x's list, which holds
+    // the chunks of panels x, x + 8, x + 16, ... in order (-1: idle)
+    std::vector<std::vector<int>> lists(8);
+    for (int c = 0; c < nc; ++c) lists[(size_t)cpanel[c] % 8].push_back(c);
+    size_t m = 0;
+    for (auto &l : lists) m = std::max(m, l.size());
+    std::vector<int> perm(8 * std::max<size_t>(m, 1), -1);
+    for (int x = 0; x < 8; ++x)
+        for (size_t j = 0; j < lists[x].size(); ++j) perm[8 * j + x] = lists[x][j];
+    A->pn_nchunk = nc;
+    A->pn_nslot = (int)perm.size();
+    A->pn_nseg = nseg;
+    A->pn_grid = (int)std::min<long>(std::max<long>((n + kBlock - 1) / kBlock, 1), kSpmvGrid);
+    auto put = [](void **d, const void *h, size_t bytes) -> int {
+        if (hipMalloc(d, bytes ? bytes : 8) != hipSuccess) return TRLHIP_ENOMEM;
+        if (bytes && hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return TRLHIP_EHIP;
+        return 0;
+    };
+    int rc = put((void **)&A->pn_perm, perm.data(), sizeof(int) * perm.size());
+    if (!rc) rc = put((void **)&A->pn_cstart, cstart.data(), sizeof(long) * cstart.size());
+    if (!rc) rc = put((void **)&A->pn_segbase, segbase.data(), sizeof(int) * segbase.size());
+    if (!rc) rc = put((void **)&A->pn_erow, erow.data(), sizeof(int) * erow.size());
+    if (!rc) rc = put((void **)&A->pn_ecol, ecol.data(), sizeof(int) * ecol.size());
+    if (!rc) rc = put((void **)&A->pn_eval, ev.data(), sizeof(double) * ev.size());
+    if (!rc) rc = put((void **)&A->pn_rsptr, rcount.data(), sizeof(int) * rcount.size());
+    if (!rc) rc = put((void **)&A->pn_rsidx, rsidx.data(), sizeof(int) * rsidx.size());
+    if (!rc && hipMalloc((void **)&A->pn_sp, sizeof(double) * (size_t)std::max<long>(nseg, 1)) != hipSuccess)
+        rc = TRLHIP_ENOMEM;
+    return rc;
+}
+
+// Whether the column-panel SpMV pays: x larger than a few L2s and most
+// nonzeros reaching beyond half a panel's width from their row.
+bool wants_panels(int n, long nnz, const int *rp, const int *ci, int W)
+{
+    if ((long)n * 8 < (32L << 20) || nnz == 0) return false;
+    long far = 0;
+    const int step = std::max(1, n / 4096);  // a sample of rows
+    long seen = 0;
+    for (int i = 0; i < n; i += step)
+        for (long k = rp[i]; k < rp[i + 1]; ++k) {
+            ++seen;
+            if (std::labs((long)ci[k] - i) > W / 2) ++far;
+        }
+    return seen > 0 && 2 * far > seen;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2507,7 +2960,25 @@ int trlhip_csr_create(trlhip_csr **out, int n, long nnz, const int *row_ptr, con
                         hipMemcpy(A->dg, dg.data(), sizeof(double) * dg.size(), hipMemcpyHostToDevice) != hipSuccess ||
                         hipMemcpy(A->tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess))
                 rc = TRLHIP_EHIP;
+            // a periodic stencil needs neither the packed words nor the
+            // row pointers (TRLHIP_CSR_STENCIL=0: never)
+            const char *es = getenv("TRLHIP_CSR_STENCIL");
+            if (!rc && !(es && es[0] == '0'))
+                A->st_dim = detect_stencil(n, nnz, row_ptr, col_idx, (const double *)values, &A->st_L,
+                                           &A->st_off);
         }
+    }
+    // column panels for large, widely gathering real operators
+    // (TRLHIP_CSR_PANEL=0: never; 1: always; TRLHIP_CSR_PANEL_COLS: the
+    // panel width, a multiple of 64)
+    const char *epn = getenv("TRLHIP_CSR_PANEL");
+    const char *epw = getenv("TRLHIP_CSR_PANEL_COLS");
+    int W = kPanelCols;
+    if (epw && atoi(epw) >= 64) W = atoi(epw) & ~63;
+    if (!rc && A->E == 1 && nnz > 0 && !A->pk && !(epn && epn[0] == '0') &&
+        ((epn && epn[0] == '1') || wants_panels(n, nnz, row_ptr, col_idx, W))) {
+        rc = build_panels(A, row_ptr, col_idx, (const double *)values, W);
+        if (!rc) A->pn_on = 1;
     }
     if (rc) {
         trlhip_csr_destroy(A);
@@ -2532,12 +3003,21 @@ int trlhip_csr_autotune(trlhip_csr *A, int reps, int *lpr_out, float *ms_out)
         rc = TRLHIP_ENOMEM;
     if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = TRLHIP_EHIP;
     if (!rc && hipMemsetAsync(x, 0, sizeof(double) * ne, A->s) != hipSuccess) rc = TRLHIP_EHIP;
-    const int prev = A->lpr;
+    // candidates: 1 the column-panel kernel (when A has panels), 0 the
+    // CSR-stream kernel (stencil or packed when A allows), 2..32 lanes per
+    // row of the vector kernel
+    const int prev = A->lpr ? A->lpr : (A->pn_on ? 1 : 0);
     int best = prev;
     float best_ms = 0.f;
-    const int cand[6] = {0, 2, 4, 8, 16, 32};
-    for (int i = 0; i < 6 && !rc; ++i) {
-        set_lpr(A, cand[i]);
+    int cand[7], nc = 0;
+    if (A->pn_nchunk) cand[nc++] = 1;
+    for (int c : {0, 2, 4, 8, 16, 32}) cand[nc++] = c;
+    auto use = [&](int c) {
+        set_lpr(A, c == 1 ? 0 : c);
+        A->pn_on = c == 1;
+    };
+    for (int i = 0; i < nc && !rc; ++i) {
+        use(cand[i]);
         for (int w = 0; w < 2 && !rc; ++w) rc = launch_spmv(A, A->vtype, x, y, part, A->s);
         if (!rc && hipEventRecord(e0, A->s) != hipSuccess) rc = TRLHIP_EHIP;
         for (int r = 0; r < reps && !rc; ++r) rc = launch_spmv(A, A->vtype, x, y, part, A->s);
@@ -2550,7 +3030,7 @@ int trlhip_csr_autotune(trlhip_csr *A, int reps, int *lpr_out, float *ms_out)
             best = cand[i];
         }
     }
-    set_lpr(A, rc ? prev : best);
+    use(rc ? prev : best);
     if (!rc && lpr_out) *lpr_out = best;
     if (!rc && ms_out) *ms_out = best_ms / (float)reps;
     if (e0) (void)hipEventDestroy(e0);
@@ -2564,8 +3044,15 @@ int trlhip_csr_autotune(trlhip_csr *A, int reps, int *lpr_out, float *ms_out)
 int trlhip_csr_set_spmv(trlhip_csr *A, int lpr)
 {
     if (!A || A->magic != kCsrMagic) return TRLHIP_EINVAL;
+    if (lpr == 1) {  // the column-panel kernel, when A has panels
+        if (!A->pn_nchunk) return TRLHIP_EINVAL;
+        set_lpr(A, 0);
+        A->pn_on = 1;
+        return 0;
+    }
     if (lpr != 0 && lpr != 2 && lpr != 4 && lpr != 8 && lpr != 16 && lpr != 32) return TRLHIP_EINVAL;
     set_lpr(A, lpr);
+    if (lpr == 0) A->pn_on = 0;
     return 0;
 }
 
@@ -2594,16 +3081,17 @@ int trlhip_proj_autotune(int n, int m, int reps, int *cb_out, float *ms_out)
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = 0;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = TRLHIP_EHIP;
-    const int saved[2] = {g_proj_cb[0], g_proj_cb[1]};
     const int cand[3] = {4, 8, 16};
     for (int prec = 0; prec < 2 && !rc; ++prec) {
         int best = 8;
         float best_ms = 0.f;
         for (int i = 0; i < 3 && !rc; ++i) {
-            g_proj_cb[prec] = cand[i];
-            for (int w = 0; w < 2 && !rc; ++w) rc = run_proj(c, 0, m, m, c->dh, prec == 1);
+            // trial widths go to the launches directly: solves running
+            // meanwhile keep the process-wide setting
+            for (int w = 0; w < 2 && !rc; ++w) rc = run_proj(c, 0, m, m, c->dh, prec == 1, cand[i]);
             if (!rc && hipEventRecord(e0, c->s) != hipSuccess) rc = TRLHIP_EHIP;
-            for (int r = 0; r < reps && !rc; ++r) rc = run_proj(c, 0, m, m, c->dh, prec == 1);
+            for (int r = 0; r < reps && !rc; ++r)
+                rc = run_proj(c, 0, m, m, c->dh, prec == 1, cand[i]);
             if (!rc && hipEventRecord(e1, c->s) != hipSuccess) rc = TRLHIP_EHIP;
             float ms = 0.f;
             if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
@@ -2613,13 +3101,9 @@ int trlhip_proj_autotune(int n, int m, int reps, int *cb_out, float *ms_out)
                 best = cand[i];
             }
         }
-        g_proj_cb[prec] = rc ? saved[prec] : best;
+        if (!rc) g_proj_cb[prec] = best;
         if (!rc && cb_out) cb_out[prec] = best;
         if (!rc && ms_out) ms_out[prec] = best_ms / (float)reps;
-    }
-    if (rc) {
-        g_proj_cb[0] = saved[0];
-        g_proj_cb[1] = saved[1];
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -2638,6 +3122,10 @@ int trlhip_csr_destroy(trlhip_csr *A)
     if (A->pk) (void)hipFree(A->pk);
     if (A->dg) (void)hipFree(A->dg);
     if (A->tab) (void)hipFree(A->tab);
+    for (void *q : {(void *)A->pn_perm, (void *)A->pn_cstart, (void *)A->pn_segbase,
+                    (void *)A->pn_erow, (void *)A->pn_ecol, (void *)A->pn_eval, (void *)A->pn_rsptr,
+                    (void *)A->pn_rsidx, (void *)A->pn_sp})
+        if (q) (void)hipFree(q);
     if (A->dx) (void)hipFree(A->dx);
     if (A->dy) (void)hipFree(A->dy);
     if (A->part) (void)hipFree(A->part);
@@ -3153,7 +3641,7 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y, int ldy)
     const int tpw = nrows >= (1L << 20) ? 4 : kRitzTiles;
     const long rows_per_block = 16L * tpw * (kBlock / 64);
     const int g = (int)((nrows + rows_per_block - 1) / rows_per_block);
-    Timed t(TRLHIP_K_RITZ, c->s, (double)c->esz * c->n * (jl + k));
+    Timed t(TRLHIP_K_RITZ, c->s, (double)c->esz * c->n * (jl + k), 2.0 * (double)nrows * jl * k);
 #define RITZ_CASE(K)                                                                          \
     case K:                                                                                   \
         KLAUNCH((k_ritz_mfma<K>), dim3(g), dim3(kBlock), shm, c->s, nrows,                     \
@@ -3209,7 +3697,15 @@ void trlhip_stats_enable(int on)
         g_stats.launches[k] = 0;
         g_stats.ms[k] = 0;
         g_stats.bytes[k] = 0;
+        g_stats.flops[k] = 0;
     }
+}
+
+int trlhip_stats_flops(double *out, int nout)
+{
+    stats_resolve();
+    for (int k = 0; k < TRLHIP_K_COUNT && k < nout; ++k) out[k] = g_stats.flops[k];
+    return TRLHIP_K_COUNT;
 }
 
 int trlhip_stats_get(double *out, int nout)
@@ -3437,7 +3933,7 @@ int noalloc_launch(int n, int m, const VS *V, long ld, double *r, double *scr, h
     // the same kernels, tiles and grids as trlhip_cgs_pass on a context of
     // n rows, so the two agree bit for bit
     const bool s32 = std::is_same<VS, float>::value;
-    const int cb = E == 2 ? 4 : g_proj_cb[s32 ? 1 : 0];
+    const int cb = E == 2 ? 4 : g_proj_cb[s32 ? 1 : 0].load(std::memory_order_relaxed);
     const int gx = proj_gx_rows(n, m, cb);
     const int gy = (m + cb - 1) / cb;
 #define NA_PROJ(CB)                                                                              \

@@ -1,13 +1,17 @@
 """Trains the learned start-vector net and writes the shipped weights
 (nutrlan_amd/data/trlnet_anderson2d.npy, with a log beside it).
 
-Training data is made by the solver itself: `count` periodic 2D Anderson
-lattices (L = 64, hopping -1, on-site U[-W/2, W/2], W ~ U[1, 12]) are solved
-for their 8 lowest eigenpairs by the batched mixed-precision solver
-(trlan_batch), and the net learns sqrt(sum_k psi_k^2) of those states from
-the potential (nutrlan_amd/learned.py).  All of it runs on one GPU:
+Training data is made by the solver itself: periodic 2D Anderson lattices
+(hopping -1, on-site U[-W/2, W/2], W ~ U[1, 12]) are solved for their 8
+lowest eigenpairs by the batched mixed-precision solver (trlan_batch), and
+the net learns sqrt(sum_k psi_k^2) of those states from the potential
+(nutrlan_amd/learned.py).  The net is convolutional, so one set of weights
+serves every lattice size; --sizes trains on several (count at the first
+size, and the same number of lattice sites at each further size), each
+epoch visiting every size.  All of it runs on one GPU:
 
     python -m nutrlan_amd.train_start_net --count 20000 --epochs 12
+    python -m nutrlan_amd.train_start_net --sizes 64,96,128,192,256 --count 12000
 """
 import argparse
 import os
@@ -44,6 +48,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--count", type=int, default=20000)
     ap.add_argument("--L", type=int, default=64)
+    ap.add_argument("--sizes", default=None,
+                    help="comma-separated lattice sizes (overrides --L): --count lattices at the "
+                         "first, as many sites at each further size")
     ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -60,15 +67,23 @@ def main():
         lines.append(msg)
 
     t0 = time.perf_counter()
-    X, Y = make_set(args.count, args.L, args.seed, log=log, kind=args.target)
-    log(f"training set: {args.count} lattices {args.L}x{args.L} in {time.perf_counter() - t0:.1f} s")
+    sizes = [int(x) for x in args.sizes.split(",")] if args.sizes else [args.L]
+    sets = []
+    for i, L in enumerate(sizes):
+        cnt = max(1, args.count * sizes[0] ** 2 // (L * L))
+        X, Y = make_set(cnt, L, args.seed + 7919 * i, log=log, kind=args.target)
+        sets.append((L, X, Y))
+        log(f"training set: {cnt} lattices {L}x{L} ({time.perf_counter() - t0:.1f} s so far)")
     net = LN.Net(seed=args.seed)
     t1 = time.perf_counter()
     hist = []
     for ep in range(args.epochs):
         lr = args.lr * (0.5 if ep >= args.epochs * 2 // 3 else 1.0)
-        hist += LN.train(net, X, Y, epochs=1, batch=args.batch, lr=lr, seed=args.seed + ep)
-        log(f"epoch {ep} loss {hist[-1]:.4f} ({time.perf_counter() - t1:.1f} s)")
+        for i, (L, X, Y) in enumerate(sets):
+            # the same number of sites per step at every size
+            b = max(1, args.batch * sizes[0] ** 2 // (L * L))
+            hist += LN.train(net, X, Y, epochs=1, batch=b, lr=lr, seed=args.seed + 31 * ep + i)
+            log(f"epoch {ep} L {L} loss {hist[-1]:.4f} ({time.perf_counter() - t1:.1f} s)")
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     np.save(args.out, net.params())
     with open(os.path.splitext(args.out)[0] + ".txt", "w") as f:

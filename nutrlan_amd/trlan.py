@@ -166,6 +166,27 @@ class Csr:
         L.trlhip_csr_is_packed.restype = ctypes.c_int
         return bool(L.trlhip_csr_is_packed(self.handle))
 
+    @property
+    def paneled(self):
+        """True when the SpMV in use is the column-panel kernel
+        (trlhip_csr_is_paneled)."""
+        L = lib()
+        L.trlhip_csr_is_paneled.argtypes = [ctypes.c_void_p]
+        L.trlhip_csr_is_paneled.restype = ctypes.c_int
+        return bool(L.trlhip_csr_is_paneled(self.handle))
+
+    @property
+    def stencil(self):
+        """(D, L) when the operator is a periodic nearest-neighbour stencil on
+        an L^D lattice whose SpMV forms its columns from the lattice
+        (trlhip_csr_stencil), else (0, 0)."""
+        L = lib()
+        L.trlhip_csr_stencil.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.trlhip_csr_stencil.restype = ctypes.c_int
+        side = ctypes.c_int(0)
+        d = L.trlhip_csr_stencil(self.handle, ctypes.byref(side))
+        return (d, side.value) if d else (0, 0)
+
     def set_spmv(self, lpr):
         """Use the SpMV kernel with `lpr` lanes per row (0: CSR-stream), as
         a cached autotuner choice (trlhip_csr_set_spmv)."""

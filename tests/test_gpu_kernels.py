@@ -445,3 +445,209 @@ def test_projection_autotune_and_cache(gpu_lib, tmp_path):
         A.close()
     finally:
         T.proj_tiles()
+
+
+# ------------------------------------------------------------ stencils
+
+def _csr_three(T, rp, ci, v, monkeypatch, mixed=False):
+    """The same matrix with the stencil SpMV (default), with packed
+    nonzeros only (TRLHIP_CSR_STENCIL=0) and unpacked (TRLHIP_CSR_PACK=0)."""
+    S = T.Csr(rp, ci, v, mixed=mixed)
+    monkeypatch.setenv("TRLHIP_CSR_STENCIL", "0")
+    P = T.Csr(rp, ci, v, mixed=mixed)
+    monkeypatch.delenv("TRLHIP_CSR_STENCIL")
+    monkeypatch.setenv("TRLHIP_CSR_PACK", "0")
+    U = T.Csr(rp, ci, v, mixed=mixed)
+    monkeypatch.delenv("TRLHIP_CSR_PACK")
+    return S, P, U
+
+
+@pytest.mark.parametrize("builder", ["lap3d", "and2d", "lap3d_odd"])
+def test_stencil_spmv_bitwise(gpu_lib, builder, monkeypatch):
+    """A periodic 2D/3D stencil's SpMV, which forms its columns from the
+    lattice, equals the packed and the unpacked CSR-stream SpMV bit for bit
+    (y and alpha), and numpy to rounding."""
+    T = gpu_lib
+    if builder == "lap3d":
+        rp, ci, v = W.laplace3d_csr(20, seed=2)
+        want = (3, 20)
+    elif builder == "lap3d_odd":
+        rp, ci, v = W.laplace3d_csr(3, seed=5)
+        want = (3, 3)
+    else:
+        (rp, ci, v), _ = W.anderson2d_csr(150, 7.5, seed=3)
+        want = (2, 150)
+    n = len(rp) - 1
+    S, P, U = _csr_three(T, rp, ci, v, monkeypatch)
+    print(f"stencil spmv {builder}: n={n} stencil={S.stencil}", flush=True)
+    assert S.stencil == want and P.stencil == (0, 0) and U.stencil == (0, 0)
+    assert P.packed and not U.packed
+    x = W.start_vector(n, seed=11)
+    c = T.Ctx(n, 4, False)
+    c.set(0, x)
+    a = [c.spmv(A, 0, j) for j, A in zip((1, 2, 3), (S, P, U))]
+    ys = [c.get(j) for j in (1, 2, 3)]
+    assert np.array_equal(ys[0], ys[1]) and np.array_equal(ys[0], ys[2])
+    assert a[0] == a[1] == a[2]
+    y = W.csr_matvec(rp, ci, v)(x)
+    assert np.abs(ys[0] - y).max() <= 1e-13 * np.abs(y).max()
+    c.close()
+    for A in (S, P, U):
+        A.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_stencil_solves_bitwise(gpu_lib, mixed, monkeypatch):
+    """Whole solves with the stencil SpMV equal those with the packed
+    CSR-stream SpMV bit for bit: single (trlan) and batched (trlan_batch,
+    kb_spmv_st against kb_spmv_pk), plain FP64 and mixed precision."""
+    T = gpu_lib
+    rp, ci, v = W.laplace3d_csr(16, seed=4)
+    n = len(rp) - 1
+    x0 = W.start_vector(n, seed=12)
+    S, P, U = _csr_three(T, rp, ci, v, monkeypatch, mixed=mixed)
+    print(f"stencil solves mixed={mixed} n={n}", flush=True)
+    gs = T.solve_csr(S, -1, 6, 30, 1e-9, 1, 20000, 6, x0)
+    gp = T.solve_csr(P, -1, 6, 30, 1e-9, 1, 20000, 6, x0)
+    assert gs["nec"] == gp["nec"] == 6
+    assert (gs["matvec"], gs["nloop"]) == (gp["matvec"], gp["nloop"])
+    assert np.array_equal(gs["eval"][:6], gp["eval"][:6])
+    assert np.array_equal(gs["evec"][:, :6], gp["evec"][:, :6])
+    lat = [W.anderson2d_csr(L, 6.0, seed=20 + L)[0] for L in (48, 64, 70)]
+    opsS, opsP = [], []
+    for (r, c_, vv) in lat:
+        a, b, u = _csr_three(T, r, c_, vv, monkeypatch, mixed=mixed)
+        u.close()
+        assert a.stencil[0] == 2 and b.stencil == (0, 0)
+        opsS.append(a)
+        opsP.append(b)
+    probs = lambda ops: [(A, -1, 4, 24, 1e-8, 1, 20000, 4,
+                          W.start_vector(len(lat[i][0]) - 1, seed=30 + i)) for i, A in enumerate(ops)]
+    rcS, rS, _ = T.solve_batch(probs(opsS))
+    rcP, rP, _ = T.solve_batch(probs(opsP))
+    assert rcS == 0 and rcP == 0
+    for a, b in zip(rS, rP):
+        assert a["matvec"] == b["matvec"] and a["nec"] == b["nec"] == 4
+        assert np.array_equal(a["eval"][:4], b["eval"][:4])
+    for A in opsS + opsP + [S, P, U]:
+        A.close()
+
+
+def test_stencil_detection_declines(gpu_lib):
+    """Only periodic lattices with every off-diagonal value equal (bit for
+    bit) and ascending columns use the stencil SpMV; others keep the CSR
+    kernels."""
+    T = gpu_lib
+    rp, ci, v = W.random_sym_csr(3000, seed=5)
+    A = T.Csr(rp, ci, v)
+    assert A.stencil == (0, 0)
+    A.close()
+    (rp, ci, v), _ = W.anderson2d_csr(32, 4.0, seed=1)
+    off = np.flatnonzero(ci != np.repeat(np.arange(32 * 32), 5))
+    v2 = v.copy()
+    v2[off[7]] = np.nextafter(v2[off[7]], 0.0)  # one off-diagonal value differs
+    A = T.Csr(rp, ci, v2)
+    assert A.stencil == (0, 0) and A.packed
+    A.close()
+    ci2 = ci.copy()
+    ci2[[0, 1]] = ci2[[1, 0]]  # row 0's columns out of order
+    v3 = v.copy()
+    v3[[0, 1]] = v3[[1, 0]]
+    A = T.Csr(rp, ci2, v3)
+    assert A.stencil == (0, 0)
+    A.close()
+    A = T.Csr(rp, ci, v)
+    assert A.stencil == (2, 32)
+    A.close()
+
+
+# ------------------------------------------------------- column panels
+
+def _paneled(T, rp, ci, v, monkeypatch, cols=4096, mixed=False):
+    """The operator with its nonzeros split into column panels of `cols`
+    columns (forced at this size by TRLHIP_CSR_PANEL=1)."""
+    monkeypatch.setenv("TRLHIP_CSR_PANEL", "1")
+    monkeypatch.setenv("TRLHIP_CSR_PANEL_COLS", str(cols))
+    A = T.Csr(rp, ci, v, mixed=mixed)
+    monkeypatch.delenv("TRLHIP_CSR_PANEL")
+    monkeypatch.delenv("TRLHIP_CSR_PANEL_COLS")
+    return A
+
+
+@pytest.mark.parametrize("n,cols", [(50000, 4096), (30001, 64), (5000, 1 << 18)])
+def test_panel_spmv(gpu_lib, n, cols, monkeypatch):
+    """The column-panel SpMV (many panels, panels of one block of columns,
+    a single panel) against numpy, its alpha against numpy, the same bits
+    run to run, and against the CSR-stream kernel to rounding."""
+    T = gpu_lib
+    rp, ci, v = W.random_sym_csr(n, seed=7)
+    A = _paneled(T, rp, ci, v, monkeypatch, cols)
+    B = T.Csr(rp, ci, v)
+    print(f"panel spmv n={n} cols={cols} paneled={A.paneled}", flush=True)
+    assert A.paneled and not B.paneled
+    x = W.start_vector(n, seed=8) - 0.5
+    c = T.Ctx(n, 4)
+    c.set(0, x)
+    a1 = c.spmv(A, 0, 1)
+    a2 = c.spmv(A, 0, 2)
+    a3 = c.spmv(B, 0, 3)
+    y1, y2, y3 = c.get(1), c.get(2), c.get(3)
+    assert np.array_equal(y1, y2) and a1 == a2
+    y = W.csr_matvec(rp, ci, v)(x)
+    scale = np.abs(y).max()
+    assert np.abs(y1 - y).max() <= 1e-13 * scale
+    assert np.abs(y1 - y3).max() <= 1e-13 * scale
+    assert abs(a1 - x @ y) <= 1e-12 * abs(x) @ abs(y)
+    c.close()
+    A.close()
+    B.close()
+
+
+def test_panel_fused_steps_and_solve(gpu_lib, monkeypatch):
+    """Lanczos steps whose normalisation is fused into the column-panel
+    SpMV, against numpy; and a whole solve on a paneled operator against
+    the FP64 oracle (eigenvalues within 1e-10 ||A||, true residuals)."""
+    T = gpu_lib
+    from oracle import trlan_oracle as O
+
+    n = 20000
+    rp, ci, v = W.random_sym_csr(n, seed=9)
+    f = W.csr_matvec(rp, ci, v)
+    A = _paneled(T, rp, ci, v, monkeypatch, 2048)
+    assert A.paneled
+    x = W.start_vector(n, seed=6)
+    x = x / np.linalg.norm(x)
+    ncols = 6
+    c = T.Ctx(n, ncols)
+    c.set(0, x)
+    Q = [x]
+    beta_prev = 0.0
+    for j in range(1, 4):
+        qa, qb = j - 1, (j - 2 if j >= 2 else -1)
+        out = c.step_cgs(A, 1 if j == 1 else 0, qa, qb, beta_prev, None, j, j)
+        q = Q[qa]
+        y = f(q)
+        alpha = q @ y
+        r = y - alpha * q - (beta_prev * Q[qb] if qb >= 0 else 0.0)
+        Vm = np.stack(Q[:j], axis=1)
+        s = r - Vm @ (Vm.T @ r)
+        assert abs(out[0] - alpha) <= 1e-12 * max(1.0, abs(alpha))
+        assert np.abs(c.get(j) - r).max() <= 1e-12 * max(np.abs(r).max(), 1.0)
+        beta = np.sqrt(out[3])
+        c.scale_copy(j, ncols, 1.0 / beta)
+        Q.append(s / beta)
+        beta_prev = beta
+    assert np.abs(c.get(3) - Q[3]).max() <= 1e-10
+    c.close()
+    x0 = W.start_vector(n, seed=10)
+    g = T.solve_csr(A, 1, 12, 48, 1e-8, 1, 20000, 12, x0)
+    A.close()
+    info = O.init_info(n, 48, 1, 12, 1e-8, 1, 20000)
+    O.set_iguess(info, 0, 1)
+    o = O.trlan(f, info, n, 12, x0)
+    assert g["stat"] == 0 and g["nec"] == 12 == info.nec
+    assert np.abs(g["eval"][:12] - o["eval"][:12]).max() <= 1e-10 * info.anrm
+    for i in range(12):
+        xx = g["evec"][:, i]
+        assert np.linalg.norm(f(xx) - g["eval"][i] * xx) <= 1.01e-8 * g["anrm"]
+    print(f"paneled solve: matvec {g['matvec']} oracle {info.matvec}", flush=True)
