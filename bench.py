@@ -46,17 +46,27 @@ STATS_EVERY = 7
 METRIC = "eigenpairs/sec at resid<=1e-8||A||: mixed+learned-start vs plain FP64, 1 GPU"
 
 
-def kernel_names(T, path):
-    """Device kernel behind each timed kernel class of a solver path, as
+def kernel_names(T, path, A=None):
+    """Device kernel(s) behind each timed kernel class of a solver path, as
     rocprofv3 names the dispatches (the projection's columns per block come
-    from the library's current tile setting)."""
+    from the library's current tile setting; the SpMV from operator A's
+    format: column panels run two kernels per product, joined by '+')."""
     cb64, cb32 = T.proj_tiles_get()
     if path == "complex":
         return {"proj": "k_proj<double, double, 2, 4, 4>", "update": "k_update<double, double, 2, 4>",
                 "spmv": "k_spmv<double, double, 2, true>"}
     vs, cb = ("float", cb32) if path == "mixed" else ("double", cb64)
+    spmv = "k_spmv<double, double, 1, true>"
+    if A is not None and A.paneled:
+        spmv = "k_spmv_panel<true>+k_spmv_panel_rows<true>"
+    elif A is not None and A.stencil[0]:
+        # up to 2048 row blocks of at most 256 rows: four row blocks per workgroup
+        D, L = A.stencil
+        spmv = f"k_spmv_st_rb<{D}, 4>" if L ** D <= 2048 * 256 else f"k_spmv_st<{D}, true>"
+    elif A is not None and A.packed:
+        spmv = "k_spmv_pk<true>"
     return {"proj": f"k_proj<{vs}, double, 1, {cb}, 4>", "update": f"k_update<{vs}, double, 1, 4>",
-            "spmv": "k_spmv<double, double, 1, true>"}
+            "spmv": spmv}
 
 
 def calibration_copy(T, n):
@@ -78,19 +88,27 @@ BATCH_TRAFFIC = "hbm_traffic_anderson2d_batch_1024.txt"
 
 
 def measured_traffic(kernel, summary=LAPLACE_TRAFFIC):
-    """HBM bytes per launch of `kernel` from a committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None
-    (with a warning on stderr when the summary exists but does not list the
-    kernel: a renamed kernel must not pass silently)."""
+    """HBM bytes per launch of `kernel` ('a+b': the sum over kernels a and
+    b, dispatched once each) from a committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None (with a
+    warning on stderr when the summary exists but does not list a kernel: a
+    renamed kernel must not pass silently)."""
     path = os.path.join(ROOT, "profiles", summary)
     if kernel is None or not os.path.exists(path):
         return None
+    found = {}
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch ") and line.split(" ", 1)[1].rsplit(" ", 1)[0] == kernel:
-            return int(line.split()[-1])
-    print(f"bench.py: profiles/{summary} has no traffic line for kernel {kernel!r}",
-          file=sys.stderr, flush=True)
-    return None
+        if line.startswith("traffic_bytes_per_launch "):
+            name, val = line.split(" ", 1)[1].rsplit(" ", 1)
+            found[name] = int(val)
+    total = 0
+    for k in kernel.split("+"):
+        if k not in found:
+            print(f"bench.py: profiles/{summary} has no traffic line for kernel {k!r}",
+                  file=sys.stderr, flush=True)
+            return None
+        total += found[k]
+    return total
 
 
 # Single-solve workloads (one step = one complete trlan()/ztrlan() solve):
@@ -548,9 +566,7 @@ def single_solve(T, args):
     pb = sum(stats[k][2] for k in pk)
     pt = sum(stats[k][1] for k in pk)
     path_gbs = pb / (pt * 1e-3) / 1e9 if pt > 0 else 0.0
-    kname = kernel_names(T, "complex" if cplx else path).get(dom)
-    if dom == "spmv" and kname and ops[path].packed:
-        kname = "k_spmv_pk<true>"
+    kname = kernel_names(T, "complex" if cplx else path, ops[path]).get(dom)
     roofline = {
         "bound": "hbm", "kernel": kname or dom, "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

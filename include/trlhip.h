@@ -97,18 +97,18 @@ int trlhip_csr_autotune(trlhip_csr *A, int reps, int *lpr_out, float *ms_out);
  * panels, when A has them), 0 (CSR-stream) or 2, 4, 8, 16, 32 lanes per
  * row; for a cached autotuner choice.  No counterpart in the reference.
  *
- * Column panels: a real operator with at least 4 M rows whose nonzeros
- * mostly lie farther than 128 K columns from their row (a random sparse
- * pattern) also gets its nonzeros split into column panels of 256 K
- * columns (about 16 bytes per nonzero more device memory); its SpMV then
- * reads each panel's share of x through one XCD's L2 and combines per-row
- * partials in panel order, which changes the sums' grouping (results
+ * Column panels: with TRLHIP_CSR_PANEL=1 set at creation, a real unpacked
+ * operator also gets its nonzeros split into column panels of 256 K
+ * columns (TRLHIP_CSR_PANEL_COLS: another width, a multiple of 64; about 16
+ * bytes per nonzero more device memory) and its SpMV reads each panel's
+ * share of x through one XCD's L2, combining per-row partials in panel
+ * order in a second kernel; this changes the sums' grouping (results
  * differ from the CSR-stream kernel's in the last bits; run to run they
- * are the same).  Batched solves of such operators use the CSR-stream
- * kernel.  The panel kernel keeps its partials in the operator: one such
- * operator serves one solve at a time.  TRLHIP_CSR_PANEL=0 turns panels
- * off, =1 builds them for any real unpacked operator; TRLHIP_CSR_PANEL_COLS
- * sets the panel width (a multiple of 64). */
+ * are the same).  Opt-in only: on a random n = 2^23 pattern the two
+ * kernels take 2.94 ms a product against CSR-stream's 2.62 ms.  Batched
+ * solves of such operators use the CSR-stream kernel.  The panel kernel
+ * keeps its partials in the operator: one such operator serves one solve
+ * at a time. */
 int trlhip_csr_set_spmv(trlhip_csr *A, int lpr);
 
 /* Gram-Schmidt projection tiles (h = V^T r): columns per block, 4, 8 or

@@ -365,11 +365,18 @@ __global__ __launch_bounds__(kBlock) void k_spmv_pk(
 // spmv_stream_body, so y and the partials equal the CSR-stream kernel's on
 // the same matrix bit for bit, while the only matrix bytes read are the
 // diagonal's.
+// One row of the periodic-stencil product: its K = 2D+1 columns formed
+// from the lattice position and sorted ascending (the CSR column order),
+// the products rounded and summed left to right as add_prod does in the
+// CSR-stream kernel, so y[row] has the CSR kernel's bits.  Returns y[row]
+// and sets xr = the (scaled) x[row] for the alpha product; FUSE stores the
+// scaled x to xq / xq32.
 template <int D, bool FUSE>
-__device__ __forceinline__ void spmv_stencil_body(
-    int nb, const int *__restrict__ rb, int L, double off, const double *__restrict__ dg,
-    const double *__restrict__ x, double *__restrict__ y, double *__restrict__ part, double xscale,
-    double *__restrict__ xq, float *__restrict__ xq32, int bid, int nblk)
+__device__ __forceinline__ double stencil_row(int row, int L, double off,
+                                              const double *__restrict__ dg,
+                                              const double *__restrict__ x, double xscale,
+                                              double *__restrict__ xq, float *__restrict__ xq32,
+                                              double &xr)
 {
     constexpr int K = 2 * D + 1;
     auto xv = [&](int c) -> double {
@@ -378,54 +385,132 @@ __device__ __forceinline__ void spmv_stencil_body(
         else
             return x[c];
     };
+    int c[K];
+    {
+        const int xs = row % L, rest = row / L;
+        const int ys = D == 2 ? rest : rest % L;
+        const int base = row - xs;  // the site with x = 0 on this row's line
+        c[0] = row;
+        c[1] = base + (xs == 0 ? L - 1 : xs - 1);
+        c[2] = base + (xs == L - 1 ? 0 : xs + 1);
+        const int ybase = row - ys * L;
+        c[3] = ybase + (ys == 0 ? L - 1 : ys - 1) * L;
+        c[4] = ybase + (ys == L - 1 ? 0 : ys + 1) * L;
+        if constexpr (D == 3) {
+            const int zs = rest / L;
+            const int L2 = L * L;
+            const int zbase = row - zs * L2;
+            c[5] = zbase + (zs == 0 ? L - 1 : zs - 1) * L2;
+            c[6] = zbase + (zs == L - 1 ? 0 : zs + 1) * L2;
+        }
+    }
+    // odd-even transposition sort: fixed indices, so c stays in registers
+#pragma unroll
+    for (int r = 0; r < K; ++r)
+#pragma unroll
+        for (int i = r & 1; i + 1 < K; i += 2) {
+            const int lo = min(c[i], c[i + 1]), hi = max(c[i], c[i + 1]);
+            c[i] = lo;
+            c[i + 1] = hi;
+        }
+    xr = xv(row);
+    if constexpr (FUSE) {
+        if (xq) xq[row] = xr;
+        if (xq32) xq32[row] = (float)xr;
+    }
+    const double d = dg[row];
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) s = add_prod(s, c[k] == row ? d : off, xv(c[k]));
+    return s;
+}
+
+// y = A x for a periodic nearest-neighbour stencil (no matrix arrays: the
+// columns come from the lattice, the off-diagonal is one value, the
+// diagonal dg).  Same row blocks and block -> row-block map as the
+// CSR-stream kernel (block bid takes row blocks bid, bid + nblk, ...), one
+// row per thread, alpha accumulated over a thread's rows in that order, one
+// partial per block: results, alpha partials included, have the CSR-stream
+// kernel's bits.  A thread's next kStU row blocks are formed together so
+// their loads are in flight at once; the accumulation order is unchanged.
+constexpr int kStU = 4;
+
+template <int D, bool FUSE>
+__device__ __forceinline__ void spmv_stencil_body(
+    int nb, const int *__restrict__ rb, int L, double off, const double *__restrict__ dg,
+    const double *__restrict__ x, double *__restrict__ y, double *__restrict__ part, double xscale,
+    double *__restrict__ xq, float *__restrict__ xq32, int bid, int nblk)
+{
+    const int t = threadIdx.x;
     double acc = 0.0;
-    for (int blk = bid; blk < nb; blk += nblk) {
-        const int r0 = rb[blk], r1 = rb[blk + 1];
-        const int t = threadIdx.x;
-        if (t >= r1 - r0) continue;
-        const int row = r0 + t;
-        int c[K];
-        {
-            const int xs = row % L, rest = row / L;
-            const int ys = D == 2 ? rest : rest % L;
-            const int base = row - xs;  // the site with x = 0 on this row's line
-            c[0] = row;
-            c[1] = base + (xs == 0 ? L - 1 : xs - 1);
-            c[2] = base + (xs == L - 1 ? 0 : xs + 1);
-            const int ybase = row - ys * L;
-            c[3] = ybase + (ys == 0 ? L - 1 : ys - 1) * L;
-            c[4] = ybase + (ys == L - 1 ? 0 : ys + 1) * L;
-            if constexpr (D == 3) {
-                const int zs = rest / L;
-                const int L2 = L * L;
-                const int zbase = row - zs * L2;
-                c[5] = zbase + (zs == 0 ? L - 1 : zs - 1) * L2;
-                c[6] = zbase + (zs == L - 1 ? 0 : zs + 1) * L2;
+    for (int blk0 = bid; blk0 < nb; blk0 += kStU * nblk) {
+        double sv[kStU], xr[kStU];
+        bool on[kStU];
+#pragma unroll
+        for (int u = 0; u < kStU; ++u) {
+            const int blk = blk0 + u * nblk;
+            on[u] = false;
+            if (blk < nb) {
+                const int r0 = rb[blk], r1 = rb[blk + 1];
+                if (t < r1 - r0) {
+                    on[u] = true;
+                    sv[u] = stencil_row<D, FUSE>(r0 + t, L, off, dg, x, xscale, xq, xq32, xr[u]);
+                    y[r0 + t] = sv[u];
+                }
             }
         }
-        // odd-even transposition sort: fixed indices, so c stays in registers
 #pragma unroll
-        for (int r = 0; r < K; ++r)
-#pragma unroll
-            for (int i = r & 1; i + 1 < K; i += 2) {
-                const int lo = min(c[i], c[i + 1]), hi = max(c[i], c[i + 1]);
-                c[i] = lo;
-                c[i + 1] = hi;
-            }
-        if constexpr (FUSE) {
-            const double v = xv(row);
-            if (xq) xq[row] = v;
-            if (xq32) xq32[row] = (float)v;
-        }
-        const double d = dg[row];
-        double s = 0.0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) s = add_prod(s, c[k] == row ? d : off, xv(c[k]));
-        y[row] = s;
-        acc = add_prod(acc, xv(row), s);
+        for (int u = 0; u < kStU; ++u)
+            if (on[u]) acc = add_prod(acc, xr[u], sv[u]);
     }
     const double bs = block_sum(acc);
     if (threadIdx.x == 0) part[bid] = bs;
+}
+
+// The batched form: workgroup w of a problem takes row blocks w * RB ..
+// w * RB + RB - 1 and writes each one's alpha partial to part[blk], as RB
+// separate one-row-block blocks would (same per-row arithmetic, same
+// wave_sum / block_sum order), so with nblk = nb partials the results equal
+// spmv_stencil_body's bit for bit while a workgroup keeps RB rows' loads in
+// flight per thread.
+template <int D, int RB>
+__device__ __forceinline__ void spmv_stencil_rb(int nb, const int *__restrict__ rb, int L, double off,
+                                                const double *__restrict__ dg,
+                                                const double *__restrict__ x, double *__restrict__ y,
+                                                double *__restrict__ part, double xscale,
+                                                double *__restrict__ xq, float *__restrict__ xq32,
+                                                int w)
+{
+    __shared__ double sm[RB][kBlock / 64];
+    const int t = threadIdx.x;
+    double acc[RB];
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+        const int blk = w * RB + u;
+        acc[u] = 0.0;
+        if (blk < nb) {
+            const int r0 = rb[blk], r1 = rb[blk + 1];
+            if (t < r1 - r0) {
+                double xr;
+                const double s = stencil_row<D, true>(r0 + t, L, off, dg, x, xscale, xq, xq32, xr);
+                y[r0 + t] = s;
+                acc[u] = add_prod(0.0, xr, s);
+            }
+        }
+    }
+    const int lane = t & 63, wv = t >> 6;
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+        const double v = wave_sum(acc[u]);
+        if (lane == 0) sm[u][wv] = v;
+    }
+    __syncthreads();
+    if (t < RB && w * RB + t < nb) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < kBlock / 64; ++i) s += sm[t][i];
+        part[w * RB + t] = s;
+    }
 }
 
 template <int D, bool FUSE>
@@ -436,6 +521,18 @@ __global__ __launch_bounds__(kBlock) void k_spmv_st(
 {
     spmv_stencil_body<D, FUSE>(nb, rb, L, off, dg, x, y, part, xscale, xq, xq32, blockIdx.x,
                                gridDim.x);
+}
+
+// Operators of at most kSpmvGrid row blocks: workgroup w takes RB row
+// blocks (alpha partials one per row block, as above with one row block per
+// block).  xscale = 1 with no xq / xq32 is the plain product: x * 1 is x.
+template <int D, int RB>
+__global__ __launch_bounds__(kBlock) void k_spmv_st_rb(
+    int nb, const int *__restrict__ rb, int L, double off, const double *__restrict__ dg,
+    const double *__restrict__ x, double *__restrict__ y, double *__restrict__ part, double xscale,
+    double *__restrict__ xq, float *__restrict__ xq32)
+{
+    spmv_stencil_rb<D, RB>(nb, rb, L, off, dg, x, y, part, xscale, xq, xq32, blockIdx.x);
 }
 
 // Column-panel SpMV, for large operators whose rows gather x from all
@@ -1198,6 +1295,16 @@ thread_local LaunchEvents g_launch_ev;
         hipExtLaunchKernelGGL(kernel, grid, block, shm, stream, ev_.a, ev_.b, 0, __VA_ARGS__); \
     } while (0)
 
+// The first of several kernels that one timed scope launches back to back:
+// it takes the start event and leaves the stop event armed for the scope's
+// last launch (a plain KLAUNCH), so the timing spans all of them.
+#define KLAUNCH_FIRST(kernel, grid, block, shm, stream, ...)                                    \
+    do {                                                                                        \
+        const LaunchEvents ev_ = g_launch_ev;                                                   \
+        g_launch_ev = LaunchEvents{nullptr, ev_.b};                                             \
+        hipExtLaunchKernelGGL(kernel, grid, block, shm, stream, ev_.a, nullptr, 0, __VA_ARGS__); \
+    } while (0)
+
 // Times the first kernel launched in its scope when enabled: one scope in
 // g_stats.on of each kind, picked by a multiplicative hash of its ordinal
 // (a sample that does not alias with the periodic growth of the basis
@@ -1518,11 +1625,10 @@ size_t part_need(int E, int ncols)
 double spmv_bytes(const trlhip_csr *A, int E, size_t vsz, size_t xsz)
 {
     // values + col_idx + row_ptr + x read once + y written once; packed:
-    // one word per nonzero + row_ptr + the diagonal + x + y
-    if (A->pn_on && !A->lpr)  // panels: nonzeros (row, col, value) + the runs' partials
-                              // written and gathered + row lists + x + y
-        return 16.0 * (double)A->nnz + 20.0 * (double)A->pn_nseg + 4.0 * (A->n + 1) +
-               12.0 * A->pn_nchunk + 4.0 * A->pn_nslot + 2.0 * (double)xsz * A->n;
+    // one word per nonzero + row_ptr + the diagonal + x + y.  Column panels
+    // count as the CSR they were built from: their (row, col, value)
+    // triples and partial sums are the method's overhead, not the product's
+    // bytes.
     if (A->st_dim && !A->lpr)  // stencil: row blocks + the diagonal + x + y
         return 4.0 * (A->nb + 1) + 8.0 * A->n + 2.0 * (double)xsz * A->n;
     if (A->pk && !A->lpr)
@@ -1555,6 +1661,15 @@ int spmv_grid(const trlhip_csr *A)
     if (A->pn_on) return A->pn_grid;
     const int g = (A->nb + kMinSpmvRowBlocks - 1) / kMinSpmvRowBlocks;
     return g < kSpmvGrid ? g : kSpmvGrid;
+}
+
+// Workgroups of the stencil SpMV: kStRB row blocks each (one alpha partial
+// per row block) when A has at most kSpmvGrid row blocks, else the
+// CSR-stream grid.
+constexpr int kStRB = 4;
+int stencil_wgs(const trlhip_csr *A)
+{
+    return A->nb <= kSpmvGrid ? (A->nb + kStRB - 1) / kStRB : spmv_grid(A);
 }
 
 template <int E, bool FUSE>
@@ -1600,19 +1715,29 @@ int launch_spmv(const trlhip_csr *A, int vtype, const void *x, void *y, double *
     if (A->pn_on && !cplx) {
         const dim3 g1(A->pn_nslot);
         if (xq) {
-            KLAUNCH((k_spmv_panel<true>), g1, b, 0, s, A->pn_perm, A->pn_cstart, A->pn_segbase,
-                    A->pn_erow, A->pn_ecol, A->pn_eval, xs, xscale, A->pn_sp);
+            KLAUNCH_FIRST((k_spmv_panel<true>), g1, b, 0, s, A->pn_perm, A->pn_cstart, A->pn_segbase,
+                          A->pn_erow, A->pn_ecol, A->pn_eval, xs, xscale, A->pn_sp);
             KLAUNCH((k_spmv_panel_rows<true>), g, b, 0, s, A->n, A->pn_rsptr, A->pn_rsidx, A->pn_sp,
                     xs, ys, part, xscale, q, xq32);
         } else {
-            KLAUNCH((k_spmv_panel<false>), g1, b, 0, s, A->pn_perm, A->pn_cstart, A->pn_segbase,
-                    A->pn_erow, A->pn_ecol, A->pn_eval, xs, xscale, A->pn_sp);
+            KLAUNCH_FIRST((k_spmv_panel<false>), g1, b, 0, s, A->pn_perm, A->pn_cstart, A->pn_segbase,
+                          A->pn_erow, A->pn_ecol, A->pn_eval, xs, xscale, A->pn_sp);
             KLAUNCH((k_spmv_panel_rows<false>), g, b, 0, s, A->n, A->pn_rsptr, A->pn_rsidx,
                     A->pn_sp, xs, ys, part, xscale, (double *)nullptr, (float *)nullptr);
         }
         return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
     }
     if (A->st_dim && !cplx) {
+        if (A->nb <= kSpmvGrid) {  // one alpha partial per row block
+            const dim3 gw(stencil_wgs(A));
+            if (A->st_dim == 2)
+                KLAUNCH((k_spmv_st_rb<2, kStRB>), gw, b, 0, s, A->nb, A->rb, A->st_L, A->st_off, A->dg,
+                        xs, ys, part, xq ? xscale : 1.0, q, xq ? xq32 : (float *)nullptr);
+            else
+                KLAUNCH((k_spmv_st_rb<3, kStRB>), gw, b, 0, s, A->nb, A->rb, A->st_L, A->st_off, A->dg,
+                        xs, ys, part, xq ? xscale : 1.0, q, xq ? xq32 : (float *)nullptr);
+            return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
+        }
 #define ST_CASE(D, F)                                                                             \
     KLAUNCH((k_spmv_st<D, F>), g, b, 0, s, A->nb, A->rb, A->st_L, A->st_off, A->dg, xs, ys, part, \
             xscale, q, F ? xq32 : (float *)nullptr)
@@ -1997,8 +2122,12 @@ __global__ __launch_bounds__(kBlock) void kb_spmv_st(const BDesc *__restrict__ d
                                                     int nreq)
 {
     BATCH_LOCATE(BP_SPMV)
-    spmv_stencil_body<D, true>(q.nb, q.rb, q.st_L, q.st_off, q.dg, q.x, q.r, q.part_alpha, q.xscale,
-                               q.xq, q.xq32, bid_, nblk_);
+    if (q.nb <= kSpmvGrid)
+        spmv_stencil_rb<D, kStRB>(q.nb, q.rb, q.st_L, q.st_off, q.dg, q.x, q.r, q.part_alpha,
+                                  q.xscale, q.xq, q.xq32, bid_);
+    else
+        spmv_stencil_body<D, true>(q.nb, q.rb, q.st_L, q.st_off, q.dg, q.x, q.r, q.part_alpha,
+                                   q.xscale, q.xq, q.xq32, bid_, nblk_);
 }
 
 template <int LPR>
@@ -2124,7 +2253,9 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
         d.dh = c->dh;
         d.dscal = c->dscal;
         d.dres = c->dres;
-        const int nb[BP_COUNT] = {d.np_alpha, d.g_upd, d.gx * d.gy, jnd, d.g_upd, 2};
+        const int kind = spmv_kind_batch(A);
+        const int spmv_wg = (kind == -2 || kind == -3) ? stencil_wgs(A) : d.np_alpha;
+        const int nb[BP_COUNT] = {spmv_wg, d.g_upd, d.gx * d.gy, jnd, d.g_upd, 2};
         for (int ph = 0; ph < BP_COUNT; ++ph) {
             int *pp = pre + (size_t)ph * (R + 1);
             pp[i + 1] = pp[i] + nb[ph];
@@ -2893,20 +3024,6 @@ int build_panels(trlhip_csr *A, const int *rp, const int *ci, const double *val,
 
 // Whether the column-panel SpMV pays: x larger than a few L2s and most
 // nonzeros reaching beyond half a panel's width from their row.
-bool wants_panels(int n, long nnz, const int *rp, const int *ci, int W)
-{
-    if ((long)n * 8 < (32L << 20) || nnz == 0) return false;
-    long far = 0;
-    const int step = std::max(1, n / 4096);  // a sample of rows
-    long seen = 0;
-    for (int i = 0; i < n; i += step)
-        for (long k = rp[i]; k < rp[i + 1]; ++k) {
-            ++seen;
-            if (std::labs((long)ci[k] - i) > W / 2) ++far;
-        }
-    return seen > 0 && 2 * far > seen;
-}
-
 }  // namespace
 
 extern "C" {
@@ -2968,15 +3085,16 @@ int trlhip_csr_create(trlhip_csr **out, int n, long nnz, const int *row_ptr, con
                                            &A->st_off);
         }
     }
-    // column panels for large, widely gathering real operators
-    // (TRLHIP_CSR_PANEL=0: never; 1: always; TRLHIP_CSR_PANEL_COLS: the
-    // panel width, a multiple of 64)
+    // column panels on request (TRLHIP_CSR_PANEL=1; TRLHIP_CSR_PANEL_COLS:
+    // the panel width, a multiple of 64).  Not by default: on a random
+    // symmetric pattern with n = 2^23 and 17 nonzeros per row the two panel
+    // kernels together take 2.94 ms a product against the CSR-stream
+    // kernel's 2.62 ms.
     const char *epn = getenv("TRLHIP_CSR_PANEL");
     const char *epw = getenv("TRLHIP_CSR_PANEL_COLS");
     int W = kPanelCols;
     if (epw && atoi(epw) >= 64) W = atoi(epw) & ~63;
-    if (!rc && A->E == 1 && nnz > 0 && !A->pk && !(epn && epn[0] == '0') &&
-        ((epn && epn[0] == '1') || wants_panels(n, nnz, row_ptr, col_idx, W))) {
+    if (!rc && A->E == 1 && nnz > 0 && !A->pk && epn && epn[0] == '1') {
         rc = build_panels(A, row_ptr, col_idx, (const double *)values, W);
         if (!rc) A->pn_on = 1;
     }
