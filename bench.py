@@ -334,7 +334,7 @@ def batched_anderson2d(args):
     starts = [W.start_vector(len(c[0]) - 1, seed=9000 + i) for i, (c, d) in enumerate(models)]
     plain = [T.Csr(*c) for c, d in models]
     mixed = [T.Csr(*c, mixed=True) for c, d in models]
-    net = LN.load_net()
+    net = LN.load_net(args.start_weights)
     assert net is not None, "learned-start weights missing (run nutrlan_amd/train_start_net.py)"
 
     def total_good(res, n=None):
@@ -378,6 +378,7 @@ def batched_anderson2d(args):
     mv_net = sum(r["matvec"] for r in res)
     res = [dict(r, eval=r["eval"].copy(), evec=None) for r in res]  # the buffers are reused
     value = good * args.steps / elapsed
+    band_ratio = {}
     if args.headline_only:  # a profiling run: no legs beside the headline
         el_r = el_p = el_s = float("nan")
         good_r = mv_rand = good_p = mv_p = 0
@@ -389,6 +390,13 @@ def batched_anderson2d(args):
         res_r = batch(mixed, starts)
         el_r = time.perf_counter() - t0
         good_r, mv_rand = total_good(res_r), sum(r["matvec"] for r in res_r)
+        # operator applications, learned over random start, by lattice side
+        for lo, hi in ((64, 112), (112, 160), (160, 208), (208, 257)):
+            sel = [i for i, (L, _, _) in enumerate(params) if lo <= L < hi]
+            a = sum(res[i]["matvec"] for i in sel)
+            b = sum(res_r[i]["matvec"] for i in sel)
+            band_ratio[f"L{lo}-{hi - 1}"] = {"lattices": len(sel),
+                                             "ratio": round(a / b, 4) if b else None}
         t0 = time.perf_counter()
         res_p = batch(plain, starts)
         el_p = time.perf_counter() - t0
@@ -408,6 +416,7 @@ def batched_anderson2d(args):
         dstat = {"pairs": int(dall.size), "max": dl, "median": float(np.median(dall)),
                  "pairs_over_1e-10": int((dall > 1e-10).sum())}
     packed = all(A.packed for A in mixed)
+    stencil = all(A.stencil[0] == 2 for A in mixed)
     for A in plain + mixed:
         A.close()
     net.close()
@@ -420,7 +429,8 @@ def batched_anderson2d(args):
     pb = sum(stats[k][2] for k in path)
     pt = sum(stats[k][1] for k in path)
     path_gbs = pb / (pt * 1e-3) / 1e9 if pt > 0 else 0.0
-    kname = {"b_spmv": "kb_spmv_pk" if packed else "kb_spmv_stream", "b_recur": "kb_recur",
+    kname = {"b_spmv": "kb_spmv_st<2>" if stencil else "kb_spmv_pk" if packed else "kb_spmv_stream",
+             "b_recur": "kb_recur",
              "b_proj": f"kb_proj<float, {T.proj_tiles_get()[1]}>",
              "b_update": "kb_update<float>"}[dom]
     roofline = {
@@ -468,6 +478,7 @@ def batched_anderson2d(args):
         "mixed_batched_random_start": {"value": round(good_r / el_r, 1), "s": round(el_r, 3),
                                        "matvec_total": mv_rand, "converged_at_1e-8": good_r},
         "learned_vs_random_matvec": round(mv_net / mv_rand, 4) if mv_rand else None,
+        "learned_vs_random_matvec_by_L": band_ratio or None,
         "max_eigenvalue_diff_vs_plain_fp64": dl,
         "eigenvalue_diff_vs_plain_fp64_per_pair": dstat,
         "roofline": roofline,
@@ -506,6 +517,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=64)
     ap.add_argument("--stats-every", type=int, default=STATS_EVERY,
                     help="time one launch in N of each kernel kind with HIP events (1: all)")
+    ap.add_argument("--start-weights", default=None,
+                    help="anderson2d-batch: learned-start weights (.npy; default the shipped ones)")
     ap.add_argument("--headline-only", action="store_true",
                     help="anderson2d-batch: time the headline path only (profiling runs)")
     args = ap.parse_args()

@@ -23,6 +23,21 @@
 extern "C" {
 #endif
 
+#ifdef TRL_FORTRAN_COMPATIBLE
+/* The reference's pointer-argument callbacks (INC/trlan.h:9-36): every
+ * scalar by address, no mvparam.  The library exports them as separate
+ * entry points (trlan_fc ...; the adapters pass the library's own C
+ * callbacks through info->mvparam for the duration of a call and restore
+ * it), and the reference's names map onto them here, so sources written
+ * for the reference's TRL_FORTRAN_COMPATIBLE build compile unchanged. */
+typedef void (*trl_matvec)(int *pnrow, int *pncol, double *x, int *pldx,
+                           double *y, int *pldy);
+typedef void (*trl_uo)(int *pnrow, double *x, double *y);
+#define trlan trlan_fc
+#define trl_check_ritz trl_check_ritz_fc
+#define trl_ritz_projection trl_ritz_projection_fc
+#define trl_rayleigh_quotients trl_rayleigh_quotients_fc
+#else
 /* y(:,j) = A x(:,j) for j < ncol; column j of x starts at x + j*ldx.
  * Replaces: reference INC/trlan.h:67-69 (non-Fortran callback).
  * The solver always calls it with ncol == 1. */
@@ -33,6 +48,7 @@ typedef void (*trl_matvec)(const int nrow, const int ncol,
 /* Optional extra orthogonalization of a host vector; y (may be NULL)
  * receives the 2-norm of the change.  Replaces: INC/trlan.h:70. */
 typedef void (*trl_uo)(const int nrow, double *x, double *y);
+#endif
 
 /* Problem description, progress counters and bookkeeping of one solve.
  * Field order and types are the ABI (reference INC/trlan.h:77-245). */

@@ -1,6 +1,7 @@
 /*
  * ztrlan.h -- complex Hermitian entry points of the MI355X thick-restart
- * Lanczos solver (drop-in for reference INC/ztrlan.h, non-Fortran build).
+ * Lanczos solver (drop-in for reference INC/ztrlan.h; TRL_FORTRAN_COMPATIBLE
+ * selects the pointer-argument callback, as there).
  *
  * The basis is complex (interleaved re/im doubles, like trl_dcomplex) and
  * lives on the device; the projected tridiagonal stays real FP64 on the host
@@ -21,10 +22,19 @@ typedef struct {
     double r, i;
 } trl_dcomplex;
 
+#ifdef TRL_FORTRAN_COMPATIBLE
+/* Pointer-argument callback of the reference's TRL_FORTRAN_COMPATIBLE
+ * build (INC/ztrlan.h:30-31); see trlan.h. */
+typedef void (*ztrl_matvec)(int *pnrow, int *pncol, trl_dcomplex *x, int *pldx,
+                            trl_dcomplex *y, int *pldy);
+#define ztrlan ztrlan_fc
+#define ztrl_check_ritz ztrl_check_ritz_fc
+#else
 /* Replaces: reference INC/ztrlan.h:62-64. */
 typedef void (*ztrl_matvec)(const int nrow, const int ncol,
                             const trl_dcomplex *x, const int ldx,
                             trl_dcomplex *y, const int ldy, void *mvparam);
+#endif
 
 /* Thick-restart Lanczos for a complex Hermitian operator.  misc/nmis and
  * dwrk/ldwrk are accepted for ABI compatibility; on return misc[i].r holds

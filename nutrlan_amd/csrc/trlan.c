@@ -17,6 +17,7 @@
 #define _GNU_SOURCE
 #include <float.h>
 #include <stdio.h>
+#include <limits.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -572,9 +573,10 @@ static void lz_write_checkpoint(lz_t *z, int jnd, int rcol)
  * basis) and trl_initial_guess gets what follows alpha, beta, alfrot and
  * betrot in it (SRC/trlcore.c:805-806); ztrlan() passes the caller's nmis
  * whole (SRC/ztrlcore.c:617-618). */
-static long lz_guess_cap(const lz_t *z, int lwrk)
+/* The reference's misc workspace (trlanczos' wrk, of nmis doubles) for a
+ * caller's lwrk, as trlan() partitions it (SRC/trlan.c:172-226). */
+static long lz_ref_nmis(const lz_t *z, int lwrk)
 {
-    if (z->cplx) return lwrk;
     const long maxlan = z->info->maxlan;
     long ldb = ((long)(z->n + 3) / 4) * 4;
     if (ldb % 4096 == 0) ldb += 4;
@@ -588,7 +590,13 @@ static long lz_guess_cap(const lz_t *z, int lwrk)
         } else if (ii > nmis)
             nmis = lwrk;
     }
-    return nmis - 4 * maxlan;
+    return nmis;
+}
+
+static long lz_guess_cap(const lz_t *z, int lwrk)
+{
+    if (z->cplx) return lwrk;
+    return lz_ref_nmis(z, lwrk) - 4L * z->info->maxlan;
 }
 
 static int lz_initial_guess(lz_t *z, const double *evec, int lde, int lwrk, int *j1)
@@ -1067,6 +1075,21 @@ static void lz_convergence_test(lz_t *z, int jnd)
 
 /* The thick-restart iteration; mirrors the decision sequence of the
  * reference's trlanczos (SRC/trlcore.c:723-1532). */
+/* The reference's debug checks of the basis (SRC/trlcore.c:1025-1029,
+ * 1199-1210, 1497-1505, 1748-1757): vectors 0..tot-1 with the last one
+ * in column `last`, split into the reference's evec and base parts
+ * (j1 = min(tot, mev)); the recurrence check at verbose > 7.  avail is
+ * the reference's workspace there; the complex solver's checks always run
+ * (its workspace is not modelled). */
+static void lz_debug_basis(lz_t *z, int tot, int last, int kept, int recurrence, long avail)
+{
+    const int j1 = DMIN(tot, z->mev), j2 = tot - j1;
+    trl_dbg_check_orth(z->info, z->dev, z->E, j1, j2, tot - 1, last, avail);
+    if (recurrence && z->info->verbose > 7)
+        trl_dbg_check_recurrence(z->info, z->dev, z->n, z->E, z->op, z->zop, j1, j2, kept,
+                                 z->alpha, z->beta, tot - 1, last, avail);
+}
+
 static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk, int lwrk)
 {
     trl_info *info = z->info;
@@ -1084,7 +1107,17 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
     }
     int j1 = 0;
     if (lz_initial_guess(z, evec, lde, lwrk, &j1) != 0) return;
+    /* the reference's workspace (trlanczos' lwrk) for the debug checks */
+    const long ws = z->cplx ? LONG_MAX / 4 : lz_ref_nmis(z, lwrk);
+    const long ws4 = ws - 4L * info->maxlan;
+    if (info->verbose > 6) {
+        /* the reference checks its j1 = nec vectors, plus the start vector
+         * only when it lies in base (trlcore.c:1748-1757) */
+        const int tot = j1 < mev ? j1 : j1 + 1;
+        if (tot > 0) lz_debug_basis(z, tot, tot - 1, 0, 0, ws4);
+    }
     int jnd = j1, kept = jnd;
+    long ws_cycle = ws4;  /* lwrk2 of the current cycle (trlcore.c:860-869) */
     int i1 = info->ned - jnd;
     int next_test = i1 + DMIN(i1, DMIN(6, info->ned / 2));
     /* checkpoints every maxmv/cpflag operator applications */
@@ -1101,11 +1134,13 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
             return;
         }
         double a, rr2, pass1[2];
+        ws_cycle = ws4 - (long)(jnd - locked) * (jnd - locked);
         if (lz_matvec_step(z, 1, qa, -1, 0.0, jnd, rr, &a, &rr2, pass1)) return;
         alpha[jnd - 1] = a;
         beta[jnd - 1] = a;
         clock_t t0 = lz_clock();
         lz_orth(z, jnd, kept, rr, rr2, pass1);
+        if (info->verbose > 8 && info->stat == 0) lz_debug_basis(z, jnd + 1, rr, kept, 0, ws_cycle);
         add_ticks(info, &info->clk_orth, &info->tick_h, t0);
         if (info->stat != 0) return;
         memcpy(alfrot, alpha, sizeof(double) * (size_t)locked);
@@ -1150,7 +1185,9 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
             }
         }
         /* ---- restart or finish */
+        if (info->verbose > 6) lz_debug_basis(z, jnd + 1, rr, kept, 1, ws_cycle - 2L * jnd);
         clock_t tr = lz_clock();
+        long ws_end = ws4;  /* lwrk2 after the restart (trlcore.c:1354, 1460) */
         const long tr0 = g_prof ? lz_ns() : 0;
         const int prek = kept;
         const int jml = jnd - locked;
@@ -1206,6 +1243,7 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
                 return;
             }
             info->flop += 2 * z->n * jml * kept;
+            ws_end = ws4 - (long)kept * jml - (jml + locked);
             kept += locked;
             locked += i2;
             info->locked = locked;
@@ -1251,6 +1289,7 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
                 }
             }
             if (kept > 0) memcpy(alpha, eval, sizeof(double) * (size_t)kept);
+            ws_end = ws4 - (long)kept * jnd;
             for (int i = 0; i < kept; ++i) beta[i] = betrot[jnd - 1] * z->yy[(size_t)(i + 1) * jnd - 1];
             if (kept > 0 && trlhip_ritz(z->dev, 0, jnd, kept, z->yy, jnd) != 0) {
                 info->stat = TRLHIP_STAT_DEVICE;
@@ -1275,6 +1314,7 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
                 if (info->stat != 0) return;
             }
         }
+        if (info->verbose > 6) lz_debug_basis(z, jnd + 1, rr, kept, 1, ws_end);
         add_ticks(info, &info->clk_res, &info->tick_r, tr);
         info->nloop++;
     }
@@ -1425,8 +1465,22 @@ void trl_solve_device(trl_matvec op, ztrl_matvec zop, trl_uo user_ortho, trl_inf
 {
     lz_t z;
     if (!lz_setup(&z, op, zop, user_ortho, info, nrow, mev, lde, cplx, NULL)) return;
+    /* the debug checks write to the log file named by trl_set_debug, with
+     * the PE number appended, as the reference's trl_open_logfile does
+     * (SRC/trlaux.c:53-84; opened for the solve, closed after it) */
+    FILE *log = NULL;
+    if (info->verbose > 6 && !info->log_fp && info->log_file[0]) {
+        char name[160];
+        snprintf(name, sizeof name, "%.128s%d", info->log_file, info->my_pe);
+        log = fopen(name, "w");
+        info->log_fp = log;
+    }
     lz_iterate(&z, eval, evec, lde, wrk, lwrk);
     lz_teardown(&z);
+    if (log) {
+        fclose(log);
+        info->log_fp = NULL;
+    }
 }
 
 /* ------------------------------------------------------------- batches

@@ -52,6 +52,19 @@ int trlhip_batch_ready(trlhip_batch *b);
 int trlhip_batch_after(trlhip_batch *b, trlhip_batch *prev);
 int trlhip_ctx_create_batched(trlhip_ctx **c, int n, int ncols, int vtype, trlhip_batch *b);
 
+/* The reference's basis debug checks on a device basis (trl_debug.c):
+ * check vector i is context column i for i < k and column `last` for
+ * i == k.  trl_check_orth / ztrl_check_orth over j1 + j2 vectors (j1 in
+ * the place of the reference's evec part), and trl_check_recurrence /
+ * ztrl_check_recurrence over m1 + m2 vectors with `kept` from the last
+ * restart, applying op (E == 1) or zop (E == 2) to host copies.  avail:
+ * the reference's workspace at that point (too little: its message). */
+void trl_dbg_check_orth(trl_info *info, trlhip_ctx *dev, int E, int j1, int j2, int k, int last,
+                        long avail);
+void trl_dbg_check_recurrence(trl_info *info, trlhip_ctx *dev, int n, int E, trl_matvec op,
+                              ztrl_matvec zop, int m1, int m2, int kept, const double *alpha,
+                              const double *beta, int k, int last, long avail);
+
 /* One thick-restart solve on a device basis; cplx selects ztrlan (zop,
  * evec/wrk interleaved complex) or trlan (op).  Implemented in trlan.c. */
 void trl_solve_device(trl_matvec op, ztrl_matvec zop, trl_uo user_ortho, trl_info *info,

@@ -47,11 +47,12 @@ def info_layout():
 
 
 def solve_csr(rp, ci, val, lohi, ned, maxlan, tol, restart, maxmv, mev, start,
-              nthreads=1, complex_=False, guess=1, checkpoint=None, resume=None):
+              nthreads=1, complex_=False, guess=1, checkpoint=None, resume=None, debug=None):
     """Run the reference trlan()/ztrlan() on a CSR operator (info->guess =
     `guess`; 1: `start` is the start vector).  checkpoint=(cpflag, prefix)
     writes checkpoint files (trl_set_checkpoint); resume=prefix restarts
-    from the file under that prefix (guess = 2, info->oldcpf).
+    from the file under that prefix (guess = 2, info->oldcpf);
+    debug=(verbose, prefix): trl_set_debug, the log in prefix + "0".
 
     Returns dict(eval, evec (n x mev), resid, stat, nec, matvec, nloop, north,
     nrand, locked, klan, maxlan, ned, anrm, wall_s, crat, tol).
@@ -78,7 +79,11 @@ def solve_csr(rp, ci, val, lohi, ned, maxlan, tol, restart, maxmv, mev, start,
         env["TRL_REF_CPFILE"] = str(checkpoint[1])
     if resume is not None:
         env["TRL_REF_OLDCPF"] = str(resume)
-    for k in ("TRL_REF_CPFLAG", "TRL_REF_CPFILE", "TRL_REF_OLDCPF"):
+    if debug is not None:
+        env["TRL_REF_VERBOSE"] = str(int(debug[0]))
+        env["TRL_REF_LOGFILE"] = str(debug[1])
+    for k in ("TRL_REF_CPFLAG", "TRL_REF_CPFILE", "TRL_REF_OLDCPF", "TRL_REF_VERBOSE",
+              "TRL_REF_LOGFILE"):
         os.environ.pop(k, None)
     os.environ.update(env)
     f(n, _ptr(rp), _ptr(ci), _ptr(v), lohi, ned, maxlan, tol, restart, maxmv, mev,

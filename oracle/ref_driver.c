@@ -86,10 +86,14 @@ static int ref_guess(void)
 
 /* Checkpoint settings from the environment: TRL_REF_CPFLAG and
  * TRL_REF_CPFILE (write, trl_set_checkpoint), TRL_REF_OLDCPF (the prefix a
- * guess = 2 solve resumes from), then info->guess as ref_guess() says. */
+ * guess = 2 solve resumes from), then info->guess as ref_guess() says;
+ * TRL_REF_VERBOSE with TRL_REF_LOGFILE: trl_set_debug (the reference
+ * writes its log to that prefix + the PE number). */
 static void ref_setup(trl_info *info)
 {
     static char cpf[160];
+    const char *v = getenv("TRL_REF_VERBOSE"), *lf = getenv("TRL_REF_LOGFILE");
+    if (v && lf) trl_set_debug(info, atoi(v), (char *)lf);
     const char *f = getenv("TRL_REF_CPFILE"), *fl = getenv("TRL_REF_CPFLAG");
     if (f && fl) trl_set_checkpoint(info, atoi(fl), (char *)f);
     const char *old = getenv("TRL_REF_OLDCPF");

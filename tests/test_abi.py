@@ -70,6 +70,49 @@ def test_info_layout_matches_reference(tmp_path):
     assert got == [lay[0], lay[12], lay[17], lay[-1]]
 
 
+def test_fortran_compatible_build_maps_to_exported_entry_points(tmp_path):
+    """Sources written for the reference's TRL_FORTRAN_COMPATIBLE build
+    (pointer-argument callbacks, INC/trlan.h:9-36, INC/ztrlan.h:30-31)
+    compile unchanged against include/, with -Werror, and every function
+    they call resolves to a symbol the library exports."""
+    src = str(tmp_path / "fortran_probe.c")
+    with open(src, "w") as fh:
+        fh.write(
+            '#include "trlan.h"\n#include "ztrlan.h"\n'
+            "static void op(int *n, int *nc, double *x, int *lx, double *y, int *ly)\n"
+            "{ for (int j = 0; j < *nc; ++j) for (int i = 0; i < *n; ++i)"
+            " y[j * *ly + i] = 2.0 * x[j * *lx + i]; }\n"
+            "static void uo(int *n, double *x, double *y) { (void)n; (void)x;"
+            " if (y) *y = 0.0; }\n"
+            "static void zop(int *n, int *nc, trl_dcomplex *x, int *lx, trl_dcomplex *y,"
+            " int *ly) { (void)nc; (void)lx; (void)ly; for (int i = 0; i < *n; ++i) y[i] = x[i]; }\n"
+            "void probe(trl_info *info, double *ev, double *evec, double *wrk, int *chk,"
+            " trl_dcomplex *zv, trl_dcomplex *zw)\n"
+            "{\n"
+            "  trlan(op, uo, info, 10, 4, ev, evec, 10, 4, wrk);\n"
+            "  trl_check_ritz(op, info, 10, 2, evec, 10, ev, chk, wrk, ev, wrk, 40);\n"
+            "  trl_rayleigh_quotients(op, info, 2, evec, 10, wrk, wrk);\n"
+            "  trl_ritz_projection(op, info, 4, evec, 10, wrk, wrk, 40, wrk);\n"
+            "  ztrlan(zop, info, 10, 4, ev, zv, 10, zw, 4, wrk, 40);\n"
+            "  ztrl_check_ritz(zop, info, 10, 2, zv, 10, ev, chk, wrk, ev, zw, 40);\n"
+            "}\n")
+    obj = src[:-2] + ".o"
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-DTRL_FORTRAN_COMPATIBLE", "-I",
+                    os.path.join(ROOT, "include"), "-c", src, "-o", obj], check=True)
+    undef = {line.split()[-1] for line in subprocess.run(
+        ["nm", "-u", obj], capture_output=True, text=True, check=True).stdout.splitlines()
+        if line.strip()}
+    want = {"trlan_fc", "trl_check_ritz_fc", "trl_rayleigh_quotients_fc",
+            "trl_ritz_projection_fc", "ztrlan_fc", "ztrl_check_ritz_fc"}
+    assert want <= undef, undef
+    assert not ({"trlan", "ztrlan", "trl_check_ritz"} & undef)
+    T.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", T.LIBPATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    assert want <= exported
+
+
 @pytest.mark.parametrize("args", [
     (1000, 40, 1, 5, 1.4e-8, 1, 5000),
     (500, 5, -1, 8, 0.0, 0, 0),         # maxlan <= ned, default tol and maxmv

@@ -554,3 +554,33 @@ def test_complex_checkpoint_files_interchange_with_reference(kit, tmp_path):
                                   mev + nbas - 1, p(be)) == 0
     assert j1.value == jnd and np.array_equal(al[:jnd], alpha)
     assert np.array_equal(ev[: nrow * (jnd + 1)].reshape(jnd + 1, nrow).T, Z)
+
+
+def test_debug_fixture_structure():
+    """The reference's debug logs (tests/golden/debug_cases.json) have the
+    shape its call sites give (SRC/trlcore.c:1025-1029, 1199-1210,
+    1497-1505): per restart cycle one orthogonality check after the inner
+    loop (maxlan + 1 vectors) and one after the restart (kept + 1), plus
+    one after the cycle's first step at debug level 9; recurrence checks at
+    level 8 and up, each over one vector fewer than its orthogonality
+    check; per-vector levels at level 8 and up."""
+    import json
+    import os
+
+    path = os.path.join(os.path.dirname(__file__), "golden", "debug_cases.json")
+    for c in json.load(open(path))["cases"]:
+        log, nloop = c["log"], c["counters"]["nloop"]
+        per = 3 if c["verbose"] > 8 else 2
+        assert len(log["orth"]) == per * nloop, c["name"]
+        after_inner = log["orth"][per - 2::per]
+        assert all(j == c["maxlan"] + 1 for _, j in after_inner[:-1]), c["name"]
+        assert log["orth"][-1][0] == c["counters"]["matvec"]
+        if c["verbose"] > 7:
+            assert log["vlevel"] == sum(j for _, j in log["orth"])
+            assert len(log["rec"]) + log["rec_too_small"] == 2 * nloop
+            if log["rec"]:
+                assert log["rec"] == [j - 1 for k, (_, j) in enumerate(log["orth"])
+                                      if k % per != (0 if per == 3 else -1)][:len(log["rec"])]
+        else:
+            assert log["vlevel"] == 0 and log["rec"] == [] and log["rec_too_small"] == 0
+        assert 0 < log["max_fro"] < 1e-13
