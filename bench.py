@@ -228,10 +228,25 @@ def leg(T, A, x0, steps, rp, ci, v, stats_every=0, sp=None):
     return t1 - t0, g, good, stats
 
 
-def restart_gemm(stats, flops, key, kernel):
+def mfma_ceiling():
+    """The FP64 matrix-core rate this GPU sustains (TFLOP/s), from
+    nutrlan_amd/mfma_f64_probe (register-only MFMA streams filling every CU),
+    or None when the probe is not built."""
+    exe = os.path.join(ROOT, "nutrlan_amd", "mfma_f64_probe")
+    if not os.path.exists(exe):
+        return None
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    for line in out.stdout.splitlines():
+        if line.startswith("mfma_f64_tflops "):
+            return float(line.split()[1])
+    return None
+
+
+def restart_gemm(stats, flops, key, kernel, ceiling=None):
     """FP64-MFMA use of the restart GEMM (V(:, lck:lck+k) <- V Y): its
     2 n jl k flops per timed launch over their event time, against the
-    dense FP64 matrix-core peak, with its HBM rate beside it."""
+    dense FP64 matrix-core peak and the rate the probe measured on this GPU,
+    with its HBM rate beside it."""
     launches, ms, byts = stats[key]
     if launches <= 0 or ms <= 0:
         return None
@@ -239,6 +254,8 @@ def restart_gemm(stats, flops, key, kernel):
     return {"kernel": kernel, "launches_timed": launches, "avg_launch_us": round(ms * 1e3 / launches, 2),
             "achieved_tflops": round(tf, 3), "peak_tflops": FP64_MFMA_PEAK_TFLOPS,
             "mfma_frac": round(tf / FP64_MFMA_PEAK_TFLOPS, 4),
+            "measured_ceiling_tflops": ceiling,
+            "ceiling_frac": round(tf / ceiling, 4) if ceiling else None,
             "GBps": round(byts / (ms * 1e-3) / 1e9, 1),
             "hbm_frac": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
@@ -326,6 +343,7 @@ def batched_anderson2d(args):
     from nutrlan_amd import trlan as T
     from nutrlan_amd import workloads as W
 
+    ceiling = mfma_ceiling()  # its own process, before this one's first dispatch
     calibration_copy(T, 1 << 21)
     params = anderson_params(args.models)
     models = [LN.lattice(L, w, seed) for L, w, seed in params]
@@ -444,7 +462,7 @@ def batched_anderson2d(args):
         "kernels": {k: {"launches": s[0], "ms": round(s[1], 3), "bytes": round(s[2]),
                         "GBps": round(s[2] / (s[1] * 1e-3) / 1e9, 1) if s[1] > 0 else 0.0}
                     for k, s in stats.items() if s[0] > 0},
-        "restart_gemm": restart_gemm(stats, flops, "b_ritz", "kb_ritz"),
+        "restart_gemm": restart_gemm(stats, flops, "b_ritz", "kb_ritz", ceiling),
     }
     cpu = cpu_seq = None
     if not args.no_cpu_baseline and not args.headline_only:
@@ -550,6 +568,7 @@ def single_solve(T, args):
     print(f"building {sp['name']}", file=sys.stderr, flush=True)
     rp, ci, v, x0 = single_problem(args.workload)
     n = len(rp) - 1
+    ceiling = mfma_ceiling()
     calibration_copy(T, min(n, 1 << 21) if n >= 1 << 16 else 1 << 16)
     cplx = sp["cplx"]
     path = "plain" if cplx else args.path
@@ -591,7 +610,7 @@ def single_solve(T, args):
         "kernels": {k: {"launches": s_[0], "ms": round(s_[1], 3), "bytes": round(s_[2]),
                         "GBps": round(s_[2] / (s_[1] * 1e-3) / 1e9, 1) if s_[1] > 0 else 0.0}
                     for k, s_ in stats.items() if s_[0] > 0},
-        "restart_gemm": restart_gemm(stats, flops, "ritz", "k_ritz_mfma"),
+        "restart_gemm": restart_gemm(stats, flops, "ritz", "k_ritz_mfma", ceiling),
     }
     cpu = None
     if not args.no_cpu_baseline:

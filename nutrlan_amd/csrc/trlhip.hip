@@ -3759,7 +3759,9 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y, int ldy)
     const int tpw = nrows >= (1L << 20) ? 4 : kRitzTiles;
     const long rows_per_block = 16L * tpw * (kBlock / 64);
     const int g = (int)((nrows + rows_per_block - 1) / rows_per_block);
-    Timed t(TRLHIP_K_RITZ, c->s, (double)c->esz * c->n * (jl + k), 2.0 * (double)nrows * jl * k);
+    Timed t(TRLHIP_K_RITZ, c->s,
+            (double)c->esz * c->n * (jl + k) + (c->V32 ? 4.0 * c->n * k : 0.0),
+            2.0 * (double)nrows * jl * k);
 #define RITZ_CASE(K)                                                                          \
     case K:                                                                                   \
         KLAUNCH((k_ritz_mfma<K>), dim3(g), dim3(kBlock), shm, c->s, nrows,                     \
