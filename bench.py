@@ -379,8 +379,13 @@ def batched_anderson2d(args):
         t_inf[0] += time.perf_counter() - t0
         return batch(mixed, None, x_dev)
 
+    def progress(msg):
+        print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+    progress(f"{count} lattices built; warm-up")
     for _ in range(args.warmup):
         step()
+    progress("timed steps")
     t_inf[0] = 0.0
     # headline: learned start + mixed precision, batched; one launch in
     # STATS_EVERY of each kernel kind carries HIP start/stop events
@@ -405,6 +410,7 @@ def batched_anderson2d(args):
         ns = 0
     else:  # beside it, untimed by events
         t0 = time.perf_counter()
+        progress("mixed batch from random starts")
         res_r = batch(mixed, starts)
         el_r = time.perf_counter() - t0
         good_r, mv_rand = total_good(res_r), sum(r["matvec"] for r in res_r)
@@ -416,12 +422,14 @@ def batched_anderson2d(args):
             band_ratio[f"L{lo}-{hi - 1}"] = {"lattices": len(sel),
                                              "ratio": round(a / b, 4) if b else None}
         t0 = time.perf_counter()
+        progress("plain FP64 batch")
         res_p = batch(plain, starts)
         el_p = time.perf_counter() - t0
         good_p, mv_p = total_good(res_p), sum(r["matvec"] for r in res_p)
         res_p = [dict(r, eval=r["eval"].copy(), evec=None) for r in res_p]
         ns = min(args.plain_sample, count)
         t0 = time.perf_counter()
+        progress(f"plain FP64 one lattice at a time ({ns})")
         res_s = [T.solve_csr(plain[i], -1, nev, maxlan, tol, 1, 100000, nev, starts[i],
                              out=bufs[i]) for i in range(ns)]
         el_s = time.perf_counter() - t0
@@ -576,12 +584,15 @@ def single_solve(T, args):
     if not cplx:
         ops["mixed"] = T.Csr(rp, ci, v, mixed=True)
     other = None if cplx else ("plain" if path == "mixed" else "mixed")
+    print("bench.py: warm-up", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         for k in ops:
             leg(T, ops[k], x0, 1, rp, ci, v, 0, sp)
     o = None
     if other:  # the path beside the headline one, timed on its own (no events)
+        print(f"bench.py: {other} path", file=sys.stderr, flush=True)
         o = leg(T, ops[other], x0, args.steps, rp, ci, v, 0, sp)
+    print(f"bench.py: {path} path (headline)", file=sys.stderr, flush=True)
     # headline path: one launch in stats_every of each kernel kind (a hashed,
     # non-periodic pick) carries HIP start/stop events stamped from its own
     # dispatch on the solver's stream; timing every launch would cost ~10%.
