@@ -51,6 +51,13 @@ int trlhip_batch_ready(trlhip_batch *b);
  * (lanes on their own streams that still run one at a time). */
 int trlhip_batch_after(trlhip_batch *b, trlhip_batch *prev);
 int trlhip_ctx_create_batched(trlhip_ctx **c, int n, int ncols, int vtype, trlhip_batch *b);
+/* The solver allows its next (non-first) batched step to bring up to
+ * `steps` - 1 followers (steps jnd + 1, ... on consecutive columns) into the
+ * same round: each runs only if the device finds the solver would accept
+ * the step before it after its one speculative Gram-Schmidt pass (kept and
+ * ntot as lz_orth sees them), and the solver then takes its results from
+ * trlhip_step_cgs without a round of its own.  No effect outside a batch. */
+int trlhip_step_chain(trlhip_ctx *c, int steps, int kept, int ntot);
 
 /* The reference's basis debug checks on a device basis (trl_debug.c):
  * check vector i is context column i for i < k and column `last` for

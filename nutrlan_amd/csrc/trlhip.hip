@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <atomic>
 #include <cstdint>
@@ -45,6 +46,7 @@ constexpr int kRowsBlk = kBlock;       // rows per SpMV row block (at most)
 // blocks) run 3 %, 4 % and 10 % slower.
 constexpr int kMinRowsUpd = 4 * kBlock;
 constexpr int kMinSpmvRowBlocks = 1;
+constexpr int kChainMax = 32;  // Lanczos steps one batched context may record per round
 constexpr uint64_t kCsrMagic = 0x7472686970637372ull;  // "trhipcsr"
 
 
@@ -1427,6 +1429,12 @@ struct trlhip_ctx {
     char *hops, *dops;    // staging of a refinement outside a batch (pinned / device)
     size_t ocap;
     char *hstage;         // batch: pinned staging of one column for col_set / col_get
+    // chained steps (batch): the solver's hint for its next step, and the
+    // followers recorded with the last step
+    int ch_budget, ch_kept, ch_ntot;  // hint: steps the solver allows (<= 1: none)
+    int ch_n, ch_next;                // steps in the chain; next follower to hand out (0: none)
+    int ch_copy_ok;                   // that follower's fused normalisation was claimed
+    int ch_qa[kChainMax], ch_qb[kChainMax], ch_rcol[kChainMax], ch_jnd[kChainMax];
 };
 
 // One recorded Lanczos step of a batched context (see the batch section).
@@ -1439,6 +1447,9 @@ struct BatchStep {
     double fuse_s;
     long coff;        // first step after a restart: its jnd-1 coefficients at cbuf[coff]
     int fuse_src;
+    int ci;           // position in its context's chain (0: the step the solver asked for)
+    int decide;       // a follower comes next: decide on the device whether it runs
+    int kept, ntot;   // the solver's kept and info->ntot (for that decision)
 };
 
 // A recorded restart GEMM or scale-copy of a batched context; `phase` is
@@ -1518,6 +1529,10 @@ struct trlhip_batch {
 namespace {
 
 enum { S_ALPHA = 0, S_RR2 = 1, S_HH = 2, S_RR2B = 3 };
+// Chained batched steps (see trlhip_step_cgs): the step before a follower
+// leaves the follower's go flag, its fused normalisation 1/beta and beta
+// (the follower's beta_prev) in these device scalars.
+enum { S_GO = 4, S_XS = 5, S_BETA = 6 };
 
 int grid_for(long work, int per_block_min, int cap)
 {
@@ -1616,7 +1631,11 @@ double *part_alpha(trlhip_ctx *c) { return c->part; }
 double *part_update(trlhip_ctx *c) { return c->part + kSpmvGrid; }
 double *part_recur(trlhip_ctx *c) { return c->part + kSpmvGrid + kMaxGrid; }
 double *part_proj(trlhip_ctx *c) { return c->part + kSpmvGrid + 2 * kMaxGrid; }
-size_t hres_count(int ncols) { return 8 + 3 * (size_t)(ncols + 1); }
+size_t hres_chain(int ncols) { return 8 + 3 * (size_t)(ncols + 1); }
+// host-coherent results: a step's 4 scalars, 3 per refined pair from index
+// 8, then 7 per chain follower from hres_chain (alpha, ||r||^2, ||h||^2,
+// ||r - V h||^2, go, 1/beta, beta)
+size_t hres_count(int ncols) { return hres_chain(ncols) + 7 * (size_t)kChainMax; }
 size_t part_need(int E, int ncols)
 {
     return (size_t)kSpmvGrid + 2 * (size_t)kMaxGrid + (size_t)kProjGrid * E * (ncols + 1) + 64;
@@ -2001,8 +2020,18 @@ int ctx_wait(trlhip_ctx *c)
 // program order.
 int batch_drain(trlhip_ctx *c) { return (c->batch && c->nrec) ? ctx_wait(c) : 0; }
 
+// A chain follower the device ran that the solver has not taken yet: the
+// solver's own decision for the step before it must be the common one the
+// device made (chain_go), so any other device operation on the context
+// now is a mismatch, reported as an error rather than run out of order.
+bool chain_live(const trlhip_ctx *c)
+{
+    return c->ch_next > 0 && c->hres[hres_chain(c->ncols) + 7 * (size_t)(c->ch_next - 1) + 4] == 1.0;
+}
+
 int settle(trlhip_ctx *c)
 {
+    if (chain_live(c)) return TRLHIP_EINVAL;
     if (int rc = flush_pending(c)) return rc;
     return batch_drain(c);
 }
@@ -2069,6 +2098,14 @@ struct BDesc {
     double rh0;           // beta_prev
     double *part_alpha, *part_recur, *part_update, *part_proj;
     double *dh, *dscal, *dres;
+    // a chain follower reads its go flag, normalisation and beta_prev from
+    // the device scalars the step before it left (null: xscale / rh0 hold
+    // them and the step runs)
+    const double *go, *dxscale, *drh0;
+    double *hm;           // host-coherent mirror of this step's 4 results
+    double *hm_next;      // decide: the follower's mirror slot (go, 1/beta, beta at 4..6)
+    int decide, kept, ntot;
+    double eps_cgs;       // the Gram-Schmidt pass test's epsilon (FP32 basis: FLT_EPSILON)
 };
 
 enum { BP_SPMV, BP_RECUR, BP_PROJ, BP_COLSUM, BP_UPDATE, BP_SUM2, BP_COUNT };
@@ -2098,14 +2135,16 @@ __device__ __forceinline__ int batch_find(const int *__restrict__ pre, int nreq,
     const int *pre_ = pre + (size_t)(phase) * (nreq + 1);                     \
     const int p_ = batch_find(pre_, nreq, blockIdx.x);                        \
     const int bid_ = blockIdx.x - pre_[p_], nblk_ = pre_[p_ + 1] - pre_[p_];  \
-    const BDesc &q = d[p_];
+    const BDesc &q = d[p_];                                                   \
+    if (q.go && *q.go == 0.0) return; /* a chain follower that does not run */
 
 __global__ __launch_bounds__(kBlock) void kb_spmv_stream(const BDesc *__restrict__ d, const int *pre,
                                                         int nreq)
 {
     BATCH_LOCATE(BP_SPMV)
     spmv_stream_body<double, double, 1, true>(q.nb, q.rb, q.rp, q.ci, q.val, q.x, q.r, q.part_alpha,
-                                              q.xscale, q.xq, q.xq32, bid_, nblk_);
+                                              q.dxscale ? *q.dxscale : q.xscale, q.xq, q.xq32, bid_,
+                                              nblk_);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_spmv_pk(const BDesc *__restrict__ d, const int *pre,
@@ -2113,8 +2152,8 @@ __global__ __launch_bounds__(kBlock) void kb_spmv_pk(const BDesc *__restrict__ d
 {
     BATCH_LOCATE(BP_SPMV)
     spmv_stream_body<double, double, 1, true, true>(q.nb, q.rb, q.rp, nullptr, nullptr, q.x, q.r,
-                                                    q.part_alpha, q.xscale, q.xq, q.xq32, bid_, nblk_,
-                                                    q.pk, q.dg, q.tab);
+                                                    q.part_alpha, q.dxscale ? *q.dxscale : q.xscale,
+                                                    q.xq, q.xq32, bid_, nblk_, q.pk, q.dg, q.tab);
 }
 
 template <int D>
@@ -2122,12 +2161,13 @@ __global__ __launch_bounds__(kBlock) void kb_spmv_st(const BDesc *__restrict__ d
                                                     int nreq)
 {
     BATCH_LOCATE(BP_SPMV)
+    const double xs = q.dxscale ? *q.dxscale : q.xscale;
     if (q.nb <= kSpmvGrid)
-        spmv_stencil_rb<D, kStRB>(q.nb, q.rb, q.st_L, q.st_off, q.dg, q.x, q.r, q.part_alpha,
-                                  q.xscale, q.xq, q.xq32, bid_);
+        spmv_stencil_rb<D, kStRB>(q.nb, q.rb, q.st_L, q.st_off, q.dg, q.x, q.r, q.part_alpha, xs,
+                                  q.xq, q.xq32, bid_);
     else
-        spmv_stencil_body<D, true>(q.nb, q.rb, q.st_L, q.st_off, q.dg, q.x, q.r, q.part_alpha,
-                                   q.xscale, q.xq, q.xq32, bid_, nblk_);
+        spmv_stencil_body<D, true>(q.nb, q.rb, q.st_L, q.st_off, q.dg, q.x, q.r, q.part_alpha, xs,
+                                   q.xq, q.xq32, bid_, nblk_);
 }
 
 template <int LPR>
@@ -2135,15 +2175,16 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(const BDesc *__restrict__ d, c
 {
     BATCH_LOCATE(BP_SPMV)
     spmv_vec_body<double, double, 1, LPR, 4, true>(q.n, q.rp, q.ci, q.val, q.x, q.r, q.part_alpha,
-                                                   q.xscale, q.xq, q.xq32, bid_, nblk_);
+                                                   q.dxscale ? *q.dxscale : q.xscale, q.xq, q.xq32,
+                                                   bid_, nblk_);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_recur(const BDesc *__restrict__ d, const int *pre, int nreq)
 {
     BATCH_LOCATE(BP_RECUR)
-    update_body<double, double, 1, 4>(q.n, q.V, q.ld, q.rc0, q.rm, q.rh, q.rh0, q.part_alpha,
-                                      q.np_alpha, q.rm - 1, q.r, q.r, q.part_recur, nullptr,
-                                      q.dscal + S_ALPHA, bid_, nblk_);
+    update_body<double, double, 1, 4>(q.n, q.V, q.ld, q.rc0, q.rm, q.rh, q.drh0 ? *q.drh0 : q.rh0,
+                                      q.part_alpha, q.np_alpha, q.rm - 1, q.r, q.r, q.part_recur,
+                                      nullptr, q.dscal + S_ALPHA, bid_, nblk_);
 }
 
 template <typename VS, int CB>
@@ -2169,11 +2210,63 @@ __global__ __launch_bounds__(kBlock) void kb_update(const BDesc *__restrict__ d,
                                   nblk_);
 }
 
+// Whether a chain follower runs: the solver's own decisions after a step
+// (SRC/trlcore.c:1872-1898 and trl_cgs's first pass, 2913-3117, as
+// lz_orth / lz_cgs make them), in the same FP64 operations, for the
+// common case only -- the step is accepted after its one speculative
+// Gram-Schmidt pass with beta = sqrt(||r - V h||^2) > eps |alpha|.  Any
+// other outcome stops the chain and the solver takes over.
+__device__ __forceinline__ bool chain_go(double alpha, double rr2, double hh, double rr2b,
+                                         double bprev, int jnd, int kept, int ntot, double eps_cgs)
+{
+#pragma clang fp contract(off)
+    if (!(rr2 >= 0.0) || !(rr2 <= DBL_MAX)) return false;
+    const int jm1 = jnd - 1;
+    if (jm1 <= kept) return false;  // a cycle's first step (not chained)
+    if (jnd >= ntot) return false;
+    double tmp = alpha * alpha;
+    tmp += bprev * bprev;
+    if (DBL_EPSILON * rr2 >= tmp) return false;  // local re-orthogonalisation
+    if (!(eps_cgs * rr2b > hh)) return false;    // another Gram-Schmidt pass
+    const double rnrm = sqrt(rr2b);
+    if (!(rnrm > DBL_MIN)) return false;
+    if (rnrm <= DBL_EPSILON * fabs(alpha)) return false;
+    return true;
+}
+
+// Both norms of a step (before and after its speculative Gram-Schmidt
+// pass), in one block per request: the same fixed-order sums k_sum2 forms,
+// then the step's four results to its host-coherent mirror, and, when a
+// follower comes next, the decision whether it runs.
 __global__ __launch_bounds__(kBlock) void kb_sum2(const BDesc *__restrict__ d, const int *pre, int nreq)
 {
     BATCH_LOCATE(BP_SUM2)
-    sum2_body(q.part_recur, q.g_upd, q.dscal + S_RR2, q.part_update, q.g_upd, q.dscal + S_RR2B,
-              q.dscal, q.dres, bid_, nblk_);
+    (void)bid_;
+    (void)nblk_;
+    const double t0 = ordered_sum(q.part_recur, q.g_upd);
+    const double t1 = ordered_sum(q.part_update, q.g_upd);
+    if (threadIdx.x == 0) {
+        double *sc = q.dscal;
+        sc[S_RR2] = t0;
+        sc[S_RR2B] = t1;
+        q.hm[S_ALPHA] = sc[S_ALPHA];
+        q.hm[S_RR2] = t0;
+        q.hm[S_HH] = sc[S_HH];
+        q.hm[S_RR2B] = t1;
+        if (q.decide) {
+            const double bprev = q.drh0 ? *q.drh0 : q.rh0;
+            const bool go = chain_go(sc[S_ALPHA], t0, sc[S_HH], t1, bprev, q.jnd, q.kept, q.ntot,
+                                     q.eps_cgs);
+            const double beta = go ? sqrt(t1) : 0.0;
+            const double xs = go ? 1.0 / beta : 0.0;
+            sc[S_GO] = go ? 1.0 : 0.0;
+            sc[S_XS] = xs;
+            sc[S_BETA] = beta;
+            q.hm_next[4] = go ? 1.0 : 0.0;
+            q.hm_next[5] = xs;
+            q.hm_next[6] = beta;
+        }
+    }
 }
 #undef BATCH_LOCATE
 
@@ -2240,6 +2333,17 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
         d.st_off = A->st_off;
         d.x = colp<double>(c, st.fuse ? st.fuse_src : st.qa);
         d.xscale = st.fuse ? st.fuse_s : 1.0;
+        // a chain follower: its fused normalisation and beta_prev come from
+        // the step before it, on the device, and it runs only if told to
+        d.go = st.ci > 0 ? c->dscal + S_GO : nullptr;
+        d.dxscale = st.ci > 0 ? c->dscal + S_XS : nullptr;
+        d.drh0 = st.ci > 0 ? c->dscal + S_BETA : nullptr;
+        d.hm = st.ci > 0 ? c->dres + hres_chain(c->ncols) + 7 * (size_t)(st.ci - 1) : c->dres;
+        d.hm_next = c->dres + hres_chain(c->ncols) + 7 * (size_t)st.ci;
+        d.decide = st.decide;
+        d.kept = st.kept;
+        d.ntot = st.ntot;
+        d.eps_cgs = gi.mixed ? (double)FLT_EPSILON : DBL_EPSILON;
         d.xq = st.fuse ? colp<double>(c, st.qa) : nullptr;
         d.xq32 = st.fuse ? shadow32(c, st.qa) : nullptr;
         d.r = colp<double>(c, st.rcol);
@@ -2255,7 +2359,7 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
         d.dres = c->dres;
         const int kind = spmv_kind_batch(A);
         const int spmv_wg = (kind == -2 || kind == -3) ? stencil_wgs(A) : d.np_alpha;
-        const int nb[BP_COUNT] = {spmv_wg, d.g_upd, d.gx * d.gy, jnd, d.g_upd, 2};
+        const int nb[BP_COUNT] = {spmv_wg, d.g_upd, d.gx * d.gy, jnd, d.g_upd, 1};
         for (int ph = 0; ph < BP_COUNT; ++ph) {
             int *pp = pre + (size_t)ph * (R + 1);
             pp[i + 1] = pp[i] + nb[ph];
@@ -2348,25 +2452,31 @@ void steps_plan(const trlhip_batch *b, StepsPlan &P)
     order.reserve(b->steps.size());
     std::vector<char> taken(b->steps.size(), 0);
     size_t npre = 0;
-    for (size_t i = 0; i < b->steps.size(); ++i) {
-        if (taken[i]) continue;
-        GroupInfo gi{};
-        gi.lpr = spmv_kind_batch(b->steps[i].A);
-        gi.mixed = b->steps[i].c->V32 != nullptr;
-        gi.cb = g_proj_cb[gi.mixed ? 1 : 0].load(std::memory_order_relaxed);
-        gi.doff = order.size();
-        gi.poff = npre;
-        for (size_t k = i; k < b->steps.size(); ++k) {
-            const BatchStep &st = b->steps[k];
-            if (!taken[k] && spmv_kind_batch(st.A) == gi.lpr && (st.c->V32 != nullptr) == gi.mixed) {
-                order.push_back(st);
-                taken[k] = 1;
+    // chain position first: a context's follower runs after its step
+    // before it, so all steps at position 0 launch first, then position 1 ...
+    int maxci = 0;
+    for (const BatchStep &st : b->steps) maxci = std::max(maxci, st.ci);
+    for (int ci = 0; ci <= maxci; ++ci)
+        for (size_t i = 0; i < b->steps.size(); ++i) {
+            if (taken[i] || b->steps[i].ci != ci) continue;
+            GroupInfo gi{};
+            gi.lpr = spmv_kind_batch(b->steps[i].A);
+            gi.mixed = b->steps[i].c->V32 != nullptr;
+            gi.cb = g_proj_cb[gi.mixed ? 1 : 0].load(std::memory_order_relaxed);
+            gi.doff = order.size();
+            gi.poff = npre;
+            for (size_t k = i; k < b->steps.size(); ++k) {
+                const BatchStep &st = b->steps[k];
+                if (!taken[k] && st.ci == ci && spmv_kind_batch(st.A) == gi.lpr &&
+                    (st.c->V32 != nullptr) == gi.mixed) {
+                    order.push_back(st);
+                    taken[k] = 1;
+                }
             }
+            gi.R = (int)(order.size() - gi.doff);
+            npre += (size_t)BP_COUNT * (gi.R + 1);
+            groups.push_back(gi);
         }
-        gi.R = (int)(order.size() - gi.doff);
-        npre += (size_t)BP_COUNT * (gi.R + 1);
-        groups.push_back(gi);
-    }
     auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
     P.npre = npre;
     P.off_pre = up(sizeof(BDesc) * order.size());
@@ -3493,6 +3603,17 @@ int trlhip_col_scale(trlhip_ctx *c, int col, double s)
 int trlhip_col_scale_copy(trlhip_ctx *c, int dst, int src, double s)
 {
     if (!c || !valid_col(c, dst) || src < 0 || src > c->ncols) return TRLHIP_EINVAL;
+    if (chain_live(c)) {
+        // the follower's fused SpMV stored exactly this normalisation of the
+        // scratch column into dst (same 1/beta, bit for bit), so it is done
+        const double *slot = c->hres + hres_chain(c->ncols) + 7 * (size_t)(c->ch_next - 1);
+        const int k = c->ch_next;
+        if (c->ch_copy_ok || src != c->ncols || dst != c->ch_qa[k] ||
+            memcmp(&s, &slot[5], sizeof(double)) != 0)
+            return TRLHIP_EINVAL;
+        c->ch_copy_ok = 1;
+        return 0;
+    }
     if (int rc = flush_pending(c)) return rc;
     if (src == c->ncols) {  // deferred: the next SpMV of column dst applies it
         c->pend_dst = dst;
@@ -3604,10 +3725,40 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
     if (!first && qb >= 0 && qb != qa - 1) return TRLHIP_EINVAL;
     if (jnd < 1 || jnd > c->ncols || (rcol >= 0 && rcol < jnd)) return TRLHIP_EINVAL;
     if (A && !csr_fits(c, A)) return TRLHIP_EINVAL;
+    if (c->ch_next > 0) {
+        // the step a follower already ran (it must be exactly that step)
+        const int k = c->ch_next;
+        const double *slot = c->hres + hres_chain(c->ncols) + 7 * (size_t)(k - 1);
+        if (slot[4] == 1.0) {
+            if (!c->ch_copy_ok || first || qa != c->ch_qa[k] || qb != c->ch_qb[k] ||
+                rcol != c->ch_rcol[k] || jnd != c->ch_jnd[k] ||
+                memcmp(&beta_prev, &slot[6], sizeof(double)) != 0)
+                return TRLHIP_EINVAL;
+            out[0] = slot[S_ALPHA];
+            out[1] = slot[S_RR2];
+            out[2] = slot[S_HH];
+            out[3] = slot[S_RR2B];
+            c->ch_copy_ok = 0;
+            c->ch_next = k + 1 < c->ch_n ? k + 1 : 0;
+            return 0;
+        }
+        c->ch_next = 0;  // the chain stopped before this step: record it
+    }
     if (c->batch && A && c->E == 1) {
         // batched: record the step; trlhip_batch_flush runs it with the
         // other contexts' steps
-        BatchStep st{c, A, first, qa, qb, rcol, jnd, beta_prev, 0, 1.0, -1};
+        BatchStep st{c, A, first, qa, qb, rcol, jnd, beta_prev, 0, 1.0, -1, 0, 0, 0, 0};
+        // followers the solver allows (trlhip_step_chain): steps jnd + 1 ...
+        // on consecutive columns, each run only if the device decides the
+        // solver would accept the step before it as it is
+        int K = first ? 1 : std::min(c->ch_budget, kChainMax);
+        while (K > 1 && (!valid_col(c, rcol + K - 1) || jnd + K - 1 > c->ncols)) --K;
+        c->ch_budget = 0;
+        if (K > 1) {
+            st.decide = 1;
+            st.kept = c->ch_kept;
+            st.ntot = c->ch_ntot;
+        }
         if (c->pend_dst == qa && rcol != c->pend_src) {
             st.fuse = 1;
             st.fuse_s = c->pend_s;
@@ -3625,7 +3776,20 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
                 c->batch->cbuf.push_back(0.0);
             }
             c->batch->steps.push_back(st);
+            for (int k = 1; k < K; ++k) {
+                BatchStep f{c, A, 0, rcol + k - 1, k >= 2 ? rcol + k - 2 : qa, rcol + k, jnd + k, 0.0,
+                            1, 1.0, -1, c->ncols, k, k + 1 < K, c->ch_kept, c->ch_ntot};
+                c->ch_qa[k] = f.qa;
+                c->ch_qb[k] = f.qb;
+                c->ch_rcol[k] = f.rcol;
+                c->ch_jnd[k] = f.jnd;
+                c->hres[hres_chain(c->ncols) + 7 * (size_t)(k - 1) + 4] = 0.0;  // set if it runs
+                c->batch->steps.push_back(f);
+            }
         }
+        c->ch_n = K;
+        c->ch_next = K > 1 ? 1 : 0;
+        c->ch_copy_ok = 0;
         if (int rc = ctx_wait(c)) return rc;
         const volatile double *hr = c->hres;
         out[0] = hr[S_ALPHA];
@@ -3655,6 +3819,16 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
     out[1] = hr[S_RR2];
     out[2] = hr[S_HH];
     out[3] = hr[S_RR2B];
+    return 0;
+}
+
+int trlhip_step_chain(trlhip_ctx *c, int steps, int kept, int ntot)
+{
+    if (!c) return TRLHIP_EINVAL;
+    if (!c->batch || c->E != 1) return 0;  // chains are a batched-step device layout
+    c->ch_budget = steps;
+    c->ch_kept = kept;
+    c->ch_ntot = ntot;
     return 0;
 }
 

@@ -141,3 +141,44 @@ def test_batch_heterogeneous_groups(gpu_lib, monkeypatch):
         assert np.array_equal(g["evec"], s["evec"])
     for A in ops:
         A.close()
+
+
+def test_batch_chained_steps(gpu_lib, monkeypatch):
+    """Chained inner steps (several Lanczos steps of a solve in one round,
+    each run only when the device finds the solver would accept the step
+    before it) change nothing: every job equals its single solve bit for
+    bit with chains of 8 (the default), of 2 and with none, including jobs
+    whose chains stop early -- a basis as large as the problem (the last
+    step takes the local re-orthogonalisation, n = 30 = maxlan), frequent
+    convergence tests (ned close to nec), and mixed-precision bases."""
+    T = gpu_lib
+    specs = []
+    for i, (L, Wd, nev, maxlan, mixed) in enumerate([(24, 3.0, 8, 32, False), (40, 9.0, 8, 32, True),
+                                                     (56, 1.5, 4, 20, False), (32, 6.0, 12, 40, True),
+                                                     (64, 11.0, 8, 32, False)]):
+        rp, ci, v = W.anderson2d_csr(L, Wd, seed=300 + i)[0]
+        specs.append(((rp, ci, v), mixed, nev, maxlan))
+    rp, ci, v = W.random_sym_csr(30, seed=310)
+    specs.append(((rp, ci, v), False, 6, 30))
+    ops, probs = [], []
+    for i, (csr, mixed, nev, maxlan) in enumerate(specs):
+        A = T.Csr(*csr, mixed=mixed)
+        x0 = W.start_vector(len(csr[0]) - 1, seed=320 + i)
+        ops.append(A)
+        probs.append((A, -1, nev, maxlan, 1e-8, 1, 40000, nev, x0))
+    singles = [T.solve_csr(*p) for p in probs]
+    keys = ["stat", "nec", "matvec", "nloop", "north", "nrand", "locked"]
+    for chain in (None, "2", "0"):
+        if chain is None:
+            monkeypatch.delenv("TRLHIP_BATCH_CHAIN", raising=False)
+        else:
+            monkeypatch.setenv("TRLHIP_BATCH_CHAIN", chain)
+        rc, res, wall = T.solve_batch(probs)
+        assert rc == 0
+        print(f"chain {chain or 'default'}: matvec {[g['matvec'] for g in res]}", flush=True)
+        for g, s in zip(res, singles):
+            assert [g[k] for k in keys] == [s[k] for k in keys]
+            assert np.array_equal(g["eval"], s["eval"])
+            assert np.array_equal(g["evec"], s["evec"])
+    for A in ops:
+        A.close()
