@@ -943,7 +943,8 @@ __global__ __launch_bounds__(kBlock) void k_proj(
 // sums its columns in the order of update_rows.
 template <typename VS, typename RS>
 __device__ __forceinline__ double update_rows4(int row, const VS *__restrict__ Vc, long ld, int m,
-                                               const double *hs, const RS *rin, RS *rout)
+                                               const double *hs, const RS *rin, RS *rout,
+                                               double *ro_out = nullptr)
 {
     double s[4][4];
 #pragma unroll
@@ -973,6 +974,9 @@ __device__ __forceinline__ double update_rows4(int row, const VS *__restrict__ V
         nrm += ro[j] * ro[j];
     }
     store4(rout + row, ro);
+    if (ro_out)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ro_out[j] = ro[j];
     return nrm;
 }
 
@@ -1112,6 +1116,102 @@ __global__ __launch_bounds__(kBlock) void k_update(
     double *__restrict__ part, double *__restrict__ out_hh, double *__restrict__ out_coef)
 {
     update_body<VS, RS, E, RU>(n, V, ld, c0, m, h, h0, cpart, cnp, cslot, rin, rout, part, out_hh, out_coef, blockIdx.x, gridDim.x);
+}
+
+// The recurrence and the first Gram-Schmidt projection of a real step on
+// an FP32 shadow basis in one pass (n <= kMaxGrid * kMinRowsUpd rows):
+// block rb owns rows [1024 rb, 1024 rb + 1024), four consecutive rows per
+// thread.  It forms alpha from the SpMV partials and r = y - V(:, c0:c0+rm)
+// h in place exactly as update_body does for the recurrence (the same
+// rows per block and thread, so its ||r||^2 partial is the recurrence's,
+// at part_recur[rb]), keeps its rows of r in registers, and for each chunk
+// of CB columns of the shadow basis forms the projection partial exactly
+// as proj_body does for one row block (part_proj[p * gx + rb], gx the row
+// blocks): r is neither written nor read twice, and one kernel replaces
+// two.
+template <int CB>
+__device__ __forceinline__ void recproj_body(int n, const double *__restrict__ V, long ld, int c0,
+                                             int rm, const double *__restrict__ h, double h0,
+                                             const double *__restrict__ cpart, int cnp,
+                                             double *__restrict__ r, double *__restrict__ part_recur,
+                                             double *__restrict__ out_coef,
+                                             const float *__restrict__ VG, int m,
+                                             double *__restrict__ part_proj, int gx, int rb)
+{
+    extern __shared__ double hs[];
+    __shared__ double sm[kBlock / 64][CB];
+    for (int i = threadIdx.x; i < rm; i += kBlock) hs[i] = h ? h[i] : (i == 0 ? h0 : 0.0);
+    {
+        const double a = ordered_sum(cpart, cnp);  // contains __syncthreads
+        if (threadIdx.x == 0) {
+            hs[rm - 1] = a;
+            if (rb == 0 && out_coef) out_coef[0] = a;
+        }
+    }
+    __syncthreads();
+    const double *Vc = V + (size_t)c0 * ld;
+    const int row = (rb * kBlock + threadIdx.x) * 4;
+    double rv[4] = {0.0, 0.0, 0.0, 0.0};
+    double acc = 0.0;
+    const bool four = row + 3 < n;
+    if (four) {
+        acc += update_rows4<double, double>(row, Vc, ld, rm, hs, r, r, rv);
+    } else {
+        for (int j = 0; row + j < n; ++j) {
+            acc += update_rows<double, double, 1, 1>(row + j, Vc, ld, rm, hs, r, r);
+            rv[j] = r[row + j];
+        }
+    }
+    const double bs = block_sum(acc);
+    if (threadIdx.x == 0) part_recur[rb] = bs;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int cb = 0; cb < m; cb += CB) {
+        const int last = min(CB, m - cb) - 1;
+        const float *col[CB];
+#pragma unroll
+        for (int c = 0; c < CB; ++c) col[c] = VG + (size_t)(cb + min(c, last)) * ld;
+        double pacc[CB];
+#pragma unroll
+        for (int c = 0; c < CB; ++c) pacc[c] = 0.0;
+        if (four) {
+            double vv[CB][4];
+#pragma unroll
+            for (int c = 0; c < CB; ++c) load4(col[c] + row, vv[c]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int c = 0; c < CB; ++c) pacc[c] += vv[c][j] * rv[j];
+        } else {
+            for (int j = 0; row + j < n; ++j)
+#pragma unroll
+                for (int c = 0; c < CB; ++c) pacc[c] += (double)col[c][row + j] * rv[j];
+        }
+        const double ws = wave_transpose_sum<CB>(pacc, lane);
+        constexpr int grp = 64 / CB;
+        if (lane % grp == 0) sm[w][lane / grp] = ws;
+        __syncthreads();
+        if (threadIdx.x <= last) {
+            double t = 0.0;
+#pragma unroll
+            for (int i = 0; i < kBlock / 64; ++i) t += sm[i][threadIdx.x];
+            part_proj[(size_t)(cb + threadIdx.x) * gx + rb] = t;
+        }
+        __syncthreads();  // sm is reused by the next chunk
+    }
+}
+
+template <int CB>
+__global__ __launch_bounds__(kBlock) void k_recproj(int n, const double *__restrict__ V, long ld,
+                                                    int c0, int rm, const double *__restrict__ h,
+                                                    double h0, const double *__restrict__ cpart,
+                                                    int cnp, double *__restrict__ r,
+                                                    double *__restrict__ part_recur,
+                                                    double *__restrict__ out_coef,
+                                                    const float *__restrict__ VG, int m,
+                                                    double *__restrict__ part_proj)
+{
+    recproj_body<CB>(n, V, ld, c0, rm, h, h0, cpart, cnp, r, part_recur, out_coef, VG, m, part_proj,
+                     gridDim.x, blockIdx.x);
 }
 
 // dst = s * src over ne storage elements (dst may equal src); with dst32
@@ -1682,6 +1782,14 @@ int spmv_grid(const trlhip_csr *A)
     return g < kSpmvGrid ? g : kSpmvGrid;
 }
 
+// The fused recurrence + first projection (recproj_body) applies to real
+// contexts with an FP32 shadow basis whose rows fit kMaxGrid row blocks of
+// kMinRowsUpd (one recurrence block per row block).
+bool recproj_ok(const trlhip_ctx *c)
+{
+    return c->V32 && c->E == 1 && (long)c->n <= (long)kMaxGrid * kMinRowsUpd;
+}
+
 // Workgroups of the stencil SpMV: kStRB row blocks each (one alpha partial
 // per row block) when A has at most kSpmvGrid row blocks, else the
 // CSR-stream grid.
@@ -2195,6 +2303,17 @@ __global__ __launch_bounds__(kBlock) void kb_proj(const BDesc *__restrict__ d, c
                                     q.gy, bid_, nblk_);
 }
 
+template <int CB>
+__global__ __launch_bounds__(kBlock) void kb_recproj(const BDesc *__restrict__ d, const int *pre,
+                                                    int nreq)
+{
+    BATCH_LOCATE(BP_PROJ)
+    (void)nblk_;
+    recproj_body<CB>(q.n, q.V, q.ld, q.rc0, q.rm, q.rh, q.drh0 ? *q.drh0 : q.rh0, q.part_alpha,
+                     q.np_alpha, q.r, q.part_recur, q.dscal + S_ALPHA, (const float *)q.VG, q.jnd,
+                     q.part_proj, q.gx, bid_);
+}
+
 __global__ __launch_bounds__(kBlock) void kb_colsum(const BDesc *__restrict__ d, const int *pre, int nreq)
 {
     BATCH_LOCATE(BP_COLSUM)
@@ -2278,6 +2397,7 @@ struct GroupInfo {
     int R, lpr;  // lpr: the SpMV kernel (spmv_kind: lanes per row, 0: CSR-stream, -1: packed
                  // CSR-stream, -2 / -3: 2D / 3D stencil)
     bool mixed;
+    bool fused;         // recurrence and first projection in one kernel (recproj_ok)
     int cb;             // projection columns per block, read once for the group
     size_t doff, poff;  // offsets into the staging arrays
     double bytes[BP_COUNT];
@@ -2303,8 +2423,8 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
         c->np_alpha = d.np_alpha;
         d.g_upd = grid_for(n, kMinRowsUpd, kMaxGrid);
         const int cb = gi.cb;
-        d.gx = proj_gx(c, jnd, cb);
-        d.gy = (jnd + cb - 1) / cb;
+        d.gx = gi.fused ? d.g_upd : proj_gx(c, jnd, cb);  // fused: one block per row block
+        d.gy = gi.fused ? 1 : (jnd + cb - 1) / cb;
         if (st.first) {
             d.rm = jnd;
             d.rc0 = 0;
@@ -2369,8 +2489,12 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
                               ? spmv_bytes(A, 1, 8, 8)
                               : 12.0 * (double)A->nnz + 4.0 * (n + 1) + 16.0 * n;
         gi.bytes[BP_SPMV] += sb + (st.fuse ? (gi.mixed ? 12.0 : 8.0) * n : 0.0);
-        gi.bytes[BP_RECUR] += 8.0 * n * (d.rm + 2);
-        gi.bytes[BP_PROJ] += (vs * jnd + 8.0) * n;
+        if (gi.fused) {  // y, the recurrence columns, r written once; the shadow basis
+            gi.bytes[BP_PROJ] += 8.0 * n * (d.rm + 2) + vs * jnd * n;
+        } else {
+            gi.bytes[BP_RECUR] += 8.0 * n * (d.rm + 2);
+            gi.bytes[BP_PROJ] += (vs * jnd + 8.0) * n;
+        }
         gi.bytes[BP_UPDATE] += (vs * jnd + 16.0) * n;
         gi.maxrm = std::max(gi.maxrm, d.rm);
         gi.maxj = std::max(gi.maxj, jnd);
@@ -2397,7 +2521,7 @@ int batch_run(hipStream_t s, const GroupInfo &gi, const BDesc *dd, const int *dp
         default: KLAUNCH((kb_spmv<32>), blocks(BP_SPMV), dim3(kBlock), 0, s, dd, dp, R); break;
         }
     }
-    {
+    if (!gi.fused) {
         Timed t(TRLHIP_K_BRECUR, s, gi.bytes[BP_RECUR]);
         KLAUNCH(kb_recur, blocks(BP_RECUR), dim3(kBlock), sizeof(double) * gi.maxrm, s, dd, dp, R);
     }
@@ -2405,7 +2529,10 @@ int batch_run(hipStream_t s, const GroupInfo &gi, const BDesc *dd, const int *dp
         Timed t(TRLHIP_K_BPROJ, s, gi.bytes[BP_PROJ]);
 #define KB_PROJ_CB(CB)                                                                        \
     case CB:                                                                                  \
-        if (gi.mixed)                                                                         \
+        if (gi.fused)                                                                         \
+            KLAUNCH((kb_recproj<CB>), blocks(BP_PROJ), dim3(kBlock), sizeof(double) * gi.maxrm, \
+                    s, dd, dp, R);                                                            \
+        else if (gi.mixed)                                                                    \
             KLAUNCH((kb_proj<float, CB>), blocks(BP_PROJ), dim3(kBlock), 0, s, dd, dp, R);    \
         else                                                                                  \
             KLAUNCH((kb_proj<double, CB>), blocks(BP_PROJ), dim3(kBlock), 0, s, dd, dp, R);   \
@@ -2462,13 +2589,14 @@ void steps_plan(const trlhip_batch *b, StepsPlan &P)
             GroupInfo gi{};
             gi.lpr = spmv_kind_batch(b->steps[i].A);
             gi.mixed = b->steps[i].c->V32 != nullptr;
+            gi.fused = recproj_ok(b->steps[i].c);
             gi.cb = g_proj_cb[gi.mixed ? 1 : 0].load(std::memory_order_relaxed);
             gi.doff = order.size();
             gi.poff = npre;
             for (size_t k = i; k < b->steps.size(); ++k) {
                 const BatchStep &st = b->steps[k];
                 if (!taken[k] && st.ci == ci && spmv_kind_batch(st.A) == gi.lpr &&
-                    (st.c->V32 != nullptr) == gi.mixed) {
+                    (st.c->V32 != nullptr) == gi.mixed && recproj_ok(st.c) == gi.fused) {
                     order.push_back(st);
                     taken[k] = 1;
                 }
@@ -3642,9 +3770,17 @@ int trlhip_spmv(trlhip_ctx *c, const trlhip_csr *A, int xcol, int ycol, double *
 // coefficient (filled on the device from the partials).  ||r||^2 ->
 // dscal[S_RR2] (defer: its partials stay in part_recur, np_recur of them),
 // alpha -> dscal[S_ALPHA].
+// The recurrence's columns and coefficients, for a caller that runs it
+// fused with the projection (form_residual with `rec`: SpMV only).
+struct RecArgs {
+    int c0, m;
+    const double *coef;
+    double h0;
+};
+
 static int form_residual(trlhip_ctx *c, const trlhip_csr *A, int qa, int first, int qb,
                          double beta_prev, const double *beta, int jnd, int rcol,
-                         bool defer = false)
+                         bool defer = false, RecArgs *rec = nullptr)
 {
     const int E = c->E;
     int c0, m;
@@ -3678,6 +3814,10 @@ static int form_residual(trlhip_ctx *c, const trlhip_csr *A, int qa, int first, 
         if (!rc) rc = A ? run_spmv(c, A, qa, rcol) : run_dot(c, qa, rcol);
     }
     if (rc) return rc;
+    if (rec) {
+        *rec = RecArgs{c0, m, coef, h0};
+        return 0;
+    }
     if (defer)
         return run_update(c, c0, m, coef, m - 1, rcol, rcol, nullptr, nullptr, TRLHIP_K_RECUR, h0,
                           part_recur(c), &c->np_recur);
@@ -3799,11 +3939,41 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
         return 0;
     }
     if (int rc = batch_drain(c)) return rc;
-    int rc = form_residual(c, A, qa, first, qb, beta_prev, beta, jnd, rcol, true);
-    if (rc) return rc;
-    // speculative first Gram-Schmidt pass into the scratch column
-    rc = run_proj(c, 0, jnd, rcol, c->dh, true);
-    if (rc) return rc;
+    int rc;
+    if (recproj_ok(c)) {
+        // SpMV, then the recurrence fused with the first projection
+        RecArgs ra{};
+        rc = form_residual(c, A, qa, first, qb, beta_prev, beta, jnd, rcol, true, &ra);
+        if (rc) return rc;
+        const int gx = grid_for(c->n, kMinRowsUpd, kMaxGrid);
+        {
+            Timed t(TRLHIP_K_PROJ, c->s, 8.0 * c->n * (ra.m + 2) + 4.0 * jnd * c->n);
+            const int cb = g_proj_cb[1].load(std::memory_order_relaxed);
+#define RECPROJ_CB(CB)                                                                          \
+    case CB:                                                                                    \
+        KLAUNCH((k_recproj<CB>), dim3(gx), dim3(kBlock), sizeof(double) * ra.m, c->s, c->n,      \
+                colp<double>(c, 0), (long)c->ld, ra.c0, ra.m, ra.coef, ra.h0, part_alpha(c),      \
+                c->np_alpha, colp<double>(c, rcol), part_recur(c), c->dscal + S_ALPHA, c->V32, jnd, \
+                part_proj(c));                                                                  \
+        break;
+            switch (cb) {
+                RECPROJ_CB(4)
+                RECPROJ_CB(16)
+                default: RECPROJ_CB(8)
+            }
+#undef RECPROJ_CB
+            if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
+        }
+        c->np_recur = gx;
+        KLAUNCH(k_colsum, dim3(jnd), dim3(kBlock), 0, c->s, part_proj(c), gx, c->dh);
+        if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
+    } else {
+        rc = form_residual(c, A, qa, first, qb, beta_prev, beta, jnd, rcol, true);
+        if (rc) return rc;
+        // speculative first Gram-Schmidt pass into the scratch column
+        rc = run_proj(c, 0, jnd, rcol, c->dh, true);
+        if (rc) return rc;
+    }
     int g2 = 0;
     rc = run_update(c, 0, jnd, c->dh, -1, rcol, c->ncols, nullptr, c->dscal + S_HH,
                     TRLHIP_K_UPDATE, 0.0, nullptr, &g2, true);
