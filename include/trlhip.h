@@ -262,6 +262,19 @@ int trlhip_ritz(trlhip_ctx *c, int lck, int jl, int k, const double *Y,
 int trlhip_refine(trlhip_ctx *c, const trlhip_csr *A, int k, int avoid,
                   double *rho, double *res);
 
+/* The two device passes of the Rayleigh-Ritz polish of a mixed-precision
+ * solve's Ritz vectors (real bases, FP64 basis columns and operator), one
+ * device round trip each.  trlhip_residuals: column k+i = A v_i - rho[i]
+ * v_i for the columns v_i, i < k, and rr2[i] = its squared norm (the same
+ * kernels and sums as trlhip_spmv then trlhip_update per column).
+ * trlhip_gram: G = Q'Q and H = Q'AQ (m x m, column-major) over the
+ * columns Q = [0, m), with A Q_j formed in column tmp >= m (the same sums
+ * as trlhip_proj per column).  No counterpart in the reference: the polish
+ * follows the final refinement (trlhip_refine). */
+int trlhip_residuals(trlhip_ctx *c, const trlhip_csr *A, int k, const double *rho,
+                     double *rr2);
+int trlhip_gram(trlhip_ctx *c, const trlhip_csr *A, int m, int tmp, double *G, double *H);
+
 /* ---- measurement ---------------------------------------------------- */
 
 /* Kernel classes timed with HIP events on the solver stream when enabled. */

@@ -60,6 +60,7 @@ double trl_rand_uniform(uint64_t *state)
  * cycle, and the arrowhead reduction at a cycle's start. */
 static int g_prof;
 static long g_ns_restart, g_ns_check, g_ns_tridiag, g_n_restart;
+static long g_ns_alloc, g_ns_ctx; /* job setup: host arrays, device context */
 
 static long lz_ns(void)
 {
@@ -734,8 +735,11 @@ static int lz_alloc(lz_t *z, int maxlan)
     z->dsc = (double *)calloc(m, sizeof(double));
     z->esc = (double *)calloc(m, sizeof(double));
     z->iwrk = (int *)calloc(4 * m, sizeof(int));
-    z->hx = (double *)calloc((size_t)z->n * z->E, sizeof(double));
-    z->hy = (double *)calloc((size_t)z->n * z->E, sizeof(double));
+    /* host staging columns: every use writes all n entries before reading
+     * them, so no zeroing (calloc would clear 16n bytes per solve, most of
+     * a batch's setup time) */
+    z->hx = (double *)malloc(sizeof(double) * (size_t)z->n * z->E);
+    z->hy = (double *)malloc(sizeof(double) * (size_t)z->n * z->E);
     return (z->alpha && z->beta && z->alfrot && z->betrot && z->rot && z->lam && z->res &&
             z->tag && z->ctw && z->shw && z->yy && z->zq && z->dsc && z->esc && z->iwrk && z->hx && z->hy)
                ? 0
@@ -914,23 +918,19 @@ static int lz_polish(lz_t *z, int k, int avoid, const double *rho)
     double *Y = rnorm + m;
     int rc = 0, used = 0;
     if (save >= 0 && trlhip_col_copy(z->dev, save, avoid)) rc = TRLHIP_STAT_DEVICE;
+    /* the k residuals and their squared norms in one device pass (rnorm
+     * takes the norms in place), then G and H in another */
+    if (!rc && trlhip_residuals(z->dev, z->A, k, rho, rnorm)) rc = TRLHIP_STAT_DEVICE;
     for (int i = 0; i < k && !rc; ++i) {
-        double vav = 0.0, rr2 = 0.0;
-        if (trlhip_spmv(z->dev, z->A, i, k + i, &vav) ||
-            trlhip_update(z->dev, i, 1, &rho[i], k + i, &rr2))
-            rc = TRLHIP_STAT_DEVICE;
-        else if (!(rr2 > DBL_MIN))
+        const double rr2 = rnorm[i];
+        if (!(rr2 > DBL_MIN))
             break;
         else if (trlhip_col_scale(z->dev, k + i, 1.0 / sqrt(rr2)))
             rc = TRLHIP_STAT_DEVICE;
         else
             rnorm[used++] = sqrt(rr2);
     }
-    for (int j = 0; j < m && !rc && used == k; ++j) {
-        if (trlhip_proj(z->dev, 0, m, j, G + (size_t)j * m) || trlhip_spmv(z->dev, z->A, j, tmp, NULL) ||
-            trlhip_proj(z->dev, 0, m, tmp, H + (size_t)j * m))
-            rc = TRLHIP_STAT_DEVICE;
-    }
+    if (!rc && used == k && trlhip_gram(z->dev, z->A, m, tmp, G, H)) rc = TRLHIP_STAT_DEVICE;
     z->info->matvec += used + (used == k ? m : 0);
     int done = 0;
     if (!rc && used == k) {
@@ -1442,15 +1442,21 @@ static int lz_setup(lz_t *z, trl_matvec op, ztrl_matvec zop, trl_uo user_ortho, 
                               cplx ? TRLHIP_C128 : TRLHIP_R64);
     z->mixed = !cplx && trlhip_csr_is_mixed(z->A);
     const int vt = cplx ? TRLHIP_C128 : (z->mixed ? TRLHIP_R32 : TRLHIP_R64);
+    const long ta0 = g_prof ? lz_ns() : 0;
     if (lz_alloc(z, DMAX(info->maxlan, mev)) != 0) {
         info->stat = -4;
         lz_free(z);
         return 0;
     }
+    const long ta1 = g_prof ? lz_ns() : 0;
     if (trlhip_ctx_create_batched(&z->dev, nrow, z->ncols, vt, b) != 0) {
         info->stat = TRLHIP_STAT_DEVICE;
         lz_free(z);
         return 0;
+    }
+    if (g_prof) {
+        g_ns_alloc += ta1 - ta0;
+        g_ns_ctx += lz_ns() - ta1;
     }
     /* batched real solves chain up to 8 inner steps per round (measured
      * on the batched 2D Anderson workload: 4 steps 8,344, 6: 8,560,
@@ -1747,6 +1753,12 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     if (njobs <= 0) return 0;
     if (!jobs) return TRLHIP_EINVAL;
     const double t_begin = lz_now();
+    /* TRLHIP_BATCH_PROFILE=1: host time in the setup, in the solves and
+     * waiting for the device */
+    const char *prof = getenv("TRLHIP_BATCH_PROFILE");
+    g_prof = prof && prof[0] == '1';
+    g_ns_restart = g_ns_check = g_ns_tridiag = g_n_restart = 0;
+    g_ns_alloc = g_ns_ctx = 0;
     const int nlanes = lz_batch_lanes(njobs);
     trlhip_batch *bl[4] = {NULL, NULL, NULL, NULL};
     int rc = 0;
@@ -1870,11 +1882,6 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
             for (int k = 0; k < pool.nlane[l]; ++k)
                 pool.lanes[l][k]->home = (int)((long)k * pool.nthr / pool.nlane[l]);
     }
-    /* TRLHIP_BATCH_PROFILE=1: host time in the solves vs waiting for the
-     * device */
-    const char *prof = getenv("TRLHIP_BATCH_PROFILE");
-    g_prof = prof && prof[0] == '1';
-    g_ns_restart = g_ns_check = g_ns_tridiag = g_n_restart = 0;
     double t_co = 0.0, t_wait = 0.0;
     long rounds = 0;
     const double t_loop = lz_now();
@@ -2016,8 +2023,10 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     for (int l = nlanes - 1; l >= 0; --l) trlhip_batch_destroy(bl[l]); /* lane 0 owns the stream */
     if (prof && prof[0] == '1')
         fprintf(stderr, "trlan_batch: setup %.3f s (lanes %.3f, jobs %.3f of which job setup %.3f "
-                "and stacks %.3f), rounds %.3f s, teardown %.3f s\n",
-                t_loop - t_begin, t_jobs - t_begin, t_pool - t_jobs, t_su, t_st,
+                "[host arrays %.3f, device contexts %.3f] and stacks %.3f), rounds %.3f s, "
+                "teardown %.3f s\n",
+                t_loop - t_begin, t_jobs - t_begin, t_pool - t_jobs, t_su, 1e-9 * g_ns_alloc,
+                1e-9 * g_ns_ctx, t_st,
                 t_end_loop - t_loop, lz_now() - t_end_loop);
     return rc;
 }

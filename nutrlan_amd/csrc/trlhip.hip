@@ -439,7 +439,7 @@ constexpr int kStU = 4;
 
 template <int D, bool FUSE>
 __device__ __forceinline__ void spmv_stencil_body(
-    int nb, const int *__restrict__ rb, int L, double off, const double *__restrict__ dg,
+    int n, int nb, int L, double off, const double *__restrict__ dg,
     const double *__restrict__ x, double *__restrict__ y, double *__restrict__ part, double xscale,
     double *__restrict__ xq, float *__restrict__ xq32, int bid, int nblk)
 {
@@ -453,7 +453,7 @@ __device__ __forceinline__ void spmv_stencil_body(
             const int blk = blk0 + u * nblk;
             on[u] = false;
             if (blk < nb) {
-                const int r0 = rb[blk], r1 = rb[blk + 1];
+                const int r0 = blk * kRowsBlk, r1 = min(n, r0 + kRowsBlk);
                 if (t < r1 - r0) {
                     on[u] = true;
                     sv[u] = stencil_row<D, FUSE>(r0 + t, L, off, dg, x, xscale, xq, xq32, xr[u]);
@@ -476,7 +476,7 @@ __device__ __forceinline__ void spmv_stencil_body(
 // spmv_stencil_body's bit for bit while a workgroup keeps RB rows' loads in
 // flight per thread.
 template <int D, int RB>
-__device__ __forceinline__ void spmv_stencil_rb(int nb, const int *__restrict__ rb, int L, double off,
+__device__ __forceinline__ void spmv_stencil_rb(int n, int nb, int L, double off,
                                                 const double *__restrict__ dg,
                                                 const double *__restrict__ x, double *__restrict__ y,
                                                 double *__restrict__ part, double xscale,
@@ -491,7 +491,7 @@ __device__ __forceinline__ void spmv_stencil_rb(int nb, const int *__restrict__ 
         const int blk = w * RB + u;
         acc[u] = 0.0;
         if (blk < nb) {
-            const int r0 = rb[blk], r1 = rb[blk + 1];
+            const int r0 = blk * kRowsBlk, r1 = min(n, r0 + kRowsBlk);
             if (t < r1 - r0) {
                 double xr;
                 const double s = stencil_row<D, true>(r0 + t, L, off, dg, x, xscale, xq, xq32, xr);
@@ -517,11 +517,11 @@ __device__ __forceinline__ void spmv_stencil_rb(int nb, const int *__restrict__ 
 
 template <int D, bool FUSE>
 __global__ __launch_bounds__(kBlock) void k_spmv_st(
-    int nb, const int *__restrict__ rb, int L, double off, const double *__restrict__ dg,
+    int n, int nb, int L, double off, const double *__restrict__ dg,
     const double *__restrict__ x, double *__restrict__ y, double *__restrict__ part, double xscale,
     double *__restrict__ xq, float *__restrict__ xq32)
 {
-    spmv_stencil_body<D, FUSE>(nb, rb, L, off, dg, x, y, part, xscale, xq, xq32, blockIdx.x,
+    spmv_stencil_body<D, FUSE>(n, nb, L, off, dg, x, y, part, xscale, xq, xq32, blockIdx.x,
                                gridDim.x);
 }
 
@@ -530,11 +530,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_st(
 // block).  xscale = 1 with no xq / xq32 is the plain product: x * 1 is x.
 template <int D, int RB>
 __global__ __launch_bounds__(kBlock) void k_spmv_st_rb(
-    int nb, const int *__restrict__ rb, int L, double off, const double *__restrict__ dg,
+    int n, int nb, int L, double off, const double *__restrict__ dg,
     const double *__restrict__ x, double *__restrict__ y, double *__restrict__ part, double xscale,
     double *__restrict__ xq, float *__restrict__ xq32)
 {
-    spmv_stencil_rb<D, RB>(nb, rb, L, off, dg, x, y, part, xscale, xq, xq32, blockIdx.x);
+    spmv_stencil_rb<D, RB>(n, nb, L, off, dg, x, y, part, xscale, xq, xq32, blockIdx.x);
 }
 
 // Column-panel SpMV, for large operators whose rows gather x from all
@@ -1514,8 +1514,9 @@ struct trlhip_ctx {
     double *dY;     // restart matrix
     size_t dYcap;
     double *hbuf;   // pinned host mirror: 16 scalars + coefficients
-    double *hres;   // pinned host-coherent results: a fused step's 4 scalars, then
-                    // 3 per refined pair from index 8 (hres_count doubles)
+    double *hres;   // pinned host-coherent results: a fused step's 4 scalars, a
+                    // batched column norm at kHresNorm, then 3 per refined pair
+                    // from index 8 (hres_count doubles)
     double *dres;   // its device address
     int pend_dst;   // column of a deferred scale-copy (-1: none)
     double pend_s;  // its scale
@@ -1552,16 +1553,17 @@ struct BatchStep {
     int kept, ntot;   // the solver's kept and info->ntot (for that decision)
 };
 
-// A recorded restart GEMM or scale-copy of a batched context; `phase` is
-// its ordinal among the context's recorded operations since the last
-// flush (a flush runs phase 0 of every context, then phase 1, ...).
-enum { OP_RITZ = 0, OP_COPY = 1, OP_REFINE = 2 };
+// A recorded restart GEMM, scale-copy, refinement or column norm of a
+// batched context; `phase` is its ordinal among the context's recorded
+// operations since the last flush (a flush runs phase 0 of every context,
+// then phase 1, ...).
+enum { OP_RITZ = 0, OP_COPY = 1, OP_REFINE = 2, OP_NORM = 3 };
 struct BatchOp {
     trlhip_ctx *c;
     int type, phase;
     int lck, jl, k, ldy;  // restart GEMM; Y at ybuf[yoff], ldy x k
     size_t yoff;
-    int dst, src;         // scale-copy dst = s * src
+    int dst, src;         // scale-copy dst = s * src; norm: ||dst||^2
     double s;
     const float *src32;   // copy from an FP32 device vector instead of column src
     const trlhip_csr *A;  // refinement of columns [0, k) (k above), `avoid` kept
@@ -1732,6 +1734,7 @@ double *part_update(trlhip_ctx *c) { return c->part + kSpmvGrid; }
 double *part_recur(trlhip_ctx *c) { return c->part + kSpmvGrid + kMaxGrid; }
 double *part_proj(trlhip_ctx *c) { return c->part + kSpmvGrid + 2 * kMaxGrid; }
 size_t hres_chain(int ncols) { return 8 + 3 * (size_t)(ncols + 1); }
+constexpr int kHresNorm = 4;  // a batched column norm (OP_NORM)
 // host-coherent results: a step's 4 scalars, 3 per refined pair from index
 // 8, then 7 per chain follower from hres_chain (alpha, ||r||^2, ||h||^2,
 // ||r - V h||^2, go, 1/beta, beta)
@@ -1858,15 +1861,15 @@ int launch_spmv(const trlhip_csr *A, int vtype, const void *x, void *y, double *
         if (A->nb <= kSpmvGrid) {  // one alpha partial per row block
             const dim3 gw(stencil_wgs(A));
             if (A->st_dim == 2)
-                KLAUNCH((k_spmv_st_rb<2, kStRB>), gw, b, 0, s, A->nb, A->rb, A->st_L, A->st_off, A->dg,
+                KLAUNCH((k_spmv_st_rb<2, kStRB>), gw, b, 0, s, A->n, A->nb, A->st_L, A->st_off, A->dg,
                         xs, ys, part, xq ? xscale : 1.0, q, xq ? xq32 : (float *)nullptr);
             else
-                KLAUNCH((k_spmv_st_rb<3, kStRB>), gw, b, 0, s, A->nb, A->rb, A->st_L, A->st_off, A->dg,
+                KLAUNCH((k_spmv_st_rb<3, kStRB>), gw, b, 0, s, A->n, A->nb, A->st_L, A->st_off, A->dg,
                         xs, ys, part, xq ? xscale : 1.0, q, xq ? xq32 : (float *)nullptr);
             return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
         }
 #define ST_CASE(D, F)                                                                             \
-    KLAUNCH((k_spmv_st<D, F>), g, b, 0, s, A->nb, A->rb, A->st_L, A->st_off, A->dg, xs, ys, part, \
+    KLAUNCH((k_spmv_st<D, F>), g, b, 0, s, A->n, A->nb, A->st_L, A->st_off, A->dg, xs, ys, part, \
             xscale, q, F ? xq32 : (float *)nullptr)
         if (A->st_dim == 2) {
             if (xq) ST_CASE(2, true);
@@ -2246,45 +2249,57 @@ __device__ __forceinline__ int batch_find(const int *__restrict__ pre, int nreq,
     const BDesc &q = d[p_];                                                   \
     if (q.go && *q.go == 0.0) return; /* a chain follower that does not run */
 
+// The batched SpMVs' locate: a chain follower's go flag and fused scale
+// 1/beta are adjacent device scalars (S_GO, S_XS), read in one load.
+static_assert(S_XS == S_GO + 1 && S_GO % 2 == 0, "go flag and scale adjacent, 16-byte aligned");
+#define BATCH_LOCATE_SPMV                                                     \
+    const int *pre_ = pre + (size_t)BP_SPMV * (nreq + 1);                     \
+    const int p_ = batch_find(pre_, nreq, blockIdx.x);                        \
+    const int bid_ = blockIdx.x - pre_[p_], nblk_ = pre_[p_ + 1] - pre_[p_];  \
+    const BDesc &q = d[p_];                                                   \
+    double xs_ = q.xscale;                                                    \
+    if (q.go) {                                                               \
+        const double2 gs_ = *reinterpret_cast<const double2 *>(q.go);         \
+        if (gs_.x == 0.0) return; /* a chain follower that does not run */    \
+        xs_ = gs_.y;                                                          \
+    }
+
 __global__ __launch_bounds__(kBlock) void kb_spmv_stream(const BDesc *__restrict__ d, const int *pre,
                                                         int nreq)
 {
-    BATCH_LOCATE(BP_SPMV)
+    BATCH_LOCATE_SPMV
     spmv_stream_body<double, double, 1, true>(q.nb, q.rb, q.rp, q.ci, q.val, q.x, q.r, q.part_alpha,
-                                              q.dxscale ? *q.dxscale : q.xscale, q.xq, q.xq32, bid_,
-                                              nblk_);
+                                              xs_, q.xq, q.xq32, bid_, nblk_);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_spmv_pk(const BDesc *__restrict__ d, const int *pre,
                                                     int nreq)
 {
-    BATCH_LOCATE(BP_SPMV)
+    BATCH_LOCATE_SPMV
     spmv_stream_body<double, double, 1, true, true>(q.nb, q.rb, q.rp, nullptr, nullptr, q.x, q.r,
-                                                    q.part_alpha, q.dxscale ? *q.dxscale : q.xscale,
-                                                    q.xq, q.xq32, bid_, nblk_, q.pk, q.dg, q.tab);
+                                                    q.part_alpha, xs_, q.xq, q.xq32, bid_, nblk_,
+                                                    q.pk, q.dg, q.tab);
 }
 
 template <int D>
 __global__ __launch_bounds__(kBlock) void kb_spmv_st(const BDesc *__restrict__ d, const int *pre,
                                                     int nreq)
 {
-    BATCH_LOCATE(BP_SPMV)
-    const double xs = q.dxscale ? *q.dxscale : q.xscale;
+    BATCH_LOCATE_SPMV
     if (q.nb <= kSpmvGrid)
-        spmv_stencil_rb<D, kStRB>(q.nb, q.rb, q.st_L, q.st_off, q.dg, q.x, q.r, q.part_alpha, xs,
+        spmv_stencil_rb<D, kStRB>(q.n, q.nb, q.st_L, q.st_off, q.dg, q.x, q.r, q.part_alpha, xs_,
                                   q.xq, q.xq32, bid_);
     else
-        spmv_stencil_body<D, true>(q.nb, q.rb, q.st_L, q.st_off, q.dg, q.x, q.r, q.part_alpha, xs,
+        spmv_stencil_body<D, true>(q.n, q.nb, q.st_L, q.st_off, q.dg, q.x, q.r, q.part_alpha, xs_,
                                    q.xq, q.xq32, bid_, nblk_);
 }
 
 template <int LPR>
 __global__ __launch_bounds__(kBlock) void kb_spmv(const BDesc *__restrict__ d, const int *pre, int nreq)
 {
-    BATCH_LOCATE(BP_SPMV)
+    BATCH_LOCATE_SPMV
     spmv_vec_body<double, double, 1, LPR, 4, true>(q.n, q.rp, q.ci, q.val, q.x, q.r, q.part_alpha,
-                                                   q.dxscale ? *q.dxscale : q.xscale, q.xq, q.xq32,
-                                                   bid_, nblk_);
+                                                   xs_, q.xq, q.xq32, bid_, nblk_);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_recur(const BDesc *__restrict__ d, const int *pre, int nreq)
@@ -2388,6 +2403,7 @@ __global__ __launch_bounds__(kBlock) void kb_sum2(const BDesc *__restrict__ d, c
     }
 }
 #undef BATCH_LOCATE
+#undef BATCH_LOCATE_SPMV
 
 // Host staging of one group of recorded steps (same lanes per row, same
 // basis precision): descriptors d[0:R) and BP_COUNT prefix tables of R + 1
@@ -2650,6 +2666,9 @@ struct ODesc {
     const double *Y;
     double s;
     int lck, jl, k, ldy;
+    double *part;        // norm: its g block partials, the sum to o1 and o2
+    double *o1, *o2;
+    int g;
 };
 
 template <int KT>
@@ -2671,6 +2690,29 @@ __global__ __launch_bounds__(kBlock) void kb_copy(const ODesc *__restrict__ d, c
         const double v = (q.src32 ? (double)q.src32[i] : q.src[i]) * q.s;
         q.V[i] = v;
         if (q.V32) q.V32[i] = (float)v;
+    }
+}
+
+// ||V||^2 of one column per request, in place: exactly the single
+// context's trlhip_update with no coefficients (k_update over the same g
+// row blocks, then k_sum), so batched and single solves agree bit for bit.
+__global__ __launch_bounds__(kBlock) void kb_norm(const ODesc *__restrict__ d, const int *pre, int nreq)
+{
+    const int p = batch_find(pre, nreq, blockIdx.x);
+    const ODesc &q = d[p];
+    update_body<double, double, 1, 4>((int)q.n, q.V, q.ld, 0, 0, nullptr, 0.0, nullptr, 0, -1, q.V,
+                                      q.V, q.part, nullptr, nullptr, blockIdx.x - pre[p],
+                                      pre[p + 1] - pre[p]);
+}
+
+// One block per request: the norm's fixed-order sum (k_sum's).
+__global__ __launch_bounds__(kBlock) void kb_norm_fin(const ODesc *__restrict__ d)
+{
+    const ODesc &q = d[blockIdx.x];
+    const double t = ordered_sum(q.part, q.g);
+    if (threadIdx.x == 0) {
+        q.o1[0] = t;
+        q.o2[0] = t;
     }
 }
 
@@ -2775,7 +2817,7 @@ int refine_free_col(const trlhip_ctx *c, int k, int avoid, int j)
 // staging block of `bytes` bytes, copied to the device at once.  Operations
 // of one phase belong to different contexts, so their order inside the
 // phase is free.
-enum { L_COPY = 0, L_REFINE = 9 };  // 1..8: restart GEMMs of that width
+enum { L_COPY = 0, L_REFINE = 9, L_NORM = 10 };  // 1..8: restart GEMMs of that width
 struct OpsPlan {
     struct Launch {
         int kind, R;
@@ -2808,7 +2850,7 @@ void ops_plan(const std::vector<BatchOp> &ops, const std::vector<double> &ybuf, 
         for (int kind = 0; kind <= 8; ++kind) {
             Launch l{kind, 0, order.size(), npre, 0, 0.0};
             for (const BatchOp &op : ops) {
-                if (op.phase != ph || op.type == OP_REFINE) continue;
+                if (op.phase != ph || op.type == OP_REFINE || op.type == OP_NORM) continue;
                 const int okind = op.type == OP_COPY ? L_COPY : (op.k + 15) / 16;
                 if (okind == kind) order.push_back(&op);
             }
@@ -2816,6 +2858,16 @@ void ops_plan(const std::vector<BatchOp> &ops, const std::vector<double> &ybuf, 
             if (l.R == 0) continue;
             npre += (size_t)l.R + 1;
             ls.push_back(l);
+        }
+        {
+            Launch l{L_NORM, 0, order.size(), npre, 0, 0.0};
+            for (const BatchOp &op : ops)
+                if (op.phase == ph && op.type == OP_NORM) order.push_back(&op);
+            l.R = (int)(order.size() - l.doff);
+            if (l.R > 0) {
+                npre += (size_t)l.R + 1;
+                ls.push_back(l);
+            }
         }
         for (int chunk = 0;; ++chunk) {
             Launch l{L_REFINE, 0, pairs.size(), npre, 0, 0.0};
@@ -2892,7 +2944,17 @@ void ops_fill(OpsPlan &P, const std::vector<double> &ybuf, char *h, const char *
             ODesc &o = ho[l.doff + i];
             o = ODesc{};
             int nb;
-            if (op.type == OP_COPY) {
+            if (op.type == OP_NORM) {
+                o.n = c->n;
+                o.ld = c->ld;
+                o.V = colp<double>(c, op.dst);
+                o.part = part_update(c);
+                o.o1 = c->dscal + S_RR2;
+                o.o2 = c->dres + kHresNorm;
+                nb = grid_for(c->n, kMinRowsUpd, kMaxGrid);
+                o.g = nb;
+                l.bytes += 16.0 * c->n;
+            } else if (op.type == OP_COPY) {
                 o.n = c->n;
                 o.V = colp<double>(c, op.dst);
                 o.src = op.src32 ? nullptr : colp<double>(c, op.src);
@@ -2954,6 +3016,14 @@ int ops_launch(const OpsPlan &P, const char *h, const char *d, hipStream_t s)
         if (l.kind == L_COPY) {
             Timed t(TRLHIP_K_MISC, s, l.bytes);
             KLAUNCH(kb_copy, g, dim3(kBlock), 0, s, o, dpp, l.R);
+            continue;
+        }
+        if (l.kind == L_NORM) {
+            {
+                Timed t(TRLHIP_K_MISC, s, l.bytes);
+                KLAUNCH(kb_norm, g, dim3(kBlock), 0, s, o, dpp, l.R);
+            }
+            KLAUNCH(kb_norm_fin, dim3(l.R), dim3(kBlock), 0, s, o);
             continue;
         }
         Timed t(TRLHIP_K_BRITZ, s, l.bytes, l.flops);
@@ -3321,6 +3391,11 @@ int trlhip_csr_create(trlhip_csr **out, int n, long nnz, const int *row_ptr, con
             if (!rc && !(es && es[0] == '0'))
                 A->st_dim = detect_stencil(n, nnz, row_ptr, col_idx, (const double *)values, &A->st_L,
                                            &A->st_off);
+            // the stencil kernels form row block b as rows [kRowsBlk b,
+            // kRowsBlk (b + 1)) instead of reading rb (2D + 1 <= 7 nonzeros
+            // a row always fill kRowsBlk rows a block)
+            for (int b = 0; A->st_dim && b < A->nb; ++b)
+                if (rb[b] != b * kRowsBlk || rb[b + 1] != std::min(n, (b + 1) * kRowsBlk)) A->st_dim = 0;
         }
     }
     // column panels on request (TRLHIP_CSR_PANEL=1; TRLHIP_CSR_PANEL_COLS:
@@ -3597,7 +3672,8 @@ int trlhip_ctx_create_batched(trlhip_ctx **out, int n, int ncols, int vtype, trl
         c->s = b->side;
         c->arena = 1;
         const size_t np = part_need(c->E, ncols);
-        const size_t ny = (size_t)(ncols + 1) * (ncols + 1);
+        // restart matrices, and the polish's two Gram matrices (trlhip_gram)
+        const size_t ny = std::max((size_t)(ncols + 1) * (ncols + 1), 2 * (size_t)ncols * ncols);
         c->V = (char *)arena_get(b, false, vbytes);
         if (vtype == TRLHIP_R32)
             c->V32 = (float *)arena_get(b, false, sizeof(float) * (size_t)c->ld * (ncols + 1));
@@ -4040,6 +4116,26 @@ int trlhip_update(trlhip_ctx *c, int c0, int m, const double *h, int rcol, doubl
     if (!c || !h || m < 0 || !valid_cols(c, c0, m) || !valid_col(c, rcol) ||
         (m > 0 && rcol >= c0 && rcol < c0 + m))
         return TRLHIP_EINVAL;
+    if (c->batch && c->E == 1 && m == 0) {
+        // a column norm in a batch (a start vector's): recorded after the
+        // context's other records, run with every context's norms at the
+        // flush (kb_norm, the same sums as below)
+        if (chain_live(c)) return TRLHIP_EINVAL;
+        if (int rc = flush_pending(c)) return rc;
+        BatchOp op{};
+        op.c = c;
+        op.type = OP_NORM;
+        op.dst = rcol;
+        {
+            std::lock_guard<SpinLock> lk(c->batch->mu);
+            op.phase = c->nrec++;
+            c->batch->ops.push_back(op);
+        }
+        if (int rc = ctx_wait(c)) return rc;
+        c->hbuf[4] = c->hres[kHresNorm];
+        if (rr2) *rr2 = c->hbuf[4];
+        return 0;
+    }
     if (int rc = settle(c)) return rc;
     const int E = c->E;
     memcpy(c->hbuf + 16, h, sizeof(double) * E * (size_t)m);
@@ -4050,6 +4146,60 @@ int trlhip_update(trlhip_ctx *c, int c0, int m, const double *h, int rcol, doubl
     rc = fetch(c, c->dscal + S_RR2, c->hbuf + 4, 1);
     if (rc) return rc;
     if (rr2) *rr2 = c->hbuf[4];
+    return 0;
+}
+
+int trlhip_residuals(trlhip_ctx *c, const trlhip_csr *A, int k, const double *rho, double *rr2)
+{
+    if (!c || !rho || !rr2 || k < 1 || c->E != 1 || !csr_fits(c, A) || 2 * k > c->ncols)
+        return TRLHIP_EINVAL;
+    if (int rc = settle(c)) return rc;
+    // rho staged once (hbuf holds 16 + 2 (ncols + 2) doubles); each norm's
+    // fixed-order sum goes straight to the host-coherent results (the
+    // refinement's slots, free once trlhip_refine has returned)
+    memcpy(c->hbuf + 16, rho, sizeof(double) * (size_t)k);
+    TRY(hipMemcpyAsync(c->dh, c->hbuf + 16, sizeof(double) * (size_t)k, hipMemcpyHostToDevice, c->s));
+    for (int i = 0; i < k; ++i) {
+        if (int rc = run_spmv(c, A, i, k + i)) return rc;
+        if (int rc = run_update(c, i, 1, c->dh + i, -1, k + i, k + i, c->dres + 8 + i, nullptr,
+                                TRLHIP_K_UPDATE))
+            return rc;
+    }
+    if (int rc = ctx_wait(c)) return rc;
+    const volatile double *hr = c->hres + 8;
+    for (int i = 0; i < k; ++i) rr2[i] = hr[i];
+    return 0;
+}
+
+int trlhip_gram(trlhip_ctx *c, const trlhip_csr *A, int m, int tmp, double *G, double *H)
+{
+    if (!c || !G || !H || m < 1 || c->E != 1 || !csr_fits(c, A) || !valid_cols(c, 0, m) ||
+        !valid_col(c, tmp) || tmp < m)
+        return TRLHIP_EINVAL;
+    if (int rc = settle(c)) return rc;
+    const size_t mm = (size_t)m * m;
+    if (c->dYcap < 2 * mm) {
+        if (c->arena) return TRLHIP_EINVAL;  // sized for 2 ncols^2 at creation
+        if (c->dY) (void)hipFree(c->dY);
+        if (c->hY) (void)hipHostFree(c->hY);
+        c->dY = nullptr;
+        c->hY = nullptr;
+        c->dYcap = 0;
+        if (hipMalloc((void **)&c->dY, sizeof(double) * 2 * mm) != hipSuccess) return TRLHIP_ENOMEM;
+        if (hipHostMalloc((void **)&c->hY, sizeof(double) * 2 * mm, 0) != hipSuccess) return TRLHIP_ENOMEM;
+        c->dYcap = 2 * mm;
+    }
+    // column j of G and of H by the projection kernels (trlhip_proj's sums),
+    // both matrices back in one copy
+    for (int j = 0; j < m; ++j) {
+        if (int rc = run_proj(c, 0, m, j, c->dY + (size_t)j * m)) return rc;
+        if (int rc = run_spmv(c, A, j, tmp)) return rc;
+        if (int rc = run_proj(c, 0, m, tmp, c->dY + mm + (size_t)j * m)) return rc;
+    }
+    TRY(hipMemcpyAsync(c->hY, c->dY, sizeof(double) * 2 * mm, hipMemcpyDeviceToHost, c->s));
+    if (int rc = ctx_wait(c)) return rc;
+    memcpy(G, c->hY, sizeof(double) * mm);
+    memcpy(H, c->hY + mm, sizeof(double) * mm);
     return 0;
 }
 
