@@ -2322,9 +2322,19 @@ template <int CB>
 __global__ __launch_bounds__(kBlock) void kb_recproj(const BDesc *__restrict__ d, const int *pre,
                                                     int nreq)
 {
-    BATCH_LOCATE(BP_PROJ)
-    (void)nblk_;
-    recproj_body<CB>(q.n, q.V, q.ld, q.rc0, q.rm, q.rh, q.drh0 ? *q.drh0 : q.rh0, q.part_alpha,
+    const int *pre_ = pre + (size_t)BP_PROJ * (nreq + 1);
+    const int p_ = batch_find(pre_, nreq, blockIdx.x);
+    const int bid_ = blockIdx.x - pre_[p_];
+    const BDesc &q = d[p_];
+    double h0 = q.rh0;
+    if (q.go) {  // a chain follower: go flag (S_GO) and beta_prev (S_BETA) loaded together
+        static_assert(S_BETA == S_GO + 2, "go flag and beta in one 32-byte block");
+        const double2 a = reinterpret_cast<const double2 *>(q.go)[0];
+        const double2 b = reinterpret_cast<const double2 *>(q.go)[1];
+        if (a.x == 0.0) return;  // a follower that does not run
+        h0 = b.x;
+    }
+    recproj_body<CB>(q.n, q.V, q.ld, q.rc0, q.rm, q.rh, h0, q.part_alpha,
                      q.np_alpha, q.r, q.part_recur, q.dscal + S_ALPHA, (const float *)q.VG, q.jnd,
                      q.part_proj, q.gx, bid_);
 }
