@@ -479,6 +479,8 @@ class Ctx:
             ("trlhip_col_scale_copy", [P, I, I, D]),
             ("trlhip_step_cgs", [P, P, I, I, I, D, P, I, I, P]),
             ("trlhip_refine", [P, P, I, I, P, P]),
+            ("trlhip_residuals", [P, P, I, P, P]),
+            ("trlhip_gram", [P, P, I, I, P, P]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -554,6 +556,22 @@ class Ctx:
         res = np.zeros(k)
         self._chk(lib().trlhip_refine(self.h, A.handle, k, avoid, _ptr(rho), _ptr(res)), "refine")
         return rho, res
+
+    def residuals(self, A, rho):
+        """Columns k+i = A v_i - rho[i] v_i for i < k = len(rho) and their
+        squared norms (trlhip_residuals)."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        rr2 = np.zeros(len(rho))
+        self._chk(lib().trlhip_residuals(self.h, A.handle, len(rho), _ptr(rho), _ptr(rr2)), "residuals")
+        return rr2
+
+    def gram(self, A, m, tmp):
+        """(Q'Q, Q'AQ) over columns [0, m), A Q_j formed in column tmp
+        (trlhip_gram)."""
+        G = np.zeros(m * m)
+        H = np.zeros(m * m)
+        self._chk(lib().trlhip_gram(self.h, A.handle, m, tmp, _ptr(G), _ptr(H)), "gram")
+        return G.reshape(m, m, order="F"), H.reshape(m, m, order="F")
 
     def close(self):
         if self.h:

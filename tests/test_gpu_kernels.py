@@ -197,6 +197,47 @@ def test_refine_rayleigh_quotients(gpu_lib, n, k, ncols, avoid):
     c.close()
 
 
+@pytest.mark.parametrize("n,k", [(4096, 8), (20000, 5)])
+def test_polish_residuals_and_gram(gpu_lib, n, k):
+    """The polish's two device passes against numpy: trlhip_residuals puts
+    A v_i - rho_i v_i in columns k+i with its squared norm; trlhip_gram
+    returns Q'Q and Q'AQ over the first m = 2k columns (A Q_j formed in
+    column m), leaving those columns as they were."""
+    T = gpu_lib
+    rng = np.random.default_rng(n + 17 * k)
+    if n == 4096:
+        rp, ci, v = W.anderson2d_csr(64, 7.0, seed=5)[0]
+    else:
+        rp, ci, v = W.random_sym_csr(n, seed=k)
+    import scipy.sparse as sp
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    X = rng.standard_normal((n, k))
+    rho = rng.standard_normal(k)
+    m = 2 * k
+    print(f"polish passes n={n} k={k}", flush=True)
+    c = T.Ctx(n, m + 2)
+    for j in range(k):
+        c.set(j, X[:, j])
+    Ad = T.Csr(rp, ci, v)
+    rr2 = c.residuals(Ad, rho)
+    R = np.empty((n, k))
+    scale = abs(A).max() * 6
+    for i in range(k):
+        ref = A @ X[:, i] - rho[i] * X[:, i]
+        R[:, i] = c.get(k + i)
+        assert np.allclose(R[:, i], ref, rtol=0, atol=1e-13 * scale * (1 + abs(rho[i])))
+        assert abs(rr2[i] - ref @ ref) <= 1e-12 * (ref @ ref), (i, rr2[i], ref @ ref)
+    Q = np.hstack([X, R])
+    G, H = c.gram(Ad, m, m)
+    Gr, Hr = Q.T @ Q, Q.T @ (A @ Q)
+    assert np.allclose(G, Gr, rtol=0, atol=1e-12 * abs(Gr).max())
+    assert np.allclose(H, Hr, rtol=0, atol=1e-12 * abs(Hr).max())
+    for j in range(m):
+        assert np.array_equal(c.get(j), Q[:, j])
+    Ad.close()
+    c.close()
+
+
 @pytest.mark.parametrize("builder", ["lap", "rand", "anderson3d"])
 def test_autotune_spmv(gpu_lib, builder):
     """trlhip_csr_autotune picks one of the SpMV kernels (0 = CSR-stream,
