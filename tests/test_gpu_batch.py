@@ -182,3 +182,43 @@ def test_batch_chained_steps(gpu_lib, monkeypatch):
             assert np.array_equal(g["evec"], s["evec"])
     for A in ops:
         A.close()
+
+
+def test_batch_polish_matches_single_solves(gpu_lib):
+    """Mixed-precision jobs whose refined residual misses tol take the
+    Rayleigh-Ritz polish over [X, residuals] (trlhip_residuals and
+    trlhip_gram, a device round trip each) inside the batch; with a few
+    jobs that do not, every job equals its lone trlan() call bit for bit,
+    and each returned pair's true residual meets tol (numpy)."""
+    import scipy.sparse as sp
+
+    T = gpu_lib
+    specs = [(80, 11.35921550186609, 7718, 9718),  # ground state needs the polish
+             (75, 9.673365812110031, 7380, 9380),  # polish with a singular [X, R]
+             (64, 6.0, 31, 41), (48, 3.0, 32, 42)]
+    jobs = []
+    for L, Wd, seed, sseed in specs:
+        rp, ci, v = W.anderson2d_csr(L, Wd, seed=seed)[0]
+        n = len(rp) - 1
+        A = T.Csr(rp, ci, v, mixed=True)
+        x0 = W.start_vector(n, seed=sseed)
+        jobs.append(((rp, ci, v), A, (A, -1, 8, 32, 1e-8, 1, 40000, 8, x0)))
+    print(f"batch polish jobs {[s[:2] for s in specs]}", flush=True)
+    rc, res, wall = T.solve_batch([j[2] for j in jobs])
+    assert rc == 0
+    for ((rp, ci, v), A, args), g in zip(jobs, res):
+        s = T.solve_csr(*args)
+        assert g["stat"] == 0 and s["stat"] == 0
+        keys = ["nec", "matvec", "nloop", "north", "nrand", "locked"]
+        assert [g[k] for k in keys] == [s[k] for k in keys]
+        assert np.array_equal(g["eval"], s["eval"])
+        assert np.array_equal(g["evec"], s["evec"])
+        n = len(rp) - 1
+        M = sp.csr_matrix((v, ci, rp), shape=(n, n))
+        anrm = abs(g["anrm"])
+        for i in range(g["nec"]):
+            x = g["evec"][:, i]
+            r = np.linalg.norm(M @ x - g["eval"][i] * x) / np.linalg.norm(x)
+            assert r <= 1e-8 * anrm, (i, r / anrm)
+    for _, A, _ in jobs:
+        A.close()
