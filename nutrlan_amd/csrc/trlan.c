@@ -1442,19 +1442,20 @@ static int lz_setup(lz_t *z, trl_matvec op, ztrl_matvec zop, trl_uo user_ortho, 
                               cplx ? TRLHIP_C128 : TRLHIP_R64);
     z->mixed = !cplx && trlhip_csr_is_mixed(z->A);
     const int vt = cplx ? TRLHIP_C128 : (z->mixed ? TRLHIP_R32 : TRLHIP_R64);
-    const long ta0 = g_prof ? lz_ns() : 0;
+    const int prof = g_prof && b; /* batched setup runs on one thread */
+    const long ta0 = prof ? lz_ns() : 0;
     if (lz_alloc(z, DMAX(info->maxlan, mev)) != 0) {
         info->stat = -4;
         lz_free(z);
         return 0;
     }
-    const long ta1 = g_prof ? lz_ns() : 0;
+    const long ta1 = prof ? lz_ns() : 0;
     if (trlhip_ctx_create_batched(&z->dev, nrow, z->ncols, vt, b) != 0) {
         info->stat = TRLHIP_STAT_DEVICE;
         lz_free(z);
         return 0;
     }
-    if (g_prof) {
+    if (prof) {
         g_ns_alloc += ta1 - ta0;
         g_ns_ctx += lz_ns() - ta1;
     }
