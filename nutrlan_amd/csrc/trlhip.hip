@@ -37,7 +37,11 @@ constexpr int kBlock = 256;            // 4 waves of 64
 constexpr int kMaxGrid = 1024;         // partial slots per streaming reduction
 constexpr int kSpmvGrid = 2048;        // SpMV blocks (and alpha partial slots)
 constexpr int kProjGrid = 2048;        // row blocks (partial rows) of the projection
-constexpr int kProjBlocks = 4096;      // projection blocks wanted over all column chunks
+// projection blocks wanted over all column chunks, by the bytes of a basis
+// element read (FP32 shadow, FP64, c128); measured on the 128^3 Laplacian
+// (n = 2^21, m up to 64): FP32 reads 8192 blocks 25.30, 4096 25.15, 2048
+// 24.99 eigenpairs/s; FP64 reads 2048 17.56, 4096 16.96, 8192 17.16
+constexpr int kProjBlocks32 = 8192, kProjBlocks64 = 2048, kProjBlocksC128 = 4096;
 constexpr int kNnzBlk = 2048;          // nonzeros staged per SpMV row block
 constexpr int kRowsBlk = kBlock;       // rows per SpMV row block (at most)
 // Smallest share of work per block: rows per update or projection block
@@ -2023,17 +2027,19 @@ void launch_proj(trlhip_ctx *c, int gx, const VS *V, int c0, int m, int rcol)
 // change from another thread never splits one launch between two widths.
 std::atomic<int> g_proj_cb[2] = {8, 8};
 
-// Row blocks of the projection: about kProjBlocks blocks over all column
-// chunks, at most kProjGrid (the partial rows), a multiple of 8 (the XCD
-// count) when there are at least 8.
-int proj_gx_rows(int n, int m, int cb)
+// Row blocks of the projection: about kProjBlocks32/64/C128 blocks (by vsz,
+// the bytes of a basis element read: 4, 8 or 16) over all column chunks, at
+// most kProjGrid (the partial rows), a multiple of 8 (the XCD count) when
+// there are at least 8.
+int proj_gx_rows(int n, int m, int cb, int vsz)
 {
+    const int blocks = vsz == 4 ? kProjBlocks32 : vsz == 8 ? kProjBlocks64 : kProjBlocksC128;
     const int chunks = (m + cb - 1) / cb;
-    const int want = (kProjBlocks + chunks - 1) / chunks;
+    const int want = (blocks + chunks - 1) / chunks;
     const int g = grid_for(n, kMinRowsUpd, want < kProjGrid ? want : kProjGrid);
     return g >= 8 ? g & ~7 : g;
 }
-int proj_gx(const trlhip_ctx *c, int m, int cb) { return proj_gx_rows(c->n, m, cb); }
+int proj_gx(const trlhip_ctx *c, int m, int cb, int vsz) { return proj_gx_rows(c->n, m, cb, vsz); }
 
 // hout = V(:, c0:c0+m)^H r (device, E*m doubles).  shadow: a Gram-Schmidt
 // projection, which reads the FP32 shadow of V in a mixed-precision context.
@@ -2051,7 +2057,7 @@ int run_proj(trlhip_ctx *c, int c0, int m, int rcol, double *hout, bool shadow =
         // (about 5.7 TB/s for both element types)
         if (s32 || c->E == 1) {
             const int cb = cb_force ? cb_force : g_proj_cb[s32 ? 1 : 0].load(std::memory_order_relaxed);
-            g = proj_gx(c, m, cb);
+            g = proj_gx(c, m, cb, s32 ? 4 : 8);
 #define PROJ_CB(CB)                                                                      \
     case CB:                                                                             \
         if (s32)                                                                         \
@@ -2066,7 +2072,7 @@ int run_proj(trlhip_ctx *c, int c0, int m, int rcol, double *hout, bool shadow =
             }
 #undef PROJ_CB
         } else {
-            g = proj_gx(c, m, 4);
+            g = proj_gx(c, m, 4, 16);
             launch_proj<double, 2, 4, 4>(c, g, colp<double>(c, 0), c0, m, rcol);
         }
         if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
@@ -2449,7 +2455,7 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
         c->np_alpha = d.np_alpha;
         d.g_upd = grid_for(n, kMinRowsUpd, kMaxGrid);
         const int cb = gi.cb;
-        d.gx = gi.fused ? d.g_upd : proj_gx(c, jnd, cb);  // fused: one block per row block
+        d.gx = gi.fused ? d.g_upd : proj_gx(c, jnd, cb, gi.mixed ? 4 : 8);  // fused: one block per row block
         d.gy = gi.fused ? 1 : (jnd + cb - 1) / cb;
         if (st.first) {
             d.rm = jnd;
@@ -4558,7 +4564,7 @@ int noalloc_launch(int n, int m, const VS *V, long ld, double *r, double *scr, h
     // n rows, so the two agree bit for bit
     const bool s32 = std::is_same<VS, float>::value;
     const int cb = E == 2 ? 4 : g_proj_cb[s32 ? 1 : 0].load(std::memory_order_relaxed);
-    const int gx = proj_gx_rows(n, m, cb);
+    const int gx = proj_gx_rows(n, m, cb, s32 ? 4 : E == 2 ? 16 : 8);
     const int gy = (m + cb - 1) / cb;
 #define NA_PROJ(CB)                                                                              \
     case CB:                                                                                     \
