@@ -1133,13 +1133,13 @@ __global__ __launch_bounds__(kBlock) void k_update(
 // as proj_body does for one row block (part_proj[p * gx + rb], gx the row
 // blocks): r is neither written nor read twice, and one kernel replaces
 // two.
-template <int CB>
+template <int CB, typename VS>
 __device__ __forceinline__ void recproj_body(int n, const double *__restrict__ V, long ld, int c0,
                                              int rm, const double *__restrict__ h, double h0,
                                              const double *__restrict__ cpart, int cnp,
                                              double *__restrict__ r, double *__restrict__ part_recur,
                                              double *__restrict__ out_coef,
-                                             const float *__restrict__ VG, int m,
+                                             const VS *__restrict__ VG, int m,
                                              double *__restrict__ part_proj, int gx, int rb)
 {
     extern __shared__ double hs[];
@@ -1171,7 +1171,7 @@ __device__ __forceinline__ void recproj_body(int n, const double *__restrict__ V
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int cb = 0; cb < m; cb += CB) {
         const int last = min(CB, m - cb) - 1;
-        const float *col[CB];
+        const VS *col[CB];
 #pragma unroll
         for (int c = 0; c < CB; ++c) col[c] = VG + (size_t)(cb + min(c, last)) * ld;
         double pacc[CB];
@@ -1204,18 +1204,18 @@ __device__ __forceinline__ void recproj_body(int n, const double *__restrict__ V
     }
 }
 
-template <int CB>
+template <int CB, typename VS>
 __global__ __launch_bounds__(kBlock) void k_recproj(int n, const double *__restrict__ V, long ld,
                                                     int c0, int rm, const double *__restrict__ h,
                                                     double h0, const double *__restrict__ cpart,
                                                     int cnp, double *__restrict__ r,
                                                     double *__restrict__ part_recur,
                                                     double *__restrict__ out_coef,
-                                                    const float *__restrict__ VG, int m,
+                                                    const VS *__restrict__ VG, int m,
                                                     double *__restrict__ part_proj)
 {
-    recproj_body<CB>(n, V, ld, c0, rm, h, h0, cpart, cnp, r, part_recur, out_coef, VG, m, part_proj,
-                     gridDim.x, blockIdx.x);
+    recproj_body<CB, VS>(n, V, ld, c0, rm, h, h0, cpart, cnp, r, part_recur, out_coef, VG, m,
+                         part_proj, gridDim.x, blockIdx.x);
 }
 
 // dst = s * src over ne storage elements (dst may equal src); with dst32
@@ -1789,12 +1789,12 @@ int spmv_grid(const trlhip_csr *A)
     return g < kSpmvGrid ? g : kSpmvGrid;
 }
 
-// The fused recurrence + first projection (recproj_body) applies to real
-// contexts with an FP32 shadow basis whose rows fit kMaxGrid row blocks of
-// kMinRowsUpd (one recurrence block per row block).
+// The fused recurrence + first projection (recproj_body, over the FP32
+// shadow or the FP64 basis) applies to real contexts whose rows fit
+// kMaxGrid row blocks of kMinRowsUpd (one recurrence block per row block).
 bool recproj_ok(const trlhip_ctx *c)
 {
-    return c->V32 && c->E == 1 && (long)c->n <= (long)kMaxGrid * kMinRowsUpd;
+    return c->E == 1 && (long)c->n <= (long)kMaxGrid * kMinRowsUpd;
 }
 
 // Workgroups of the stencil SpMV: kStRB row blocks each (one alpha partial
@@ -2324,7 +2324,7 @@ __global__ __launch_bounds__(kBlock) void kb_proj(const BDesc *__restrict__ d, c
                                     q.gy, bid_, nblk_);
 }
 
-template <int CB>
+template <int CB, typename VS>
 __global__ __launch_bounds__(kBlock) void kb_recproj(const BDesc *__restrict__ d, const int *pre,
                                                     int nreq)
 {
@@ -2340,9 +2340,9 @@ __global__ __launch_bounds__(kBlock) void kb_recproj(const BDesc *__restrict__ d
         if (a.x == 0.0) return;  // a follower that does not run
         h0 = b.x;
     }
-    recproj_body<CB>(q.n, q.V, q.ld, q.rc0, q.rm, q.rh, h0, q.part_alpha,
-                     q.np_alpha, q.r, q.part_recur, q.dscal + S_ALPHA, (const float *)q.VG, q.jnd,
-                     q.part_proj, q.gx, bid_);
+    recproj_body<CB, VS>(q.n, q.V, q.ld, q.rc0, q.rm, q.rh, h0, q.part_alpha,
+                         q.np_alpha, q.r, q.part_recur, q.dscal + S_ALPHA, (const VS *)q.VG, q.jnd,
+                         q.part_proj, q.gx, bid_);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_colsum(const BDesc *__restrict__ d, const int *pre, int nreq)
@@ -2561,9 +2561,12 @@ int batch_run(hipStream_t s, const GroupInfo &gi, const BDesc *dd, const int *dp
         Timed t(TRLHIP_K_BPROJ, s, gi.bytes[BP_PROJ]);
 #define KB_PROJ_CB(CB)                                                                        \
     case CB:                                                                                  \
-        if (gi.fused)                                                                         \
-            KLAUNCH((kb_recproj<CB>), blocks(BP_PROJ), dim3(kBlock), sizeof(double) * gi.maxrm, \
-                    s, dd, dp, R);                                                            \
+        if (gi.fused && gi.mixed)                                                             \
+            KLAUNCH((kb_recproj<CB, float>), blocks(BP_PROJ), dim3(kBlock),                    \
+                    sizeof(double) * gi.maxrm, s, dd, dp, R);                                 \
+        else if (gi.fused)                                                                    \
+            KLAUNCH((kb_recproj<CB, double>), blocks(BP_PROJ), dim3(kBlock),                   \
+                    sizeof(double) * gi.maxrm, s, dd, dp, R);                                 \
         else if (gi.mixed)                                                                    \
             KLAUNCH((kb_proj<float, CB>), blocks(BP_PROJ), dim3(kBlock), 0, s, dd, dp, R);    \
         else                                                                                  \
@@ -4039,14 +4042,20 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
         if (rc) return rc;
         const int gx = grid_for(c->n, kMinRowsUpd, kMaxGrid);
         {
-            Timed t(TRLHIP_K_PROJ, c->s, 8.0 * c->n * (ra.m + 2) + 4.0 * jnd * c->n);
-            const int cb = g_proj_cb[1].load(std::memory_order_relaxed);
+            const bool s32 = c->V32 != nullptr;  // mixed: the FP32 shadow, else the FP64 basis
+            Timed t(TRLHIP_K_PROJ, c->s, 8.0 * c->n * (ra.m + 2) + (s32 ? 4.0 : 8.0) * jnd * c->n);
+            const int cb = g_proj_cb[s32 ? 1 : 0].load(std::memory_order_relaxed);
+#define RECPROJ_LAUNCH(CB, VS, VG)                                                              \
+    KLAUNCH((k_recproj<CB, VS>), dim3(gx), dim3(kBlock), sizeof(double) * ra.m, c->s, c->n,      \
+            colp<double>(c, 0), (long)c->ld, ra.c0, ra.m, ra.coef, ra.h0, part_alpha(c),          \
+            c->np_alpha, colp<double>(c, rcol), part_recur(c), c->dscal + S_ALPHA, VG, jnd,       \
+            part_proj(c))
 #define RECPROJ_CB(CB)                                                                          \
     case CB:                                                                                    \
-        KLAUNCH((k_recproj<CB>), dim3(gx), dim3(kBlock), sizeof(double) * ra.m, c->s, c->n,      \
-                colp<double>(c, 0), (long)c->ld, ra.c0, ra.m, ra.coef, ra.h0, part_alpha(c),      \
-                c->np_alpha, colp<double>(c, rcol), part_recur(c), c->dscal + S_ALPHA, c->V32, jnd, \
-                part_proj(c));                                                                  \
+        if (s32)                                                                                \
+            RECPROJ_LAUNCH(CB, float, c->V32);                                                  \
+        else                                                                                    \
+            RECPROJ_LAUNCH(CB, double, colp<double>(c, 0));                                     \
         break;
             switch (cb) {
                 RECPROJ_CB(4)
@@ -4054,6 +4063,7 @@ int trlhip_step_cgs(trlhip_ctx *c, const trlhip_csr *A, int first, int qa, int q
                 default: RECPROJ_CB(8)
             }
 #undef RECPROJ_CB
+#undef RECPROJ_LAUNCH
             if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
         }
         c->np_recur = gx;
