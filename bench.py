@@ -56,9 +56,10 @@ def kernel_names(T, path, A=None):
         return {"proj": "k_proj<double, double, 2, 4, 4>", "update": "k_update<double, double, 2, 4>",
                 "spmv": "k_spmv<double, double, 2, true>"}
     vs, cb = ("float", cb32) if path == "mixed" else ("double", cb64)
-    # mixed precision up to 2^20 rows: the recurrence runs fused with the
-    # first projection (k_recproj), timed as the projection
-    fused = path == "mixed" and A is not None and A.n <= 1 << 20
+    # real operators up to 2^20 rows: the recurrence runs fused with the
+    # first projection (k_recproj, over the FP32 shadow or the FP64 basis),
+    # timed as the projection
+    fused = A is not None and A.n <= 1 << 20
     spmv = "k_spmv<double, double, 1, true>"
     if A is not None and A.paneled:
         spmv = "k_spmv_panel<true>+k_spmv_panel_rows<true>"
@@ -68,7 +69,7 @@ def kernel_names(T, path, A=None):
         spmv = f"k_spmv_st_rb<{D}, 4>" if L ** D <= 2048 * 256 else f"k_spmv_st<{D}, true>"
     elif A is not None and A.packed:
         spmv = "k_spmv_pk<true>"
-    return {"proj": f"k_recproj<{cb}>" if fused else f"k_proj<{vs}, double, 1, {cb}, 4>",
+    return {"proj": f"k_recproj<{cb}, {vs}>" if fused else f"k_proj<{vs}, double, 1, {cb}, 4>",
             "update": f"k_update<{vs}, double, 1, 4>", "spmv": spmv}
 
 
@@ -460,7 +461,7 @@ def batched_anderson2d(args):
     path_gbs = pb / (pt * 1e-3) / 1e9 if pt > 0 else 0.0
     kname = {"b_spmv": "kb_spmv_st<2>" if stencil else "kb_spmv_pk" if packed else "kb_spmv_stream",
              "b_recur": "kb_recur",
-             "b_proj": f"kb_recproj<{T.proj_tiles_get()[1]}>",
+             "b_proj": f"kb_recproj<{T.proj_tiles_get()[1]}, float>",
              "b_update": "kb_update<float>"}[dom]
     roofline = {
         "bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
