@@ -152,6 +152,10 @@ int trlhip_ctx_ld(const trlhip_ctx *c);
  * vectors between evec/base (SRC/trlcore.c:1395-1403). */
 int trlhip_col_set(trlhip_ctx *c, int col, const void *x);
 int trlhip_col_get(trlhip_ctx *c, int col, void *x);
+/* Columns c0 .. c0+k-1 to x (column i at x + i*ldx elements, ldx >= n) in
+ * one device-to-host transfer: the returned Ritz vectors (trlcore.c:
+ * 1395-1403 copies them one by one). */
+int trlhip_cols_get(trlhip_ctx *c, int c0, int k, void *x, long ldx);
 
 /* Column col = the FP32 device vector dsrc (n entries) converted to FP64,
  * without a host round trip (real bases).  dsrc must stay valid until the

@@ -1308,12 +1308,12 @@ static void lz_iterate(lz_t *z, double *eval, double *evec, int lde, double *wrk
             if (z->mixed && lz_refine(z, kept, rr, eval, beta) != 0) return;
             jnd = kept;
             j1 = kept;
-            /* Ritz vectors (and the last residual after them) to the caller */
-            for (int i = 0; i < kept && i < mev; ++i)
-                if (trlhip_col_get(z->dev, i, evec + (size_t)i * lde * z->E)) {
-                    info->stat = TRLHIP_STAT_DEVICE;
-                    return;
-                }
+            /* Ritz vectors (and the last residual after them) to the caller,
+             * the vectors in one transfer */
+            if (DMIN(kept, mev) > 0 && trlhip_cols_get(z->dev, 0, DMIN(kept, mev), evec, lde)) {
+                info->stat = TRLHIP_STAT_DEVICE;
+                return;
+            }
             if (kept < mev && trlhip_col_get(z->dev, rr, evec + (size_t)kept * lde * z->E)) {
                 info->stat = TRLHIP_STAT_DEVICE;
                 return;

@@ -1667,6 +1667,41 @@ S *colp(trlhip_ctx *c, int col)
 std::mutex g_chunk_mu;
 std::vector<trlhip_batch::Chunk> g_free_chunks[2];  // device, host
 
+// Pinned staging buffers for batched multi-column transfers to the host
+// (trlhip_cols_get): taken for one transfer and given back, sizes powers of
+// two from 1 MiB, kept until trlhip_batch_trim() or the end of the process
+// (no zeroing: a transfer overwrites what it reads back).
+struct StageBuf {
+    char *p;
+    size_t size;
+};
+std::mutex g_stage_mu;
+std::vector<StageBuf> g_stage_free;
+
+StageBuf stage_take(size_t bytes)
+{
+    size_t sz = (size_t)1 << 20;
+    while (sz < bytes) sz <<= 1;
+    {
+        std::lock_guard<std::mutex> lk(g_stage_mu);
+        for (size_t i = 0; i < g_stage_free.size(); ++i)
+            if (g_stage_free[i].size == sz) {
+                const StageBuf b = g_stage_free[i];
+                g_stage_free.erase(g_stage_free.begin() + (long)i);
+                return b;
+            }
+    }
+    StageBuf b{nullptr, sz};
+    if (hipHostMalloc((void **)&b.p, sz, 0) != hipSuccess) b.p = nullptr;
+    return b;
+}
+
+void stage_give(const StageBuf &b)
+{
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    g_stage_free.push_back(b);
+}
+
 void *arena_get(trlhip_batch *b, bool host, size_t bytes, void **dptr = nullptr)
 {
     bytes = (bytes + 255) & ~(size_t)255;
@@ -3778,6 +3813,34 @@ int trlhip_col_get(trlhip_ctx *c, int col, void *x)
     return ctx_wait(c);
 }
 
+int trlhip_cols_get(trlhip_ctx *c, int c0, int k, void *x, long ldx)
+{
+    if (!c || !x || k < 1 || !valid_cols(c, c0, k) || ldx < c->n) return TRLHIP_EINVAL;
+    if (int rc = settle(c)) return rc;
+    const size_t colb = c->esz * (size_t)c->ld;  // a device column with its padding
+    const size_t bytes = colb * (size_t)(k - 1) + c->esz * (size_t)c->n;
+    const char *src = c->V + (size_t)c0 * colb;
+    if (!c->batch) {
+        TRY(hipMemcpy2DAsync(x, c->esz * (size_t)ldx, src, colb, c->esz * (size_t)c->n, k,
+                             hipMemcpyDeviceToHost, c->s));
+        return ctx_wait(c);
+    }
+    // batched: one transfer into pinned staging (stage_take), a single
+    // yield, then the columns out to x
+    const StageBuf st = stage_take(bytes);
+    if (!st.p) return TRLHIP_ENOMEM;
+    int rc = 0;
+    if (hipMemcpyAsync(st.p, src, bytes, hipMemcpyDeviceToHost, c->s) != hipSuccess)
+        rc = TRLHIP_EHIP;
+    if (!rc) rc = ctx_wait(c);
+    if (!rc)
+        for (int i = 0; i < k; ++i)
+            memcpy((char *)x + (size_t)i * c->esz * (size_t)ldx, st.p + (size_t)i * colb,
+                   c->esz * (size_t)c->n);
+    stage_give(st);
+    return rc;
+}
+
 int trlhip_col_set_device_f32(trlhip_ctx *c, int col, const float *dsrc)
 {
     if (!c || !dsrc || !valid_col(c, col) || c->E != 1) return TRLHIP_EINVAL;
@@ -4408,11 +4471,16 @@ void trlhip_batch_destroy(trlhip_batch *b)
 
 void trlhip_batch_trim(void)
 {
-    std::lock_guard<std::mutex> lk(g_chunk_mu);
-    for (auto &ch : g_free_chunks[0]) (void)hipFree(ch.p);
-    for (auto &ch : g_free_chunks[1]) (void)hipHostFree(ch.p);
-    g_free_chunks[0].clear();
-    g_free_chunks[1].clear();
+    {
+        std::lock_guard<std::mutex> lk(g_chunk_mu);
+        for (auto &ch : g_free_chunks[0]) (void)hipFree(ch.p);
+        for (auto &ch : g_free_chunks[1]) (void)hipHostFree(ch.p);
+        g_free_chunks[0].clear();
+        g_free_chunks[1].clear();
+    }
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    for (auto &b : g_stage_free) (void)hipHostFree(b.p);
+    g_stage_free.clear();
 }
 
 void trlhip_batch_set_yield(trlhip_batch *b, void (*yield)(void *), void *user)

@@ -480,6 +480,7 @@ class Ctx:
             ("trlhip_step_cgs", [P, P, I, I, I, D, P, I, I, P]),
             ("trlhip_refine", [P, P, I, I, P, P]),
             ("trlhip_residuals", [P, P, I, P, P]),
+            ("trlhip_cols_get", [P, I, I, P, ctypes.c_long]),
             ("trlhip_gram", [P, P, I, I, P, P]),
         ]:
             f = getattr(L, name)
@@ -556,6 +557,14 @@ class Ctx:
         res = np.zeros(k)
         self._chk(lib().trlhip_refine(self.h, A.handle, k, avoid, _ptr(rho), _ptr(res)), "refine")
         return rho, res
+
+    def cols_get(self, c0, k):
+        """Columns c0 .. c0+k-1 in one transfer (trlhip_cols_get), as an
+        (n, k) array."""
+        dt = np.complex128 if self.complex else np.float64
+        x = np.zeros((k, self.n), dtype=dt)
+        self._chk(lib().trlhip_cols_get(self.h, c0, k, _ptr(x), self.n), "cols_get")
+        return x.T
 
     def residuals(self, A, rho):
         """Columns k+i = A v_i - rho[i] v_i for i < k = len(rho) and their

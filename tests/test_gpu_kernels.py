@@ -197,6 +197,26 @@ def test_refine_rayleigh_quotients(gpu_lib, n, k, ncols, avoid):
     c.close()
 
 
+@pytest.mark.parametrize("complex_", [False, True])
+def test_cols_get_matches_col_get(gpu_lib, complex_):
+    """trlhip_cols_get (the returned Ritz vectors in one transfer) returns
+    exactly what trlhip_col_get returns column by column."""
+    T = gpu_lib
+    n, ncols = 1000, 9
+    rng = np.random.default_rng(5)
+    c = T.Ctx(n, ncols, complex_=complex_)
+    cols = []
+    for j in range(ncols):
+        x = rng.standard_normal(n) + (1j * rng.standard_normal(n) if complex_ else 0.0)
+        c.set(j, x)
+        cols.append(x)
+    got = c.cols_get(2, 5)
+    for i in range(5):
+        assert np.array_equal(got[:, i], c.get(2 + i))
+        assert np.array_equal(got[:, i], cols[2 + i])
+    c.close()
+
+
 @pytest.mark.parametrize("n,k", [(4096, 8), (20000, 5)])
 def test_polish_residuals_and_gram(gpu_lib, n, k):
     """The polish's two device passes against numpy: trlhip_residuals puts
