@@ -1122,14 +1122,15 @@ __global__ __launch_bounds__(kBlock) void k_update(
     update_body<VS, RS, E, RU>(n, V, ld, c0, m, h, h0, cpart, cnp, cslot, rin, rout, part, out_hh, out_coef, blockIdx.x, gridDim.x);
 }
 
-// The recurrence and the first Gram-Schmidt projection of a real step on
-// an FP32 shadow basis in one pass (n <= kMaxGrid * kMinRowsUpd rows):
+// The recurrence and the first Gram-Schmidt projection of a real step in
+// one pass, over the FP32 shadow (VS = float) or the FP64 basis (VS =
+// double) (n <= kMaxGrid * kMinRowsUpd rows):
 // block rb owns rows [1024 rb, 1024 rb + 1024), four consecutive rows per
 // thread.  It forms alpha from the SpMV partials and r = y - V(:, c0:c0+rm)
 // h in place exactly as update_body does for the recurrence (the same
 // rows per block and thread, so its ||r||^2 partial is the recurrence's,
 // at part_recur[rb]), keeps its rows of r in registers, and for each chunk
-// of CB columns of the shadow basis forms the projection partial exactly
+// of CB columns of the basis forms the projection partial exactly
 // as proj_body does for one row block (part_proj[p * gx + rb], gx the row
 // blocks): r is neither written nor read twice, and one kernel replaces
 // two.
