@@ -1442,7 +1442,7 @@ static int lz_setup(lz_t *z, trl_matvec op, ztrl_matvec zop, trl_uo user_ortho, 
                               cplx ? TRLHIP_C128 : TRLHIP_R64);
     z->mixed = !cplx && trlhip_csr_is_mixed(z->A);
     const int vt = cplx ? TRLHIP_C128 : (z->mixed ? TRLHIP_R32 : TRLHIP_R64);
-    const int prof = g_prof && b; /* batched setup runs on one thread */
+    const int prof = g_prof && b; /* batched setups (several threads, atomic sums) */
     const long ta0 = prof ? lz_ns() : 0;
     if (lz_alloc(z, DMAX(info->maxlan, mev)) != 0) {
         info->stat = -4;
@@ -1456,8 +1456,8 @@ static int lz_setup(lz_t *z, trl_matvec op, ztrl_matvec zop, trl_uo user_ortho, 
         return 0;
     }
     if (prof) {
-        g_ns_alloc += ta1 - ta0;
-        g_ns_ctx += lz_ns() - ta1;
+        __atomic_add_fetch(&g_ns_alloc, ta1 - ta0, __ATOMIC_RELAXED);
+        __atomic_add_fetch(&g_ns_ctx, lz_ns() - ta1, __ATOMIC_RELAXED);
     }
     /* batched real solves chain up to 8 inner steps per round (measured
      * on the batched 2D Anderson workload: 4 steps 8,344, 6: 8,560,
@@ -1736,6 +1736,56 @@ static int lz_batch_lanes(int njobs)
 
 /* Host threads for the coroutines: TRLHIP_BATCH_THREADS, else up to 16
  * (and no more than the online CPUs or the jobs). */
+/* A coroutine's context on its stack, starting at lz_co_main (the context
+ * getcontext saves is only a template: nothing returns to it). */
+static __attribute__((noinline)) int lz_co_make(lz_co_t *co)
+{
+    if (getcontext(&co->uc) != 0) return -1;
+    co->uc.uc_stack.ss_sp = co->stack;
+    co->uc.uc_stack.ss_size = LZ_STACK;
+    makecontext(&co->uc, lz_co_main, 0);
+    return 0;
+}
+
+/* One thread's share of a batch's job setup: jobs t, t + nt, ... */
+typedef struct {
+    trl_batch_job *jobs;
+    lz_co_t *cos;
+    trlhip_batch **bl;
+    int njobs, nlanes, t, nt;
+    double t_su, t_st; /* profile: job setup, coroutine stacks */
+} lz_setup_share_t;
+
+static void *lz_setup_share(void *arg)
+{
+    lz_setup_share_t *a = (lz_setup_share_t *)arg;
+    for (int i = a->t; i < a->njobs; i += a->nt) {
+        trl_batch_job *j = &a->jobs[i];
+        lz_co_t *co = &a->cos[i];
+        co->job = j;
+        co->done = 1;
+        co->ended = 1;
+        co->lane = i % a->nlanes;
+        j->info->mvparam = j->A;
+        const double ts0 = lz_now();
+        const int ok = lz_setup(&co->z, trlhip_csr_matvec, NULL, NULL, j->info, j->nrow, j->mev,
+                                j->lde, 0, a->bl[co->lane]);
+        a->t_su += lz_now() - ts0;
+        if (!ok) continue;
+        co->z.dstart = j->dstart;
+        co->ended = 0;
+        const double ts1 = lz_now();
+        co->stack = malloc(LZ_STACK);
+        a->t_st += lz_now() - ts1;
+        if (!co->stack || lz_co_make(co) != 0) {
+            j->info->stat = -4;
+            continue;
+        }
+        co->done = 0;
+    }
+    return NULL;
+}
+
 static int lz_batch_threads(int njobs)
 {
     const char *e = getenv("TRLHIP_BATCH_THREADS");
@@ -1784,33 +1834,29 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     int live[4] = {0, 0, 0, 0};
     double t_su = 0.0, t_st = 0.0;  /* profile: job setup, coroutine stacks */
     const double t_jobs = lz_now();
-    for (int i = 0; i < njobs; ++i) {
-        trl_batch_job *j = &jobs[i];
-        lz_co_t *co = &cos[i];
-        co->job = j;
-        co->done = 1;
-        co->ended = 1;
-        co->lane = i % nlanes;
-        j->info->mvparam = j->A;
-        const double ts0 = lz_now();
-        const int ok = lz_setup(&co->z, trlhip_csr_matvec, NULL, NULL, j->info, j->nrow, j->mev,
-                                j->lde, 0, bl[co->lane]);
-        t_su += lz_now() - ts0;
-        if (!ok) continue;
-        co->z.dstart = j->dstart;
-        co->ended = 0;
-        const double ts1 = lz_now();
-        co->stack = malloc(LZ_STACK);
-        t_st += lz_now() - ts1;
-        if (!co->stack || getcontext(&co->uc) != 0) {
-            j->info->stat = -4;
-            continue;
+    {
+        /* the jobs' setups (host arrays, device contexts from the lanes'
+         * arenas, coroutine stacks) on up to as many threads as the
+         * scheduler's; a share that cannot start a thread runs here */
+        enum { LZ_SETUP_MAX = 64 };
+        const int nt = DMIN(lz_batch_threads(njobs), (int)LZ_SETUP_MAX);
+        lz_setup_share_t sh[LZ_SETUP_MAX];
+        pthread_t sth[LZ_SETUP_MAX];
+        int started[LZ_SETUP_MAX];
+        for (int t = 0; t < nt; ++t) {
+            sh[t] = (lz_setup_share_t){jobs, cos, bl, njobs, nlanes, t, nt, 0.0, 0.0};
+            started[t] = t > 0 && pthread_create(&sth[t], NULL, lz_setup_share, &sh[t]) == 0;
         }
-        co->uc.uc_stack.ss_sp = co->stack;
-        co->uc.uc_stack.ss_size = LZ_STACK;
-        makecontext(&co->uc, lz_co_main, 0);
-        co->done = 0;
-        ++live[co->lane];
+        for (int t = 0; t < nt; ++t)
+            if (!started[t]) lz_setup_share(&sh[t]);
+        for (int t = 1; t < nt; ++t)
+            if (started[t]) pthread_join(sth[t], NULL);
+        for (int t = 0; t < nt; ++t) {
+            t_su += sh[t].t_su;
+            t_st += sh[t].t_st;
+        }
+        for (int i = 0; i < njobs; ++i)
+            if (!cos[i].done) ++live[cos[i].lane];
     }
     const double t_pool = lz_now();
     lz_pool_t pool;

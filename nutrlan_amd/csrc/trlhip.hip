@@ -1615,6 +1615,7 @@ struct trlhip_batch {
     void *user = nullptr;
     int err = 0;                  // sticky device error of a flush
     SpinLock mu;                  // guards the records (coroutines run on several host threads)
+    std::mutex arena_mu;          // guards the arenas (contexts are created on several threads)
     std::vector<BatchStep> steps;
     std::vector<double> cbuf;     // restart-step coefficients of the recorded steps
     std::vector<BatchOp> ops;     // recorded restart GEMMs and scale-copies
@@ -3723,6 +3724,7 @@ int trlhip_ctx_create_batched(trlhip_ctx **out, int n, int ncols, int vtype, trl
     c->hcap = 16 + 2 * ncoef;
     if (b) {
         // batched: everything from the batch's arenas, nothing to synchronize
+        std::lock_guard<std::mutex> lk(b->arena_mu);
         c->batch = b;
         c->s = b->side;
         c->arena = 1;
