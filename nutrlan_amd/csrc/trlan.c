@@ -1786,6 +1786,45 @@ static void *lz_setup_share(void *arg)
     return NULL;
 }
 
+/* One thread's share of a batch's teardown: jobs t, t + nt, ... */
+static void *lz_teardown_share(void *arg)
+{
+    lz_setup_share_t *a = (lz_setup_share_t *)arg;
+    for (int i = a->t; i < a->njobs; i += a->nt) {
+        lz_co_t *co = &a->cos[i];
+        if (!co->ended) lz_teardown(&co->z);
+        free(co->stack);
+    }
+    return NULL;
+}
+
+/* Runs fn over jobs 0..njobs-1 on nt threads (shares t, t + nt, ...); a
+ * share whose thread cannot start runs on the calling thread.  Returns the
+ * summed profile times in *t_su, *t_st (may be NULL). */
+static void lz_run_shares(void *(*fn)(void *), trl_batch_job *jobs, lz_co_t *cos,
+                          trlhip_batch **bl, int njobs, int nlanes, int nt, double *t_su,
+                          double *t_st)
+{
+    enum { LZ_SHARES_MAX = 64 };
+    if (nt > LZ_SHARES_MAX) nt = LZ_SHARES_MAX;
+    if (nt < 1) nt = 1;
+    lz_setup_share_t sh[LZ_SHARES_MAX];
+    pthread_t sth[LZ_SHARES_MAX];
+    int started[LZ_SHARES_MAX];
+    for (int t = 0; t < nt; ++t) {
+        sh[t] = (lz_setup_share_t){jobs, cos, bl, njobs, nlanes, t, nt, 0.0, 0.0};
+        started[t] = t > 0 && pthread_create(&sth[t], NULL, fn, &sh[t]) == 0;
+    }
+    for (int t = 0; t < nt; ++t)
+        if (!started[t]) fn(&sh[t]);
+    for (int t = 1; t < nt; ++t)
+        if (started[t]) pthread_join(sth[t], NULL);
+    for (int t = 0; t < nt; ++t) {
+        if (t_su) *t_su += sh[t].t_su;
+        if (t_st) *t_st += sh[t].t_st;
+    }
+}
+
 static int lz_batch_threads(int njobs)
 {
     const char *e = getenv("TRLHIP_BATCH_THREADS");
@@ -1834,30 +1873,12 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
     int live[4] = {0, 0, 0, 0};
     double t_su = 0.0, t_st = 0.0;  /* profile: job setup, coroutine stacks */
     const double t_jobs = lz_now();
-    {
-        /* the jobs' setups (host arrays, device contexts from the lanes'
-         * arenas, coroutine stacks) on up to as many threads as the
-         * scheduler's; a share that cannot start a thread runs here */
-        enum { LZ_SETUP_MAX = 64 };
-        const int nt = DMIN(lz_batch_threads(njobs), (int)LZ_SETUP_MAX);
-        lz_setup_share_t sh[LZ_SETUP_MAX];
-        pthread_t sth[LZ_SETUP_MAX];
-        int started[LZ_SETUP_MAX];
-        for (int t = 0; t < nt; ++t) {
-            sh[t] = (lz_setup_share_t){jobs, cos, bl, njobs, nlanes, t, nt, 0.0, 0.0};
-            started[t] = t > 0 && pthread_create(&sth[t], NULL, lz_setup_share, &sh[t]) == 0;
-        }
-        for (int t = 0; t < nt; ++t)
-            if (!started[t]) lz_setup_share(&sh[t]);
-        for (int t = 1; t < nt; ++t)
-            if (started[t]) pthread_join(sth[t], NULL);
-        for (int t = 0; t < nt; ++t) {
-            t_su += sh[t].t_su;
-            t_st += sh[t].t_st;
-        }
-        for (int i = 0; i < njobs; ++i)
-            if (!cos[i].done) ++live[cos[i].lane];
-    }
+    /* the jobs' setups (host arrays, device contexts from the lanes'
+     * arenas, coroutine stacks) on as many threads as the scheduler's */
+    lz_run_shares(lz_setup_share, jobs, cos, bl, njobs, nlanes, lz_batch_threads(njobs), &t_su,
+                  &t_st);
+    for (int i = 0; i < njobs; ++i)
+        if (!cos[i].done) ++live[cos[i].lane];
     const double t_pool = lz_now();
     lz_pool_t pool;
     memset(&pool, 0, sizeof pool);
@@ -2055,12 +2076,10 @@ int trlan_batch(trl_batch_job *jobs, int njobs)
                 res ? 1e6 * bsum / res : 0.0);
     }
     const double t_end_loop = lz_now();
-    /* teardown after the last round: freeing device memory synchronizes */
-    for (int i = 0; i < njobs; ++i) {
-        lz_co_t *co = &cos[i];
-        if (!co->ended) lz_teardown(&co->z);
-        free(co->stack);
-    }
+    /* teardown after the last round (the jobs' host arrays and stacks;
+     * their device memory is the lanes' arenas), on the same threads */
+    lz_run_shares(lz_teardown_share, jobs, cos, bl, njobs, nlanes, lz_batch_threads(njobs), NULL,
+                  NULL);
     free(th);
     free(pool.w);
     free(pool.share);

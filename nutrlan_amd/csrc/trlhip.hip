@@ -1680,6 +1680,16 @@ struct StageBuf {
 std::mutex g_stage_mu;
 std::vector<StageBuf> g_stage_free;
 
+// A batch's record staging (pinned + device, `cap` bytes each) outlives it
+// too: the next batch takes it instead of allocating (a pinned and a
+// device allocation cost milliseconds, and freeing device memory
+// synchronizes the device).  Guarded by g_chunk_mu.
+struct StagePair {
+    char *h, *d;
+    size_t cap;
+};
+std::vector<StagePair> g_stage_pairs;
+
 StageBuf stage_take(size_t bytes)
 {
     size_t sz = (size_t)1 << 20;
@@ -4458,10 +4468,13 @@ void trlhip_batch_destroy(trlhip_batch *b)
     if (!b) return;
     (void)hipStreamSynchronize(b->side);
     (void)hipStreamSynchronize(b->s);
-    if (b->hstg) (void)hipHostFree(b->hstg);
-    if (b->dstg) (void)hipFree(b->dstg);
     {
         std::lock_guard<std::mutex> lk(g_chunk_mu);
+        if (b->hstg && b->dstg) g_stage_pairs.push_back(StagePair{b->hstg, b->dstg, b->scap});
+        else {
+            if (b->hstg) (void)hipHostFree(b->hstg);
+            if (b->dstg) (void)hipFree(b->dstg);
+        }
         for (auto &ch : b->dev) g_free_chunks[0].push_back(ch);
         for (auto &ch : b->host) g_free_chunks[1].push_back(ch);
     }
@@ -4480,6 +4493,11 @@ void trlhip_batch_trim(void)
         for (auto &ch : g_free_chunks[1]) (void)hipHostFree(ch.p);
         g_free_chunks[0].clear();
         g_free_chunks[1].clear();
+        for (auto &sp : g_stage_pairs) {
+            (void)hipHostFree(sp.h);
+            (void)hipFree(sp.d);
+        }
+        g_stage_pairs.clear();
     }
     std::lock_guard<std::mutex> lk(g_stage_mu);
     for (auto &b : g_stage_free) (void)hipHostFree(b.p);
@@ -4522,16 +4540,37 @@ int trlhip_batch_launch(trlhip_batch *b)
         if (!b->steps.empty()) steps_plan(b, sp);
         total = o_bytes + sp.total;
         if (total > b->scap) {
+            // a kept staging pair that fits (the largest), else a new one
+            // (this lane's previous launch has finished: its staging is free)
+            StagePair got{nullptr, nullptr, 0};
+            {
+                std::lock_guard<std::mutex> lk(g_chunk_mu);
+                size_t best = g_stage_pairs.size();
+                for (size_t i = 0; i < g_stage_pairs.size(); ++i)
+                    if (g_stage_pairs[i].cap >= total &&
+                        (best == g_stage_pairs.size() || g_stage_pairs[i].cap > g_stage_pairs[best].cap))
+                        best = i;
+                if (best < g_stage_pairs.size()) {
+                    got = g_stage_pairs[best];
+                    g_stage_pairs.erase(g_stage_pairs.begin() + (long)best);
+                }
+            }
             if (b->hstg) (void)hipHostFree(b->hstg);
             if (b->dstg) (void)hipFree(b->dstg);
             b->hstg = b->dstg = nullptr;
             b->scap = 0;
-            const size_t cap = ((total + total / 2 + 65536) + 255) & ~(size_t)255;
-            if (hipHostMalloc((void **)&b->hstg, cap, 0) != hipSuccess ||
-                hipMalloc((void **)&b->dstg, cap) != hipSuccess)
-                rc = TRLHIP_ENOMEM;
-            else
-                b->scap = cap;
+            if (got.h) {
+                b->hstg = got.h;
+                b->dstg = got.d;
+                b->scap = got.cap;
+            } else {
+                const size_t cap = ((total + total / 2 + 65536) + 255) & ~(size_t)255;
+                if (hipHostMalloc((void **)&b->hstg, cap, 0) != hipSuccess ||
+                    hipMalloc((void **)&b->dstg, cap) != hipSuccess)
+                    rc = TRLHIP_ENOMEM;
+                else
+                    b->scap = cap;
+            }
         }
         if (!rc) {
             if (!b->ops.empty()) ops_fill(op, b->ybuf, b->hstg, b->dstg);
