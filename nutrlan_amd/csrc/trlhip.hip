@@ -2392,10 +2392,26 @@ __global__ __launch_bounds__(kBlock) void kb_recproj(const BDesc *__restrict__ d
                          q.part_proj, q.gx, bid_);
 }
 
+// Four coefficients per block, one per wave: wave w sums coefficient
+// 4 bid + w's gx partials exactly as ordered_sum does with a block (the
+// four 64-lane wave totals of its strided sums, each a butterfly
+// wave_sum, added to 0.0 in wave order), so dh equals k_colsum's bit for
+// bit with a quarter of the blocks.
 __global__ __launch_bounds__(kBlock) void kb_colsum(const BDesc *__restrict__ d, const int *pre, int nreq)
 {
     BATCH_LOCATE(BP_COLSUM)
-    colsum_body(q.part_proj, q.gx, q.dh, bid_, nblk_);
+    (void)nblk_;
+    const int lane = threadIdx.x & 63, c = 4 * bid_ + (threadIdx.x >> 6);
+    if (c >= q.jnd) return;
+    const double *pc = q.part_proj + (size_t)c * q.gx;
+    double t = 0.0;
+#pragma unroll
+    for (int v = 0; v < kBlock / 64; ++v) {
+        // a virtual wave past the partials holds zeros: its total is +0.0
+        const double wv = v * 64 < q.gx ? wave_sum(strided_sum(pc, q.gx, 1, 0, v * 64 + lane, kBlock)) : 0.0;
+        t += wv;
+    }
+    if (lane == 0) q.dh[c] = t;
 }
 
 template <typename VS>
@@ -2558,7 +2574,7 @@ void batch_fill(const BatchStep *steps, int R, BDesc *desc, int *pre, GroupInfo 
         d.dres = c->dres;
         const int kind = spmv_kind_batch(A);
         const int spmv_wg = (kind == -2 || kind == -3) ? stencil_wgs(A) : d.np_alpha;
-        const int nb[BP_COUNT] = {spmv_wg, d.g_upd, d.gx * d.gy, jnd, d.g_upd, 1};
+        const int nb[BP_COUNT] = {spmv_wg, d.g_upd, d.gx * d.gy, (jnd + 3) / 4, d.g_upd, 1};
         for (int ph = 0; ph < BP_COUNT; ++ph) {
             int *pp = pre + (size_t)ph * (R + 1);
             pp[i + 1] = pp[i] + nb[ph];
