@@ -2456,9 +2456,23 @@ __global__ __launch_bounds__(kBlock) void kb_sum2(const BDesc *__restrict__ d, c
     BATCH_LOCATE(BP_SUM2)
     (void)bid_;
     (void)nblk_;
-    const double t0 = ordered_sum(q.part_recur, q.g_upd);
-    const double t1 = ordered_sum(q.part_update, q.g_upd);
+    // wave 0 forms ||r||^2 and wave 1 ||r - V h||^2 from their partials
+    // exactly as ordered_sum does with a block (k_sum2's bits)
+    __shared__ double t1s;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double t = 0.0;
+    if (w < 2) {
+        const double *p = w == 0 ? q.part_recur : q.part_update;
+#pragma unroll
+        for (int v = 0; v < kBlock / 64; ++v) {
+            const double wv = v * 64 < q.g_upd ? wave_sum(strided_sum(p, q.g_upd, 1, 0, v * 64 + lane, kBlock)) : 0.0;
+            t += wv;
+        }
+        if (w == 1 && lane == 0) t1s = t;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        const double t0 = t, t1 = t1s;
         double *sc = q.dscal;
         sc[S_RR2] = t0;
         sc[S_RR2B] = t1;
