@@ -781,9 +781,11 @@ __global__ __launch_bounds__(kBlock) void k_dot(long ne, const S *__restrict__ x
 // mixed-precision basis), RS = storage of r.
 // Four consecutive elements p[0..3] (p 16-B aligned for float, 32-B for
 // double) with 16-byte loads/stores, as doubles.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void load4(const float *p, double v[4])
 {
-    const float4 t = *reinterpret_cast<const float4 *>(p);
+    // the FP32 shadow is streamed once per kernel: a non-temporal load
+    const f32x4 t = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(p));
     v[0] = t.x;
     v[1] = t.y;
     v[2] = t.z;
