@@ -800,6 +800,31 @@ __device__ __forceinline__ void load4(const double *p, double v[4])
     v[2] = b.x;
     v[3] = b.y;
 }
+// Four consecutive basis entries.  The FP32 shadow is always streamed
+// (load4 above); FP64 columns are streamed when NT: the recurrence's few
+// columns (measured: the 128^3 Laplacian's mixed path +6.7 %), but not
+// the plain path's Gram-Schmidt reads of the FP64 basis (its batch 14 %
+// slower streamed).
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <bool NT>
+__device__ __forceinline__ void loadv4(const float *p, double v[4])
+{
+    load4(p, v);
+}
+template <bool NT>
+__device__ __forceinline__ void loadv4(const double *p, double v[4])
+{
+    if constexpr (NT) {
+        const f64x2 a = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(p));
+        const f64x2 b = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(p) + 1);
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = b.x;
+        v[3] = b.y;
+    } else {
+        load4(p, v);
+    }
+}
 __device__ __forceinline__ void store4(double *p, const double v[4])
 {
     reinterpret_cast<double2 *>(p)[0] = double2{v[0], v[1]};
@@ -848,7 +873,7 @@ __device__ __forceinline__ void proj_body(
             double rv[4], vv[CB][4];
             load4(r + row, rv);
 #pragma unroll
-            for (int c = 0; c < CB; ++c) load4(col[c] + row, vv[c]);
+            for (int c = 0; c < CB; ++c) loadv4<false>(col[c] + row, vv[c]);
 #pragma unroll
             for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -947,7 +972,7 @@ __global__ __launch_bounds__(kBlock) void k_proj(
 // r -= V h on the four consecutive rows [row, row+4) of a real basis (row
 // a multiple of 4): one 16-byte load per column (two for FP64).  Each row
 // sums its columns in the order of update_rows.
-template <typename VS, typename RS>
+template <typename VS, typename RS, bool NTV = false>
 __device__ __forceinline__ double update_rows4(int row, const VS *__restrict__ Vc, long ld, int m,
                                                const double *hs, const RS *rin, RS *rout,
                                                double *ro_out = nullptr)
@@ -959,7 +984,7 @@ __device__ __forceinline__ double update_rows4(int row, const VS *__restrict__ V
     for (; c + 8 <= m; c += 8) {
         double v[8][4];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) load4(Vc + (size_t)(c + q) * ld + row, v[q]);
+        for (int q = 0; q < 8; ++q) loadv4<NTV>(Vc + (size_t)(c + q) * ld + row, v[q]);
 #pragma unroll
         for (int q = 0; q < 8; ++q)
 #pragma unroll
@@ -967,7 +992,7 @@ __device__ __forceinline__ double update_rows4(int row, const VS *__restrict__ V
     }
     for (; c < m; ++c) {
         double v[4];
-        load4(Vc + (size_t)c * ld + row, v);
+        loadv4<NTV>(Vc + (size_t)c * ld + row, v);
 #pragma unroll
         for (int j = 0; j < 4; ++j) s[j][0] += v[j] * hs[c];
     }
@@ -1071,7 +1096,7 @@ __device__ __forceinline__ double update_rows(int row, const VS *__restrict__ Vc
 // coefficients are (h0, 0, ..., 0) apart from cslot.  A block takes RU
 // rows of its grid stride per iteration.  rin may equal rout.  VS = basis
 // storage, RS = storage of r.
-template <typename VS, typename RS, int E, int RU>
+template <typename VS, typename RS, int E, int RU, bool NTV = false>
 __device__ __forceinline__ void update_body(
     int n, const VS *__restrict__ V, long ld, int c0, int m, const double *__restrict__ h, double h0,
     const double *__restrict__ cpart, int cnp, int cslot, const RS *rin, RS *rout,
@@ -1094,7 +1119,7 @@ __device__ __forceinline__ void update_body(
         // four consecutive rows per thread (16-byte loads)
         const int stride = nblk * kBlock * 4;
         int row = (bid * kBlock + threadIdx.x) * 4;
-        for (; row + 3 < n; row += stride) acc += update_rows4<VS, RS>(row, Vc, ld, m, hs, rin, rout);
+        for (; row + 3 < n; row += stride) acc += update_rows4<VS, RS, NTV>(row, Vc, ld, m, hs, rin, rout);
         for (; row < n; ++row) acc += update_rows<VS, RS, E, 1>(row, Vc, ld, m, hs, rin, rout);
     } else {
         const int stride = nblk * kBlock * RU;
@@ -1115,13 +1140,13 @@ __device__ __forceinline__ void update_body(
     }
 }
 
-template <typename VS, typename RS, int E, int RU>
+template <typename VS, typename RS, int E, int RU, bool NTV = false>
 __global__ __launch_bounds__(kBlock) void k_update(
     int n, const VS *__restrict__ V, long ld, int c0, int m, const double *__restrict__ h, double h0,
     const double *__restrict__ cpart, int cnp, int cslot, const RS *rin, RS *rout,
     double *__restrict__ part, double *__restrict__ out_hh, double *__restrict__ out_coef)
 {
-    update_body<VS, RS, E, RU>(n, V, ld, c0, m, h, h0, cpart, cnp, cslot, rin, rout, part, out_hh, out_coef, blockIdx.x, gridDim.x);
+    update_body<VS, RS, E, RU, NTV>(n, V, ld, c0, m, h, h0, cpart, cnp, cslot, rin, rout, part, out_hh, out_coef, blockIdx.x, gridDim.x);
 }
 
 // The recurrence and the first Gram-Schmidt projection of a real step in
@@ -1162,7 +1187,7 @@ __device__ __forceinline__ void recproj_body(int n, const double *__restrict__ V
     double acc = 0.0;
     const bool four = row + 3 < n;
     if (four) {
-        acc += update_rows4<double, double>(row, Vc, ld, rm, hs, r, r, rv);
+        acc += update_rows4<double, double, true>(row, Vc, ld, rm, hs, r, r, rv);
     } else {
         for (int j = 0; row + j < n; ++j) {
             acc += update_rows<double, double, 1, 1>(row + j, Vc, ld, rm, hs, r, r);
@@ -1183,7 +1208,7 @@ __device__ __forceinline__ void recproj_body(int n, const double *__restrict__ V
         if (four) {
             double vv[CB][4];
 #pragma unroll
-            for (int c = 0; c < CB; ++c) load4(col[c] + row, vv[c]);
+            for (int c = 0; c < CB; ++c) loadv4<false>(col[c] + row, vv[c]);
 #pragma unroll
             for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -2018,13 +2043,13 @@ int run_dot(trlhip_ctx *c, int xcol, int ycol)
     return hipGetLastError() == hipSuccess ? 0 : TRLHIP_EHIP;
 }
 
-template <typename VS, int E, int RU>
+template <typename VS, int E, int RU, bool NTV = false>
 void launch_update(trlhip_ctx *c, int g, const VS *V, int c0, int m, const double *dh, double h0,
                    int aslot, int rin, int rout, double *o_hh, double *pdst)
 {
     const size_t shm = sizeof(double) * (size_t)c->E * (m > 0 ? m : 1);
     const double *cp = aslot >= 0 ? part_alpha(c) : nullptr;
-    KLAUNCH((k_update<VS, double, E, RU>), dim3(g), dim3(kBlock), shm, c->s, c->n, V, (long)c->ld,
+    KLAUNCH((k_update<VS, double, E, RU, NTV>), dim3(g), dim3(kBlock), shm, c->s, c->n, V, (long)c->ld,
             c0, m, dh, h0, cp, c->np_alpha, aslot, colp<double>(c, rin), colp<double>(c, rout),
             pdst, o_hh, c->dscal + S_ALPHA);
 }
@@ -2048,9 +2073,12 @@ int run_update(trlhip_ctx *c, int c0, int m, const double *dh, int aslot, int ri
         Timed t(kind, c->s, s32 ? (4.0 * m + 16.0) * c->n : (double)c->esz * c->n * (m + 2));
         if (s32)
             launch_update<float, 1, 4>(c, g, c->V32, c0, m, dh, h0, aslot, rin, rout, o_hh, pdst);
-        else if (c->E == 1)
+        else if (c->E == 1 && shadow)  // the plain path's Gram-Schmidt update
             launch_update<double, 1, 4>(c, g, colp<double>(c, 0), c0, m, dh, h0, aslot, rin, rout,
                                         o_hh, pdst);
+        else if (c->E == 1)  // the recurrence (and other short updates): streamed columns
+            launch_update<double, 1, 4, true>(c, g, colp<double>(c, 0), c0, m, dh, h0, aslot, rin,
+                                              rout, o_hh, pdst);
         else
             launch_update<double, 2, 4>(c, g, colp<double>(c, 0), c0, m, dh, h0, aslot, rin, rout,
                                         o_hh, pdst);
@@ -2360,7 +2388,7 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(const BDesc *__restrict__ d, c
 __global__ __launch_bounds__(kBlock) void kb_recur(const BDesc *__restrict__ d, const int *pre, int nreq)
 {
     BATCH_LOCATE(BP_RECUR)
-    update_body<double, double, 1, 4>(q.n, q.V, q.ld, q.rc0, q.rm, q.rh, q.drh0 ? *q.drh0 : q.rh0,
+    update_body<double, double, 1, 4, true>(q.n, q.V, q.ld, q.rc0, q.rm, q.rh, q.drh0 ? *q.drh0 : q.rh0,
                                       q.part_alpha, q.np_alpha, q.rm - 1, q.r, q.r, q.part_recur,
                                       nullptr, q.dscal + S_ALPHA, bid_, nblk_);
 }
