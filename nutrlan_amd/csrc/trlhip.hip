@@ -377,21 +377,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv_pk(
 // CSR-stream kernel, so y[row] has the CSR kernel's bits.  Returns y[row]
 // and sets xr = the (scaled) x[row] for the alpha product; FUSE stores the
 // scaled x to xq / xq32.
-template <int D, bool FUSE>
-__device__ __forceinline__ double stencil_row(int row, int L, double off,
-                                              const double *__restrict__ dg,
-                                              const double *__restrict__ x, double xscale,
-                                              double *__restrict__ xq, float *__restrict__ xq32,
-                                              double &xr)
+template <int D>
+__device__ __forceinline__ void stencil_cols(int row, int L, int *c)
 {
     constexpr int K = 2 * D + 1;
-    auto xv = [&](int c) -> double {
-        if constexpr (FUSE)
-            return (double)(x[c] * xscale);
-        else
-            return x[c];
-    };
-    int c[K];
     {
         const int xs = row % L, rest = row / L;
         const int ys = D == 2 ? rest : rest % L;
@@ -419,6 +408,24 @@ __device__ __forceinline__ double stencil_row(int row, int L, double off,
             c[i] = lo;
             c[i + 1] = hi;
         }
+}
+
+template <int D, bool FUSE>
+__device__ __forceinline__ double stencil_row(int row, int L, double off,
+                                              const double *__restrict__ dg,
+                                              const double *__restrict__ x, double xscale,
+                                              double *__restrict__ xq, float *__restrict__ xq32,
+                                              double &xr)
+{
+    constexpr int K = 2 * D + 1;
+    auto xv = [&](int c) -> double {
+        if constexpr (FUSE)
+            return (double)(x[c] * xscale);
+        else
+            return x[c];
+    };
+    int c[K];
+    stencil_cols<D>(row, L, c);
     xr = xv(row);
     if constexpr (FUSE) {
         if (xq) xq[row] = xr;
@@ -473,6 +480,8 @@ __device__ __forceinline__ void spmv_stencil_body(
     if (threadIdx.x == 0) part[bid] = bs;
 }
 
+constexpr int kStMaxL = 512;  // widest 2D lattice whose x window is staged in LDS
+
 // The batched form: workgroup w of a problem takes row blocks w * RB ..
 // w * RB + RB - 1 and writes each one's alpha partial to part[blk], as RB
 // separate one-row-block blocks would (same per-row arithmetic, same
@@ -490,17 +499,74 @@ __device__ __forceinline__ void spmv_stencil_rb(int n, int nb, int L, double off
     __shared__ double sm[RB][kBlock / 64];
     const int t = threadIdx.x;
     double acc[RB];
+    bool staged = false;
+    if constexpr (D == 2) {
+        // 2D lattices of side L <= kStMaxL: the workgroup's rows
+        // [R0, R0 + RB * kRowsBlk) read x only in the window
+        // [R0 - L, R0 + RB * kRowsBlk + L) but for the periodic wrap in y,
+        // so the window is staged (scaled) in LDS with coalesced loads and
+        // the 2D+1 neighbours are read from there: one global x load per
+        // row and a halo instead of five.  Same products in the same order.
+        staged = L <= kStMaxL;
+        if (staged) {
+            __shared__ double xw[RB * kRowsBlk + 2 * kStMaxL];
+            constexpr int kW = (RB * kRowsBlk + 2 * kStMaxL + kBlock - 1) / kBlock;
+            const int R0 = w * RB * kRowsBlk;
+            const int lo = max(0, R0 - L), hi = min(n, R0 + RB * kRowsBlk + L);
+            double xt[kW], dv[RB];
 #pragma unroll
-    for (int u = 0; u < RB; ++u) {
-        const int blk = w * RB + u;
-        acc[u] = 0.0;
-        if (blk < nb) {
-            const int r0 = blk * kRowsBlk, r1 = min(n, r0 + kRowsBlk);
-            if (t < r1 - r0) {
-                double xr;
-                const double s = stencil_row<D, true>(r0 + t, L, off, dg, x, xscale, xq, xq32, xr);
-                y[r0 + t] = s;
-                acc[u] = add_prod(0.0, xr, s);
+            for (int k = 0; k < kW; ++k) {
+                const int i = lo + k * kBlock + t;
+                if (i < hi) xt[k] = (double)(x[i] * xscale);
+            }
+#pragma unroll
+            for (int u = 0; u < RB; ++u) {
+                const int row = R0 + u * kRowsBlk + t;
+                if (row < n) dv[u] = dg[row];
+            }
+#pragma unroll
+            for (int k = 0; k < kW; ++k) {
+                const int i = lo + k * kBlock + t;
+                if (i < hi) xw[i - lo] = xt[k];
+            }
+            __syncthreads();
+            auto xv = [&](int c) -> double {
+                return (c >= lo && c < hi) ? xw[c - lo] : (double)(x[c] * xscale);
+            };
+#pragma unroll
+            for (int u = 0; u < RB; ++u) {
+                const int row = R0 + u * kRowsBlk + t;
+                acc[u] = 0.0;
+                if (row < n) {
+                    int c[2 * D + 1];
+                    stencil_cols<D>(row, L, c);
+                    const double xr = xw[row - lo];
+                    if (xq) xq[row] = xr;
+                    if (xq32) xq32[row] = (float)xr;
+                    double s = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 2 * D + 1; ++k)
+                        s = add_prod(s, c[k] == row ? dv[u] : off, xv(c[k]));
+                    y[row] = s;
+                    acc[u] = add_prod(0.0, xr, s);
+                }
+            }
+        }
+    }
+    if (!staged) {
+#pragma unroll
+        for (int u = 0; u < RB; ++u) {
+            const int blk = w * RB + u;
+            acc[u] = 0.0;
+            if (blk < nb) {
+                const int r0 = blk * kRowsBlk, r1 = min(n, r0 + kRowsBlk);
+                if (t < r1 - r0) {
+                    double xr;
+                    const double s =
+                        stencil_row<D, true>(r0 + t, L, off, dg, x, xscale, xq, xq32, xr);
+                    y[r0 + t] = s;
+                    acc[u] = add_prod(0.0, xr, s);
+                }
             }
         }
     }
