@@ -4399,6 +4399,50 @@ int trlhip_residuals(trlhip_ctx *c, const trlhip_csr *A, int k, const double *rh
     return 0;
 }
 
+// G = W^T W for the first m <= kGramMaxM FP64 basis columns W, in one
+// pass: block b takes the 64-row tiles b, b + gridDim.x, ... (staged in
+// LDS); thread t owns entries e = t + q * kBlock < m^2 (G(i, j) at
+// e = j m + i) and accumulates w_i[r] * w_j[r] over its tiles' rows in
+// order; partial of entry e at part[e * gridDim.x + b], summed in block
+// order by k_colsum.  G(i, j) and G(j, i) are the same products in the same
+// order, so G is exactly symmetric.
+constexpr int kGramMaxM = 32, kGramRows = 64, kGramBlocks = 256;
+
+__global__ __launch_bounds__(kBlock) void k_gram(int n, const double *__restrict__ V, long ld, int m,
+                                                 double *__restrict__ part)
+{
+    constexpr int EPT = kGramMaxM * kGramMaxM / kBlock;
+    __shared__ double w[kGramRows][kGramMaxM + 1];
+    const int t = threadIdx.x, mm = m * m;
+    double acc[EPT];
+    int ei[EPT], ej[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = t + q * kBlock;
+        acc[q] = 0.0;
+        ei[q] = e < mm ? e % m : 0;
+        ej[q] = e < mm ? e / m : 0;
+    }
+    for (int r0 = blockIdx.x * kGramRows; r0 < n; r0 += gridDim.x * kGramRows) {
+        const int rows = min(kGramRows, n - r0);
+        __syncthreads();  // the previous tile has been read
+        for (int idx = t; idx < kGramRows * m; idx += kBlock) {
+            const int col = idx / kGramRows, row = idx % kGramRows;
+            if (row < rows) w[row][col] = V[(size_t)col * ld + r0 + row];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < EPT; ++q)
+            if (t + q * kBlock < mm)
+                for (int r = 0; r < rows; ++r) acc[q] = fma(w[r][ei[q]], w[r][ej[q]], acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = t + q * kBlock;
+        if (e < mm) part[(size_t)e * gridDim.x + blockIdx.x] = acc[q];
+    }
+}
+
 int trlhip_gram(trlhip_ctx *c, const trlhip_csr *A, int m, int tmp, double *G, double *H)
 {
     if (!c || !G || !H || m < 1 || c->E != 1 || !csr_fits(c, A) || !valid_cols(c, 0, m) ||
@@ -4417,10 +4461,22 @@ int trlhip_gram(trlhip_ctx *c, const trlhip_csr *A, int m, int tmp, double *G, d
         if (hipHostMalloc((void **)&c->hY, sizeof(double) * 2 * mm, 0) != hipSuccess) return TRLHIP_ENOMEM;
         c->dYcap = 2 * mm;
     }
-    // column j of G and of H by the projection kernels (trlhip_proj's sums),
-    // both matrices back in one copy
+    // G in one pass over W (k_gram + k_colsum) when m <= kGramMaxM, else
+    // column by column with the projection kernels; column j of H = W^T A w_j
+    // by the projection kernels; both matrices back in one copy
+    const size_t avail = c->partcap - ((size_t)kSpmvGrid + 2 * (size_t)kMaxGrid);
+    const int gb = (int)std::min<size_t>({(size_t)(c->n + kGramRows - 1) / kGramRows,
+                                          avail / mm, (size_t)kGramBlocks});
+    const bool one_pass = m <= kGramMaxM && gb >= 1;
+    if (one_pass) {
+        KLAUNCH(k_gram, dim3(gb), dim3(kBlock), 0, c->s, c->n, colp<double>(c, 0), (long)c->ld, m,
+                part_proj(c));
+        KLAUNCH(k_colsum, dim3((unsigned)mm), dim3(kBlock), 0, c->s, part_proj(c), gb, c->dY);
+        if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
+    }
     for (int j = 0; j < m; ++j) {
-        if (int rc = run_proj(c, 0, m, j, c->dY + (size_t)j * m)) return rc;
+        if (!one_pass)
+            if (int rc = run_proj(c, 0, m, j, c->dY + (size_t)j * m)) return rc;
         if (int rc = run_spmv(c, A, j, tmp)) return rc;
         if (int rc = run_proj(c, 0, m, tmp, c->dY + mm + (size_t)j * m)) return rc;
     }
