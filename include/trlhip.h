@@ -272,8 +272,10 @@ int trlhip_refine(trlhip_ctx *c, const trlhip_csr *A, int k, int avoid,
  * v_i for the columns v_i, i < k, and rr2[i] = its squared norm (the same
  * kernels and sums as trlhip_spmv then trlhip_update per column).
  * trlhip_gram: G = Q'Q and H = Q'AQ (m x m, column-major) over the
- * columns Q = [0, m), with A Q_j formed in column tmp >= m (the same sums
- * as trlhip_proj per column).  No counterpart in the reference: the polish
+ * columns Q = [0, m), with A Q_j formed in column tmp >= m: H by the
+ * same sums as trlhip_proj per column, G (for m <= 32) in one pass over Q
+ * with fixed-order per-block sums, exactly symmetric, else as H).  No
+ * counterpart in the reference: the polish
  * follows the final refinement (trlhip_refine). */
 int trlhip_residuals(trlhip_ctx *c, const trlhip_csr *A, int k, const double *rho,
                      double *rr2);
