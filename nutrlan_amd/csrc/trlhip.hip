@@ -1077,7 +1077,19 @@ __device__ __forceinline__ double update_rows4(int row, const VS *__restrict__ V
     return nrm;
 }
 
-template <typename VS, typename RS, int E, int R>
+// One complex basis entry; NT streams it (the recurrence's columns).
+template <bool NT>
+__device__ __forceinline__ double2 ldc(const double2 *p)
+{
+    if constexpr (NT) {
+        const f64x2 t = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(p));
+        return make_double2(t.x, t.y);
+    } else {
+        return *p;
+    }
+}
+
+template <typename VS, typename RS, int E, int R, bool NTV = false>
 __device__ __forceinline__ double update_rows(int row, const VS *__restrict__ Vc, long ld, int m,
                                               const double *hs, const RS *rin, RS *rout)
 {
@@ -1119,7 +1131,7 @@ __device__ __forceinline__ double update_rows(int row, const VS *__restrict__ Vc
 #pragma unroll
             for (int j = 0; j < R; ++j)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) v[j][q] = V2[(size_t)(c + q) * ld + row + j * kBlock];
+                for (int q = 0; q < 4; ++q) v[j][q] = ldc<NTV>(V2 + (size_t)(c + q) * ld + row + j * kBlock);
 #pragma unroll
             for (int j = 0; j < R; ++j)
 #pragma unroll
@@ -1135,7 +1147,7 @@ __device__ __forceinline__ double update_rows(int row, const VS *__restrict__ Vc
         for (; c < m; ++c)
 #pragma unroll
             for (int j = 0; j < R; ++j) {
-                const double2 v = V2[(size_t)c * ld + row + j * kBlock];
+                const double2 v = ldc<NTV>(V2 + (size_t)c * ld + row + j * kBlock);
                 ar[j] += v.x * hs[2 * c] - v.y * hs[2 * c + 1];
                 ai[j] += v.x * hs[2 * c + 1] + v.y * hs[2 * c];
             }
@@ -1191,11 +1203,11 @@ __device__ __forceinline__ void update_body(
         const int stride = nblk * kBlock * RU;
         int row = bid * kBlock * RU + threadIdx.x;
         for (; row + (RU - 1) * kBlock < n; row += stride)
-            acc += update_rows<VS, RS, E, RU>(row, Vc, ld, m, hs, rin, rout);
+            acc += update_rows<VS, RS, E, RU, NTV>(row, Vc, ld, m, hs, rin, rout);
 #pragma unroll
         for (int j = 0; j < RU - 1; ++j)
             if (row + j * kBlock < n)
-                acc += update_rows<VS, RS, E, 1>(row + j * kBlock, Vc, ld, m, hs, rin, rout);
+                acc += update_rows<VS, RS, E, 1, NTV>(row + j * kBlock, Vc, ld, m, hs, rin, rout);
     }
     const double bs = block_sum(acc);
     if (threadIdx.x == 0) part[bid] = bs;
@@ -2145,9 +2157,12 @@ int run_update(trlhip_ctx *c, int c0, int m, const double *dh, int aslot, int ri
         else if (c->E == 1)  // the recurrence (and other short updates): streamed columns
             launch_update<double, 1, 4, true>(c, g, colp<double>(c, 0), c0, m, dh, h0, aslot, rin,
                                               rout, o_hh, pdst);
-        else
+        else if (shadow)  // complex Gram-Schmidt update
             launch_update<double, 2, 4>(c, g, colp<double>(c, 0), c0, m, dh, h0, aslot, rin, rout,
                                         o_hh, pdst);
+        else  // complex recurrence: streamed columns
+            launch_update<double, 2, 4, true>(c, g, colp<double>(c, 0), c0, m, dh, h0, aslot, rin,
+                                              rout, o_hh, pdst);
         if (hipGetLastError() != hipSuccess) return TRLHIP_EHIP;
     }
     return o_rr2 ? launch_sum(pdst, g, o_rr2, c->s) : 0;
